@@ -1,0 +1,203 @@
+/*
+ * pbbfmm3d_amd.h -- C ABI of the MI355X (gfx950) black-box FMM matvec.
+ *
+ * Plain C types only: pointers, sizes and status codes.  Every entry point
+ * names the PBBFMM3D interface it replaces (paths relative to the PBBFMM3D
+ * source tree).  The C++ drop-in headers (H2_3D_Tree.hpp, kernel_Types.hpp,
+ * compute.hpp) are thin wrappers over these calls; Python binds them with
+ * ctypes.
+ *
+ * Model: one point set is both the sources and the targets (the reference
+ * supports nothing else: compute.hpp:1195-1196).  Weights and outputs are
+ * column-major, w[c*N + j] and out[c*N + i] (compute.hpp:316, :747), and
+ * outputs are ACCUMULATED (+=), as compute.hpp:747 does.
+ *
+ * Errors: every call returns a pbb_status; pbb_last_error() returns a
+ * thread-local message for the last failing call on that thread.  Nothing
+ * calls exit() or abort().
+ */
+#ifndef PBBFMM3D_AMD_H
+#define PBBFMM3D_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBB_API __attribute__((visibility("default")))
+
+#define PBB_ABI_VERSION 1
+
+typedef enum pbb_status {
+    PBB_OK = 0,
+    PBB_ERR_INVALID_ARG = 1,  /* bad size, null pointer, bad option      */
+    PBB_ERR_UNSUPPORTED = 2,  /* valid but not implemented here          */
+    PBB_ERR_HIP = 3,          /* a HIP runtime call failed               */
+    PBB_ERR_OUT_OF_MEMORY = 4,
+    PBB_ERR_NO_DEVICE = 5,    /* no usable gfx950 device                 */
+    PBB_ERR_INTERNAL = 6
+} pbb_status;
+
+/* Device kernel functors (the P2P near field runs these on the GPU; the
+ * operator precompute evaluates them on the host in FP64 unless a host
+ * callback is given).  r is |source - target|.                          */
+typedef enum pbb_kernel_id {
+    PBB_KERNEL_NONE = 0,           /* host callback only: no device P2P     */
+    PBB_KERNEL_LAPLACIAN = 1,      /* 1/r          kernel_Types.cpp:15-28   */
+    PBB_KERNEL_ONEOVERR2 = 2,      /* 1/r^2        kernel_Types.cpp:109-121 */
+    PBB_KERNEL_GAUSSIAN = 3,       /* exp(-r^2/p0^2) (p0=1: :64-74)         */
+    PBB_KERNEL_EXPONENTIAL = 4,    /* p1*exp(-r/p0)                         */
+    PBB_KERNEL_SOFT_LAPLACIAN = 5, /* 1/sqrt(r^2+p0)                        */
+    PBB_KERNEL_QUADRIC = 6,        /* 1+r^2        kernel_Types.cpp:131-138 */
+    PBB_KERNEL_INV_QUADRIC = 7,    /* 1/(1+r^2)    kernel_Types.cpp:148-155 */
+    PBB_KERNEL_LOGARITHM = 8,      /* 0.5 log r^2  kernel_Types.cpp:85-99   */
+    PBB_KERNEL_THIN_PLATE = 9,     /* 0.5 r^2 log r^2  :165-178             */
+    PBB_KERNEL_R2 = 10,            /* r^2 (degree-2 homogeneous test kernel)*/
+    PBB_KERNEL_CONSTANT = 11,      /* 1 (test kernel)                       */
+    PBB_KERNEL_POLY_XY = 12        /* (tx-sx)(ty+sy)+p0 (non-symmetric test)*/
+} pbb_kernel_id;
+
+/* What H2_3D_Tree::SetKernelProperty sets (H2_3D_Tree.hpp:96,
+ * kernel_Types.cpp:7-13) plus the device functor.                        */
+typedef struct pbb_kernel_desc {
+    int32_t id;        /* pbb_kernel_id                                    */
+    int32_t symmetry;  /* 1 symmetric, -1 antisymmetric, 0 neither         */
+    double homogen;    /* degree as declared (1/r: -1); |h| > 0.1 selects
+                          one unit-length operator set (H2_3D_Tree.cpp:8,
+                          :134-136), else one set per level 2..tree_level  */
+    double param[4];   /* functor parameters (see pbb_kernel_id)           */
+} pbb_kernel_desc;
+
+/* Host FP64 kernel evaluation K(target, source); stands in for the virtual
+ * H2_3D_Tree::EvaluateKernel (H2_3D_Tree.hpp:94) during the precompute.   */
+typedef double (*pbb_host_kernel_fn)(void *ctx, const double *target3,
+                                     const double *source3);
+
+typedef struct pbb_ops pbb_ops;   /* operators: weights, U, VT, E          */
+typedef struct pbb_plan pbb_plan; /* one point set's tree, device-resident */
+
+typedef struct pbb_ops_info {
+    int32_t order;        /* n                                            */
+    int32_t n3;           /* n^3                                          */
+    int32_t rank_f;       /* cutoff.f                                     */
+    int32_t rank_s;       /* cutoff.s                                     */
+    int32_t nsets;        /* operator sets (1 or tree_level-1)            */
+    int32_t homogeneous;  /* 1 if one unit-length set                     */
+    int32_t tree_level;
+    int32_t reserved;
+    double L;
+    double homogen;       /* as declared                                  */
+    double eps;
+} pbb_ops_info;
+
+/* Precision modes of pbb_plan_matvec.                                    */
+typedef enum pbb_precision {
+    PBB_PREC_FP64 = 0,    /* every phase in FP64 (the drop-in path)       */
+    PBB_PREC_MIXED = 1    /* FP32 expansions + split-BF16 MFMA M2L, FP32
+                             P2P with compensated sums, FP64 output sum   */
+} pbb_precision;
+
+#define PBB_NUM_PHASES 8
+/* phase order of pbb_plan_timings: P2M, M2M, VT proxy, M2L, U expand, L2L,
+ * L2P+P2P, total (milliseconds, HIP events on the plan's stream)         */
+
+typedef struct pbb_plan_info {
+    int64_t npoints;
+    int32_t tree_level;
+    int32_t nonempty_levels;
+    int64_t boxes[16];      /* non-empty boxes per level 0..tree_level     */
+    int64_t m2l_pairs;      /* far-field (target, source) box pairs        */
+    int64_t near_pairs;     /* ordered near-field point pairs incl. self   */
+    int64_t max_leaf_points;
+    double center[3];       /* bbox centre subtracted (compute.hpp:108-111)*/
+    int64_t outside_points; /* points outside the root cube (extrapolated
+                               as the reference does)                      */
+} pbb_plan_info;
+
+/* ---- library ---------------------------------------------------------- */
+
+/* Thread-local message for the last failing call on this thread.          */
+PBB_API const char *pbb_last_error(void);
+PBB_API int pbb_abi_version(void);
+/* Number of visible gfx950 devices (0 without a GPU; never fails).        */
+PBB_API int pbb_device_count(void);
+
+/* ---- operators -------------------------------------------------------- */
+
+/* Replaces H2_3D_Tree::H2_3D_Tree + buildFMMTree (H2_3D_Tree.cpp:12-66):
+ * Chebyshev weights (ComputeWeights :715-826) and the SVD-compressed M2L
+ * operators (PrecomputeM2L :74-99, ComputeKernelCheb :251-575), computed on
+ * the host in FP64, then uploaded to the current device.  host_eval may be
+ * NULL, then kernel->id must name a functor.  use_chebyshev must be 1: the
+ * uniform-grid/FFT branch is not provided.                                */
+PBB_API pbb_status pbb_ops_create(const pbb_kernel_desc *kernel,
+                                  pbb_host_kernel_fn host_eval, void *ctx,
+                                  double L, int32_t tree_level, int32_t order,
+                                  double eps, int32_t use_chebyshev,
+                                  pbb_ops **out);
+
+/* Builds operators from given arrays (the layout FMMReadMatrices reads,
+ * H2_3D_Tree.cpp:186-239): U nsets x (n3 x rank_f), VT nsets x (rank_s x
+ * n3), E nsets x 316 x (rank_f x rank_s), all column-major FP64.          */
+PBB_API pbb_status pbb_ops_create_from_arrays(
+    const pbb_kernel_desc *kernel, double L, int32_t tree_level, int32_t order,
+    double eps, int32_t rank_f, int32_t rank_s, int32_t nsets, const double *U,
+    const double *VT, const double *E, pbb_ops **out);
+
+PBB_API pbb_status pbb_ops_query(const pbb_ops *ops, pbb_ops_info *info);
+
+/* Copies the host operators out (any pointer may be NULL).  sigma_f and
+ * sigma_s receive the n3 singular values of K_fat and K_thin of the first
+ * operator set (H2_3D_Tree.cpp:400, :471).                               */
+PBB_API pbb_status pbb_ops_export(const pbb_ops *ops, double *U, double *VT,
+                                  double *E, double *sigma_f, double *sigma_s);
+
+PBB_API void pbb_ops_destroy(pbb_ops *ops);
+
+/* ---- plans (prepared tree, device resident) --------------------------- */
+
+/* Replaces FMMDistribute + InteractionList (compute.hpp:157-190,
+ * :1042-1203) and the bbox re-centring (compute.hpp:92-127) for one point
+ * set (host AoS xyz, N points).  The tree is kept on the device; unlike
+ * the reference a plan can serve any number of matvecs.                  */
+PBB_API pbb_status pbb_plan_create(pbb_ops *ops, const double *xyz,
+                                   int64_t npoints, pbb_plan **out);
+
+PBB_API pbb_status pbb_plan_query(const pbb_plan *plan, pbb_plan_info *info);
+
+/* Replaces FMMCompute (compute.hpp:192-222): upward pass, M2L, downward
+ * pass, near field.  w_dev: device FP64 weights, column-major N x ncols;
+ * out_dev: device FP64, column-major N x ncols, accumulated (+=).
+ * stream: a hipStream_t (NULL = the plan's own stream).  Asynchronous with
+ * respect to the host; the call returns once all work is enqueued.        */
+PBB_API pbb_status pbb_plan_matvec(pbb_plan *plan, const double *w_dev,
+                                   int32_t ncols, double *out_dev,
+                                   int32_t precision, void *stream);
+
+/* Phase times of the last pbb_plan_matvec on this plan, in ms
+ * (PBB_NUM_PHASES values).  Synchronises the plan's events.               */
+PBB_API pbb_status pbb_plan_timings(pbb_plan *plan, double *ms);
+
+/* Enables (1) or disables (0, default) per-phase event timing.            */
+PBB_API pbb_status pbb_plan_set_timing(pbb_plan *plan, int32_t enable);
+
+PBB_API void pbb_plan_destroy(pbb_plan *plan);
+
+/* ---- one-shot drop-in ------------------------------------------------- */
+
+/* Replaces the H2_3D_Compute<T> constructor (compute.hpp:67-149): host
+ * FP64 buffers, target and source must hold the same N points, weights
+ * column-major N x ncols, output accumulated.  Builds a plan, runs the
+ * FP64 matvec, copies back, frees the plan.                              */
+PBB_API pbb_status pbb_matvec(pbb_ops *ops, const double *target,
+                              const double *source, int64_t npoints,
+                              const double *weight, int32_t ncols,
+                              double *output);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PBBFMM3D_AMD_H */

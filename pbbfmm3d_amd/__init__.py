@@ -1,0 +1,13 @@
+"""pbbfmm3d_amd -- MI355X (gfx950) black-box FMM matvec with PBBFMM3D's API.
+
+The compute path is hand-written HIP in ``csrc/`` behind the C ABI of
+``include/pbbfmm3d_amd.h``; this package binds it with ctypes.
+"""
+from . import _capi as capi
+from .core import (Kernel, Operators, Plan, constant, exponential, gaussian, inverse_quadric,
+                   laplacian, logarithm, matvec_host, oneoverr2, quadric, r2_homogeneous,
+                   soft_laplacian, thin_plate_spline)
+
+__all__ = ["capi", "Kernel", "Operators", "Plan", "matvec_host", "laplacian", "oneoverr2",
+           "gaussian", "exponential", "soft_laplacian", "quadric", "inverse_quadric", "logarithm",
+           "thin_plate_spline", "r2_homogeneous", "constant"]

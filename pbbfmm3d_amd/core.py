@@ -1,0 +1,267 @@
+"""Python objects over the C ABI: kernels, operators, plans.
+
+``Operators`` is what ``H2_3D_Tree::buildFMMTree`` produces
+(H2_3D_Tree.cpp:40-66); ``Plan`` is one point set's tree, kept on the GPU
+(FMMDistribute, compute.hpp:157-190); ``Plan.matvec`` is FMMCompute
+(compute.hpp:192-222) on device-resident weights.  Weights and outputs are
+column-major ``w[c*N + i]``, outputs accumulated, as in compute.hpp:747.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+from . import _capi as capi
+
+
+@dataclass(frozen=True)
+class Kernel:
+    """A kernel with a device functor (pbb_kernel_id) and the properties
+    SetKernelProperty sets (kernel_Types.cpp:7-13)."""
+    id: int
+    symmetry: int = 1
+    homogen: float = 0.0
+    params: Sequence[float] = ()
+
+    def desc(self) -> capi.KernelDesc:
+        d = capi.KernelDesc()
+        d.id = self.id
+        d.symmetry = self.symmetry
+        d.homogen = float(self.homogen)
+        for i, v in enumerate(tuple(self.params)[:4]):
+            d.param[i] = float(v)
+        return d
+
+    def eval(self, t, s) -> float:
+        """FP64 reference formula of the functor, K(target, source)."""
+        p = list(self.params) + [0.0] * (4 - len(self.params))
+        d = np.asarray(s, dtype=np.float64) - np.asarray(t, dtype=np.float64)
+        r2 = float(d @ d)
+        i = self.id
+        with np.errstate(divide="ignore"):
+            if i == capi.KERNEL_LAPLACIAN: return 1.0 / np.sqrt(r2)
+            if i == capi.KERNEL_ONEOVERR2: return (1.0 / np.sqrt(r2)) ** 2
+            if i == capi.KERNEL_GAUSSIAN: return float(np.exp(-r2 / p[0] ** 2))
+            if i == capi.KERNEL_EXPONENTIAL: return p[1] * float(np.exp(-np.sqrt(r2) / p[0]))
+            if i == capi.KERNEL_SOFT_LAPLACIAN: return 1.0 / np.sqrt(r2 + p[0])
+            if i == capi.KERNEL_QUADRIC: return 1.0 + r2
+            if i == capi.KERNEL_INV_QUADRIC: return 1.0 / (1.0 + r2)
+            if i == capi.KERNEL_LOGARITHM: return 0.0 if r2 == 0 else 0.5 * np.log(r2)
+            if i == capi.KERNEL_THIN_PLATE: return 0.0 if r2 == 0 else 0.5 * r2 * np.log(r2)
+            if i == capi.KERNEL_R2: return r2
+            if i == capi.KERNEL_CONSTANT: return 1.0
+        raise ValueError(f"unknown kernel id {i}")
+
+
+# The reference's built-in kernels (kernel_Types.cpp) and the parameterised
+# kernels the workloads use.
+def laplacian() -> Kernel:
+    return Kernel(capi.KERNEL_LAPLACIAN, 1, -1.0)
+
+
+def oneoverr2() -> Kernel:
+    return Kernel(capi.KERNEL_ONEOVERR2, 1, -2.0)
+
+
+def gaussian(length: float = 1.0) -> Kernel:
+    """exp(-r^2 / length^2); length 1 is kernel_Gaussian."""
+    return Kernel(capi.KERNEL_GAUSSIAN, 1, 0.0, (length,))
+
+
+def exponential(length: float = 1.0, scale: float = 1.0) -> Kernel:
+    """scale * exp(-r / length)."""
+    return Kernel(capi.KERNEL_EXPONENTIAL, 1, 0.0, (length, scale))
+
+
+def soft_laplacian(delta: float) -> Kernel:
+    """1 / sqrt(r^2 + delta)."""
+    return Kernel(capi.KERNEL_SOFT_LAPLACIAN, 1, 0.0, (delta,))
+
+
+def quadric() -> Kernel:
+    return Kernel(capi.KERNEL_QUADRIC, 1, 0.0)
+
+
+def inverse_quadric() -> Kernel:
+    return Kernel(capi.KERNEL_INV_QUADRIC, 1, 0.0)
+
+
+def logarithm() -> Kernel:
+    return Kernel(capi.KERNEL_LOGARITHM, 1, 0.0)
+
+
+def thin_plate_spline() -> Kernel:
+    return Kernel(capi.KERNEL_THIN_PLATE, 1, 0.0)
+
+
+def r2_homogeneous() -> Kernel:
+    """r^2 declared homogeneous of degree 2 (a test kernel)."""
+    return Kernel(capi.KERNEL_R2, 1, 2.0)
+
+
+def constant() -> Kernel:
+    return Kernel(capi.KERNEL_CONSTANT, 1, 0.0)
+
+
+class Operators:
+    """Chebyshev weights + SVD-compressed M2L operators (host FP64),
+    uploaded to the GPU when the first plan is made."""
+
+    def __init__(self, kernel: Kernel, L: float, tree_level: int, order: int, eps: float,
+                 use_chebyshev: int = 1,
+                 host_eval: Optional[Callable[[np.ndarray, np.ndarray], float]] = None):
+        self.kernel = kernel
+        self._cb = None
+        L_ = capi.lib()
+        h = C.c_void_p()
+        if host_eval is not None:
+            def _fn(ctx, t, s):
+                return float(host_eval(np.ctypeslib.as_array(t, (3,)),
+                                       np.ctypeslib.as_array(s, (3,))))
+            self._cb = capi.HOST_KERNEL_FN(_fn)
+            cb = self._cb
+        else:
+            cb = capi.HOST_KERNEL_FN()
+        d = kernel.desc()
+        capi.check(L_.pbb_ops_create(C.byref(d), cb, None, float(L), int(tree_level), int(order),
+                                     float(eps), int(use_chebyshev), C.byref(h)))
+        self._h = h
+        self._info()
+
+    @classmethod
+    def from_arrays(cls, kernel: Kernel, L, tree_level, order, eps, rank_f, rank_s, nsets,
+                    U, VT, E) -> "Operators":
+        self = cls.__new__(cls)
+        self.kernel = kernel
+        self._cb = None
+        h = C.c_void_p()
+        d = kernel.desc()
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        VT = np.ascontiguousarray(VT, dtype=np.float64)
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        dp = C.POINTER(C.c_double)
+        capi.check(capi.lib().pbb_ops_create_from_arrays(
+            C.byref(d), float(L), int(tree_level), int(order), float(eps), int(rank_f),
+            int(rank_s), int(nsets), U.ctypes.data_as(dp), VT.ctypes.data_as(dp),
+            E.ctypes.data_as(dp), C.byref(h)))
+        self._h = h
+        self._info()
+        return self
+
+    def _info(self):
+        info = capi.OpsInfo()
+        capi.check(capi.lib().pbb_ops_query(self._h, C.byref(info)))
+        self.order = info.order
+        self.n3 = info.n3
+        self.rank_f = info.rank_f
+        self.rank_s = info.rank_s
+        self.nsets = info.nsets
+        self.homogeneous = bool(info.homogeneous)
+        self.tree_level = info.tree_level
+        self.L = info.L
+        self.homogen = info.homogen
+        self.eps = info.eps
+
+    def export(self):
+        """(U, VT, E, sigma_f, sigma_s) as flat column-major arrays."""
+        n3, rf, rs, ns = self.n3, self.rank_f, self.rank_s, self.nsets
+        U = np.zeros(ns * n3 * rf)
+        VT = np.zeros(ns * rs * n3)
+        E = np.zeros(ns * 316 * rf * rs)
+        sf = np.zeros(n3)
+        ss = np.zeros(n3)
+        dp = C.POINTER(C.c_double)
+        capi.check(capi.lib().pbb_ops_export(self._h, U.ctypes.data_as(dp), VT.ctypes.data_as(dp),
+                                             E.ctypes.data_as(dp), sf.ctypes.data_as(dp),
+                                             ss.ctypes.data_as(dp)))
+        return U, VT, E, sf, ss
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and capi._lib is not None:
+            capi.lib().pbb_ops_destroy(h)
+            self._h = None
+
+
+class Plan:
+    """One point set's tree on the current GPU (sources are the targets)."""
+
+    def __init__(self, ops: Operators, xyz):
+        self.ops = ops
+        xyz = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
+        self.npoints = xyz.shape[0]
+        h = C.c_void_p()
+        capi.check(capi.lib().pbb_plan_create(ops.handle, xyz.ctypes.data_as(C.POINTER(C.c_double)),
+                                              self.npoints, C.byref(h)))
+        self._h = h
+
+    def info(self) -> dict:
+        I = capi.PlanInfo()
+        capi.check(capi.lib().pbb_plan_query(self._h, C.byref(I)))
+        return {"npoints": I.npoints, "tree_level": I.tree_level,
+                "boxes": [I.boxes[l] for l in range(I.tree_level + 1)],
+                "m2l_pairs": I.m2l_pairs, "near_pairs": I.near_pairs,
+                "max_leaf_points": I.max_leaf_points, "center": tuple(I.center),
+                "outside_points": I.outside_points}
+
+    def matvec_ptr(self, w_ptr: int, ncols: int, out_ptr: int, precision: int = capi.PREC_FP64,
+                   stream: int = 0) -> None:
+        """Enqueues out += A w on raw device pointers (FP64, column-major)."""
+        capi.check(capi.lib().pbb_plan_matvec(self._h, C.c_void_p(w_ptr), int(ncols),
+                                              C.c_void_p(out_ptr), int(precision),
+                                              C.c_void_p(stream) if stream else None))
+
+    def matvec(self, w, out=None, precision: int = capi.PREC_FP64):
+        """torch CUDA tensors: w (ncols, N) or (N,) float64, contiguous;
+        returns out (+=) of the same shape, on torch's current stream."""
+        import torch
+        if w.dtype != torch.float64 or not w.is_cuda:
+            raise TypeError("w must be a float64 CUDA tensor")
+        w = w.contiguous()
+        ncols = 1 if w.dim() == 1 else w.shape[0]
+        if w.numel() != ncols * self.npoints:
+            raise ValueError("w must hold ncols * npoints values")
+        if out is None:
+            out = torch.zeros_like(w)
+        elif not out.is_contiguous() or out.shape != w.shape or out.dtype != torch.float64:
+            raise ValueError("out must be a contiguous float64 tensor shaped like w")
+        stream = torch.cuda.current_stream(w.device).cuda_stream
+        self.matvec_ptr(w.data_ptr(), ncols, out.data_ptr(), precision, stream)
+        return out
+
+    def set_timing(self, enable: bool = True):
+        capi.check(capi.lib().pbb_plan_set_timing(self._h, 1 if enable else 0))
+
+    def timings(self) -> dict:
+        ms = np.zeros(capi.NUM_PHASES)
+        capi.check(capi.lib().pbb_plan_timings(self._h, ms.ctypes.data_as(C.POINTER(C.c_double))))
+        return dict(zip(capi.PHASE_NAMES, ms.tolist()))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and capi._lib is not None:
+            capi.lib().pbb_plan_destroy(h)
+            self._h = None
+
+
+def matvec_host(ops: Operators, xyz, weight, ncols: int, output=None) -> np.ndarray:
+    """The drop-in path (pbb_matvec): host FP64 arrays, output accumulated."""
+    xyz = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
+    N = xyz.shape[0]
+    w = np.ascontiguousarray(np.asarray(weight, dtype=np.float64).reshape(-1))
+    if w.size != N * ncols:
+        raise ValueError("weight must hold ncols * N values")
+    if output is None:
+        output = np.zeros(N * ncols)
+    dp = C.POINTER(C.c_double)
+    p = xyz.ctypes.data_as(dp)
+    capi.check(capi.lib().pbb_matvec(ops.handle, p, p, N, w.ctypes.data_as(dp), int(ncols),
+                                     output.ctypes.data_as(dp)))
+    return output

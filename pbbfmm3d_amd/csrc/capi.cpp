@@ -1,0 +1,593 @@
+// capi.cpp -- the C ABI (include/pbbfmm3d_amd.h): operator objects, plans
+// (device-resident trees) and the launch sequence of one matvec.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+#include "host.hpp"
+#include "kernels.hpp"
+
+using namespace pbb;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+pbb_status set_error(pbb_status st, const std::string &msg)
+{
+    g_last_error = msg;
+    return st;
+}
+
+template <typename F>
+pbb_status guarded(F &&f)
+{
+    try {
+        f();
+        return PBB_OK;
+    } catch (const Error &e) {
+        return set_error(e.status, e.msg);
+    } catch (const std::bad_alloc &) {
+        return set_error(PBB_ERR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::exception &e) {
+        return set_error(PBB_ERR_INTERNAL, e.what());
+    }
+}
+
+template <typename T>
+T *dalloc(size_t count, const char *what)
+{
+    if (count == 0) count = 1;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(e == hipErrorOutOfMemory ? PBB_ERR_OUT_OF_MEMORY : PBB_ERR_HIP,
+             std::string("hipMalloc(") + what + ", " + std::to_string(count * sizeof(T)) + " bytes): " +
+                 hipGetErrorString(e));
+    }
+    return (T *)p;
+}
+
+template <typename T>
+T *dupload(const std::vector<T> &v, const char *what)
+{
+    T *p = dalloc<T>(v.size(), what);
+    if (!v.empty()) check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), what);
+    return p;
+}
+
+void dfree(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+int current_device()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        fail(PBB_ERR_NO_DEVICE, "no HIP device is visible");
+    }
+    int d = 0;
+    check(hipGetDevice(&d), "hipGetDevice");
+    return d;
+}
+
+// Makes `dev` current for the scope, restoring the caller's device after.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        check(hipGetDevice(&prev), "hipGetDevice");
+        if (prev != dev) check(hipSetDevice(dev), "hipSetDevice");
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+struct pbb_ops {
+    HostOps h;
+    std::mutex mu;
+    int device = -1;
+    double *d_Tkz = nullptr, *d_Cw = nullptr, *d_VTw = nullptr, *d_Uw = nullptr, *d_E = nullptr;
+
+    void upload(int dev)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (device == dev) return;
+        if (device >= 0) fail(PBB_ERR_UNSUPPORTED, "operators are already resident on another device");
+        const int n3 = h.n3, rf = h.rf, rs = h.rs;
+        std::vector<double> VTw(h.VT.size()), Uw(h.U.size());
+        for (int s = 0; s < h.nsets; s++)
+            for (int q = 0; q < n3; q++) {
+                for (int a = 0; a < rs; a++) {
+                    size_t i = (size_t)s * rs * n3 + a + (size_t)q * rs;
+                    VTw[i] = h.VT[i] * h.w.Kw[q];
+                }
+                for (int a = 0; a < rf; a++) {
+                    size_t i = (size_t)s * n3 * rf + q + (size_t)a * n3;
+                    Uw[i] = h.U[i] * h.w.Kw[q];
+                }
+            }
+        d_Tkz = dupload(h.w.Tkz, "Tkz");
+        d_Cw = dupload(h.w.Cw, "Cweights");
+        d_VTw = dupload(VTw, "VT");
+        d_Uw = dupload(Uw, "U");
+        d_E = dupload(h.E, "E");
+        device = dev;
+    }
+    ~pbb_ops()
+    {
+        if (device >= 0) {
+            int prev = -1;
+            (void)hipGetDevice(&prev);
+            (void)hipSetDevice(device);
+            dfree(d_Tkz); dfree(d_Cw); dfree(d_VTw); dfree(d_Uw); dfree(d_E);
+            if (prev >= 0) (void)hipSetDevice(prev);
+        }
+    }
+};
+
+struct DevLevel {
+    int64_t nb = 0, off = 0;
+    int32_t *coord = nullptr, *map = nullptr, *child = nullptr, *order = nullptr;
+    double *ctab = nullptr;
+    double len = 0;
+    int64_t class_count[8] = {0};
+};
+
+struct pbb_plan {
+    pbb_ops *ops = nullptr;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int Lv = 0;
+    int64_t N = 0;
+    pbb_plan_info info{};
+    double *x = nullptr, *y = nullptr, *z = nullptr;
+    int32_t *perm = nullptr, *leaf_start = nullptr;
+    std::vector<DevLevel> lv;
+    int64_t Btot = 0;
+    int ws_ncols = 0;
+    double *ws = nullptr, *S = nullptr, *P = nullptr, *Pf = nullptr;
+    bool timing = false;
+    hipEvent_t ev[PBB_NUM_PHASES] = {};
+    bool events_made = false;
+    double last_ms[PBB_NUM_PHASES] = {0};
+
+    ~pbb_plan()
+    {
+        if (device < 0) return;
+        int prev = -1;
+        (void)hipGetDevice(&prev);
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        dfree(x); dfree(y); dfree(z); dfree(perm); dfree(leaf_start);
+        for (auto &L : lv) {
+            dfree(L.coord); dfree(L.map); dfree(L.child); dfree(L.order); dfree(L.ctab);
+        }
+        dfree(ws); dfree(S); dfree(P); dfree(Pf);
+        if (events_made)
+            for (auto &e : ev) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+
+    void ensure_workspace(int ncols)
+    {
+        if (ncols <= ws_ncols) return;
+        dfree(ws); dfree(S); dfree(P); dfree(Pf);
+        ws = S = P = Pf = nullptr;
+        ws_ncols = 0;
+        const HostOps &h = ops->h;
+        ws = dalloc<double>((size_t)N * ncols, "sorted weights");
+        if (Lv >= 2) {
+            S = dalloc<double>((size_t)Btot * ncols * h.n3, "expansions");
+            P = dalloc<double>((size_t)Btot * ncols * h.rs, "proxies");
+            Pf = dalloc<double>((size_t)Btot * ncols * h.rf, "M2L results");
+        }
+        ws_ncols = ncols;
+    }
+};
+
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+PBB_API const char *pbb_last_error(void) { return g_last_error.c_str(); }
+
+PBB_API int pbb_abi_version(void) { return PBB_ABI_VERSION; }
+
+PBB_API int pbb_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+static void validate_params(const pbb_kernel_desc *kernel, double L, int tree_level, int order, double eps,
+                            int use_chebyshev)
+{
+    if (!kernel) fail(PBB_ERR_INVALID_ARG, "kernel descriptor is NULL");
+    if (use_chebyshev != 1)
+        fail(PBB_ERR_UNSUPPORTED, "only the Chebyshev branch (use_chebyshev = 1) is provided");
+    if (!(L > 0) || !std::isfinite(L)) fail(PBB_ERR_INVALID_ARG, "L must be positive");
+    if (tree_level < 1 || tree_level > 9) fail(PBB_ERR_INVALID_ARG, "tree_level must be in [1, 9]");
+    if (order < 2 || order > 12) fail(PBB_ERR_INVALID_ARG, "interpolation order must be in [2, 12]");
+    if (!(eps >= 0) || !(eps < 1)) fail(PBB_ERR_INVALID_ARG, "eps must be in [0, 1)");
+    if (kernel->symmetry < -1 || kernel->symmetry > 1) fail(PBB_ERR_INVALID_ARG, "symmetry must be -1, 0 or 1");
+    if (kernel->id != PBB_KERNEL_NONE && kernel->id != PBB_KERNEL_POLY_XY && !kernel_id_valid(kernel->id))
+        fail(PBB_ERR_INVALID_ARG, "unknown kernel id");
+}
+
+PBB_API pbb_status pbb_ops_create(const pbb_kernel_desc *kernel, pbb_host_kernel_fn host_eval, void *ctx, double L,
+                                  int32_t tree_level, int32_t order, double eps, int32_t use_chebyshev,
+                                  pbb_ops **out)
+{
+    return guarded([&] {
+        if (!out) fail(PBB_ERR_INVALID_ARG, "out is NULL");
+        *out = nullptr;
+        validate_params(kernel, L, tree_level, order, eps, use_chebyshev);
+        if (!host_eval && !kernel_id_valid(kernel->id))
+            fail(PBB_ERR_INVALID_ARG, "no host callback and no kernel functor id");
+        auto ops = std::make_unique<pbb_ops>();
+        HostOps &h = ops->h;
+        h.kernel = *kernel;
+        h.L = L;
+        h.tree_level = tree_level;
+        h.n = order;
+        h.eps = eps;
+        HostEval eval;
+        if (host_eval)
+            eval = [host_eval, ctx](const double *t, const double *s) { return host_eval(ctx, t, s); };
+        else {
+            int id = kernel->id;
+            double p[4];
+            std::memcpy(p, kernel->param, sizeof(p));
+            eval = [id, p](const double *t, const double *s) { return kernel_eval_host(id, p, t, s); };
+        }
+        precompute_ops(h, eval);
+        *out = ops.release();
+    });
+}
+
+PBB_API pbb_status pbb_ops_create_from_arrays(const pbb_kernel_desc *kernel, double L, int32_t tree_level,
+                                              int32_t order, double eps, int32_t rank_f, int32_t rank_s,
+                                              int32_t nsets, const double *U, const double *VT, const double *E,
+                                              pbb_ops **out)
+{
+    return guarded([&] {
+        if (!out) fail(PBB_ERR_INVALID_ARG, "out is NULL");
+        *out = nullptr;
+        validate_params(kernel, L, tree_level, order, eps, 1);
+        if (!U || !VT || !E) fail(PBB_ERR_INVALID_ARG, "operator arrays must not be NULL");
+        auto ops = std::make_unique<pbb_ops>();
+        HostOps &h = ops->h;
+        h.kernel = *kernel;
+        h.L = L;
+        h.tree_level = tree_level;
+        h.n = order;
+        h.n3 = order * order * order;
+        h.eps = eps;
+        h.w.compute(order);
+        h.homogen_flipped = -kernel->homogen;
+        h.homog = std::fabs(h.homogen_flipped) > 0.1;
+        int want = h.homog ? 1 : std::max(1, tree_level - 1);
+        if (nsets != want) fail(PBB_ERR_INVALID_ARG, "nsets does not match the kernel's homogeneity and tree_level");
+        if (rank_f < 1 || rank_f > h.n3 || rank_s < 1 || rank_s > h.n3) fail(PBB_ERR_INVALID_ARG, "bad ranks");
+        h.rf = rank_f;
+        h.rs = rank_s;
+        h.nsets = nsets;
+        h.U.assign(U, U + (size_t)nsets * h.n3 * rank_f);
+        h.VT.assign(VT, VT + (size_t)nsets * rank_s * h.n3);
+        h.E.assign(E, E + (size_t)nsets * 316 * rank_f * rank_s);
+        *out = ops.release();
+    });
+}
+
+PBB_API pbb_status pbb_ops_query(const pbb_ops *ops, pbb_ops_info *info)
+{
+    return guarded([&] {
+        if (!ops || !info) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        const HostOps &h = ops->h;
+        std::memset(info, 0, sizeof(*info));
+        info->order = h.n;
+        info->n3 = h.n3;
+        info->rank_f = h.rf;
+        info->rank_s = h.rs;
+        info->nsets = h.nsets;
+        info->homogeneous = h.homog ? 1 : 0;
+        info->tree_level = h.tree_level;
+        info->L = h.L;
+        info->homogen = h.kernel.homogen;
+        info->eps = h.eps;
+    });
+}
+
+PBB_API pbb_status pbb_ops_export(const pbb_ops *ops, double *U, double *VT, double *E, double *sigma_f,
+                                  double *sigma_s)
+{
+    return guarded([&] {
+        if (!ops) fail(PBB_ERR_INVALID_ARG, "ops is NULL");
+        const HostOps &h = ops->h;
+        if (U) std::memcpy(U, h.U.data(), sizeof(double) * h.U.size());
+        if (VT) std::memcpy(VT, h.VT.data(), sizeof(double) * h.VT.size());
+        if (E) std::memcpy(E, h.E.data(), sizeof(double) * h.E.size());
+        if (sigma_f) {
+            if (h.sigma_f.empty()) std::fill(sigma_f, sigma_f + h.n3, NAN);
+            else std::memcpy(sigma_f, h.sigma_f.data(), sizeof(double) * h.sigma_f.size());
+        }
+        if (sigma_s) {
+            if (h.sigma_s.empty()) std::fill(sigma_s, sigma_s + h.n3, NAN);
+            else std::memcpy(sigma_s, h.sigma_s.data(), sizeof(double) * h.sigma_s.size());
+        }
+    });
+}
+
+PBB_API void pbb_ops_destroy(pbb_ops *ops) { delete ops; }
+
+PBB_API pbb_status pbb_plan_create(pbb_ops *ops, const double *xyz, int64_t npoints, pbb_plan **out)
+{
+    return guarded([&] {
+        if (!out) fail(PBB_ERR_INVALID_ARG, "out is NULL");
+        *out = nullptr;
+        if (!ops || !xyz) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        const HostOps &h = ops->h;
+        if (h.rf > 256 || h.rs > 256) fail(PBB_ERR_UNSUPPORTED, "SVD ranks above 256 are not supported");
+        if (!kernel_id_valid(h.kernel.id) || h.kernel.id == PBB_KERNEL_POLY_XY)
+            fail(PBB_ERR_UNSUPPORTED, "the kernel has no device functor for the near field");
+        int dev = current_device();
+        HostTree T;
+        build_tree(T, xyz, npoints, h.L, h.tree_level);
+        ops->upload(dev);
+        auto plan = std::make_unique<pbb_plan>();
+        plan->ops = ops;
+        plan->device = dev;
+        check(hipStreamCreateWithFlags(&plan->stream, hipStreamNonBlocking), "hipStreamCreate");
+        plan->Lv = T.Lv;
+        plan->N = T.N;
+        plan->x = dupload(T.xs, "x");
+        plan->y = dupload(T.ys, "y");
+        plan->z = dupload(T.zs, "z");
+        plan->perm = dupload(T.perm, "perm");
+        plan->leaf_start = dupload(T.leaf_start, "leaf_start");
+        plan->lv.resize(T.Lv + 1);
+        int64_t off = 0;
+        for (int l = 0; l <= T.Lv; l++) {
+            DevLevel &D = plan->lv[l];
+            const Level &V = T.lv[l];
+            D.nb = V.nb;
+            D.len = V.len;
+            D.coord = dupload(V.coord, "coord");
+            D.map = dupload(V.map, "map");
+            D.ctab = dupload(V.ctab, "ctab");
+            if (!V.child.empty()) D.child = dupload(V.child, "child");
+            if (!V.order.empty()) D.order = dupload(V.order, "order");
+            std::copy(std::begin(V.class_count), std::end(V.class_count), D.class_count);
+            if (l >= 2) {
+                D.off = off;
+                off += V.nb;
+            }
+        }
+        plan->Btot = off;
+        int dirs[8][189][4];
+        for (int o = 0; o < 8; o++) class_directions(o, h.w.Ktable, dirs[o]);
+        dev::upload_dirs(dirs);
+
+        pbb_plan_info &I = plan->info;
+        std::memset(&I, 0, sizeof(I));
+        I.npoints = T.N;
+        I.tree_level = T.Lv;
+        for (int l = 0; l <= T.Lv && l < 16; l++) I.boxes[l] = T.lv[l].nb;
+        I.nonempty_levels = T.Lv + 1;
+        I.m2l_pairs = T.m2l_pairs;
+        I.near_pairs = T.near_pairs;
+        I.max_leaf_points = T.max_leaf;
+        for (int d = 0; d < 3; d++) I.center[d] = T.ctr[d];
+        I.outside_points = T.outside;
+        *out = plan.release();
+    });
+}
+
+PBB_API pbb_status pbb_plan_query(const pbb_plan *plan, pbb_plan_info *info)
+{
+    return guarded([&] {
+        if (!plan || !info) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        *info = plan->info;
+    });
+}
+
+PBB_API pbb_status pbb_plan_set_timing(pbb_plan *plan, int32_t enable)
+{
+    return guarded([&] {
+        if (!plan) fail(PBB_ERR_INVALID_ARG, "plan is NULL");
+        if (enable && !plan->events_made) {
+            DeviceGuard g(plan->device);
+            for (auto &e : plan->ev) check(hipEventCreate(&e), "hipEventCreate");
+            plan->events_made = true;
+        }
+        plan->timing = enable != 0;
+    });
+}
+
+PBB_API pbb_status pbb_plan_timings(pbb_plan *plan, double *ms)
+{
+    return guarded([&] {
+        if (!plan || !ms) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        if (!plan->events_made) fail(PBB_ERR_INVALID_ARG, "timing is not enabled on this plan");
+        DeviceGuard g(plan->device);
+        check(hipEventSynchronize(plan->ev[PBB_NUM_PHASES - 1]), "hipEventSynchronize");
+        for (int p = 0; p < PBB_NUM_PHASES - 1; p++) {
+            float t = 0;
+            check(hipEventElapsedTime(&t, plan->ev[p], plan->ev[p + 1]), "hipEventElapsedTime");
+            ms[p] = t;
+        }
+        float tot = 0;
+        check(hipEventElapsedTime(&tot, plan->ev[0], plan->ev[PBB_NUM_PHASES - 1]), "hipEventElapsedTime");
+        ms[PBB_NUM_PHASES - 1] = tot;
+    });
+}
+
+static void matvec_fp64(pbb_plan *p, const double *w, int ncols, double *out, hipStream_t st)
+{
+    const HostOps &h = p->ops->h;
+    pbb_ops *o = p->ops;
+    const int n = h.n, n3 = h.n3, Lv = p->Lv;
+    auto mark = [&](int k) {
+        if (p->timing) check(hipEventRecord(p->ev[k], st), "hipEventRecord");
+    };
+    p->ensure_workspace(ncols);
+    mark(0);
+    dev::launch_gather_weights(w, p->perm, p->N, ncols, p->ws, st);
+    const int cc_p2m = std::max(1, std::min({ncols, 64, 4096 / n3}));
+    const int cc_m2m = std::max(1, std::min(ncols, 2048 / n3));
+    if (Lv >= 2) {
+        const DevLevel &LL = p->lv[Lv];
+        dev::P2MArgs a{};
+        a.n = n; a.ncols = ncols; a.cc = cc_p2m; a.N = p->N; a.off = LL.off; a.len = LL.len;
+        a.leaf_start = p->leaf_start; a.coord = LL.coord; a.ctab = LL.ctab; a.Tkz = o->d_Tkz;
+        a.x = p->x; a.y = p->y; a.z = p->z; a.ws = p->ws; a.S = p->S;
+        dev::launch_p2m(a, (int)LL.nb, st);
+    }
+    mark(1);
+    for (int l = Lv - 1; l >= 2; l--) {
+        dev::M2MArgs a{};
+        a.n = n; a.ncols = ncols; a.cc = cc_m2m;
+        a.off_parent = p->lv[l].off; a.off_child = p->lv[l + 1].off;
+        a.child = p->lv[l].child; a.Cw = o->d_Cw; a.Sparent = p->S; a.Schild = p->S;
+        dev::launch_m2m(a, (int)p->lv[l].nb, st);
+    }
+    mark(2);
+    for (int l = 2; l <= Lv; l++) {
+        const DevLevel &D = p->lv[l];
+        dev::GemmArgs g{};
+        g.M = h.rs; g.K = n3; g.ncol = D.nb * ncols; g.alpha = 1.0;
+        g.A = o->d_VTw + (size_t)h.ops_set(l) * h.rs * n3;
+        g.B = p->S + (size_t)D.off * ncols * n3;
+        g.C = p->P + (size_t)D.off * ncols * h.rs;
+        dev::launch_gemm(g, st);
+    }
+    mark(3);
+    const int cpt = std::min(ncols, 64);
+    const int TB = std::max(1, 64 / cpt);
+    const int nchunks = (ncols + cpt - 1) / cpt;
+    for (int l = 2; l <= Lv; l++) {
+        const DevLevel &D = p->lv[l];
+        dev::M2LArgs a{};
+        a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = TB; a.cpt = cpt; a.side = 1 << l; a.off = D.off;
+        a.tile_prefix[0] = 0;
+        a.class_start[0] = 0;
+        for (int c = 0; c < 8; c++) {
+            a.class_start[c + 1] = a.class_start[c] + D.class_count[c];
+            a.tile_prefix[c + 1] = a.tile_prefix[c] + (int)((D.class_count[c] + TB - 1) / TB);
+        }
+        a.order = D.order; a.coord = D.coord; a.map = D.map;
+        a.E = o->d_E + (size_t)h.ops_set(l) * 316 * h.rf * h.rs;
+        a.P = p->P; a.Pf = p->Pf;
+        dev::launch_m2l(a, a.tile_prefix[8], nchunks, st);
+    }
+    mark(4);
+    // local expansions F_l = scale_l * Uw Pf_l overwrite S_l (no longer needed)
+    for (int l = 2; l <= Lv; l++) {
+        const DevLevel &D = p->lv[l];
+        dev::GemmArgs g{};
+        g.M = n3; g.K = h.rf; g.ncol = D.nb * ncols;
+        g.alpha = h.homog ? std::pow(1.0 / D.len, h.homogen_flipped) : 1.0;  // compute.hpp:433-435
+        g.A = o->d_Uw + (size_t)h.ops_set(l) * n3 * h.rf;
+        g.B = p->Pf + (size_t)D.off * ncols * h.rf;
+        g.C = p->S + (size_t)D.off * ncols * n3;
+        dev::launch_gemm(g, st);
+    }
+    mark(5);
+    // downward pass, top-down: F_l += L2L(F_{l-1}), F_{l-1} already complete
+    for (int l = 3; l <= Lv; l++) {
+        dev::M2MArgs a{};
+        a.n = n; a.ncols = ncols; a.cc = cc_m2m;
+        a.off_parent = p->lv[l - 1].off; a.off_child = p->lv[l].off;
+        a.child = p->lv[l - 1].child; a.Cw = o->d_Cw; a.Sparent = p->S; a.Schild = p->S;
+        dev::launch_l2l(a, (int)p->lv[l - 1].nb, st);
+    }
+    mark(6);
+    {
+        const DevLevel &LL = p->lv[Lv];
+        dev::NearArgs a{};
+        a.n = n; a.ncols = ncols; a.side = 1 << Lv; a.N = p->N; a.off = LL.off; a.len = LL.len;
+        std::memcpy(a.kp.p, h.kernel.param, sizeof(a.kp.p));
+        a.leaf_start = p->leaf_start; a.coord = LL.coord; a.map = LL.map; a.perm = p->perm;
+        a.ctab = LL.ctab; a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.ws = p->ws;
+        a.F = Lv >= 2 ? p->S : nullptr;
+        a.out = out;
+        dev::launch_near(a, h.kernel.id, (int)LL.nb, st);
+    }
+    mark(7);
+}
+
+PBB_API pbb_status pbb_plan_matvec(pbb_plan *plan, const double *w_dev, int32_t ncols, double *out_dev,
+                                   int32_t precision, void *stream)
+{
+    return guarded([&] {
+        if (!plan || !w_dev || !out_dev) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
+        if ((int64_t)ncols * plan->N >= (1LL << 40)) fail(PBB_ERR_INVALID_ARG, "ncols * npoints too large");
+        if (precision != PBB_PREC_FP64) fail(PBB_ERR_UNSUPPORTED, "precision mode not available");
+        DeviceGuard g(plan->device);
+        hipStream_t st = stream ? (hipStream_t)stream : plan->stream;
+        matvec_fp64(plan, w_dev, ncols, out_dev, st);
+    });
+}
+
+PBB_API void pbb_plan_destroy(pbb_plan *plan) { delete plan; }
+
+PBB_API pbb_status pbb_matvec(pbb_ops *ops, const double *target, const double *source, int64_t npoints,
+                              const double *weight, int32_t ncols, double *output)
+{
+    pbb_plan *plan = nullptr;
+    pbb_status st = guarded([&] {
+        if (!ops || !target || !source || !weight || !output) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
+        if (npoints <= 0) fail(PBB_ERR_INVALID_ARG, "npoints must be positive");
+        if (target != source && std::memcmp(target, source, sizeof(double) * 3 * (size_t)npoints) != 0)
+            fail(PBB_ERR_UNSUPPORTED, "sources must be the targets (the reference supports nothing else)");
+    });
+    if (st != PBB_OK) return st;
+    st = pbb_plan_create(ops, target, npoints, &plan);
+    if (st != PBB_OK) return st;
+    double *dw = nullptr, *dout = nullptr;
+    st = guarded([&] {
+        DeviceGuard g(plan->device);
+        size_t bytes = sizeof(double) * (size_t)npoints * ncols;
+        dw = dalloc<double>((size_t)npoints * ncols, "weights");
+        dout = dalloc<double>((size_t)npoints * ncols, "output");
+        check(hipMemcpyAsync(dw, weight, bytes, hipMemcpyHostToDevice, plan->stream), "copy weights");
+        check(hipMemsetAsync(dout, 0, bytes, plan->stream), "clear output");
+        matvec_fp64(plan, dw, ncols, dout, plan->stream);
+        std::vector<double> res((size_t)npoints * ncols);
+        check(hipMemcpyAsync(res.data(), dout, bytes, hipMemcpyDeviceToHost, plan->stream), "copy output");
+        check(hipStreamSynchronize(plan->stream), "hipStreamSynchronize");
+        for (size_t i = 0; i < res.size(); i++) output[i] += res[i];
+    });
+    {
+        int prev = -1;
+        (void)hipGetDevice(&prev);
+        (void)hipSetDevice(plan->device);
+        dfree(dw);
+        dfree(dout);
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+    pbb_plan_destroy(plan);
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,598 @@
+// kernels.hip -- gfx950 kernels of the FP64 black-box FMM matvec.
+//
+// Data layout in HBM (one plan):
+//   points    sorted by leaf (Morton order), re-centred, SoA x[N] y[N] z[N]
+//   perm      sorted position -> caller's index (weights/outputs stay in the
+//             caller's column-major order w[c*N + i])
+//   ws        weights gathered into sorted order, ws[c*N + k]
+//   S / F     per box (global id = level offset + compact id) a column-major
+//             n3 x ncols block: S[(g*ncols + c)*n3 + l].  The upward pass's
+//             S (sourceval) and the downward pass's F (targetval) share it.
+//   P         proxies (proxysval) r_s x ncols per box, P[(g*ncols + c)*rs + a]
+//   Pf        M2L result r_f x ncols per box
+// Operators: VTw = VT diag(Kweights) (r_s x n3), Uw = diag(Kweights) U
+// (n3 x r_f), E (316 blocks r_f x r_s), all column-major FP64.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace pbb {
+namespace dev {
+
+__constant__ int c_dirs[8][189][4];  // far directions of each octant class
+
+// ---------------------------------------------------------------------------
+// device kernel functors (formulas of pbb_kernel_id); r2 = |s - t|^2 with
+// the differences taken as source - target (kernel_Types.cpp:21-23)
+template <int KID>
+__device__ __forceinline__ double keval(double dx, double dy, double dz, const KParams &p)
+{
+    double r2 = dx * dx + dy * dy + dz * dz;
+    if constexpr (KID == PBB_KERNEL_LAPLACIAN) return 1.0 / sqrt(r2);
+    else if constexpr (KID == PBB_KERNEL_ONEOVERR2) { double ri = 1.0 / sqrt(r2); return ri * ri; }
+    else if constexpr (KID == PBB_KERNEL_GAUSSIAN) return exp(-r2 / (p.p[0] * p.p[0]));
+    else if constexpr (KID == PBB_KERNEL_EXPONENTIAL) return p.p[1] * exp(-sqrt(r2) / p.p[0]);
+    else if constexpr (KID == PBB_KERNEL_SOFT_LAPLACIAN) return 1.0 / sqrt(r2 + p.p[0]);
+    else if constexpr (KID == PBB_KERNEL_QUADRIC) return 1.0 + r2;
+    else if constexpr (KID == PBB_KERNEL_INV_QUADRIC) return 1.0 / (1.0 + r2);
+    else if constexpr (KID == PBB_KERNEL_LOGARITHM) return r2 == 0 ? 0.0 : 0.5 * log(r2);
+    else if constexpr (KID == PBB_KERNEL_THIN_PLATE) return r2 == 0 ? 0.0 : 0.5 * r2 * log(r2);
+    else if constexpr (KID == PBB_KERNEL_R2) return r2;
+    else if constexpr (KID == PBB_KERNEL_CONSTANT) return 1.0;
+    else return 0.0;
+}
+
+// S_m(u), m < n: Chebyshev interpolant by Clenshaw (H2_3D_Tree.cpp:830-879)
+__device__ __forceinline__ void sn_eval(int n, const double *__restrict__ Tkz, double u, double *S)
+{
+    const double pfac = 2.0 / n;
+    for (int m = 0; m < n; m++) {
+        const double *Tk = Tkz + m * n;
+        double d0 = 0, d1 = 0;
+        for (int j = n - 1; j > 0; j--) {
+            double d2 = d0;
+            d0 = 2.0 * u * d0 - d1 + Tk[j];
+            d1 = d2;
+        }
+        S[m] = pfac * (u * d0 - d1 + 0.5 * Tk[0]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ws[c*N + k] = w[c*N + perm[k]]
+__global__ void gather_weights(const double *__restrict__ w, const int32_t *__restrict__ perm, int64_t N,
+                               int ncols, double *__restrict__ ws)
+{
+    int64_t total = N * ncols;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        int64_t c = q / N, k = q - c * N;
+        ws[q] = w[c * N + perm[k]];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P2M, compute.hpp:287-323: one workgroup per non-empty leaf.
+// S[(g*ncols + c)*n3 + (l1 n + l2) n + l3] = sum_j w_jc Sx_l1(j) Sy_l2(j) Sz_l3(j)
+constexpr int P2M_PTS = 32;
+constexpr int P2M_R = 16;
+
+__global__ __launch_bounds__(256) void p2m_fp64(P2MArgs a)
+{
+    __shared__ double sS[3][16][P2M_PTS];
+    __shared__ double sW[64][P2M_PTS];
+    const int b = blockIdx.x;
+    const int n = a.n, n3 = n * n * n, tid = threadIdx.x;
+    const int t0 = a.leaf_start[b], nt = a.leaf_start[b + 1] - t0;
+    const uint32_t cc = (uint32_t)a.coord[b];
+    const double cen[3] = {a.ctab[cc & 1023], a.ctab[(cc >> 10) & 1023], a.ctab[(cc >> 20) & 1023]};
+    const double ihalfL = 2.0 / a.len;
+    const double *xyz[3] = {a.x, a.y, a.z};
+    double *Sout = a.S + (size_t)(a.off + b) * a.ncols * n3;
+
+    for (int c0 = 0; c0 < a.ncols; c0 += a.cc) {
+        const int cn = min(a.cc, a.ncols - c0);
+        double acc[P2M_R];
+#pragma unroll
+        for (int r = 0; r < P2M_R; r++) acc[r] = 0;
+        for (int j0 = 0; j0 < nt; j0 += P2M_PTS) {
+            const int jn = min(P2M_PTS, nt - j0);
+            if (tid < 3 * P2M_PTS) {
+                int p = tid % P2M_PTS, ax = tid / P2M_PTS;
+                double S[16];
+                double u = p < jn ? ihalfL * (xyz[ax][t0 + j0 + p] - cen[ax]) : 0.0;
+                sn_eval(n, a.Tkz, u, S);
+                for (int m = 0; m < n; m++) sS[ax][m][p] = p < jn ? S[m] : 0.0;
+            }
+            for (int q = tid; q < cn * P2M_PTS; q += 256) {
+                int c = q / P2M_PTS, p = q % P2M_PTS;
+                sW[c][p] = p < jn ? a.ws[(int64_t)(c0 + c) * a.N + t0 + j0 + p] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < P2M_R; r++) {
+                int q = tid + 256 * r;
+                if (q < cn * n3) {
+                    int c = q / n3, l = q - c * n3;
+                    int l1 = l / (n * n), l2 = (l / n) % n, l3 = l % n;
+                    double s = 0;
+                    for (int p = 0; p < jn; p++) s += sW[c][p] * sS[0][l1][p] * sS[1][l2][p] * sS[2][l3][p];
+                    acc[r] += s;
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int r = 0; r < P2M_R; r++) {
+            int q = tid + 256 * r;
+            if (q < cn * n3) Sout[(size_t)c0 * n3 + q] = acc[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// M2M, compute.hpp:234-283 + :956-1034: one workgroup per parent box; the
+// children's expansions are contracted mode by mode (z, y, x) in LDS.
+__global__ __launch_bounds__(256) void m2m_fp64(M2MArgs a)
+{
+    extern __shared__ double lds[];
+    const int n = a.n, n2 = n * n, n3 = n2 * n, tid = threadIdx.x;
+    const int CC = a.cc;
+    double *A = lds, *B = lds + CC * n3, *Cw = lds + 2 * CC * n3;
+    for (int q = tid; q < 2 * n2; q += 256) Cw[q] = a.Cw[q];
+    const int b = blockIdx.x;
+    double *Sp = a.Sparent + (size_t)(a.off_parent + b) * a.ncols * n3;
+    for (int c0 = 0; c0 < a.ncols; c0 += CC) {
+        const int cn = min(CC, a.ncols - c0);
+        const int tot = cn * n3;
+        double acc[P2M_R];
+#pragma unroll
+        for (int r = 0; r < P2M_R; r++) acc[r] = 0;
+        for (int o = 0; o < 8; o++) {
+            const int ch = a.child[(int64_t)b * 8 + o];
+            if (ch < 0) continue;
+            const double *Sc = a.Schild + ((size_t)(a.off_child + ch) * a.ncols + c0) * n3;
+            const double *Cx = Cw + ((o >> 2) & 1) * n2, *Cy = Cw + ((o >> 1) & 1) * n2, *Cz = Cw + (o & 1) * n2;
+            __syncthreads();
+            for (int q = tid; q < tot; q += 256) A[q] = Sc[q];
+            __syncthreads();
+            for (int q = tid; q < tot; q += 256) {  // z: B[ab n + l3] = sum_k A[ab n + k] Cz[l3 n + k]
+                int c = q / n3, l = q - c * n3, ab = l / n, l3 = l % n;
+                const double *src = A + c * n3 + ab * n;
+                double s = 0;
+                for (int k = 0; k < n; k++) s += src[k] * Cz[l3 * n + k];
+                B[q] = s;
+            }
+            __syncthreads();
+            for (int q = tid; q < tot; q += 256) {  // y: A[l1,l2,l3] = sum_k B[l1,k,l3] Cy[l2 n + k]
+                int c = q / n3, l = q - c * n3, l1 = l / n2, l2 = (l / n) % n, l3 = l % n;
+                const double *src = B + c * n3 + l1 * n2 + l3;
+                double s = 0;
+                for (int k = 0; k < n; k++) s += src[k * n] * Cy[l2 * n + k];
+                A[q] = s;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < P2M_R; r++) {  // x: acc[l1,l2,l3] += sum_k A[k,l2,l3] Cx[l1 n + k]
+                int q = tid + 256 * r;
+                if (q < tot) {
+                    int c = q / n3, l = q - c * n3, l1 = l / n2, l23 = l % n2;
+                    const double *src = A + c * n3 + l23;
+                    double s = 0;
+                    for (int k = 0; k < n; k++) s += src[k * n2] * Cx[l1 * n + k];
+                    acc[r] += s;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < P2M_R; r++) {
+            int q = tid + 256 * r;
+            if (q < tot) Sp[(size_t)c0 * n3 + q] = acc[r];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// L2L, compute.hpp:551-587 + :798-878: one workgroup per parent box, the
+// transpose of M2M added into each non-empty child.
+__global__ __launch_bounds__(256) void l2l_fp64(M2MArgs a)
+{
+    extern __shared__ double lds[];
+    const int n = a.n, n2 = n * n, n3 = n2 * n, tid = threadIdx.x;
+    const int CC = a.cc;
+    double *Pp = lds, *X = lds + CC * n3, *Y = lds + 2 * CC * n3, *Cw = lds + 3 * CC * n3;
+    for (int q = tid; q < 2 * n2; q += 256) Cw[q] = a.Cw[q];
+    const int b = blockIdx.x;
+    const double *Fp = a.Sparent + (size_t)(a.off_parent + b) * a.ncols * n3;
+    for (int c0 = 0; c0 < a.ncols; c0 += CC) {
+        const int cn = min(CC, a.ncols - c0);
+        const int tot = cn * n3;
+        __syncthreads();
+        for (int q = tid; q < tot; q += 256) Pp[q] = Fp[(size_t)c0 * n3 + q];
+        for (int o = 0; o < 8; o++) {
+            const int ch = a.child[(int64_t)b * 8 + o];
+            if (ch < 0) continue;
+            double *Fc = a.Schild + ((size_t)(a.off_child + ch) * a.ncols + c0) * n3;
+            const double *Cx = Cw + ((o >> 2) & 1) * n2, *Cy = Cw + ((o >> 1) & 1) * n2, *Cz = Cw + (o & 1) * n2;
+            __syncthreads();
+            for (int q = tid; q < tot; q += 256) {  // x: X[l1,l2,l3] = sum_k P[k,l2,l3] Cx[k n + l1]
+                int c = q / n3, l = q - c * n3, l1 = l / n2, l23 = l % n2;
+                const double *src = Pp + c * n3 + l23;
+                double s = 0;
+                for (int k = 0; k < n; k++) s += src[k * n2] * Cx[k * n + l1];
+                X[q] = s;
+            }
+            __syncthreads();
+            for (int q = tid; q < tot; q += 256) {  // y: Y[l1,l2,l3] = sum_k X[l1,k,l3] Cy[k n + l2]
+                int c = q / n3, l = q - c * n3, l1 = l / n2, l2 = (l / n) % n, l3 = l % n;
+                const double *src = X + c * n3 + l1 * n2 + l3;
+                double s = 0;
+                for (int k = 0; k < n; k++) s += src[k * n] * Cy[k * n + l2];
+                Y[q] = s;
+            }
+            __syncthreads();
+            for (int q = tid; q < tot; q += 256) {  // z: Fc[ab n + l3] += sum_k Y[ab n + k] Cz[k n + l3]
+                int c = q / n3, l = q - c * n3, ab = l / n, l3 = l % n;
+                const double *src = Y + c * n3 + ab * n;
+                double s = 0;
+                for (int k = 0; k < n; k++) s += src[k] * Cz[k * n + l3];
+                Fc[q] += s;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// C[:, j] = alpha * A B[:, j] for j < ncol: A (M x K, lda M), B (K x ncol,
+// ldb K), C (M x ncol, ldc M), column-major FP64.  64 x 64 output tiles,
+// K in chunks of 16 through LDS, 4 x 4 outputs per thread.  Serves the
+// proxy projection P = VTw S (compute.hpp:376-395) and the local expansion
+// F = scale Uw Pf (compute.hpp:497-508).
+__global__ __launch_bounds__(256) void gemm_small_a_fp64(GemmArgs g)
+{
+    __shared__ double As[16][64 + 1];
+    __shared__ double Bs[16][64 + 1];
+    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+    const int64_t n0 = (int64_t)blockIdx.x * 64;
+    const int m0 = blockIdx.y * 64;
+    double acc[4][4] = {};
+    for (int k0 = 0; k0 < g.K; k0 += 16) {
+        for (int q = tid; q < 16 * 64; q += 256) {
+            int row = q % 64, k = q / 64;
+            int gr = m0 + row, gk = k0 + k;
+            As[k][row] = (gr < g.M && gk < g.K) ? g.A[gr + (int64_t)gk * g.M] : 0.0;
+        }
+        for (int q = tid; q < 16 * 64; q += 256) {
+            int k = q % 16, col = q / 16;
+            int gk = k0 + k;
+            int64_t gc = n0 + col;
+            Bs[k][col] = (gk < g.K && gc < g.ncol) ? g.B[gk + gc * g.K] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) av[i] = As[k][tx + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; j++) bv[j] = Bs[k][ty + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] += av[i] * bv[j];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        int64_t gc = n0 + ty + 16 * j;
+        if (gc >= g.ncol) continue;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            int gr = m0 + tx + 16 * i;
+            if (gr < g.M) g.C[gr + gc * g.M] = g.alpha * acc[i][j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// M2L, compute.hpp:401-493 + Moment2Local :880-923.  A workgroup owns TB
+// target boxes of one octant class (Morton-adjacent) and the same column
+// chunk of each: a 64-wide output tile Pf (all r_f rows) that stays in
+// registers while the 189 far directions of the class stream through:
+//   Pf[:, tile] += E_d P[:, tile + d]     (empty source boxes give zeros)
+template <int RPT>
+__global__ __launch_bounds__(256) void m2l_fp64(M2LArgs a)
+{
+    __shared__ double As[16][RPT * 16];
+    __shared__ double Bs[16][64 + 1];
+    __shared__ int s_sid[64];
+    __shared__ int s_tgt[64];
+    __shared__ int s_xyz[64][3];
+    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+    const int rf = a.rf, rs = a.rs;
+    const int tile = blockIdx.x;
+    int o = 0;
+    while (o < 7 && tile >= a.tile_prefix[o + 1]) o++;
+    const int64_t first = a.class_start[o] + (int64_t)(tile - a.tile_prefix[o]) * a.TB;
+    const int64_t cend = a.class_start[o + 1];
+    const int c0 = blockIdx.y * a.cpt;
+    const int cpt = min(a.cpt, a.ncols - c0);
+    const int S = a.side;
+    if (tid < a.TB) {
+        int64_t q = first + tid;
+        int t = q < cend ? a.order[q] : -1;
+        s_tgt[tid] = t;
+        if (t >= 0) {
+            uint32_t c = (uint32_t)a.coord[t];
+            s_xyz[tid][0] = c & 1023;
+            s_xyz[tid][1] = (c >> 10) & 1023;
+            s_xyz[tid][2] = (c >> 20) & 1023;
+        }
+    }
+    double acc[RPT][4];
+#pragma unroll
+    for (int i = 0; i < RPT; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0;
+    __syncthreads();
+
+    for (int k = 0; k < 189; k++) {
+        const int dx = c_dirs[o][k][0], dy = c_dirs[o][k][1], dz = c_dirs[o][k][2];
+        const int eidx = c_dirs[o][k][3];
+        int any = 0;
+        if (tid < a.TB) {
+            int sid = -1;
+            if (s_tgt[tid] >= 0) {
+                int sx = s_xyz[tid][0] + dx, sy = s_xyz[tid][1] + dy, sz = s_xyz[tid][2] + dz;
+                if (sx >= 0 && sy >= 0 && sz >= 0 && sx < S && sy < S && sz < S)
+                    sid = a.map[((int64_t)sx * S + sy) * S + sz];
+            }
+            s_sid[tid] = sid;
+            any = sid >= 0;
+        }
+        if (!__syncthreads_or(any)) continue;
+        const double *Ed = a.E + (size_t)eidx * rf * rs;
+        for (int k0 = 0; k0 < rs; k0 += 16) {
+            for (int q = tid; q < 16 * RPT * 16; q += 256) {
+                int row = q % (RPT * 16), kk = q / (RPT * 16);
+                int gk = k0 + kk;
+                As[kk][row] = (row < rf && gk < rs) ? Ed[row + (size_t)gk * rf] : 0.0;
+            }
+            for (int q = tid; q < 16 * 64; q += 256) {
+                int kk = q % 16, col = q / 16;
+                int tb = col / cpt, cl = col - tb * cpt;
+                int gk = k0 + kk;
+                double v = 0.0;
+                if (tb < a.TB && gk < rs) {
+                    int sid = s_sid[tb];
+                    if (sid >= 0) v = a.P[((size_t)(a.off + sid) * a.ncols + c0 + cl) * rs + gk];
+                }
+                Bs[kk][col] = v;
+            }
+            __syncthreads();
+#pragma unroll 4
+            for (int kk = 0; kk < 16; kk++) {
+                double av[RPT], bv[4];
+#pragma unroll
+                for (int i = 0; i < RPT; i++) av[i] = As[kk][tx + 16 * i];
+#pragma unroll
+                for (int j = 0; j < 4; j++) bv[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+                for (int i = 0; i < RPT; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) acc[i][j] += av[i] * bv[j];
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        int col = ty + 16 * j;
+        int tb = col / cpt, cl = col - tb * cpt;
+        if (tb >= a.TB || cl >= cpt) continue;
+        int t = s_tgt[tb];
+        if (t < 0) continue;
+        double *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + cl) * rf;
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            int row = tx + 16 * i;
+            if (row < rf) dst[row] = acc[i][j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Near field, compute.hpp:590-751, as a target-owner kernel: one workgroup
+// per non-empty leaf computes, for each of its targets, the L2P of the
+// leaf's local expansion plus the direct sum over the points of the 27
+// neighbour leaves (self included, isinf -> 0), and adds both into the
+// caller's output through perm.  No partner writes, no atomics.
+constexpr int NF_TGT = 64;  // targets per pass
+constexpr int NF_SRC = 64;  // source points per LDS chunk
+constexpr int NF_CPT = 8;   // columns per thread (4 column phases)
+
+template <int KID>
+__global__ __launch_bounds__(256) void near_fp64(NearArgs a)
+{
+    __shared__ double sx[NF_SRC], sy[NF_SRC], sz[NF_SRC];
+    __shared__ double sw[4 * NF_CPT][NF_SRC];
+    __shared__ int s_nbr[27];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int slot = tid / 4, g = tid % 4;
+    const int n = a.n, n2 = n * n, n3 = n2 * n;
+    const int t0 = a.leaf_start[b], nt = a.leaf_start[b + 1] - t0;
+    const uint32_t cc = (uint32_t)a.coord[b];
+    const int ix = cc & 1023, iy = (cc >> 10) & 1023, iz = (cc >> 20) & 1023;
+    const int S = a.side;
+    const double cen[3] = {a.ctab[ix], a.ctab[iy], a.ctab[iz]};
+    const double ihalfL = 2.0 / a.len;
+    if (tid < 27) {
+        int x = ix + tid / 9 - 1, y = iy + (tid / 3) % 3 - 1, z = iz + tid % 3 - 1;
+        s_nbr[tid] = (x >= 0 && y >= 0 && z >= 0 && x < S && y < S && z < S) ? a.map[((int64_t)x * S + y) * S + z] : -1;
+    }
+    __syncthreads();
+    for (int c0 = 0; c0 < a.ncols; c0 += 4 * NF_CPT) {
+        const int cn = min(4 * NF_CPT, a.ncols - c0);
+        for (int i0 = 0; i0 < nt; i0 += NF_TGT) {
+            const int i = i0 + slot;
+            const bool active = i < nt;
+            double xi = 0, yi = 0, zi = 0;
+            if (active) { xi = a.x[t0 + i]; yi = a.y[t0 + i]; zi = a.z[t0 + i]; }
+            double acc[NF_CPT];
+#pragma unroll
+            for (int k = 0; k < NF_CPT; k++) acc[k] = 0;
+            // L2P (compute.hpp:594-648)
+            if (a.F != nullptr && active) {
+                double Sx[16], Sy[16], Sz[16];
+                sn_eval(n, a.Tkz, ihalfL * (xi - cen[0]), Sx);
+                sn_eval(n, a.Tkz, ihalfL * (yi - cen[1]), Sy);
+                sn_eval(n, a.Tkz, ihalfL * (zi - cen[2]), Sz);
+#pragma unroll
+                for (int k = 0; k < NF_CPT; k++) {
+                    int cl = g + 4 * k;
+                    if (cl >= cn) break;
+                    const double *F = a.F + ((size_t)(a.off + b) * a.ncols + c0 + cl) * n3;
+                    double sum = 0;
+                    int l = 0;
+                    for (int l1 = 0; l1 < n; l1++)
+                        for (int l2 = 0; l2 < n; l2++) {
+                            double t2 = Sx[l1] * Sy[l2];
+                            for (int l3 = 0; l3 < n; l3++) sum += F[l++] * t2 * Sz[l3];
+                        }
+                    acc[k] += sum;
+                }
+            }
+            // P2P over the 27 neighbours (compute.hpp:650-737)
+            for (int m = 0; m < 27; m++) {
+                const int s = s_nbr[m];
+                if (s < 0) continue;
+                const int s0 = a.leaf_start[s], ns = a.leaf_start[s + 1] - s0;
+                for (int j0 = 0; j0 < ns; j0 += NF_SRC) {
+                    const int jn = min(NF_SRC, ns - j0);
+                    __syncthreads();
+                    if (tid < jn) {
+                        sx[tid] = a.x[s0 + j0 + tid];
+                        sy[tid] = a.y[s0 + j0 + tid];
+                        sz[tid] = a.z[s0 + j0 + tid];
+                    }
+                    for (int q = tid; q < cn * NF_SRC; q += 256) {
+                        int c = q / NF_SRC, p = q % NF_SRC;
+                        sw[c][p] = p < jn ? a.ws[(int64_t)(c0 + c) * a.N + s0 + j0 + p] : 0.0;
+                    }
+                    __syncthreads();
+                    if (active) {
+                        for (int p = 0; p < jn; p++) {
+                            double kv = keval<KID>(sx[p] - xi, sy[p] - yi, sz[p] - zi, a.kp);
+                            if (isinf(kv)) kv = 0.0;
+#pragma unroll
+                            for (int k = 0; k < NF_CPT; k++) {
+                                int cl = g + 4 * k;
+                                if (cl < cn) acc[k] += kv * sw[cl][p];
+                            }
+                        }
+                    }
+                }
+            }
+            if (active) {
+                const int64_t oi = a.perm[t0 + i];
+#pragma unroll
+                for (int k = 0; k < NF_CPT; k++) {
+                    int cl = g + 4 * k;
+                    if (cl < cn) a.out[(int64_t)(c0 + cl) * a.N + oi] += acc[k];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+void upload_dirs(const int dirs[8][189][4])
+{
+    check(hipMemcpyToSymbol(HIP_SYMBOL(c_dirs), dirs, sizeof(int) * 8 * 189 * 4), "hipMemcpyToSymbol(c_dirs)");
+}
+
+void launch_gather_weights(const double *w, const int32_t *perm, int64_t N, int ncols, double *ws, hipStream_t st)
+{
+    int64_t total = N * ncols;
+    int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    gather_weights<<<blocks, 256, 0, st>>>(w, perm, N, ncols, ws);
+    check(hipGetLastError(), "gather_weights");
+}
+
+void launch_p2m(const P2MArgs &a, int nleaf, hipStream_t st)
+{
+    if (nleaf <= 0) return;
+    p2m_fp64<<<nleaf, 256, 0, st>>>(a);
+    check(hipGetLastError(), "p2m_fp64");
+}
+
+void launch_m2m(const M2MArgs &a, int nparent, hipStream_t st)
+{
+    if (nparent <= 0) return;
+    size_t lds = sizeof(double) * (2 * (size_t)a.cc * a.n * a.n * a.n + 2 * a.n * a.n);
+    m2m_fp64<<<nparent, 256, lds, st>>>(a);
+    check(hipGetLastError(), "m2m_fp64");
+}
+
+void launch_l2l(const M2MArgs &a, int nparent, hipStream_t st)
+{
+    if (nparent <= 0) return;
+    size_t lds = sizeof(double) * (3 * (size_t)a.cc * a.n * a.n * a.n + 2 * a.n * a.n);
+    l2l_fp64<<<nparent, 256, lds, st>>>(a);
+    check(hipGetLastError(), "l2l_fp64");
+}
+
+void launch_gemm(const GemmArgs &g, hipStream_t st)
+{
+    if (g.ncol <= 0) return;
+    dim3 grid((unsigned)((g.ncol + 63) / 64), (unsigned)((g.M + 63) / 64));
+    gemm_small_a_fp64<<<grid, 256, 0, st>>>(g);
+    check(hipGetLastError(), "gemm_small_a_fp64");
+}
+
+void launch_m2l(const M2LArgs &a, int ntiles, int ncolchunks, hipStream_t st)
+{
+    if (ntiles <= 0) return;
+    dim3 grid((unsigned)ntiles, (unsigned)ncolchunks);
+    int rpt = (a.rf + 15) / 16;
+#define PBB_M2L_CASE(R)                                   \
+    case R:                                               \
+        m2l_fp64<R><<<grid, 256, 0, st>>>(a);             \
+        break;
+    switch (rpt) {
+        PBB_M2L_CASE(1) PBB_M2L_CASE(2) PBB_M2L_CASE(3) PBB_M2L_CASE(4) PBB_M2L_CASE(5) PBB_M2L_CASE(6)
+        PBB_M2L_CASE(7) PBB_M2L_CASE(8) PBB_M2L_CASE(9) PBB_M2L_CASE(10) PBB_M2L_CASE(11)
+        PBB_M2L_CASE(12) PBB_M2L_CASE(13) PBB_M2L_CASE(14) PBB_M2L_CASE(15) PBB_M2L_CASE(16)
+    default: fail(PBB_ERR_UNSUPPORTED, "M2L rank above 256 is not supported");
+    }
+#undef PBB_M2L_CASE
+    check(hipGetLastError(), "m2l_fp64");
+}
+
+void launch_near(const NearArgs &a, int kid, int nleaf, hipStream_t st)
+{
+    if (nleaf <= 0) return;
+#define PBB_NEAR_CASE(K)                                  \
+    case K:                                               \
+        near_fp64<K><<<nleaf, 256, 0, st>>>(a);           \
+        break;
+    switch (kid) {
+        PBB_NEAR_CASE(PBB_KERNEL_LAPLACIAN) PBB_NEAR_CASE(PBB_KERNEL_ONEOVERR2) PBB_NEAR_CASE(PBB_KERNEL_GAUSSIAN)
+        PBB_NEAR_CASE(PBB_KERNEL_EXPONENTIAL) PBB_NEAR_CASE(PBB_KERNEL_SOFT_LAPLACIAN)
+        PBB_NEAR_CASE(PBB_KERNEL_QUADRIC) PBB_NEAR_CASE(PBB_KERNEL_INV_QUADRIC)
+        PBB_NEAR_CASE(PBB_KERNEL_LOGARITHM) PBB_NEAR_CASE(PBB_KERNEL_THIN_PLATE) PBB_NEAR_CASE(PBB_KERNEL_R2)
+        PBB_NEAR_CASE(PBB_KERNEL_CONSTANT)
+    default: fail(PBB_ERR_UNSUPPORTED, "no device functor for this kernel id");
+    }
+#undef PBB_NEAR_CASE
+    check(hipGetLastError(), "near_fp64");
+}
+
+}  // namespace dev
+}  // namespace pbb

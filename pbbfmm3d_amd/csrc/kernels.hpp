@@ -1,0 +1,83 @@
+// kernels.hpp -- argument blocks and launchers of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "host.hpp"
+
+namespace pbb {
+
+inline void check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) fail(PBB_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace dev {
+
+struct KParams {
+    double p[4];
+};
+
+struct P2MArgs {
+    int n, ncols, cc;
+    int64_t N, off;            // off: global box offset of the leaf level
+    double len;
+    const int32_t *leaf_start, *coord;
+    const double *ctab, *Tkz;
+    const double *x, *y, *z, *ws;
+    double *S;
+};
+
+struct M2MArgs {
+    int n, ncols, cc;
+    int64_t off_parent, off_child;
+    const int32_t *child;       // nparent*8
+    const double *Cw;
+    double *Sparent;            // M2M: written; L2L: read (parent F)
+    double *Schild;             // M2M: read;    L2L: accumulated
+};
+
+struct GemmArgs {
+    int M, K;
+    int64_t ncol;
+    double alpha;
+    const double *A, *B;
+    double *C;
+};
+
+struct M2LArgs {
+    int rf, rs, ncols, TB, cpt, side;
+    int64_t off;
+    int tile_prefix[9];
+    int64_t class_start[9];
+    const int32_t *order, *coord, *map;
+    const double *E, *P;
+    double *Pf;
+};
+
+struct NearArgs {
+    int n, ncols, side;
+    int64_t N, off;
+    double len;
+    KParams kp;
+    const int32_t *leaf_start, *coord, *map, *perm;
+    const double *ctab, *Tkz;
+    const double *x, *y, *z, *ws;
+    const double *F;            // leaf-level local expansions (nullptr: no far field)
+    double *out;
+};
+
+void upload_dirs(const int dirs[8][189][4]);
+void launch_gather_weights(const double *w, const int32_t *perm, int64_t N, int ncols, double *ws, hipStream_t st);
+void launch_p2m(const P2MArgs &a, int nleaf, hipStream_t st);
+void launch_m2m(const M2MArgs &a, int nparent, hipStream_t st);
+void launch_l2l(const M2MArgs &a, int nparent, hipStream_t st);
+void launch_gemm(const GemmArgs &g, hipStream_t st);
+void launch_m2l(const M2LArgs &a, int ntiles, int ncolchunks, hipStream_t st);
+void launch_near(const NearArgs &a, int kid, int nleaf, hipStream_t st);
+
+}  // namespace dev
+}  // namespace pbb

@@ -1,0 +1,292 @@
+// precompute.cpp -- host FP64 operator precompute of the Chebyshev bbFMM.
+//
+// Follows H2_3D_Tree::buildFMMTree (H2_3D_Tree.cpp:40-66): ComputeWeights
+// (:715-826), then for each operator set the 316 direction blocks of the
+// weighted kernel (:277-374), the SVD of K_fat (n3 x 316 n3) for U and of
+// K_thin (316 n3 x n3) for VT with the tail-energy cutoff (:376-516), and
+// E_d = U^T K_d V (:518-556).  The SVD is this library's own: a Householder
+// TSQR reduces each wide/tall matrix to an n3 x n3 triangle, and a one-sided
+// Jacobi sweep on that triangle gives singular values and right singular
+// vectors to full FP64 accuracy (svd.cpp).  No LAPACK is needed.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <exception>
+#include <cstring>
+#include <thread>
+
+#include "host.hpp"
+
+namespace pbb {
+
+void fail(pbb_status st, const std::string &msg) { throw Error{st, msg}; }
+
+void parallel_for(int64_t n, const std::function<void(int64_t)> &body)
+{
+    if (n <= 0) return;
+    int nt = (int)std::max(1u, std::thread::hardware_concurrency());
+    nt = (int)std::min<int64_t>(nt, n);
+    if (nt == 1) {
+        for (int64_t i = 0; i < n; i++) body(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::vector<std::exception_ptr> errs(nt);
+    std::atomic<int64_t> next{0};
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&, t] {
+            try {
+                for (;;) {
+                    int64_t i = next.fetch_add(1);
+                    if (i >= n) break;
+                    body(i);
+                }
+            } catch (...) {
+                errs[t] = std::current_exception();
+                next.store(n);
+            }
+        });
+    for (auto &x : th) x.join();
+    for (auto &e : errs)
+        if (e) std::rethrow_exception(e);
+}
+
+bool kernel_id_valid(int id) { return id >= PBB_KERNEL_LAPLACIAN && id <= PBB_KERNEL_POLY_XY; }
+
+// Formulas of pbb_kernel_id (kernel_Types.cpp:15-178 for the built-ins).
+double kernel_eval_host(int id, const double *p, const double *t, const double *s)
+{
+    double dx = s[0] - t[0], dy = s[1] - t[1], dz = s[2] - t[2];
+    double r2 = dx * dx + dy * dy + dz * dz;
+    switch (id) {
+    case PBB_KERNEL_LAPLACIAN: return 1. / std::sqrt(r2);
+    case PBB_KERNEL_ONEOVERR2: { double ri = 1. / std::sqrt(r2); return ri * ri; }
+    case PBB_KERNEL_GAUSSIAN: return std::exp(-r2 / (p[0] * p[0]));
+    case PBB_KERNEL_EXPONENTIAL: return p[1] * std::exp(-std::sqrt(r2) / p[0]);
+    case PBB_KERNEL_SOFT_LAPLACIAN: return 1. / std::sqrt(r2 + p[0]);
+    case PBB_KERNEL_QUADRIC: return 1.0 + r2;
+    case PBB_KERNEL_INV_QUADRIC: return 1.0 / (1.0 + r2);
+    case PBB_KERNEL_LOGARITHM: return r2 == 0 ? 0. : 0.5 * std::log(r2);
+    case PBB_KERNEL_THIN_PLATE: return r2 == 0 ? 0. : 0.5 * r2 * std::log(r2);
+    case PBB_KERNEL_R2: return r2;
+    case PBB_KERNEL_CONSTANT: return 1.0;
+    case PBB_KERNEL_POLY_XY: return (t[0] - s[0]) * (t[1] + s[1]) + p[0];
+    default: return NAN;
+    }
+}
+
+void cheb_sn(int n, const double *Tkz, double x, double *S)
+{
+    const double pfac = 2. / n;
+    for (int m = 0; m < n; m++) {
+        const double *Tk = Tkz + m * n;
+        double d0 = 0, d1 = 0;
+        for (int j = n - 1; j > 0; j--) {
+            double d2 = d0;
+            d0 = 2.0 * x * d0 - d1 + Tk[j];
+            d1 = d2;
+        }
+        S[m] = pfac * (x * d0 - d1 + 0.5 * Tk[0]);
+    }
+}
+
+void ChebWeights::compute(int order)
+{
+    n = order;
+    int n2 = n * n, n3 = n2 * n;
+    nodes.resize(n);
+    for (int m = 0; m < n; m++) nodes[m] = std::cos(M_PI * ((double)m + 0.5) / (double)n);
+    Tkz.assign(n2, 0);
+    for (int m = 0; m < n; m++) {
+        double x = nodes[m], *v = &Tkz[m * n];
+        v[0] = 1;
+        if (n > 1) v[1] = x;
+        for (int k = 2; k < n; k++) v[k] = 2.0 * x * v[k - 1] - v[k - 2];
+    }
+    Kw.resize(n3);
+    int c = 0;
+    for (int l1 = 0; l1 < n; l1++) {
+        double t1 = 1. / std::sqrt(1 - nodes[l1] * nodes[l1]);
+        for (int l2 = 0; l2 < n; l2++) {
+            double t2 = t1 / std::sqrt(1 - nodes[l2] * nodes[l2]);
+            for (int l3 = 0; l3 < n; l3++) Kw[c++] = t2 / std::sqrt(1 - nodes[l3] * nodes[l3]);
+        }
+    }
+    // Cweights: child nodes mapped into the parent frame (H2_3D_Tree.cpp:775-819)
+    Cw.assign(2 * n2, 0);
+    std::vector<double> S(n);
+    for (int half = 0; half < 2; half++)
+        for (int j = 0; j < n; j++) {
+            cheb_sn(n, Tkz.data(), nodes[j] * .5 + (half ? 0.5 : -0.5), S.data());
+            for (int m = 0; m < n; m++) Cw[half * n2 + m * n + j] = S[m];
+        }
+    Ktable.assign(343, -1);
+    int ncell = 0, ninter = 0;
+    for (int l1 = -3; l1 < 4; l1++)
+        for (int l2 = -3; l2 < 4; l2++)
+            for (int l3 = -3; l3 < 4; l3++, ncell++)
+                if (std::abs(l1) > 1 || std::abs(l2) > 1 || std::abs(l3) > 1) Ktable[ncell] = ninter++;
+}
+
+int cutoff_rank(const std::vector<double> &sigma, double eps)
+{
+    int m = (int)sigma.size();
+    double sumsqr = 0;
+    for (int i = m - 1; i >= 0; --i) sumsqr += sigma[i] * sigma[i];
+    double sum = 0, epsqr = sumsqr * eps * eps;
+    int cut = m;
+    for (int i = m - 1; i >= 0; --i) {
+        sum += sigma[i] * sigma[i];
+        if (sum < epsqr) --cut;
+        else break;
+    }
+    return cut;
+}
+
+void kernel_blocks(const HostOps &ops, const HostEval &eval, double boxLen, std::vector<double> &Kcell)
+{
+    const int n = ops.n, n3 = ops.n3;
+    const size_t n6 = (size_t)n3 * n3;
+    const int symm = ops.kernel.symmetry;
+    const int symmNum = 158 * (2 - std::abs(symm));
+    const auto &nodes = ops.w.nodes;
+    const auto &Kw = ops.w.Kw;
+    int dirs[316][3], c = 0;
+    for (int k1 = -3; k1 < 4; k1++)
+        for (int k2 = -3; k2 < 4; k2++)
+            for (int k3 = -3; k3 < 4; k3++)
+                if (std::abs(k1) > 1 || std::abs(k2) > 1 || std::abs(k3) > 1) {
+                    dirs[c][0] = k1; dirs[c][1] = k2; dirs[c][2] = k3; c++;
+                }
+    Kcell.assign(316 * n6, 0.0);
+    parallel_for(symmNum, [&](int64_t d) {
+        double *K = &Kcell[d * n6];
+        for (int j = 0; j < n3; j++) {
+            int j1 = j / (n * n), j2 = (j / n) % n, j3 = j % n;
+            double s[3] = {(dirs[d][0] + 0.5 * nodes[j1]) * boxLen, (dirs[d][1] + 0.5 * nodes[j2]) * boxLen,
+                           (dirs[d][2] + 0.5 * nodes[j3]) * boxLen};
+            for (int i = 0; i < n3; i++) {
+                int i1 = i / (n * n), i2 = (i / n) % n, i3 = i % n;
+                double t[3] = {0.5 * nodes[i1] * boxLen, 0.5 * nodes[i2] * boxLen, 0.5 * nodes[i3] * boxLen};
+                K[i + (size_t)j * n3] = eval(t, s) / (Kw[i] * Kw[j]);
+            }
+        }
+    });
+    if (symm) {
+        for (int d = symmNum; d < 316; d++) {
+            double *K = &Kcell[d * n6];
+            const double *Ks = &Kcell[(size_t)(315 - d) * n6];
+            for (int j = 0; j < n3; j++)
+                for (int i = 0; i < n3; i++) K[i + (size_t)j * n3] = symm * Ks[j + (size_t)i * n3];
+        }
+    }
+}
+
+// E_d = U^T K_d V for the 316 directions (H2_3D_Tree.cpp:529-556).
+static void compress(int n3, int rf, int rs, const double *U, const double *VT, const std::vector<double> &Kcell,
+                     double *E)
+{
+    parallel_for(316, [&](int64_t d) {
+        const double *K = &Kcell[d * (size_t)n3 * n3];
+        std::vector<double> KV((size_t)n3 * rs, 0.0);
+        // KV = K * V, V(j, b) = VT[b + j*rs]
+        for (int j = 0; j < n3; j++)
+            for (int b = 0; b < rs; b++) {
+                double v = VT[b + (size_t)j * rs];
+                const double *kc = K + (size_t)j * n3;
+                double *kv = &KV[(size_t)b * n3];
+                for (int i = 0; i < n3; i++) kv[i] += kc[i] * v;
+            }
+        double *Ed = E + d * (size_t)rf * rs;
+        for (int b = 0; b < rs; b++)
+            for (int a = 0; a < rf; a++) {
+                const double *u = U + (size_t)a * n3, *kv = &KV[(size_t)b * n3];
+                double s = 0;
+                for (int i = 0; i < n3; i++) s += u[i] * kv[i];
+                Ed[a + (size_t)b * rf] = s;
+            }
+    });
+}
+
+void precompute_ops(HostOps &ops, const HostEval &eval)
+{
+    const int n = ops.n;
+    ops.n3 = n * n * n;
+    const int n3 = ops.n3;
+    ops.w.compute(n);
+    ops.homogen_flipped = -ops.kernel.homogen;
+    ops.homog = std::fabs(ops.homogen_flipped) > 0.1;  // IsHomoKernel, H2_3D_Tree.cpp:134-136
+    ops.nsets = ops.homog ? 1 : std::max(1, ops.tree_level - 1);
+    const int64_t m = 316LL * n3;
+
+    std::vector<double> Kcell, A, R, sig, V;
+    std::vector<double> Ufull, VTfull;
+    for (int set = 0; set < ops.nsets; set++) {
+        // homogeneous: one unit-length set; otherwise levels 2.. with L/4, L/8, ...
+        // (StartPrecompute, H2_3D_Tree.cpp:138-162)
+        double boxLen = ops.homog ? 1.0 : ops.L / 4;
+        if (!ops.homog)
+            for (int s = 0; s < set; s++) boxLen /= 2;
+        kernel_blocks(ops, eval, boxLen, Kcell);
+        const size_t n6 = (size_t)n3 * n3;
+
+        // K_fat^T: row (d, j), column i -> value K_d(i, j); left singular
+        // vectors of K_fat are the right singular vectors of its R factor.
+        A.assign((size_t)m * n3, 0.0);
+        parallel_for(n3, [&](int64_t i) {
+            double *col = &A[(size_t)i * m];
+            for (int d = 0; d < 316; d++)
+                for (int j = 0; j < n3; j++) col[(size_t)d * n3 + j] = Kcell[d * n6 + i + (size_t)j * n3];
+        });
+        tsqr_r(A.data(), m, n3, R);
+        jacobi_svd_right(R.data(), n3, sig, V);
+        if (set == 0) {
+            ops.sigma_f = sig;
+            ops.rf = cutoff_rank(sig, ops.eps);
+        }
+        Ufull = V;  // n3 x n3, columns = U
+
+        // K_thin: row (d, i), column j -> value K_d(i, j)
+        A.assign((size_t)m * n3, 0.0);
+        parallel_for(n3, [&](int64_t j) {
+            double *col = &A[(size_t)j * m];
+            for (int d = 0; d < 316; d++)
+                std::memcpy(col + (size_t)d * n3, &Kcell[d * n6 + (size_t)j * n3], sizeof(double) * n3);
+        });
+        tsqr_r(A.data(), m, n3, R);
+        jacobi_svd_right(R.data(), n3, sig, V);
+        if (set == 0) {
+            ops.sigma_s = sig;
+            ops.rs = cutoff_rank(sig, ops.eps);
+            ops.U.assign((size_t)ops.nsets * n3 * ops.rf, 0.0);
+            ops.VT.assign((size_t)ops.nsets * ops.rs * n3, 0.0);
+            ops.E.assign((size_t)ops.nsets * 316 * ops.rf * ops.rs, 0.0);
+        }
+        const int rf = ops.rf, rs = ops.rs;
+        double *U = &ops.U[(size_t)set * n3 * rf];
+        double *VT = &ops.VT[(size_t)set * rs * n3];
+        std::memcpy(U, Ufull.data(), sizeof(double) * (size_t)n3 * rf);
+        for (int j = 0; j < n3; j++)
+            for (int b = 0; b < rs; b++) VT[b + (size_t)j * rs] = V[j + (size_t)b * n3];
+        compress(n3, rf, rs, U, VT, Kcell, &ops.E[(size_t)set * 316 * rf * rs]);
+    }
+}
+
+// Far directions of an octant class: the children of the parent's 27
+// neighbours that are not among the target's own 27 neighbours
+// (InteractionList, compute.hpp:1062-1100, on a uniform grid).
+void class_directions(int octant, const std::vector<int> &Ktable, int dirs[189][4])
+{
+    int o[3] = {(octant >> 2) & 1, (octant >> 1) & 1, octant & 1};
+    int k = 0;
+    for (int dx = -2 - o[0]; dx <= 3 - o[0]; dx++)
+        for (int dy = -2 - o[1]; dy <= 3 - o[1]; dy++)
+            for (int dz = -2 - o[2]; dz <= 3 - o[2]; dz++) {
+                if (std::abs(dx) <= 1 && std::abs(dy) <= 1 && std::abs(dz) <= 1) continue;
+                dirs[k][0] = dx; dirs[k][1] = dy; dirs[k][2] = dz;
+                dirs[k][3] = Ktable[(dx + 3) * 49 + (dy + 3) * 7 + (dz + 3)];
+                k++;
+            }
+}
+
+}  // namespace pbb

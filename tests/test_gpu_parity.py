@@ -1,0 +1,139 @@
+"""FP64 GPU matvec (through the C ABI) against the CPU oracle.
+
+The oracle (oracle/pbb_oracle.c) restates compute.hpp; both sides use the
+same operators (exported from the product) so that the comparison isolates
+the matvec.  Bar: relative L2 difference <= 1e-10 (FP64, different summation
+orders only).  Polynomial kernels of degree < n are reproduced exactly by
+the Chebyshev FMM, so those cases are also checked against the direct sum.
+"""
+import numpy as np
+import pytest
+
+import pbbfmm3d_amd as pbb
+from oracle import oracle as orc
+from tests._helpers import (clustered_points, gpu_matvec, oracle_ops_from, uniform_points,
+                            weights)
+
+PARITY = 1e-10
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(kernel, L, level, n, eps, xyz, w, ncols):
+    print(f"case kernel={kernel.id} L={L} level={level} n={n} eps={eps} N={len(xyz)} ncols={ncols}",
+          flush=True)
+    ops = pbb.Operators(kernel, L, level, n, eps)
+    plan = pbb.Plan(ops, xyz)
+    got = gpu_matvec(plan, w, ncols)
+    ref = orc.matvec(oracle_ops_from(ops, kernel), xyz, w, ncols)
+    return ops, plan, got, ref
+
+
+def test_laplacian_c1_shape(gpu):
+    # the reference's CPU-runnable case: 20k uniform, level 4, order 4, eps 1e-5
+    xyz = uniform_points(20000, seed=0)
+    w = weights(20000, 1, seed=1)
+    ops, plan, got, ref = _run(pbb.laplacian(), 1.0, 4, 4, 1e-5, xyz, w, 1)
+    assert ops.rank_f == 48 and ops.rank_s == 48
+    assert orc.rel_err(got, ref) <= PARITY
+    sample = np.arange(0, 20000, 97)
+    d = orc.direct(orc.LAPLACIAN, None, xyz[sample], xyz, w, 1)
+    assert orc.rel_err(got[sample], d) < 1e-3
+
+
+def test_gaussian_per_level_operators(gpu):
+    # non-homogeneous kernel: one operator set per level 2..4
+    xyz = uniform_points(12000, seed=2)
+    w = weights(12000, 3, seed=3)
+    ops, plan, got, ref = _run(pbb.gaussian(0.3), 1.0, 4, 5, 1e-7, xyz, w, 3)
+    assert ops.nsets == 3
+    assert orc.rel_err(got, ref) <= PARITY
+
+
+@pytest.mark.parametrize("kernel", [pbb.r2_homogeneous(), pbb.quadric(), pbb.constant()],
+                         ids=["r2", "quadric", "constant"])
+def test_polynomial_kernels_exact(gpu, kernel):
+    xyz = clustered_points(5000, seed=4)
+    w = weights(5000, 2, seed=5)
+    ops, plan, got, ref = _run(kernel, 1.0, 4, 4, 1e-12, xyz, w, 2)
+    assert orc.rel_err(got, ref) <= PARITY
+    d = orc.direct(kernel.id, kernel.params, xyz, xyz, w, 2)
+    assert orc.rel_err(got, d) <= 1e-12
+
+
+def test_c2_shape_ncols8(gpu):
+    xyz = uniform_points(40000, seed=6)
+    w = weights(40000, 8, seed=7)
+    ops, plan, got, ref = _run(pbb.laplacian(), 1.0, 4, 5, 1e-7, xyz, w, 8)
+    assert orc.rel_err(got, ref) <= PARITY
+
+
+def test_clustered_wide_columns(gpu):
+    # ncols > 64 exercises column chunking in M2L and the near field
+    xyz = clustered_points(6000, seed=8)
+    w = weights(6000, 70, seed=9, dist="normal")
+    ops, plan, got, ref = _run(pbb.laplacian(), 1.0, 4, 4, 1e-5, xyz, w, 70)
+    assert orc.rel_err(got, ref) <= PARITY
+
+
+@pytest.mark.parametrize("level", [1, 2, 3, 5])
+def test_levels(gpu, level):
+    xyz = uniform_points(3000, seed=10 + level)
+    w = weights(3000, 2, seed=11)
+    ops, plan, got, ref = _run(pbb.laplacian(), 1.0, level, 4, 1e-6, xyz, w, 2)
+    assert orc.rel_err(got, ref) <= PARITY
+
+
+def test_duplicates_and_tiny(gpu):
+    # coincident points: 1/r at r = 0 is inf and is zeroed (compute.hpp:767)
+    base = uniform_points(40, seed=12)
+    xyz = np.concatenate([base, base[:10]])
+    w = weights(len(xyz), 1, seed=13)
+    ops, plan, got, ref = _run(pbb.laplacian(), 1.0, 3, 4, 1e-6, xyz, w, 1)
+    assert np.all(np.isfinite(got))
+    assert orc.rel_err(got, ref) <= PARITY
+    d = orc.direct(orc.LAPLACIAN, None, xyz, xyz, w, 1)
+    assert orc.rel_err(got, d) < 1e-3
+
+
+def test_single_point(gpu):
+    xyz = np.array([[0.3, 0.2, 0.1]])
+    ops = pbb.Operators(pbb.gaussian(1.0), 1.0, 3, 4, 1e-6)
+    plan = pbb.Plan(ops, xyz)
+    got = gpu_matvec(plan, np.array([2.5]), 1)
+    assert abs(got[0] - 2.5) < 1e-14  # K(x, x) = 1 for the Gaussian
+
+
+def test_soft_laplacian_and_exponential(gpu):
+    xyz = clustered_points(8000, seed=14)
+    for kernel in (pbb.soft_laplacian(1e-6), pbb.exponential(0.5)):
+        w = weights(8000, 4, seed=15, dist="normal")
+        ops, plan, got, ref = _run(kernel, 1.0, 4, 5, 1e-7, xyz, w, 4)
+        assert orc.rel_err(got, ref) <= PARITY
+
+
+def test_output_accumulates_and_plan_reuse(gpu):
+    import torch
+    xyz = uniform_points(5000, seed=16)
+    w = weights(5000, 2, seed=17)
+    ops = pbb.Operators(pbb.laplacian(), 1.0, 4, 4, 1e-5)
+    plan = pbb.Plan(ops, xyz)
+    wd = torch.tensor(w.reshape(2, 5000), dtype=torch.float64, device="cuda:0")
+    out = torch.full_like(wd, 1.0)
+    plan.matvec(wd, out)
+    plan.matvec(wd, out)  # a plan serves any number of matvecs
+    torch.cuda.synchronize()
+    ref = orc.matvec(oracle_ops_from(ops, pbb.laplacian()), xyz, w, 2)
+    got = out.cpu().numpy().reshape(-1)
+    assert orc.rel_err(got - 1.0, 2 * ref) <= PARITY
+
+
+def test_drop_in_host_path(gpu):
+    # pbb_matvec: the H2_3D_Compute constructor's contract on host buffers
+    xyz = uniform_points(6000, seed=18)
+    w = weights(6000, 3, seed=19)
+    ops = pbb.Operators(pbb.laplacian(), 1.0, 4, 4, 1e-5)
+    out = np.full(6000 * 3, 0.5)
+    pbb.matvec_host(ops, xyz, w, 3, out)
+    ref = orc.matvec(oracle_ops_from(ops, pbb.laplacian()), xyz, w, 3)
+    assert orc.rel_err(out - 0.5, ref) <= PARITY
