@@ -1,0 +1,78 @@
+"""The C-ABI library: loads, exports every declared entry point, and turns
+bad input into status codes (no GPU needed)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pbbfmm3d_amd as pbb
+from pbbfmm3d_amd import _capi as capi
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include", "pbbfmm3d_amd.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"PBB_API\s+[\w\s\*]+?\b(pbb_\w+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol():
+    names = declared_symbols()
+    assert len(names) >= 15
+    L = C.CDLL(capi.LIB_PATH)
+    for nm in names:
+        assert hasattr(L, nm), nm
+    assert set(names) <= set(capi.exported_symbols())
+
+
+def test_abi_version_and_no_device_here():
+    assert capi.lib().pbb_abi_version() == 1
+    assert capi.lib().pbb_device_count() >= 0
+
+
+def _status(fn, *args):
+    return fn(*args)
+
+
+def test_invalid_arguments_return_status():
+    L = capi.lib()
+    h = C.c_void_p()
+    d = pbb.laplacian().desc()
+    none = capi.HOST_KERNEL_FN()
+    assert L.pbb_ops_create(C.byref(d), none, None, 1.0, 3, 4, 1e-5, 0, C.byref(h)) == \
+        capi.PBB_ERR_UNSUPPORTED  # uniform-grid branch
+    assert b"Chebyshev" in L.pbb_last_error()
+    assert L.pbb_ops_create(C.byref(d), none, None, 1.0, 3, 1, 1e-5, 1, C.byref(h)) == \
+        capi.PBB_ERR_INVALID_ARG
+    assert L.pbb_ops_create(C.byref(d), none, None, -1.0, 3, 4, 1e-5, 1, C.byref(h)) == \
+        capi.PBB_ERR_INVALID_ARG
+    assert L.pbb_ops_create(C.byref(d), none, None, 1.0, 0, 4, 1e-5, 1, C.byref(h)) == \
+        capi.PBB_ERR_INVALID_ARG
+    assert L.pbb_ops_create(None, none, None, 1.0, 3, 4, 1e-5, 1, C.byref(h)) == \
+        capi.PBB_ERR_INVALID_ARG
+    bad = capi.KernelDesc()
+    bad.id = 99
+    bad.symmetry = 1
+    assert L.pbb_ops_create(C.byref(bad), none, None, 1.0, 3, 4, 1e-5, 1, C.byref(h)) == \
+        capi.PBB_ERR_INVALID_ARG
+    assert L.pbb_plan_matvec(None, None, 1, None, 0, None) == capi.PBB_ERR_INVALID_ARG
+    assert L.pbb_ops_query(None, None) == capi.PBB_ERR_INVALID_ARG
+
+
+def test_plan_without_device_fails_cleanly():
+    if capi.lib().pbb_device_count() > 0:
+        pytest.skip("a device is visible")
+    ops = pbb.Operators(pbb.laplacian(), 1.0, 3, 4, 1e-5)
+    with pytest.raises(capi.PBBError) as e:
+        pbb.Plan(ops, np.random.default_rng(0).random((100, 3)))
+    assert e.value.status == capi.PBB_ERR_NO_DEVICE
+
+
+def test_ops_info():
+    ops = pbb.Operators(pbb.gaussian(0.3), 1.0, 5, 4, 1e-6)
+    assert ops.order == 4 and ops.n3 == 64
+    assert not ops.homogeneous and ops.nsets == 4  # levels 2..5
+    lap = pbb.Operators(pbb.laplacian(), 1.0, 5, 4, 1e-6)
+    assert lap.homogeneous and lap.nsets == 1

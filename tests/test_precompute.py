@@ -1,0 +1,100 @@
+"""The product's host FP64 operator precompute (own TSQR + Jacobi SVD),
+called through the C ABI, against the oracle's numpy/LAPACK precompute.
+
+SVD factors are unique only up to signs and, inside a cluster of equal
+singular values, up to rotation; the comparisons use what is invariant:
+singular values, cutoffs, orthonormality, U E V^T, and the projector U U^T
+wherever the cut does not split a cluster.
+"""
+import numpy as np
+import pytest
+
+import pbbfmm3d_amd as pbb
+from oracle import oracle as orc
+
+CASES = [  # kernel, L, levels, n, eps
+    (pbb.laplacian(), 1.0, 4, 4, 1e-5),
+    (pbb.laplacian(), 1.0, 4, 5, 1e-7),
+    (pbb.gaussian(0.3), 1.0, 4, 4, 1e-7),
+    (pbb.exponential(0.5), 2.0, 3, 4, 1e-6),
+    (pbb.oneoverr2(), 1.0, 3, 3, 1e-4),
+]
+
+
+def _pair(kernel, L, levels, n, eps):
+    ops = pbb.Operators(kernel, L, levels, n, eps)
+    ref = orc.precompute(kernel.id, kernel.params, kernel.symmetry, kernel.homogen, L, levels, n, eps)
+    return ops, ref
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"k{c[0].id}-n{c[3]}-eps{c[4]}")
+def test_singular_values_and_cutoffs(case):
+    ops, ref = _pair(*case)
+    U, VT, E, sf, ss = ops.export()
+    np.testing.assert_allclose(sf, ref.sigma_f, rtol=0, atol=1e-12 * ref.sigma_f[0])
+    np.testing.assert_allclose(ss, ref.sigma_s, rtol=0, atol=1e-12 * ref.sigma_s[0])
+    assert (ops.rank_f, ops.rank_s) == (ref.rf, ref.rs)
+    assert ops.nsets == ref.nsets
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"k{c[0].id}-n{c[3]}-eps{c[4]}")
+def test_operator_invariants(case):
+    ops, ref = _pair(*case)
+    n3, rf, rs = ops.n3, ops.rank_f, ops.rank_s
+    U, VT, E, sf, _ = ops.export()
+    kernel, L, levels, n, eps = case
+    for s in range(ops.nsets):
+        Us = U[s * n3 * rf:(s + 1) * n3 * rf].reshape(rf, n3).T
+        Vs = VT[s * rs * n3:(s + 1) * rs * n3].reshape(n3, rs)  # V = VT^T
+        Es = E[s * 316 * rf * rs:(s + 1) * 316 * rf * rs].reshape(316, rs, rf).transpose(0, 2, 1)
+        np.testing.assert_allclose(Us.T @ Us, np.eye(rf), atol=1e-13)
+        np.testing.assert_allclose(Vs.T @ Vs, np.eye(rs), atol=1e-13)
+        box = 1.0 if ops.homogeneous else L / 4 / 2 ** s
+        blocks = orc.kernel_blocks(kernel.id, kernel.params, kernel.symmetry, n, box)
+        Ro = ref.U[s * n3 * rf:(s + 1) * n3 * rf].reshape(rf, n3).T
+        Vo = ref.VT[s * rs * n3:(s + 1) * rs * n3].reshape(n3, rs)
+        Eo = ref.E[s * 316 * rf * rs:(s + 1) * 316 * rf * rs].reshape(316, rs, rf).transpose(0, 2, 1)
+        for d in (0, 77, 158, 315):
+            K = blocks[d].T
+            np.testing.assert_allclose(Es[d], Us.T @ K @ Vs, atol=1e-12 * np.abs(K).max())
+            # both compressions approximate K to the truncation level, and
+            # agree with each other to that level
+            A = Us @ Es[d] @ Vs.T
+            B = Ro @ Eo[d] @ Vo.T
+            assert np.linalg.norm(A - K) <= 50 * eps * np.linalg.norm(K) + 1e-13
+            assert np.linalg.norm(A - B) <= 50 * eps * np.linalg.norm(K) + 1e-13
+        # projectors agree when the cut does not split a cluster
+        gap = (sf[rf - 1] - sf[rf]) / sf[0] if rf < n3 else 1.0
+        if gap > 1e-6:
+            np.testing.assert_allclose(Us @ Us.T, Ro @ Ro.T, atol=1e-8)
+
+
+def test_host_callback_matches_functor():
+    # a host callback (the virtual EvaluateKernel path) gives the same ops
+    k = pbb.gaussian(0.3)
+    a = pbb.Operators(k, 1.0, 3, 4, 1e-6)
+    b = pbb.Operators(k, 1.0, 3, 4, 1e-6, host_eval=lambda t, s: k.eval(t, s))
+    Ua, VTa, Ea, sfa, ssa = a.export()
+    Ub, VTb, Eb, sfb, ssb = b.export()
+    np.testing.assert_allclose(sfa, sfb, rtol=0, atol=1e-13 * sfa[0])
+    np.testing.assert_allclose(ssa, ssb, rtol=0, atol=1e-13 * ssa[0])
+    assert (a.rank_f, a.rank_s) == (b.rank_f, b.rank_s)
+    n3, rf, rs = a.n3, a.rank_f, a.rank_s
+    for s in range(a.nsets):  # U E V^T is invariant under the SVD's freedom
+        def full(U, VT, E, d):
+            Us = U[s * n3 * rf:(s + 1) * n3 * rf].reshape(rf, n3).T
+            Vs = VT[s * rs * n3:(s + 1) * rs * n3].reshape(n3, rs)
+            Es = E[(s * 316 + d) * rf * rs:(s * 316 + d + 1) * rf * rs].reshape(rs, rf).T
+            return Us @ Es @ Vs.T
+        for d in (0, 200):
+            A, B = full(Ua, VTa, Ea, d), full(Ub, VTb, Eb, d)
+            assert np.linalg.norm(A - B) <= 1e-5 * np.linalg.norm(A)
+
+
+def test_from_arrays_roundtrip():
+    a = pbb.Operators(pbb.laplacian(), 1.0, 3, 4, 1e-5)
+    U, VT, E, _, _ = a.export()
+    b = pbb.Operators.from_arrays(pbb.laplacian(), 1.0, 3, 4, 1e-5, a.rank_f, a.rank_s, a.nsets,
+                                  U, VT, E)
+    for x, y in zip(a.export()[:3], b.export()[:3]):
+        np.testing.assert_array_equal(x, y)
