@@ -1,0 +1,200 @@
+#!/usr/bin/env python
+"""Benchmark of the black-box FMM matvec on one MI355X.
+
+Workload (default): 4,000,000 points, a mixture of 8 isotropic Gaussians
+(sigma 0.05, centres U[0.2,0.8]^3) plus 20% uniform background clipped to
+the unit cube; weights N(0,1), 32 columns; Gaussian kernel exp(-r^2/0.3^2);
+tree level 7, Chebyshev order 6, SVD eps 1e-8.  One step = one matvec
+(upward pass, M2L, downward pass, near field) from device-resident weights
+to device output on a prepared tree.  Prints ONE JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--precision fp64|mixed]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "targets/s at N=4M nCols=32, rel-err<=1e-6 vs FP64: mixed MFMA vs FP64 GPU"
+PEAK = {"fp64": ("mfma", 78.6, "TFLOP/s"), "mixed": ("mfma", 2500.0, "TFLOP/s")}
+
+
+def clustered_points(n, seed):
+    rng = np.random.default_rng(seed)
+    nb = int(round(0.2 * n))
+    nc = n - nb
+    centres = rng.uniform(0.2, 0.8, (8, 3))
+    lab = rng.integers(0, 8, nc)
+    pts = centres[lab] + 0.05 * rng.standard_normal((nc, 3))
+    return np.clip(np.concatenate([pts, rng.random((nb, 3))]), 0.0, 1.0)
+
+
+def make_kernel(name):
+    import pbbfmm3d_amd as pbb
+    return {"gaussian": pbb.gaussian(0.3), "laplacian": pbb.laplacian()}[name]
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(args, kernel):
+    """The oracle (plain-C restatement of the reference's CPU matvec) on a
+    bounded sample of the same workload: N/512 points of the same recipe
+    (another seed), tree level - 3 (same points per leaf), same kernel,
+    order, eps and columns."""
+    import pbbfmm3d_amd as pbb
+    from oracle import oracle as orc
+    ns = max(1000, args.npoints // 512)
+    lvl = max(2, args.level - 3)
+    xyz = clustered_points(ns, seed=3)
+    w = np.random.default_rng(4).standard_normal(ns * args.ncols)
+    ops = pbb.Operators(kernel, 1.0, lvl, args.order, args.eps)
+    U, VT, E, _, _ = ops.export()
+    p = np.zeros(4)
+    p[: len(kernel.params)] = kernel.params
+    oops = orc.Ops(kernel.id, p, kernel.symmetry, kernel.homogen, 1.0, lvl, args.order,
+                   args.eps, ops.rank_f, ops.rank_s, ops.nsets, U, VT, E)
+    t0 = time.perf_counter()
+    orc.matvec(oops, xyz, w, args.ncols)
+    dt = time.perf_counter() - t0
+    return {"value": ns / dt, "unit": "targets/s", "cores": orc.num_threads(), "kind": "port",
+            "sample": f"{ns} points of the same clustered recipe, tree level {lvl}, order "
+                      f"{args.order}, eps {args.eps}, {args.ncols} columns, {kernel_name(args)} "
+                      f"kernel; oracle matvec wall time {dt:.2f} s"}
+
+
+def kernel_name(args):
+    return {"gaussian": "exp(-r^2/0.3^2)", "laplacian": "1/r"}[args.kernel]
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--kernel", default="gaussian", choices=["gaussian", "laplacian"])
+    ap.add_argument("--npoints", type=int, default=4_000_000)
+    ap.add_argument("--ncols", type=int, default=32)
+    ap.add_argument("--level", type=int, default=7)
+    ap.add_argument("--order", type=int, default=6)
+    ap.add_argument("--eps", type=float, default=1e-8)
+    ap.add_argument("--err-targets", type=int, default=10000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import pbbfmm3d_amd as pbb
+    from pbbfmm3d_amd import _capi as capi
+
+    dev = torch.device("cuda:0")
+    kernel = make_kernel(args.kernel)
+    N, nc = args.npoints, args.ncols
+    log(f"workload N={N} ncols={nc} level={args.level} order={args.order} eps={args.eps} "
+        f"kernel={args.kernel} precision={args.precision}")
+    xyz = clustered_points(N, seed=0)
+    w_host = np.random.default_rng(1).standard_normal((nc, N))
+
+    t0 = time.perf_counter()
+    ops = pbb.Operators(kernel, 1.0, args.level, args.order, args.eps)
+    t_pre = time.perf_counter() - t0
+    log(f"precompute {t_pre:.2f} s, ranks {ops.rank_f}/{ops.rank_s}, sets {ops.nsets}")
+    t0 = time.perf_counter()
+    plan = pbb.Plan(ops, xyz)
+    t_plan = time.perf_counter() - t0
+    info = plan.info()
+    log(f"plan {t_plan:.2f} s, boxes {info['boxes']}, m2l pairs {info['m2l_pairs']}, "
+        f"near pairs {info['near_pairs']}")
+
+    prec = capi.PREC_FP64 if args.precision == "fp64" else capi.PREC_MIXED
+    w = torch.tensor(w_host, dtype=torch.float64, device=dev)
+    out = torch.zeros_like(w)
+    plan.set_timing(True)
+    for i in range(args.warmup):
+        plan.matvec(w, out, precision=prec)
+        torch.cuda.synchronize()
+        log(f"warmup {i} done")
+    phase = {k: 0.0 for k in capi.PHASE_NAMES}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        plan.matvec(w, out, precision=prec)
+        for k, v in plan.timings().items():  # waits for this step's events
+            phase[k] += v
+        log(f"step {i} done")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = N / (elapsed / args.steps)
+    phase = {k: v / args.steps for k, v in phase.items()}
+
+    # accuracy: one clean matvec against the FP64 direct sum on sampled targets
+    out.zero_()
+    plan.matvec(w, out, precision=prec)
+    ts = np.sort(np.random.default_rng(2).choice(N, size=min(args.err_targets, N), replace=False))
+    tgt = torch.tensor(xyz[ts], dtype=torch.float64, device=dev)
+    src = torch.tensor(xyz, dtype=torch.float64, device=dev)
+    ref = torch.zeros((nc, len(ts)), dtype=torch.float64, device=dev)
+    d = kernel.desc()
+    import ctypes as C
+    capi.check(capi.lib().pbb_direct_sum(C.byref(d), tgt.data_ptr(), len(ts), src.data_ptr(), N,
+                                         w.data_ptr(), nc, ref.data_ptr(), None))
+    torch.cuda.synchronize()
+    got = out[:, torch.tensor(ts, device=dev)]
+    rel_err = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
+    log(f"rel err vs FP64 direct sum on {len(ts)} targets: {rel_err:.3e}")
+
+    # roofline of the dominant kernel (M2L): algorithmic flops 2*rf*rs*ncols
+    # per far-field box pair, over the M2L phase time (HIP events on the
+    # stream the M2L launches run on)
+    flops = 2.0 * ops.rank_f * ops.rank_s * nc * info["m2l_pairs"]
+    achieved = flops / (phase["m2l"] * 1e-3) / 1e12
+    bound, peak, unit = PEAK[args.precision]
+    roofline = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+                "frac": round(achieved / peak, 4), "traffic": None,
+                "kernel": "m2l", "launches_per_step": args.level - 1,
+                "algorithmic_flops_per_step": flops}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle on a bounded sample) ...")
+        cpu = cpu_baseline(args, kernel)
+        log(f"cpu baseline {cpu['value']:.1f} targets/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "targets/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "bf16x3/f32/f64",
+        "data": "synthetic",
+        "config": {"workload": f"{N} clustered points (8 Gaussians sigma 0.05 + 20% uniform), "
+                               f"{kernel_name(args)}, {nc} columns, level {args.level}, order "
+                               f"{args.order}, eps {args.eps}",
+                   "npoints": N, "ncols": nc, "tree_level": args.level, "order": args.order,
+                   "eps": args.eps, "kernel": args.kernel, "precision": args.precision},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "rel_err_vs_direct": rel_err,
+        "phase_ms": {k: round(v, 3) for k, v in phase.items()},
+        "ranks": [ops.rank_f, ops.rank_s],
+        "m2l_pairs": info["m2l_pairs"], "near_pairs": info["near_pairs"],
+        "precompute_s": round(t_pre, 3), "plan_s": round(t_plan, 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

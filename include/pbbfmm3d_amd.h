@@ -185,6 +185,16 @@ PBB_API pbb_status pbb_plan_set_timing(pbb_plan *plan, int32_t enable);
 
 PBB_API void pbb_plan_destroy(pbb_plan *plan);
 
+/* ---- accuracy harness ------------------------------------------------- */
+
+/* Replaces DirectCalc3D (test.hpp:18-34): the O(T*N) FP64 direct sum
+ * out[c*T + i] = sum_j K(t_i, s_j) w[c*N + j] with isinf -> 0, on device
+ * buffers (targets and sources AoS xyz).  Overwrites out; synchronous.     */
+PBB_API pbb_status pbb_direct_sum(const pbb_kernel_desc *kernel, const double *tgt_dev,
+                                  int64_t ntargets, const double *src_dev, int64_t nsources,
+                                  const double *w_dev, int32_t ncols, double *out_dev,
+                                  void *stream);
+
 /* ---- one-shot drop-in ------------------------------------------------- */
 
 /* Replaces the H2_3D_Compute<T> constructor (compute.hpp:67-149): host

@@ -550,6 +550,41 @@ PBB_API pbb_status pbb_plan_matvec(pbb_plan *plan, const double *w_dev, int32_t 
 
 PBB_API void pbb_plan_destroy(pbb_plan *plan) { delete plan; }
 
+PBB_API pbb_status pbb_direct_sum(const pbb_kernel_desc *kernel, const double *tgt_dev, int64_t ntargets,
+                                  const double *src_dev, int64_t nsources, const double *w_dev, int32_t ncols,
+                                  double *out_dev, void *stream)
+{
+    return guarded([&] {
+        if (!kernel || !tgt_dev || !src_dev || !w_dev || !out_dev) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        if (ntargets < 0 || nsources < 0 || ncols < 1) fail(PBB_ERR_INVALID_ARG, "bad sizes");
+        if (!kernel_id_valid(kernel->id) || kernel->id == PBB_KERNEL_POLY_XY)
+            fail(PBB_ERR_UNSUPPORTED, "the kernel has no device functor");
+        (void)current_device();
+        hipStream_t st = (hipStream_t)stream;
+        dev::DirectArgs a{};
+        a.T = ntargets; a.Ns = nsources; a.ncols = ncols;
+        int64_t tblocks = (ntargets + 63) / 64;
+        a.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(256, (4096 + tblocks - 1) / std::max<int64_t>(1, tblocks)));
+        a.nsplit = (int)std::min<int64_t>(a.nsplit, std::max<int64_t>(1, nsources / 256));
+        std::memcpy(a.kp.p, kernel->param, sizeof(a.kp.p));
+        a.tgt = tgt_dev; a.src = src_dev; a.w = w_dev; a.out = out_dev;
+        if (ntargets == 0) return;
+        if (nsources == 0) {
+            check(hipMemsetAsync(out_dev, 0, sizeof(double) * ntargets * ncols, st), "clear output");
+            return;
+        }
+        a.part = dalloc<double>((size_t)a.nsplit * ncols * ntargets, "direct-sum partials");
+        try {
+            dev::launch_direct(a, kernel->id, st);
+            check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        } catch (...) {
+            dfree(a.part);
+            throw;
+        }
+        dfree(a.part);
+    });
+}
+
 PBB_API pbb_status pbb_matvec(pbb_ops *ops, const double *target, const double *source, int64_t npoints,
                               const double *weight, int32_t ncols, double *output)
 {

@@ -510,6 +510,76 @@ __global__ __launch_bounds__(256) void near_fp64(NearArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// Direct sum (DirectCalc3D, test.hpp:18-34, isinf -> 0): partial sums of
+// one slice of the sources per workgroup row, reduced by a second kernel.
+//   part[(split*ncols + c)*T + i] = sum_{j in slice} K(t_i, s_j) w[c*Ns + j]
+constexpr int DS_TGT = 64, DS_SRC = 64, DS_CPT = 8;
+
+template <int KID>
+__global__ __launch_bounds__(256) void direct_partial(DirectArgs a)
+{
+    __shared__ double sx[DS_SRC], sy[DS_SRC], sz[DS_SRC];
+    __shared__ double sw[4 * DS_CPT][DS_SRC];
+    const int tid = threadIdx.x, slot = tid / 4, g = tid % 4;
+    const int64_t i = (int64_t)blockIdx.x * DS_TGT + slot;
+    const int split = blockIdx.y;
+    const int64_t per = (a.Ns + a.nsplit - 1) / a.nsplit;
+    const int64_t j0s = split * per, j1s = min(a.Ns, j0s + per);
+    const bool active = i < a.T;
+    double xi = 0, yi = 0, zi = 0;
+    if (active) { xi = a.tgt[3 * i]; yi = a.tgt[3 * i + 1]; zi = a.tgt[3 * i + 2]; }
+    for (int c0 = 0; c0 < a.ncols; c0 += 4 * DS_CPT) {
+        const int cn = min(4 * DS_CPT, a.ncols - c0);
+        double acc[DS_CPT];
+#pragma unroll
+        for (int k = 0; k < DS_CPT; k++) acc[k] = 0;
+        for (int64_t j0 = j0s; j0 < j1s; j0 += DS_SRC) {
+            const int jn = (int)min((int64_t)DS_SRC, j1s - j0);
+            __syncthreads();
+            if (tid < jn) {
+                sx[tid] = a.src[3 * (j0 + tid)];
+                sy[tid] = a.src[3 * (j0 + tid) + 1];
+                sz[tid] = a.src[3 * (j0 + tid) + 2];
+            }
+            for (int q = tid; q < cn * DS_SRC; q += 256) {
+                int c = q / DS_SRC, p = q % DS_SRC;
+                sw[c][p] = p < jn ? a.w[(int64_t)(c0 + c) * a.Ns + j0 + p] : 0.0;
+            }
+            __syncthreads();
+            if (active) {
+                for (int p = 0; p < jn; p++) {
+                    double kv = keval<KID>(sx[p] - xi, sy[p] - yi, sz[p] - zi, a.kp);
+                    if (isinf(kv)) kv = 0.0;
+#pragma unroll
+                    for (int k = 0; k < DS_CPT; k++) {
+                        int cl = g + 4 * k;
+                        if (cl < cn) acc[k] += kv * sw[cl][p];
+                    }
+                }
+            }
+        }
+        if (active) {
+#pragma unroll
+            for (int k = 0; k < DS_CPT; k++) {
+                int cl = g + 4 * k;
+                if (cl < cn) a.part[((int64_t)split * a.ncols + c0 + cl) * a.T + i] = acc[k];
+            }
+        }
+    }
+}
+
+__global__ void direct_reduce(const double *__restrict__ part, int nsplit, int64_t total,
+                              double *__restrict__ out)
+{
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0;
+        for (int k = 0; k < nsplit; k++) s += part[k * total + q];
+        out[q] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 
 void upload_dirs(const int dirs[8][189][4])
@@ -592,6 +662,31 @@ void launch_near(const NearArgs &a, int kid, int nleaf, hipStream_t st)
     }
 #undef PBB_NEAR_CASE
     check(hipGetLastError(), "near_fp64");
+}
+
+void launch_direct(const DirectArgs &a, int kid, hipStream_t st)
+{
+    if (a.T <= 0) return;
+    dim3 grid((unsigned)((a.T + DS_TGT - 1) / DS_TGT), (unsigned)a.nsplit);
+#define PBB_DIRECT_CASE(K)                                \
+    case K:                                               \
+        direct_partial<K><<<grid, 256, 0, st>>>(a);       \
+        break;
+    switch (kid) {
+        PBB_DIRECT_CASE(PBB_KERNEL_LAPLACIAN) PBB_DIRECT_CASE(PBB_KERNEL_ONEOVERR2)
+        PBB_DIRECT_CASE(PBB_KERNEL_GAUSSIAN) PBB_DIRECT_CASE(PBB_KERNEL_EXPONENTIAL)
+        PBB_DIRECT_CASE(PBB_KERNEL_SOFT_LAPLACIAN) PBB_DIRECT_CASE(PBB_KERNEL_QUADRIC)
+        PBB_DIRECT_CASE(PBB_KERNEL_INV_QUADRIC) PBB_DIRECT_CASE(PBB_KERNEL_LOGARITHM)
+        PBB_DIRECT_CASE(PBB_KERNEL_THIN_PLATE) PBB_DIRECT_CASE(PBB_KERNEL_R2)
+        PBB_DIRECT_CASE(PBB_KERNEL_CONSTANT)
+    default: fail(PBB_ERR_UNSUPPORTED, "no device functor for this kernel id");
+    }
+#undef PBB_DIRECT_CASE
+    check(hipGetLastError(), "direct_partial");
+    int64_t total = a.T * a.ncols;
+    int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    direct_reduce<<<blocks, 256, 0, st>>>(a.part, a.nsplit, total, a.out);
+    check(hipGetLastError(), "direct_reduce");
 }
 
 }  // namespace dev

@@ -70,6 +70,14 @@ struct NearArgs {
     double *out;
 };
 
+struct DirectArgs {
+    int64_t T, Ns;
+    int ncols, nsplit;
+    KParams kp;
+    const double *tgt, *src, *w;   // AoS targets / sources, column-major weights
+    double *part, *out;            // part: nsplit x ncols x T;  out: ncols x T
+};
+
 void upload_dirs(const int dirs[8][189][4]);
 void launch_gather_weights(const double *w, const int32_t *perm, int64_t N, int ncols, double *ws, hipStream_t st);
 void launch_p2m(const P2MArgs &a, int nleaf, hipStream_t st);
@@ -78,6 +86,7 @@ void launch_l2l(const M2MArgs &a, int nparent, hipStream_t st);
 void launch_gemm(const GemmArgs &g, hipStream_t st);
 void launch_m2l(const M2LArgs &a, int ntiles, int ncolchunks, hipStream_t st);
 void launch_near(const NearArgs &a, int kid, int nleaf, hipStream_t st);
+void launch_direct(const DirectArgs &a, int kid, hipStream_t st);
 
 }  // namespace dev
 }  // namespace pbb
