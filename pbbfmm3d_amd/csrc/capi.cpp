@@ -480,8 +480,11 @@ static void matvec_fp64(pbb_plan *p, const double *w, int ncols, double *out, hi
         dev::launch_gemm(g, st);
     }
     mark(3);
+    // M2L tile: W columns = TB target boxes x cpt columns; 128-wide tiles
+    // halve the E traffic per flop when the columns allow it
     const int cpt = std::min(ncols, 64);
-    const int TB = std::max(1, 64 / cpt);
+    const int width = ncols >= 32 || ncols <= 16 ? 128 : 64;
+    const int TB = std::max(1, width / cpt);
     const int nchunks = (ncols + cpt - 1) / cpt;
     for (int l = 2; l <= Lv; l++) {
         const DevLevel &D = p->lv[l];

@@ -301,18 +301,30 @@ __global__ __launch_bounds__(256) void gemm_small_a_fp64(GemmArgs g)
 // ---------------------------------------------------------------------------
 // M2L, compute.hpp:401-493 + Moment2Local :880-923.  A workgroup owns TB
 // target boxes of one octant class (Morton-adjacent) and the same column
-// chunk of each: a 64-wide output tile Pf (all r_f rows) that stays in
+// chunk of each: a W-wide output tile Pf (all r_f rows) that stays in
 // registers while the 189 far directions of the class stream through:
 //   Pf[:, tile] += E_d P[:, tile + d]     (empty source boxes give zeros)
-template <int RPT>
-__global__ __launch_bounds__(256) void m2l_fp64(M2LArgs a)
+// ---------------------------------------------------------------------------
+// On the FP64 matrix cores (v_mfma_f64_16x16x4_f64): each wave owns one
+// 16-column block of the W = 16*NW column tile and all RB 16-row blocks of
+// r_f; per 4-deep k-step a wave reads one
+// B fragment and RB A fragments from LDS and issues RB independent MFMAs.
+// Fragment layout of the 16x16x4 FP64 MFMA (lane l of 64):
+//   A[i][k]: i = l % 16, k = l / 16      B[k][j]: j = l % 16, k = l / 16
+//   D[i][j]: j = l % 16, i = l / 16 + 4*v     (v = 0..3, one double each)
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+template <int RB, int NW>
+__global__ __launch_bounds__(NW * 64) void m2l_mfma_f64(M2LArgs a)
 {
-    __shared__ double As[16][RPT * 16];
-    __shared__ double Bs[16][64 + 1];
-    __shared__ int s_sid[64];
-    __shared__ int s_tgt[64];
-    __shared__ int s_xyz[64][3];
-    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+    constexpr int NT = NW * 64, W = 16 * NW, R = 16 * RB, KC = 16;
+    constexpr int AS = R + 2, BS = W + 2;  // padded LDS row strides
+    __shared__ double As[KC * AS];
+    __shared__ double Bs[KC * BS];
+    __shared__ int s_sid[W];
+    __shared__ int s_tgt[W];
+    __shared__ int s_xyz[W][3];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int rf = a.rf, rs = a.rs;
     const int tile = blockIdx.x;
     int o = 0;
@@ -333,13 +345,12 @@ __global__ __launch_bounds__(256) void m2l_fp64(M2LArgs a)
             s_xyz[tid][2] = (c >> 20) & 1023;
         }
     }
-    double acc[RPT][4];
+    dbl4 acc[RB];
 #pragma unroll
-    for (int i = 0; i < RPT; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = 0;
+    for (int i = 0; i < RB; i++) acc[i] = dbl4{0, 0, 0, 0};
     __syncthreads();
 
+    const int fr = lane % 16, fk = lane / 16;
     for (int k = 0; k < 189; k++) {
         const int dx = c_dirs[o][k][0], dy = c_dirs[o][k][1], dz = c_dirs[o][k][2];
         const int eidx = c_dirs[o][k][3];
@@ -356,14 +367,14 @@ __global__ __launch_bounds__(256) void m2l_fp64(M2LArgs a)
         }
         if (!__syncthreads_or(any)) continue;
         const double *Ed = a.E + (size_t)eidx * rf * rs;
-        for (int k0 = 0; k0 < rs; k0 += 16) {
-            for (int q = tid; q < 16 * RPT * 16; q += 256) {
-                int row = q % (RPT * 16), kk = q / (RPT * 16);
+        for (int k0 = 0; k0 < rs; k0 += KC) {
+            for (int q = tid; q < KC * R; q += NT) {
+                int row = q % R, kk = q / R;
                 int gk = k0 + kk;
-                As[kk][row] = (row < rf && gk < rs) ? Ed[row + (size_t)gk * rf] : 0.0;
+                As[kk * AS + row] = (row < rf && gk < rs) ? Ed[row + (size_t)gk * rf] : 0.0;
             }
-            for (int q = tid; q < 16 * 64; q += 256) {
-                int kk = q % 16, col = q / 16;
+            for (int q = tid; q < KC * W; q += NT) {
+                int kk = q % KC, col = q / KC;
                 int tb = col / cpt, cl = col - tb * cpt;
                 int gk = k0 + kk;
                 double v = 0.0;
@@ -371,36 +382,34 @@ __global__ __launch_bounds__(256) void m2l_fp64(M2LArgs a)
                     int sid = s_sid[tb];
                     if (sid >= 0) v = a.P[((size_t)(a.off + sid) * a.ncols + c0 + cl) * rs + gk];
                 }
-                Bs[kk][col] = v;
+                Bs[kk * BS + col] = v;
             }
             __syncthreads();
-#pragma unroll 4
-            for (int kk = 0; kk < 16; kk++) {
-                double av[RPT], bv[4];
 #pragma unroll
-                for (int i = 0; i < RPT; i++) av[i] = As[kk][tx + 16 * i];
+            for (int ks = 0; ks < KC / 4; ks++) {
+                const double b = Bs[(ks * 4 + fk) * BS + wave * 16 + fr];
 #pragma unroll
-                for (int j = 0; j < 4; j++) bv[j] = Bs[kk][ty + 16 * j];
-#pragma unroll
-                for (int i = 0; i < RPT; i++)
-#pragma unroll
-                    for (int j = 0; j < 4; j++) acc[i][j] += av[i] * bv[j];
+                for (int rb = 0; rb < RB; rb++) {
+                    const double av = As[(ks * 4 + fk) * AS + rb * 16 + fr];
+                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b, acc[rb], 0, 0, 0);
+                }
             }
             __syncthreads();
         }
     }
+    const int col = wave * 16 + fr;
+    const int tb = col / cpt, cl = col - tb * cpt;
+    if (tb < a.TB) {
+        const int t = s_tgt[tb];
+        if (t >= 0) {
+            double *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + cl) * rf;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        int col = ty + 16 * j;
-        int tb = col / cpt, cl = col - tb * cpt;
-        if (tb >= a.TB || cl >= cpt) continue;
-        int t = s_tgt[tb];
-        if (t < 0) continue;
-        double *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + cl) * rf;
+            for (int rb = 0; rb < RB; rb++)
 #pragma unroll
-        for (int i = 0; i < RPT; i++) {
-            int row = tx + 16 * i;
-            if (row < rf) dst[row] = acc[i][j];
+                for (int v = 0; v < 4; v++) {
+                    const int row = rb * 16 + fk + 4 * v;
+                    if (row < rf) dst[row] = acc[rb][v];
+                }
         }
     }
 }
@@ -630,19 +639,21 @@ void launch_m2l(const M2LArgs &a, int ntiles, int ncolchunks, hipStream_t st)
 {
     if (ntiles <= 0) return;
     dim3 grid((unsigned)ntiles, (unsigned)ncolchunks);
-    int rpt = (a.rf + 15) / 16;
-#define PBB_M2L_CASE(R)                                   \
-    case R:                                               \
-        m2l_fp64<R><<<grid, 256, 0, st>>>(a);             \
+    const int rb = (a.rf + 15) / 16;
+    const int width = a.TB * a.cpt <= 64 ? 64 : 128;  // tile width chosen by the caller
+#define PBB_M2L_CASE(R)                                                        \
+    case R:                                                                    \
+        if (width == 64) m2l_mfma_f64<R, 4><<<grid, 256, 0, st>>>(a);          \
+        else m2l_mfma_f64<R, 8><<<grid, 512, 0, st>>>(a);                      \
         break;
-    switch (rpt) {
+    switch (rb) {
         PBB_M2L_CASE(1) PBB_M2L_CASE(2) PBB_M2L_CASE(3) PBB_M2L_CASE(4) PBB_M2L_CASE(5) PBB_M2L_CASE(6)
         PBB_M2L_CASE(7) PBB_M2L_CASE(8) PBB_M2L_CASE(9) PBB_M2L_CASE(10) PBB_M2L_CASE(11)
         PBB_M2L_CASE(12) PBB_M2L_CASE(13) PBB_M2L_CASE(14) PBB_M2L_CASE(15) PBB_M2L_CASE(16)
     default: fail(PBB_ERR_UNSUPPORTED, "M2L rank above 256 is not supported");
     }
 #undef PBB_M2L_CASE
-    check(hipGetLastError(), "m2l_fp64");
+    check(hipGetLastError(), "m2l_mfma_f64");
 }
 
 void launch_near(const NearArgs &a, int kid, int nleaf, hipStream_t st)
