@@ -22,7 +22,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "targets/s at N=4M nCols=32, rel-err<=1e-6 vs FP64: mixed MFMA vs FP64 GPU"
-PEAK = {"fp64": ("mfma", 78.6, "TFLOP/s"), "mixed": ("mfma", 2500.0, "TFLOP/s")}
+PEAK = {"fp64": ("mfma", 78.6, "TFLOP/s"), "mixed": ("mfma", 2500.0, "TFLOP/s"),
+        "f16x2": ("mfma", 2500.0, "TFLOP/s"), "bf16x2": ("mfma", 2500.0, "TFLOP/s")}
+DTYPE = {"fp64": "f64", "mixed": "bf16x3->f32 (M2L), f64 elsewhere",
+         "f16x2": "f16x2->f32 (M2L), f64 elsewhere", "bf16x2": "bf16x2->f32 (M2L), f64 elsewhere"}
 
 
 def clustered_points(n, seed):
@@ -79,7 +82,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed", "f16x2", "bf16x2"],
+                    help="mixed = M2L on BF16 MFMA with 3-way split operands")
+    ap.add_argument("--fp64-steps", type=int, default=1,
+                    help="timed FP64 steps run beside a mixed run (speedup and error)")
     ap.add_argument("--kernel", default="gaussian", choices=["gaussian", "laplacian"])
     ap.add_argument("--npoints", type=int, default=4_000_000)
     ap.add_argument("--ncols", type=int, default=32)
@@ -119,31 +125,47 @@ def main():
     log(f"plan {t_plan:.2f} s, boxes {info['boxes']}, m2l pairs {info['m2l_pairs']}, "
         f"near pairs {info['near_pairs']}")
 
-    prec = capi.PREC_FP64 if args.precision == "fp64" else capi.PREC_MIXED
+    precs = {"fp64": capi.PREC_FP64, "mixed": capi.PREC_MIXED, "f16x2": capi.PREC_MIXED_F16X2,
+             "bf16x2": capi.PREC_MIXED_BF16X2}
+    prec = precs[args.precision]
     w = torch.tensor(w_host, dtype=torch.float64, device=dev)
     out = torch.zeros_like(w)
     plan.set_timing(True)
-    for i in range(args.warmup):
-        plan.matvec(w, out, precision=prec)
-        torch.cuda.synchronize()
-        log(f"warmup {i} done")
-    phase = {k: 0.0 for k in capi.PHASE_NAMES}
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        plan.matvec(w, out, precision=prec)
-        for k, v in plan.timings().items():  # waits for this step's events
-            phase[k] += v
-        log(f"step {i} done")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = N / (elapsed / args.steps)
-    phase = {k: v / args.steps for k, v in phase.items()}
 
-    # accuracy: one clean matvec against the FP64 direct sum on sampled targets
-    out.zero_()
-    plan.matvec(w, out, precision=prec)
+    def timed(p, warmup, steps, tag):
+        for i in range(warmup):
+            plan.matvec(w, out, precision=p)
+            torch.cuda.synchronize()
+            log(f"{tag} warmup {i} done")
+        phase = {k: 0.0 for k in capi.PHASE_NAMES}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            plan.matvec(w, out, precision=p)
+            for k, v in plan.timings().items():  # waits for this step's events
+                phase[k] += v
+            log(f"{tag} step {i} done")
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        return elapsed / steps, {k: v / steps for k, v in phase.items()}
+
+    def one(p):
+        o = torch.zeros_like(w)
+        plan.matvec(w, o, precision=p)
+        torch.cuda.synchronize()
+        return o
+
+    fp64 = None
+    if prec != capi.PREC_FP64 and args.fp64_steps > 0:
+        t64, ph64 = timed(capi.PREC_FP64, 1, args.fp64_steps, "fp64")
+        out64 = one(capi.PREC_FP64)
+        fp64 = {"value": N / t64, "ms_per_step": 1e3 * t64, "phase_ms": {k: round(v, 3) for k, v in ph64.items()}}
+    t_step, phase = timed(prec, args.warmup, args.steps, args.precision)
+    ms_per_step = 1e3 * t_step
+    value = N / t_step
+    out = one(prec)
+
+    # accuracy against the FP64 direct sum on sampled targets
     ts = np.sort(np.random.default_rng(2).choice(N, size=min(args.err_targets, N), replace=False))
     tgt = torch.tensor(xyz[ts], dtype=torch.float64, device=dev)
     src = torch.tensor(xyz, dtype=torch.float64, device=dev)
@@ -153,9 +175,20 @@ def main():
     capi.check(capi.lib().pbb_direct_sum(C.byref(d), tgt.data_ptr(), len(ts), src.data_ptr(), N,
                                          w.data_ptr(), nc, ref.data_ptr(), None))
     torch.cuda.synchronize()
-    got = out[:, torch.tensor(ts, device=dev)]
-    rel_err = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
+    tsd = torch.tensor(ts, device=dev)
+
+    def err(o):
+        return float(torch.linalg.norm(o[:, tsd] - ref) / torch.linalg.norm(ref))
+
+    rel_err = err(out)
     log(f"rel err vs FP64 direct sum on {len(ts)} targets: {rel_err:.3e}")
+    if fp64 is not None:
+        fp64["rel_err_vs_direct"] = err(out64)
+        fp64["speedup"] = value / fp64["value"]
+        fp64["rel_diff_vs_fp64_path"] = float(torch.linalg.norm(out - out64) / torch.linalg.norm(out64))
+        log(f"fp64 {fp64['value']:.0f} targets/s, speedup {fp64['speedup']:.2f}, "
+            f"rel diff vs FP64 path {fp64['rel_diff_vs_fp64_path']:.3e}")
+        del out64
 
     # roofline of the dominant kernel (M2L): algorithmic flops 2*rf*rs*ncols
     # per far-field box pair, over the M2L phase time (HIP events on the
@@ -178,7 +211,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "targets/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "bf16x3/f32/f64",
+        "dtype": DTYPE[args.precision],
         "data": "synthetic",
         "config": {"workload": f"{N} clustered points (8 Gaussians sigma 0.05 + 20% uniform), "
                                f"{kernel_name(args)}, {nc} columns, level {args.level}, order "
@@ -188,6 +221,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "rel_err_vs_direct": rel_err,
+        "fp64_path": fp64,
         "phase_ms": {k: round(v, 3) for k, v in phase.items()},
         "ranks": [ops.rank_f, ops.rank_s],
         "m2l_pairs": info["m2l_pairs"], "near_pairs": info["near_pairs"],

@@ -56,7 +56,10 @@ typedef enum pbb_kernel_id {
     PBB_KERNEL_THIN_PLATE = 9,     /* 0.5 r^2 log r^2  :165-178             */
     PBB_KERNEL_R2 = 10,            /* r^2 (degree-2 homogeneous test kernel)*/
     PBB_KERNEL_CONSTANT = 11,      /* 1 (test kernel)                       */
-    PBB_KERNEL_POLY_XY = 12        /* (tx-sx)(ty+sy)+p0 (non-symmetric test)*/
+    PBB_KERNEL_POLY_XY = 12,       /* (tx-sx)(ty+sy)+p0 (non-symmetric test,
+                                      host only: no device functor)        */
+    PBB_KERNEL_EXP_ANISO = 13      /* p0*exp(-sqrt(p1 dx^2 + p2 dy^2 + p3 dz^2))
+                                      (python/mykernel_python.hpp:10-17)   */
 } pbb_kernel_id;
 
 /* What H2_3D_Tree::SetKernelProperty sets (H2_3D_Tree.hpp:96,
@@ -94,9 +97,13 @@ typedef struct pbb_ops_info {
 
 /* Precision modes of pbb_plan_matvec.                                    */
 typedef enum pbb_precision {
-    PBB_PREC_FP64 = 0,    /* every phase in FP64 (the drop-in path)       */
-    PBB_PREC_MIXED = 1    /* FP32 expansions + split-BF16 MFMA M2L, FP32
-                             P2P with compensated sums, FP64 output sum   */
+    PBB_PREC_FP64 = 0,          /* every phase in FP64 (the drop-in path)  */
+    PBB_PREC_MIXED = 1,         /* M2L on the BF16 matrix cores, operands
+                                   split in three BF16 parts (6 products),
+                                   FP32 accumulation                        */
+    PBB_PREC_MIXED_F16X2 = 2,   /* M2L with operands split in two scaled
+                                   FP16 parts (3 products)                  */
+    PBB_PREC_MIXED_BF16X2 = 3   /* M2L with two BF16 parts (3 products)     */
 } pbb_precision;
 
 #define PBB_NUM_PHASES 8
@@ -170,11 +177,22 @@ PBB_API pbb_status pbb_plan_query(const pbb_plan *plan, pbb_plan_info *info);
 /* Replaces FMMCompute (compute.hpp:192-222): upward pass, M2L, downward
  * pass, near field.  w_dev: device FP64 weights, column-major N x ncols;
  * out_dev: device FP64, column-major N x ncols, accumulated (+=).
- * stream: a hipStream_t (NULL = the plan's own stream).  Asynchronous with
- * respect to the host; the call returns once all work is enqueued.        */
+ * stream: a hipStream_t (NULL = the null stream), on which all kernels run
+ * in order.  Asynchronous with respect to the host; the call returns once
+ * all work is enqueued.                                                   */
 PBB_API pbb_status pbb_plan_matvec(pbb_plan *plan, const double *w_dev,
                                    int32_t ncols, double *out_dev,
                                    int32_t precision, void *stream);
+
+/* The adjoint: out += A~^T w for the same tree, where A~ is the matrix the
+ * FMM applies.  Runs the transposed FMM (P2M <-> L2P and M2M <-> L2L swap
+ * roles, the M2L uses V^T/U swapped and E_{-d}^T) with the same kernels;
+ * this is the gradient of a loss with respect to the weights.  No reference
+ * counterpart (the reference has no adjoint); used by the autograd
+ * function.  Requires a symmetric kernel for the near field.             */
+PBB_API pbb_status pbb_plan_matvec_transpose(pbb_plan *plan, const double *w_dev,
+                                             int32_t ncols, double *out_dev,
+                                             int32_t precision, void *stream);
 
 /* Phase times of the last pbb_plan_matvec on this plan, in ms
  * (PBB_NUM_PHASES values).  Synchronises the plan's events.               */

@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(_HERE, "libpbb_oracle.so")
 # kernel ids shared with include/pbbfmm3d_amd.h
 LAPLACIAN, ONEOVERR2, GAUSSIAN, EXPONENTIAL, SOFT_LAPLACIAN = 1, 2, 3, 4, 5
 QUADRIC, INV_QUADRIC, LOGARITHM, THIN_PLATE, R2, CONSTANT, POLY_XY = 6, 7, 8, 9, 10, 11, 12
+EXP_ANISO = 13
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

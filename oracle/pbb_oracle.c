@@ -57,7 +57,8 @@ enum {
     PBBO_K_THIN_PLATE = 9,     /* 0.5 r^2 log r^2, 0 at r = 0            */
     PBBO_K_R2 = 10,            /* r^2 (homogeneous of degree 2)          */
     PBBO_K_CONSTANT = 11,      /* 1                                      */
-    PBBO_K_POLY_XY = 12        /* (tx - sx) * (ty + sy) + p0 (not symm.) */
+    PBBO_K_POLY_XY = 12,       /* (tx - sx) * (ty + sy) + p0 (not symm.) */
+    PBBO_K_EXP_ANISO = 13      /* p0 exp(-sqrt(p1 dx^2 + p2 dy^2 + p3 dz^2)) */
 };
 
 double pbbo_kernel_eval(int id, const double *p, double tx, double ty, double tz,
@@ -78,6 +79,7 @@ double pbbo_kernel_eval(int id, const double *p, double tx, double ty, double tz
     case PBBO_K_R2: return r2;
     case PBBO_K_CONSTANT: return 1.0;
     case PBBO_K_POLY_XY: return (tx - sx) * (ty + sy) + p[0];
+    case PBBO_K_EXP_ANISO: return p[0] * exp(-sqrt(p[1] * dx * dx + p[2] * dy * dy + p[3] * dz * dz));
     default: return NAN;
     }
 }

@@ -36,9 +36,12 @@ KERNEL_THIN_PLATE = 9
 KERNEL_R2 = 10
 KERNEL_CONSTANT = 11
 KERNEL_POLY_XY = 12
+KERNEL_EXP_ANISO = 13
 
 PREC_FP64 = 0
-PREC_MIXED = 1
+PREC_MIXED = 1          # BF16 x3 split M2L
+PREC_MIXED_F16X2 = 2   # FP16 x2 split M2L
+PREC_MIXED_BF16X2 = 3  # BF16 x2 split M2L
 
 NUM_PHASES = 8
 PHASE_NAMES = ("p2m", "m2m", "vt_proxy", "m2l", "u_expand", "l2l", "near", "total")
@@ -85,6 +88,7 @@ _SIGNATURES = {
     "pbb_plan_create": (C.c_int, [_vp, _dp, C.c_int64, C.POINTER(_vp)]),
     "pbb_plan_query": (C.c_int, [_vp, C.POINTER(PlanInfo)]),
     "pbb_plan_matvec": (C.c_int, [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp]),
+    "pbb_plan_matvec_transpose": (C.c_int, [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp]),
     "pbb_plan_timings": (C.c_int, [_vp, _dp]),
     "pbb_plan_set_timing": (C.c_int, [_vp, C.c_int32]),
     "pbb_plan_destroy": (None, [_vp]),

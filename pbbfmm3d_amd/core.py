@@ -53,6 +53,8 @@ class Kernel:
             if i == capi.KERNEL_THIN_PLATE: return 0.0 if r2 == 0 else 0.5 * r2 * np.log(r2)
             if i == capi.KERNEL_R2: return r2
             if i == capi.KERNEL_CONSTANT: return 1.0
+            if i == capi.KERNEL_EXP_ANISO:
+                return p[0] * float(np.exp(-np.sqrt(p[1] * d[0] ** 2 + p[2] * d[1] ** 2 + p[3] * d[2] ** 2)))
         raise ValueError(f"unknown kernel id {i}")
 
 
@@ -216,7 +218,7 @@ class Plan:
         """Enqueues out += A w on raw device pointers (FP64, column-major)."""
         capi.check(capi.lib().pbb_plan_matvec(self._h, C.c_void_p(w_ptr), int(ncols),
                                               C.c_void_p(out_ptr), int(precision),
-                                              C.c_void_p(stream) if stream else None))
+                                              C.c_void_p(stream)))
 
     def matvec(self, w, out=None, precision: int = capi.PREC_FP64):
         """torch CUDA tensors: w (ncols, N) or (N,) float64, contiguous;
@@ -234,6 +236,28 @@ class Plan:
             raise ValueError("out must be a contiguous float64 tensor shaped like w")
         stream = torch.cuda.current_stream(w.device).cuda_stream
         self.matvec_ptr(w.data_ptr(), ncols, out.data_ptr(), precision, stream)
+        return out
+
+    def matvec_transpose_ptr(self, w_ptr: int, ncols: int, out_ptr: int,
+                             precision: int = capi.PREC_FP64, stream: int = 0) -> None:
+        """Enqueues out += A~^T w (the adjoint FMM) on raw device pointers."""
+        capi.check(capi.lib().pbb_plan_matvec_transpose(
+            self._h, C.c_void_p(w_ptr), int(ncols), C.c_void_p(out_ptr), int(precision),
+            C.c_void_p(stream)))
+
+    def matvec_transpose(self, w, out=None, precision: int = capi.PREC_FP64):
+        """Like matvec, with the transposed FMM operator."""
+        import torch
+        if w.dtype != torch.float64 or not w.is_cuda:
+            raise TypeError("w must be a float64 CUDA tensor")
+        w = w.contiguous()
+        ncols = 1 if w.dim() == 1 else w.shape[0]
+        if w.numel() != ncols * self.npoints:
+            raise ValueError("w must hold ncols * npoints values")
+        if out is None:
+            out = torch.zeros_like(w)
+        stream = torch.cuda.current_stream(w.device).cuda_stream
+        self.matvec_transpose_ptr(w.data_ptr(), ncols, out.data_ptr(), precision, stream)
         return out
 
     def set_timing(self, enable: bool = True):

@@ -95,11 +95,47 @@ struct DeviceGuard {
 
 }  // namespace
 
+struct SplitE {
+    void *d = nullptr;   // [set][316][nch][nsplit][R][32] 16-bit parts
+    int R = 0, nch = 0;
+    std::vector<int> escale;  // per set
+};
+
 struct pbb_ops {
     HostOps h;
     std::mutex mu;
     int device = -1;
+    std::unique_ptr<pbb_ops> adj;  // transposed operator set, built on demand
     double *d_Tkz = nullptr, *d_Cw = nullptr, *d_VTw = nullptr, *d_Uw = nullptr, *d_E = nullptr;
+    SplitE split[4];
+
+    // E_d split into 16-bit parts for a mixed-precision variant (built once)
+    const SplitE &ensure_split(int variant, hipStream_t st)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        SplitE &S = split[variant];
+        if (S.d) return S;
+        const int nsplit = variant == PBB_PREC_MIXED ? 3 : 2;
+        S.R = 16 * ((h.rf + 15) / 16);
+        S.nch = (h.rs + 31) / 32;
+        const size_t per_set = (size_t)316 * S.nch * nsplit * S.R * 32;
+        S.d = dalloc<uint16_t>(per_set * h.nsets, "split E");
+        S.escale.assign(h.nsets, 0);
+        for (int s = 0; s < h.nsets; s++) {
+            if (variant == PBB_PREC_MIXED_F16X2) {
+                double mx = 0;
+                const size_t n = (size_t)316 * h.rf * h.rs;
+                for (size_t i = 0; i < n; i++) mx = std::max(mx, std::fabs(h.E[(size_t)s * n + i]));
+                int e = 0;
+                if (mx > 0) std::frexp(mx, &e);
+                S.escale[s] = mx > 0 ? 12 - e : 0;
+            }
+            dev::launch_split_e(variant, d_E + (size_t)s * 316 * h.rf * h.rs, h.rf, h.rs, S.R, S.nch, S.escale[s],
+                                (uint16_t *)S.d + (size_t)s * per_set, st);
+        }
+        check(hipStreamSynchronize(st), "split E");
+        return S;
+    }
 
     void upload(int dev)
     {
@@ -126,6 +162,42 @@ struct pbb_ops {
         d_E = dupload(h.E, "E");
         device = dev;
     }
+    // The operators of the transposed FMM (P2M <-> L2P, M2M <-> L2L): U' = VT^T,
+    // VT' = U^T, E'_d = E_{-d}^T with -d at index 315 - d (Ktable order is
+    // symmetric under negation).  Same weights, kernel and level scales.
+    pbb_ops *adjoint()
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (adj) return adj.get();
+        auto a = std::make_unique<pbb_ops>();
+        HostOps &g = a->h;
+        g = h;
+        g.rf = h.rs;
+        g.rs = h.rf;
+        g.sigma_f = h.sigma_s;
+        g.sigma_s = h.sigma_f;
+        const int n3 = h.n3, rf = h.rf, rs = h.rs;
+        g.U.assign((size_t)h.nsets * n3 * rs, 0.0);
+        g.VT.assign((size_t)h.nsets * rf * n3, 0.0);
+        g.E.assign(h.E.size(), 0.0);
+        for (int s = 0; s < h.nsets; s++) {
+            const double *U = &h.U[(size_t)s * n3 * rf], *VT = &h.VT[(size_t)s * rs * n3];
+            double *U2 = &g.U[(size_t)s * n3 * rs], *VT2 = &g.VT[(size_t)s * rf * n3];
+            for (int q = 0; q < n3; q++) {
+                for (int b = 0; b < rs; b++) U2[q + (size_t)b * n3] = VT[b + (size_t)q * rs];
+                for (int a2 = 0; a2 < rf; a2++) VT2[a2 + (size_t)q * rf] = U[q + (size_t)a2 * n3];
+            }
+            for (int d = 0; d < 316; d++) {
+                const double *Ed = &h.E[((size_t)s * 316 + (315 - d)) * rf * rs];
+                double *E2 = &g.E[((size_t)s * 316 + d) * rf * rs];
+                for (int a2 = 0; a2 < rf; a2++)
+                    for (int b = 0; b < rs; b++) E2[b + (size_t)a2 * rs] = Ed[a2 + (size_t)b * rf];
+            }
+        }
+        adj = std::move(a);
+        return adj.get();
+    }
+
     ~pbb_ops()
     {
         if (device >= 0) {
@@ -133,6 +205,7 @@ struct pbb_ops {
             (void)hipGetDevice(&prev);
             (void)hipSetDevice(device);
             dfree(d_Tkz); dfree(d_Cw); dfree(d_VTw); dfree(d_Uw); dfree(d_E);
+            for (auto &x : split) dfree(x.d);
             if (prev >= 0) (void)hipSetDevice(prev);
         }
     }
@@ -157,8 +230,11 @@ struct pbb_plan {
     int32_t *perm = nullptr, *leaf_start = nullptr;
     std::vector<DevLevel> lv;
     int64_t Btot = 0;
-    int ws_ncols = 0;
+    int ws_ncols = 0, ws_mixed = -1;
     double *ws = nullptr, *S = nullptr, *P = nullptr, *Pf = nullptr;
+    float *P32 = nullptr, *Pf32 = nullptr;
+    unsigned *xmax = nullptr;
+    int kpad = 0;
     bool timing = false;
     hipEvent_t ev[PBB_NUM_PHASES] = {};
     bool events_made = false;
@@ -175,27 +251,50 @@ struct pbb_plan {
         for (auto &L : lv) {
             dfree(L.coord); dfree(L.map); dfree(L.child); dfree(L.order); dfree(L.ctab);
         }
-        dfree(ws); dfree(S); dfree(P); dfree(Pf);
+        dfree(ws); dfree(S); dfree(P); dfree(Pf); dfree(P32); dfree(Pf32); dfree(xmax);
         if (events_made)
             for (auto &e : ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
         if (prev >= 0) (void)hipSetDevice(prev);
     }
 
-    void ensure_workspace(int ncols)
+    // Device buffers of one matvec: FP64 mode keeps P / Pf in FP64, the
+    // mixed modes in FP32; the other mode's buffers are released on a switch
+    // so that at most one set is resident.
+    void ensure_workspace(int ncols, bool mixed)
     {
-        if (ncols <= ws_ncols) return;
-        dfree(ws); dfree(S); dfree(P); dfree(Pf);
-        ws = S = P = Pf = nullptr;
-        ws_ncols = 0;
         const HostOps &h = ops->h;
-        ws = dalloc<double>((size_t)N * ncols, "sorted weights");
-        if (Lv >= 2) {
-            S = dalloc<double>((size_t)Btot * ncols * h.n3, "expansions");
-            P = dalloc<double>((size_t)Btot * ncols * h.rs, "proxies");
-            Pf = dalloc<double>((size_t)Btot * ncols * h.rf, "M2L results");
+        if (ncols > ws_ncols) {
+            dfree(ws); dfree(S); dfree(P); dfree(Pf); dfree(P32); dfree(Pf32);
+            ws = S = P = Pf = nullptr;
+            P32 = Pf32 = nullptr;
+            ws_ncols = 0;
+            ws_mixed = -1;
+            ws = dalloc<double>((size_t)N * ncols, "sorted weights");
+            if (Lv >= 2) S = dalloc<double>((size_t)Btot * ncols * h.n3, "expansions");
+            ws_ncols = ncols;
         }
-        ws_ncols = ncols;
+        if (ws_mixed == (int)mixed || Lv < 2) {
+            ws_mixed = (int)mixed;
+            return;
+        }
+        const int nc = ws_ncols;
+        if (mixed) {
+            dfree(P); dfree(Pf);
+            P = Pf = nullptr;
+            const int rmax = std::max(h.rf, h.rs);
+            kpad = 32 * ((rmax + 31) / 32);
+            P32 = dalloc<float>((size_t)Btot * nc * kpad, "FP32 proxies");
+            Pf32 = dalloc<float>((size_t)Btot * nc * rmax, "FP32 M2L results");
+            if (!xmax) xmax = dalloc<unsigned>(Lv + 1, "level maxima");
+        } else {
+            dfree(P32); dfree(Pf32);
+            P32 = Pf32 = nullptr;
+            const int rmax = std::max(h.rf, h.rs);
+            P = dalloc<double>((size_t)Btot * nc * rmax, "proxies");
+            Pf = dalloc<double>((size_t)Btot * nc * rmax, "M2L results");
+        }
+        ws_mixed = (int)mixed;
     }
 };
 
@@ -228,7 +327,7 @@ static void validate_params(const pbb_kernel_desc *kernel, double L, int tree_le
     if (order < 2 || order > 12) fail(PBB_ERR_INVALID_ARG, "interpolation order must be in [2, 12]");
     if (!(eps >= 0) || !(eps < 1)) fail(PBB_ERR_INVALID_ARG, "eps must be in [0, 1)");
     if (kernel->symmetry < -1 || kernel->symmetry > 1) fail(PBB_ERR_INVALID_ARG, "symmetry must be -1, 0 or 1");
-    if (kernel->id != PBB_KERNEL_NONE && kernel->id != PBB_KERNEL_POLY_XY && !kernel_id_valid(kernel->id))
+    if (kernel->id != PBB_KERNEL_NONE && !kernel_id_valid(kernel->id))
         fail(PBB_ERR_INVALID_ARG, "unknown kernel id");
 }
 
@@ -385,6 +484,7 @@ PBB_API pbb_status pbb_plan_create(pbb_ops *ops, const double *xyz, int64_t npoi
         int dirs[8][189][4];
         for (int o = 0; o < 8; o++) class_directions(o, h.w.Ktable, dirs[o]);
         dev::upload_dirs(dirs);
+        dev::upload_dirs_mixed(dirs);
 
         pbb_plan_info &I = plan->info;
         std::memset(&I, 0, sizeof(I));
@@ -440,15 +540,18 @@ PBB_API pbb_status pbb_plan_timings(pbb_plan *plan, double *ms)
     });
 }
 
-static void matvec_fp64(pbb_plan *p, const double *w, int ncols, double *out, hipStream_t st)
+static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, double *out, hipStream_t st, int prec)
 {
-    const HostOps &h = p->ops->h;
-    pbb_ops *o = p->ops;
+    const bool mixed = prec != PBB_PREC_FP64;
+    const HostOps &h = o->h;
     const int n = h.n, n3 = h.n3, Lv = p->Lv;
     auto mark = [&](int k) {
         if (p->timing) check(hipEventRecord(p->ev[k], st), "hipEventRecord");
     };
-    p->ensure_workspace(ncols);
+    p->ensure_workspace(ncols, mixed);
+    const SplitE *SE = mixed && Lv >= 2 ? &o->ensure_split(prec, st) : nullptr;
+    const int ldP = mixed ? p->kpad : h.rs;
+    if (mixed && h.rs > p->kpad) fail(PBB_ERR_INTERNAL, "proxy padding too small");
     mark(0);
     dev::launch_gather_weights(w, p->perm, p->N, ncols, p->ws, st);
     const int cc_p2m = std::max(1, std::min({ncols, 64, 4096 / n3}));
@@ -473,11 +576,20 @@ static void matvec_fp64(pbb_plan *p, const double *w, int ncols, double *out, hi
     for (int l = 2; l <= Lv; l++) {
         const DevLevel &D = p->lv[l];
         dev::GemmArgs g{};
-        g.M = h.rs; g.K = n3; g.ncol = D.nb * ncols; g.alpha = 1.0;
+        g.M = h.rs; g.K = n3; g.ldb = n3; g.ldc = ldP; g.ncol = D.nb * ncols; g.alpha = 1.0;
         g.A = o->d_VTw + (size_t)h.ops_set(l) * h.rs * n3;
         g.B = p->S + (size_t)D.off * ncols * n3;
-        g.C = p->P + (size_t)D.off * ncols * h.rs;
+        if (mixed) {
+            g.c_f32 = 1;
+            g.C = p->P32 + (size_t)D.off * ncols * ldP;
+        } else {
+            g.C = p->P + (size_t)D.off * ncols * h.rs;
+        }
         dev::launch_gemm(g, st);
+        if (prec == PBB_PREC_MIXED_F16X2) {
+            dev::launch_zero_u32(p->xmax + l, 1, st);
+            dev::launch_absmax(p->P32 + (size_t)D.off * ncols * ldP, D.nb * ncols * ldP, p->xmax + l, st);
+        }
     }
     mark(3);
     // M2L tile: W columns = TB target boxes x cpt columns; 128-wide tiles
@@ -486,7 +598,25 @@ static void matvec_fp64(pbb_plan *p, const double *w, int ncols, double *out, hi
     const int width = ncols >= 32 || ncols <= 16 ? 128 : 64;
     const int TB = std::max(1, width / cpt);
     const int nchunks = (ncols + cpt - 1) / cpt;
-    for (int l = 2; l <= Lv; l++) {
+    for (int l = 2; l <= Lv && mixed; l++) {
+        const DevLevel &D = p->lv[l];
+        dev::M2LMixArgs a{};
+        a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = std::max(1, 128 / cpt); a.cpt = cpt; a.side = 1 << l;
+        a.kpad = ldP; a.nch = SE->nch; a.escale = SE->escale[h.ops_set(l)]; a.off = D.off;
+        a.tile_prefix[0] = 0;
+        a.class_start[0] = 0;
+        for (int c = 0; c < 8; c++) {
+            a.class_start[c + 1] = a.class_start[c] + D.class_count[c];
+            a.tile_prefix[c + 1] = a.tile_prefix[c] + (int)((D.class_count[c] + a.TB - 1) / a.TB);
+        }
+        a.order = D.order; a.coord = D.coord; a.map = D.map;
+        const int nsplit = prec == PBB_PREC_MIXED ? 3 : 2;
+        a.Es = (const uint16_t *)SE->d + (size_t)h.ops_set(l) * 316 * SE->nch * nsplit * SE->R * 32;
+        a.P = p->P32; a.Pf = p->Pf32;
+        a.xmax = prec == PBB_PREC_MIXED_F16X2 ? p->xmax + l : nullptr;
+        dev::launch_m2l_mixed(prec, a, a.tile_prefix[8], nchunks, st);
+    }
+    for (int l = 2; l <= Lv && !mixed; l++) {
         const DevLevel &D = p->lv[l];
         dev::M2LArgs a{};
         a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = TB; a.cpt = cpt; a.side = 1 << l; a.off = D.off;
@@ -506,10 +636,15 @@ static void matvec_fp64(pbb_plan *p, const double *w, int ncols, double *out, hi
     for (int l = 2; l <= Lv; l++) {
         const DevLevel &D = p->lv[l];
         dev::GemmArgs g{};
-        g.M = n3; g.K = h.rf; g.ncol = D.nb * ncols;
+        g.M = n3; g.K = h.rf; g.ldb = h.rf; g.ldc = n3; g.ncol = D.nb * ncols;
         g.alpha = h.homog ? std::pow(1.0 / D.len, h.homogen_flipped) : 1.0;  // compute.hpp:433-435
         g.A = o->d_Uw + (size_t)h.ops_set(l) * n3 * h.rf;
-        g.B = p->Pf + (size_t)D.off * ncols * h.rf;
+        if (mixed) {
+            g.b_f32 = 1;
+            g.B = p->Pf32 + (size_t)D.off * ncols * h.rf;
+        } else {
+            g.B = p->Pf + (size_t)D.off * ncols * h.rf;
+        }
         g.C = p->S + (size_t)D.off * ncols * n3;
         dev::launch_gemm(g, st);
     }
@@ -544,10 +679,31 @@ PBB_API pbb_status pbb_plan_matvec(pbb_plan *plan, const double *w_dev, int32_t 
         if (!plan || !w_dev || !out_dev) fail(PBB_ERR_INVALID_ARG, "NULL argument");
         if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
         if ((int64_t)ncols * plan->N >= (1LL << 40)) fail(PBB_ERR_INVALID_ARG, "ncols * npoints too large");
-        if (precision != PBB_PREC_FP64) fail(PBB_ERR_UNSUPPORTED, "precision mode not available");
+        if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
+            fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
+        if (precision != PBB_PREC_FP64 && plan->ops->h.rf > 224)
+            fail(PBB_ERR_UNSUPPORTED, "mixed-precision M2L supports ranks up to 224");
         DeviceGuard g(plan->device);
-        hipStream_t st = stream ? (hipStream_t)stream : plan->stream;
-        matvec_fp64(plan, w_dev, ncols, out_dev, st);
+        matvec_run(plan, plan->ops, w_dev, ncols, out_dev, (hipStream_t)stream, precision);
+    });
+}
+
+PBB_API pbb_status pbb_plan_matvec_transpose(pbb_plan *plan, const double *w_dev, int32_t ncols, double *out_dev,
+                                             int32_t precision, void *stream)
+{
+    return guarded([&] {
+        if (!plan || !w_dev || !out_dev) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
+        if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
+            fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
+        if (plan->ops->h.kernel.symmetry != 1)
+            fail(PBB_ERR_UNSUPPORTED, "the transposed near field needs a symmetric kernel");
+        pbb_ops *adj = plan->ops->adjoint();
+        if (precision != PBB_PREC_FP64 && adj->h.rf > 224)
+            fail(PBB_ERR_UNSUPPORTED, "mixed-precision M2L supports ranks up to 224");
+        DeviceGuard g(plan->device);
+        adj->upload(plan->device);
+        matvec_run(plan, adj, w_dev, ncols, out_dev, (hipStream_t)stream, precision);
     });
 }
 
@@ -610,7 +766,7 @@ PBB_API pbb_status pbb_matvec(pbb_ops *ops, const double *target, const double *
         dout = dalloc<double>((size_t)npoints * ncols, "output");
         check(hipMemcpyAsync(dw, weight, bytes, hipMemcpyHostToDevice, plan->stream), "copy weights");
         check(hipMemsetAsync(dout, 0, bytes, plan->stream), "clear output");
-        matvec_fp64(plan, dw, ncols, dout, plan->stream);
+        matvec_run(plan, plan->ops, dw, ncols, dout, plan->stream, PBB_PREC_FP64);
         std::vector<double> res((size_t)npoints * ncols);
         check(hipMemcpyAsync(res.data(), dout, bytes, hipMemcpyDeviceToHost, plan->stream), "copy output");
         check(hipStreamSynchronize(plan->stream), "hipStreamSynchronize");

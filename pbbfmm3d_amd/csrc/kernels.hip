@@ -41,6 +41,8 @@ __device__ __forceinline__ double keval(double dx, double dy, double dz, const K
     else if constexpr (KID == PBB_KERNEL_THIN_PLATE) return r2 == 0 ? 0.0 : 0.5 * r2 * log(r2);
     else if constexpr (KID == PBB_KERNEL_R2) return r2;
     else if constexpr (KID == PBB_KERNEL_CONSTANT) return 1.0;
+    else if constexpr (KID == PBB_KERNEL_EXP_ANISO)
+        return p.p[0] * exp(-sqrt(p.p[1] * dx * dx + p.p[2] * dy * dy + p.p[3] * dz * dz));
     else return 0.0;
 }
 
@@ -246,15 +248,19 @@ __global__ __launch_bounds__(256) void l2l_fp64(M2MArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// C[:, j] = alpha * A B[:, j] for j < ncol: A (M x K, lda M), B (K x ncol,
-// ldb K), C (M x ncol, ldc M), column-major FP64.  64 x 64 output tiles,
-// K in chunks of 16 through LDS, 4 x 4 outputs per thread.  Serves the
-// proxy projection P = VTw S (compute.hpp:376-395) and the local expansion
-// F = scale Uw Pf (compute.hpp:497-508).
-__global__ __launch_bounds__(256) void gemm_small_a_fp64(GemmArgs g)
+// C[:, j] = alpha * A B[:, j] for j < ncol: A (M x K, lda M, FP64), B
+// (K x ncol, ldb >= K), C (ldc >= M rows per column; rows M..ldc-1 are
+// written as zeros), column-major.  64 x 64 output tiles, K in chunks of 16
+// through LDS, 4 x 4 outputs per thread, FP64 arithmetic; B and C may be
+// FP32.  Serves the proxy projection P = VTw S (compute.hpp:376-395) and
+// the local expansion F = scale Uw Pf (compute.hpp:497-508).
+template <typename TB, typename TC>
+__global__ __launch_bounds__(256) void gemm_small_a(GemmArgs g)
 {
     __shared__ double As[16][64 + 1];
     __shared__ double Bs[16][64 + 1];
+    const TB *B = (const TB *)g.B;
+    TC *C = (TC *)g.C;
     const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
     const int64_t n0 = (int64_t)blockIdx.x * 64;
     const int m0 = blockIdx.y * 64;
@@ -269,7 +275,7 @@ __global__ __launch_bounds__(256) void gemm_small_a_fp64(GemmArgs g)
             int k = q % 16, col = q / 16;
             int gk = k0 + k;
             int64_t gc = n0 + col;
-            Bs[k][col] = (gk < g.K && gc < g.ncol) ? g.B[gk + gc * g.K] : 0.0;
+            Bs[k][col] = (gk < g.K && gc < g.ncol) ? (double)B[gk + gc * g.ldb] : 0.0;
         }
         __syncthreads();
 #pragma unroll
@@ -293,7 +299,8 @@ __global__ __launch_bounds__(256) void gemm_small_a_fp64(GemmArgs g)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             int gr = m0 + tx + 16 * i;
-            if (gr < g.M) g.C[gr + gc * g.M] = g.alpha * acc[i][j];
+            if (gr < g.M) C[gr + gc * g.ldc] = (TC)(g.alpha * acc[i][j]);
+            else if (gr < g.ldc) C[gr + gc * g.ldc] = (TC)0;
         }
     }
 }
@@ -415,23 +422,28 @@ __global__ __launch_bounds__(NW * 64) void m2l_mfma_f64(M2LArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Near field, compute.hpp:590-751, as a target-owner kernel: one workgroup
+// Near field, compute.hpp:590-751, as a target-owner kernel: one wavefront
 // per non-empty leaf computes, for each of its targets, the L2P of the
 // leaf's local expansion plus the direct sum over the points of the 27
 // neighbour leaves (self included, isinf -> 0), and adds both into the
 // caller's output through perm.  No partner writes, no atomics.
-constexpr int NF_TGT = 64;  // targets per pass
-constexpr int NF_SRC = 64;  // source points per LDS chunk
-constexpr int NF_CPT = 8;   // columns per thread (4 column phases)
+// The neighbours' point ranges are concatenated into one index space and
+// streamed in 64-point chunks; each chunk's kernel values K(t_i, s_j) are
+// evaluated once into LDS (targets in passes of up to 16), then lane
+// (i, g) accumulates columns g, g+G, ... of target i (G = 64 / TT).
+constexpr int NV_SRC = 64;  // source points per chunk
+constexpr int NV_TGT = 16;  // targets per pass
+constexpr int NV_COL = 32;  // columns per pass
 
 template <int KID>
-__global__ __launch_bounds__(256) void near_fp64(NearArgs a)
+__global__ __launch_bounds__(64) void near_fp64(NearArgs a)
 {
-    __shared__ double sx[NF_SRC], sy[NF_SRC], sz[NF_SRC];
-    __shared__ double sw[4 * NF_CPT][NF_SRC];
-    __shared__ int s_nbr[27];
-    const int b = blockIdx.x, tid = threadIdx.x;
-    const int slot = tid / 4, g = tid % 4;
+    __shared__ double sK[NV_TGT][NV_SRC + 1];
+    __shared__ double sw[NV_COL][NV_SRC + 1];
+    __shared__ double sx[NV_SRC], sy[NV_SRC], sz[NV_SRC];
+    __shared__ double tx[NV_TGT], ty[NV_TGT], tz[NV_TGT];
+    __shared__ int s_start[27], s_pre[28];
+    const int b = blockIdx.x, lane = threadIdx.x;
     const int n = a.n, n2 = n * n, n3 = n2 * n;
     const int t0 = a.leaf_start[b], nt = a.leaf_start[b + 1] - t0;
     const uint32_t cc = (uint32_t)a.coord[b];
@@ -439,78 +451,101 @@ __global__ __launch_bounds__(256) void near_fp64(NearArgs a)
     const int S = a.side;
     const double cen[3] = {a.ctab[ix], a.ctab[iy], a.ctab[iz]};
     const double ihalfL = 2.0 / a.len;
-    if (tid < 27) {
-        int x = ix + tid / 9 - 1, y = iy + (tid / 3) % 3 - 1, z = iz + tid % 3 - 1;
-        s_nbr[tid] = (x >= 0 && y >= 0 && z >= 0 && x < S && y < S && z < S) ? a.map[((int64_t)x * S + y) * S + z] : -1;
+    if (lane < 27) {
+        int x = ix + lane / 9 - 1, y = iy + (lane / 3) % 3 - 1, z = iz + lane % 3 - 1;
+        int sb = (x >= 0 && y >= 0 && z >= 0 && x < S && y < S && z < S) ? a.map[((int64_t)x * S + y) * S + z] : -1;
+        s_start[lane] = sb >= 0 ? a.leaf_start[sb] : 0;
+        s_pre[lane + 1] = sb >= 0 ? a.leaf_start[sb + 1] - a.leaf_start[sb] : 0;
     }
     __syncthreads();
-    for (int c0 = 0; c0 < a.ncols; c0 += 4 * NF_CPT) {
-        const int cn = min(4 * NF_CPT, a.ncols - c0);
-        for (int i0 = 0; i0 < nt; i0 += NF_TGT) {
-            const int i = i0 + slot;
-            const bool active = i < nt;
-            double xi = 0, yi = 0, zi = 0;
-            if (active) { xi = a.x[t0 + i]; yi = a.y[t0 + i]; zi = a.z[t0 + i]; }
-            double acc[NF_CPT];
+    if (lane == 0) {
+        s_pre[0] = 0;
+        for (int m = 0; m < 27; m++) s_pre[m + 1] += s_pre[m];
+    }
+    __syncthreads();
+    const int ns = s_pre[27];
+
+    for (int c0 = 0; c0 < a.ncols; c0 += NV_COL) {
+        const int cn = min(NV_COL, a.ncols - c0);
+        for (int i0 = 0; i0 < nt; i0 += NV_TGT) {
+            const int tn = min(NV_TGT, nt - i0);
+            const int TT = tn <= 1 ? 1 : tn <= 2 ? 2 : tn <= 4 ? 4 : tn <= 8 ? 8 : 16;
+            const int G = 64 / TT;
+            const int i = lane / G, g = lane % G;
+            const bool active = i < tn;
+            __syncthreads();
+            if (lane < tn) {
+                tx[lane] = a.x[t0 + i0 + lane];
+                ty[lane] = a.y[t0 + i0 + lane];
+                tz[lane] = a.z[t0 + i0 + lane];
+            }
+            __syncthreads();
+            double acc[NV_COL / 4];
 #pragma unroll
-            for (int k = 0; k < NF_CPT; k++) acc[k] = 0;
+            for (int k = 0; k < NV_COL / 4; k++) acc[k] = 0;
             // L2P (compute.hpp:594-648)
             if (a.F != nullptr && active) {
                 double Sx[16], Sy[16], Sz[16];
-                sn_eval(n, a.Tkz, ihalfL * (xi - cen[0]), Sx);
-                sn_eval(n, a.Tkz, ihalfL * (yi - cen[1]), Sy);
-                sn_eval(n, a.Tkz, ihalfL * (zi - cen[2]), Sz);
+                sn_eval(n, a.Tkz, ihalfL * (tx[i] - cen[0]), Sx);
+                sn_eval(n, a.Tkz, ihalfL * (ty[i] - cen[1]), Sy);
+                sn_eval(n, a.Tkz, ihalfL * (tz[i] - cen[2]), Sz);
+                const double *Fb = a.F + ((size_t)(a.off + b) * a.ncols + c0) * n3;
 #pragma unroll
-                for (int k = 0; k < NF_CPT; k++) {
-                    int cl = g + 4 * k;
+                for (int k = 0; k < NV_COL / 4; k++) {
+                    const int cl = g + G * k;
                     if (cl >= cn) break;
-                    const double *F = a.F + ((size_t)(a.off + b) * a.ncols + c0 + cl) * n3;
+                    const double *F = Fb + (size_t)cl * n3;
                     double sum = 0;
                     int l = 0;
                     for (int l1 = 0; l1 < n; l1++)
                         for (int l2 = 0; l2 < n; l2++) {
-                            double t2 = Sx[l1] * Sy[l2];
+                            const double t2 = Sx[l1] * Sy[l2];
                             for (int l3 = 0; l3 < n; l3++) sum += F[l++] * t2 * Sz[l3];
                         }
                     acc[k] += sum;
                 }
             }
-            // P2P over the 27 neighbours (compute.hpp:650-737)
-            for (int m = 0; m < 27; m++) {
-                const int s = s_nbr[m];
-                if (s < 0) continue;
-                const int s0 = a.leaf_start[s], ns = a.leaf_start[s + 1] - s0;
-                for (int j0 = 0; j0 < ns; j0 += NF_SRC) {
-                    const int jn = min(NF_SRC, ns - j0);
-                    __syncthreads();
-                    if (tid < jn) {
-                        sx[tid] = a.x[s0 + j0 + tid];
-                        sy[tid] = a.y[s0 + j0 + tid];
-                        sz[tid] = a.z[s0 + j0 + tid];
+            // P2P over the concatenated neighbour points (compute.hpp:650-737)
+            for (int j0 = 0; j0 < ns; j0 += NV_SRC) {
+                const int jn = min(NV_SRC, ns - j0);
+                __syncthreads();
+                if (lane < jn) {
+                    const int q = j0 + lane;
+                    int m = 0;
+                    while (s_pre[m + 1] <= q) m++;
+                    const int gi = s_start[m] + q - s_pre[m];
+                    sx[lane] = a.x[gi];
+                    sy[lane] = a.y[gi];
+                    sz[lane] = a.z[gi];
+                    for (int c = 0; c < cn; c++) sw[c][lane] = a.ws[(int64_t)(c0 + c) * a.N + gi];
+                }
+                __syncthreads();
+                for (int q = lane; q < TT * NV_SRC; q += 64) {
+                    const int ii = q / NV_SRC, jj = q % NV_SRC;
+                    double kv = 0.0;
+                    if (ii < tn && jj < jn) {
+                        kv = keval<KID>(sx[jj] - tx[ii], sy[jj] - ty[ii], sz[jj] - tz[ii], a.kp);
+                        if (isinf(kv)) kv = 0.0;
                     }
-                    for (int q = tid; q < cn * NF_SRC; q += 256) {
-                        int c = q / NF_SRC, p = q % NF_SRC;
-                        sw[c][p] = p < jn ? a.ws[(int64_t)(c0 + c) * a.N + s0 + j0 + p] : 0.0;
-                    }
-                    __syncthreads();
-                    if (active) {
-                        for (int p = 0; p < jn; p++) {
-                            double kv = keval<KID>(sx[p] - xi, sy[p] - yi, sz[p] - zi, a.kp);
-                            if (isinf(kv)) kv = 0.0;
+                    sK[ii][jj] = kv;
+                }
+                __syncthreads();
+                if (active) {
+                    for (int jj = 0; jj < jn; jj++) {
+                        const double kv = sK[i][jj];
 #pragma unroll
-                            for (int k = 0; k < NF_CPT; k++) {
-                                int cl = g + 4 * k;
-                                if (cl < cn) acc[k] += kv * sw[cl][p];
-                            }
+                        for (int k = 0; k < NV_COL / 4; k++) {
+                            const int cl = g + G * k;
+                            if (cl < cn) acc[k] += kv * sw[cl][jj];
                         }
                     }
                 }
             }
             if (active) {
-                const int64_t oi = a.perm[t0 + i];
+                const int64_t oi = a.perm[t0 + i0 + i];
 #pragma unroll
-                for (int k = 0; k < NF_CPT; k++) {
-                    int cl = g + 4 * k;
+                for (int k = 0; k < NV_COL / 4; k++) {
+                    const int cl = g + G * k;
                     if (cl < cn) a.out[(int64_t)(c0 + cl) * a.N + oi] += acc[k];
                 }
             }
@@ -630,9 +665,13 @@ void launch_l2l(const M2MArgs &a, int nparent, hipStream_t st)
 void launch_gemm(const GemmArgs &g, hipStream_t st)
 {
     if (g.ncol <= 0) return;
-    dim3 grid((unsigned)((g.ncol + 63) / 64), (unsigned)((g.M + 63) / 64));
-    gemm_small_a_fp64<<<grid, 256, 0, st>>>(g);
-    check(hipGetLastError(), "gemm_small_a_fp64");
+    if (g.ldb < g.K || g.ldc < g.M) fail(PBB_ERR_INTERNAL, "gemm: leading dimension too small");
+    dim3 grid((unsigned)((g.ncol + 63) / 64), (unsigned)((std::max(g.M, g.ldc) + 63) / 64));
+    if (!g.b_f32 && !g.c_f32) gemm_small_a<double, double><<<grid, 256, 0, st>>>(g);
+    else if (!g.b_f32 && g.c_f32) gemm_small_a<double, float><<<grid, 256, 0, st>>>(g);
+    else if (g.b_f32 && !g.c_f32) gemm_small_a<float, double><<<grid, 256, 0, st>>>(g);
+    else gemm_small_a<float, float><<<grid, 256, 0, st>>>(g);
+    check(hipGetLastError(), "gemm_small_a");
 }
 
 void launch_m2l(const M2LArgs &a, int ntiles, int ncolchunks, hipStream_t st)
@@ -661,14 +700,14 @@ void launch_near(const NearArgs &a, int kid, int nleaf, hipStream_t st)
     if (nleaf <= 0) return;
 #define PBB_NEAR_CASE(K)                                  \
     case K:                                               \
-        near_fp64<K><<<nleaf, 256, 0, st>>>(a);           \
+        near_fp64<K><<<nleaf, 64, 0, st>>>(a);            \
         break;
     switch (kid) {
         PBB_NEAR_CASE(PBB_KERNEL_LAPLACIAN) PBB_NEAR_CASE(PBB_KERNEL_ONEOVERR2) PBB_NEAR_CASE(PBB_KERNEL_GAUSSIAN)
         PBB_NEAR_CASE(PBB_KERNEL_EXPONENTIAL) PBB_NEAR_CASE(PBB_KERNEL_SOFT_LAPLACIAN)
         PBB_NEAR_CASE(PBB_KERNEL_QUADRIC) PBB_NEAR_CASE(PBB_KERNEL_INV_QUADRIC)
         PBB_NEAR_CASE(PBB_KERNEL_LOGARITHM) PBB_NEAR_CASE(PBB_KERNEL_THIN_PLATE) PBB_NEAR_CASE(PBB_KERNEL_R2)
-        PBB_NEAR_CASE(PBB_KERNEL_CONSTANT)
+        PBB_NEAR_CASE(PBB_KERNEL_CONSTANT) PBB_NEAR_CASE(PBB_KERNEL_EXP_ANISO)
     default: fail(PBB_ERR_UNSUPPORTED, "no device functor for this kernel id");
     }
 #undef PBB_NEAR_CASE
@@ -689,7 +728,7 @@ void launch_direct(const DirectArgs &a, int kid, hipStream_t st)
         PBB_DIRECT_CASE(PBB_KERNEL_SOFT_LAPLACIAN) PBB_DIRECT_CASE(PBB_KERNEL_QUADRIC)
         PBB_DIRECT_CASE(PBB_KERNEL_INV_QUADRIC) PBB_DIRECT_CASE(PBB_KERNEL_LOGARITHM)
         PBB_DIRECT_CASE(PBB_KERNEL_THIN_PLATE) PBB_DIRECT_CASE(PBB_KERNEL_R2)
-        PBB_DIRECT_CASE(PBB_KERNEL_CONSTANT)
+        PBB_DIRECT_CASE(PBB_KERNEL_CONSTANT) PBB_DIRECT_CASE(PBB_KERNEL_EXP_ANISO)
     default: fail(PBB_ERR_UNSUPPORTED, "no device functor for this kernel id");
     }
 #undef PBB_DIRECT_CASE

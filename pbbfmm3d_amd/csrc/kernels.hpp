@@ -41,11 +41,13 @@ struct M2MArgs {
 };
 
 struct GemmArgs {
-    int M, K;
+    int M, K, ldb, ldc;
+    int b_f32, c_f32;          // B / C element type: 0 FP64, 1 FP32
     int64_t ncol;
     double alpha;
-    const double *A, *B;
-    double *C;
+    const double *A;
+    const void *B;
+    void *C;
 };
 
 struct M2LArgs {
@@ -78,7 +80,25 @@ struct DirectArgs {
     double *part, *out;            // part: nsplit x ncols x T;  out: ncols x T
 };
 
+struct M2LMixArgs {
+    int rf, rs, ncols, TB, cpt, side, kpad, nch, escale;
+    int64_t off;
+    int tile_prefix[9];
+    int64_t class_start[9];
+    const int32_t *order, *coord, *map;
+    const void *Es;             // split E of this level's operator set
+    const float *P;             // FP32 proxies, kpad per column
+    const unsigned *xmax;       // max |P| of the level (FP16 scaling) or nullptr
+    float *Pf;
+};
+
 void upload_dirs(const int dirs[8][189][4]);
+void launch_split_e(int variant, const double *E, int rf, int rs, int R, int nch, int escale, void *Es,
+                    hipStream_t st);
+void launch_absmax(const float *x, int64_t n, unsigned *out, hipStream_t st);
+void upload_dirs_mixed(const int dirs[8][189][4]);
+void launch_zero_u32(unsigned *p, int n, hipStream_t st);
+void launch_m2l_mixed(int variant, const M2LMixArgs &a, int ntiles, int ncolchunks, hipStream_t st);
 void launch_gather_weights(const double *w, const int32_t *perm, int64_t N, int ncols, double *ws, hipStream_t st);
 void launch_p2m(const P2MArgs &a, int nleaf, hipStream_t st);
 void launch_m2m(const M2MArgs &a, int nparent, hipStream_t st);

@@ -1,0 +1,313 @@
+// kernels_mixed.hip -- mixed-precision M2L on the 16-bit matrix cores.
+//
+// The M2L contraction Pf[:, t] = sum_d E_d P[:, t + d] (compute.hpp:464-493)
+// runs on v_mfma_f32_16x16x32_{bf16,f16} with FP32 accumulation.  Each FP32
+// operand x is split into NSPLIT 16-bit parts x = x0 + x1 (+ x2) and the
+// product keeps the terms x_i y_j with i + j < NSPLIT:
+//   bf16 x3: 6 MFMAs per block, ~24 significand bits per operand
+//   f16  x2: 3 MFMAs per block, ~22 bits; operands are pre-scaled by powers
+//            of two (E per operator set, P per level) into FP16's range and
+//            the scale is removed exactly from the FP32 result
+//   bf16 x2: 3 MFMAs per block, ~16 bits (the low-accuracy end)
+// Layouts:
+//   Es  [set][d][chunk][plane][R][32]  split E_d, rows padded to R = 16*RB,
+//                                     k padded to chunks of 32, zeros
+//   P32 [box][col][kpad]               FP32 proxies, k padded with zeros
+//   Pf32[box][col][rf]                 FP32 result
+// Fragment layout of v_mfma_f32_16x16x32_{bf16,f16} (lane l of 64, e < 8):
+//   A[i][k]: i = l % 16, k = 8*(l/16) + e     B[k][j]: j = l % 16, k = 8*(l/16) + e
+//   D[i][j]: j = l % 16, i = 4*(l/16) + v     (v = 0..3, FP32)
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace pbb {
+namespace dev {
+
+__constant__ int c_dirs[8][189][4];  // far directions of each octant class (this module's copy)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Vec8;
+template <>
+struct Vec8<__bf16> {
+    typedef bf16x8 type;
+};
+template <>
+struct Vec8<_Float16> {
+    typedef f16x8 type;
+};
+
+__device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 mfma16(f16x8 a, f16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Es from the FP64 operators: x = E_d[row + k*rf] * 2^escale, split in FP64.
+template <typename T, int NSPLIT>
+__global__ void split_e(const double *__restrict__ E, int rf, int rs, int R, int nch, int escale,
+                        T *__restrict__ Es)
+{
+    const int64_t per_d = (int64_t)nch * NSPLIT * R * 32;
+    const int64_t total = 316 * per_d;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        int64_t d = q / per_d, rem = q - d * per_d;
+        int ch = (int)(rem / ((int64_t)NSPLIT * R * 32));
+        rem -= (int64_t)ch * NSPLIT * R * 32;
+        int p = (int)(rem / (R * 32));
+        rem -= (int64_t)p * R * 32;
+        int row = (int)(rem / 32), kk = (int)(rem % 32);
+        int k = ch * 32 + kk;
+        double x = (row < rf && k < rs) ? ldexp(E[d * (int64_t)rf * rs + row + (int64_t)k * rf], escale) : 0.0;
+        T part = (T)0.0f;
+        for (int i = 0; i <= p; i++) {
+            part = (T)(float)x;
+            x -= (double)(float)part;
+        }
+        Es[q] = part;
+    }
+}
+
+// max |x| over n floats into *out (as ordered bits; out cleared by caller)
+__global__ void absmax_f32(const float *__restrict__ x, int64_t n, unsigned *__restrict__ out)
+{
+    float m = 0.0f;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, fabsf(x[q]));
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+// ---------------------------------------------------------------------------
+template <typename T, int NSPLIT, int RB, int NCB>
+__global__ __launch_bounds__(256) void m2l_mfma_split(M2LMixArgs a)
+{
+    typedef typename Vec8<T>::type V8;
+    constexpr int NW = 4, NT = 256, W = 16 * NW * NCB, R = 16 * RB, KS = 40;  // KS: LDS row stride
+    constexpr int NPROD = NSPLIT == 3 ? 6 : 3;
+    __shared__ __attribute__((aligned(16))) T As[NSPLIT * R * KS];
+    __shared__ __attribute__((aligned(16))) T Bs[NSPLIT * W * KS];
+    __shared__ int s_sid[W];
+    __shared__ int s_tgt[W];
+    __shared__ int s_xyz[W][3];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int rf = a.rf;
+    const int tile = blockIdx.x;
+    int o = 0;
+    while (o < 7 && tile >= a.tile_prefix[o + 1]) o++;
+    const int64_t first = a.class_start[o] + (int64_t)(tile - a.tile_prefix[o]) * a.TB;
+    const int64_t cend = a.class_start[o + 1];
+    const int c0 = blockIdx.y * a.cpt;
+    const int cpt = min(a.cpt, a.ncols - c0);
+    const int S = a.side;
+    // FP16 operands are scaled into range by 2^xs (P) and 2^escale (E)
+    int xs = 0;
+    if (a.xmax) {
+        float mx = __uint_as_float(*a.xmax);
+        if (mx > 0.0f) {
+            int e;
+            frexpf(mx, &e);
+            xs = 12 - e;
+        }
+    }
+    const float unscale = ldexpf(1.0f, -(xs + a.escale));
+    if (tid < a.TB) {
+        int64_t q = first + tid;
+        int t = q < cend ? a.order[q] : -1;
+        s_tgt[tid] = t;
+        if (t >= 0) {
+            uint32_t c = (uint32_t)a.coord[t];
+            s_xyz[tid][0] = c & 1023;
+            s_xyz[tid][1] = (c >> 10) & 1023;
+            s_xyz[tid][2] = (c >> 20) & 1023;
+        }
+    }
+    f32x4 acc[RB][NCB];
+#pragma unroll
+    for (int i = 0; i < RB; i++)
+#pragma unroll
+        for (int j = 0; j < NCB; j++) acc[i][j] = f32x4{0, 0, 0, 0};
+    __syncthreads();
+
+    const int fr = lane % 16, fq = lane / 16;
+    for (int k = 0; k < 189; k++) {
+        const int dx = c_dirs[o][k][0], dy = c_dirs[o][k][1], dz = c_dirs[o][k][2];
+        const int eidx = c_dirs[o][k][3];
+        int any = 0;
+        if (tid < a.TB) {
+            int sid = -1;
+            if (s_tgt[tid] >= 0) {
+                int sx = s_xyz[tid][0] + dx, sy = s_xyz[tid][1] + dy, sz = s_xyz[tid][2] + dz;
+                if (sx >= 0 && sy >= 0 && sz >= 0 && sx < S && sy < S && sz < S)
+                    sid = a.map[((int64_t)sx * S + sy) * S + sz];
+            }
+            s_sid[tid] = sid;
+            any = sid >= 0;
+        }
+        if (!__syncthreads_or(any)) continue;
+        const T *Ed = (const T *)a.Es + (size_t)eidx * a.nch * NSPLIT * R * 32;
+        for (int ch = 0; ch < a.nch; ch++) {
+            // A: the chunk's NSPLIT x R x 32 block, 16 bytes (8 elements) per load
+            const uint4 *src = (const uint4 *)(Ed + (size_t)ch * NSPLIT * R * 32);
+            for (int u = tid; u < NSPLIT * R * 4; u += NT) {
+                int p = u / (R * 4), row = (u / 4) % R, q = u % 4;
+                *(uint4 *)&As[(p * R + row) * KS + 8 * q] = src[u];
+            }
+            // B: gathered FP32 proxies, split on the fly
+            for (int u = tid; u < W * 8; u += NT) {
+                int col = u / 8, k4 = u % 8;
+                int tb = col / cpt, cl = col - tb * cpt;
+                float4 v = {0, 0, 0, 0};
+                if (tb < a.TB) {
+                    int sid = s_sid[tb];
+                    if (sid >= 0)
+                        v = *(const float4 *)(a.P + ((size_t)(a.off + sid) * a.ncols + c0 + cl) * a.kpad + ch * 32 + 4 * k4);
+                }
+                float x[4] = {ldexpf(v.x, xs), ldexpf(v.y, xs), ldexpf(v.z, xs), ldexpf(v.w, xs)};
+#pragma unroll
+                for (int p = 0; p < NSPLIT; p++) {
+                    T h[4];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        h[e] = (T)x[e];
+                        x[e] -= (float)h[e];
+                    }
+                    T *dst = &Bs[(p * W + col) * KS + 4 * k4];
+                    dst[0] = h[0]; dst[1] = h[1]; dst[2] = h[2]; dst[3] = h[3];
+                }
+            }
+            __syncthreads();
+            V8 bfr[NCB][NSPLIT];
+#pragma unroll
+            for (int cb = 0; cb < NCB; cb++)
+#pragma unroll
+                for (int p = 0; p < NSPLIT; p++)
+                    bfr[cb][p] = *(const V8 *)&Bs[(p * W + (wave * NCB + cb) * 16 + fr) * KS + 8 * fq];
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                V8 afr[NSPLIT];
+#pragma unroll
+                for (int p = 0; p < NSPLIT; p++) afr[p] = *(const V8 *)&As[(p * R + rb * 16 + fr) * KS + 8 * fq];
+#pragma unroll
+                for (int cb = 0; cb < NCB; cb++) {
+                    // smallest terms first
+                    if constexpr (NPROD == 6) {
+                        acc[rb][cb] = mfma16(afr[2], bfr[cb][0], acc[rb][cb]);
+                        acc[rb][cb] = mfma16(afr[1], bfr[cb][1], acc[rb][cb]);
+                        acc[rb][cb] = mfma16(afr[0], bfr[cb][2], acc[rb][cb]);
+                    }
+                    acc[rb][cb] = mfma16(afr[1], bfr[cb][0], acc[rb][cb]);
+                    acc[rb][cb] = mfma16(afr[0], bfr[cb][1], acc[rb][cb]);
+                    acc[rb][cb] = mfma16(afr[0], bfr[cb][0], acc[rb][cb]);
+                }
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int cb = 0; cb < NCB; cb++) {
+        const int col = (wave * NCB + cb) * 16 + fr;
+        const int tb = col / cpt, cl = col - tb * cpt;
+        if (tb >= a.TB) continue;
+        const int t = s_tgt[tb];
+        if (t < 0) continue;
+        float *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + cl) * rf;
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = rb * 16 + 4 * fq + v;
+                if (row < rf) dst[row] = acc[rb][cb][v] * unscale;
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+template <typename T, int NSPLIT>
+static void split_e_t(const double *E, int rf, int rs, int R, int nch, int escale, void *Es, hipStream_t st)
+{
+    int64_t total = 316LL * nch * NSPLIT * R * 32;
+    int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    split_e<T, NSPLIT><<<blocks, 256, 0, st>>>(E, rf, rs, R, nch, escale, (T *)Es);
+    check(hipGetLastError(), "split_e");
+}
+
+void launch_split_e(int variant, const double *E, int rf, int rs, int R, int nch, int escale, void *Es,
+                    hipStream_t st)
+{
+    switch (variant) {
+    case PBB_PREC_MIXED: split_e_t<__bf16, 3>(E, rf, rs, R, nch, escale, Es, st); break;
+    case PBB_PREC_MIXED_F16X2: split_e_t<_Float16, 2>(E, rf, rs, R, nch, escale, Es, st); break;
+    case PBB_PREC_MIXED_BF16X2: split_e_t<__bf16, 2>(E, rf, rs, R, nch, escale, Es, st); break;
+    default: fail(PBB_ERR_INVALID_ARG, "unknown mixed-precision variant");
+    }
+}
+
+void upload_dirs_mixed(const int dirs[8][189][4])
+{
+    check(hipMemcpyToSymbol(HIP_SYMBOL(c_dirs), dirs, sizeof(int) * 8 * 189 * 4), "hipMemcpyToSymbol(c_dirs)");
+}
+
+__global__ void zero_u32(unsigned *p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+void launch_zero_u32(unsigned *p, int n, hipStream_t st)
+{
+    if (n <= 0) return;
+    zero_u32<<<(n + 255) / 256, 256, 0, st>>>(p, n);
+    check(hipGetLastError(), "zero_u32");
+}
+
+void launch_absmax(const float *x, int64_t n, unsigned *out, hipStream_t st)
+{
+    if (n <= 0) return;
+    int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
+    absmax_f32<<<blocks, 256, 0, st>>>(x, n, out);
+    check(hipGetLastError(), "absmax_f32");
+}
+
+template <typename T, int NSPLIT>
+static void m2l_split_t(const M2LMixArgs &a, dim3 grid, hipStream_t st)
+{
+    const int rb = (a.rf + 15) / 16;
+#define PBB_CASE(RB)                                                      \
+    case RB:                                                              \
+        m2l_mfma_split<T, NSPLIT, RB, 2><<<grid, 256, 0, st>>>(a);        \
+        break;
+    switch (rb) {
+        PBB_CASE(1) PBB_CASE(2) PBB_CASE(3) PBB_CASE(4) PBB_CASE(5) PBB_CASE(6) PBB_CASE(7) PBB_CASE(8)
+        PBB_CASE(9) PBB_CASE(10) PBB_CASE(11) PBB_CASE(12) PBB_CASE(13) PBB_CASE(14)
+    default: fail(PBB_ERR_UNSUPPORTED, "mixed-precision M2L supports ranks up to 224");
+    }
+#undef PBB_CASE
+}
+
+void launch_m2l_mixed(int variant, const M2LMixArgs &a, int ntiles, int ncolchunks, hipStream_t st)
+{
+    if (ntiles <= 0) return;
+    dim3 grid((unsigned)ntiles, (unsigned)ncolchunks);
+    switch (variant) {
+    case PBB_PREC_MIXED: m2l_split_t<__bf16, 3>(a, grid, st); break;
+    case PBB_PREC_MIXED_F16X2: m2l_split_t<_Float16, 2>(a, grid, st); break;
+    case PBB_PREC_MIXED_BF16X2: m2l_split_t<__bf16, 2>(a, grid, st); break;
+    default: fail(PBB_ERR_INVALID_ARG, "unknown mixed-precision variant");
+    }
+    check(hipGetLastError(), "m2l_mfma_split");
+}
+
+}  // namespace dev
+}  // namespace pbb

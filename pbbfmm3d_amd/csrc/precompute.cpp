@@ -51,7 +51,7 @@ void parallel_for(int64_t n, const std::function<void(int64_t)> &body)
         if (e) std::rethrow_exception(e);
 }
 
-bool kernel_id_valid(int id) { return id >= PBB_KERNEL_LAPLACIAN && id <= PBB_KERNEL_POLY_XY; }
+bool kernel_id_valid(int id) { return id >= PBB_KERNEL_LAPLACIAN && id <= PBB_KERNEL_EXP_ANISO; }
 
 // Formulas of pbb_kernel_id (kernel_Types.cpp:15-178 for the built-ins).
 double kernel_eval_host(int id, const double *p, const double *t, const double *s)
@@ -71,6 +71,7 @@ double kernel_eval_host(int id, const double *p, const double *t, const double *
     case PBB_KERNEL_R2: return r2;
     case PBB_KERNEL_CONSTANT: return 1.0;
     case PBB_KERNEL_POLY_XY: return (t[0] - s[0]) * (t[1] + s[1]) + p[0];
+    case PBB_KERNEL_EXP_ANISO: return p[0] * std::exp(-std::sqrt(p[1] * dx * dx + p[2] * dy * dy + p[3] * dz * dz));
     default: return NAN;
     }
 }

@@ -1,0 +1,64 @@
+"""Mixed-precision matvec (split 16-bit MFMA M2L, FP32 accumulation) against
+the FP64 GPU path and the FP64 direct sum.
+
+Bars (relative L2): bf16 x3 and f16 x2 stay within 1e-6 of the FP64 GPU
+path; bf16 x2 (16 significand bits) within 1e-4.
+"""
+import numpy as np
+import pytest
+
+import pbbfmm3d_amd as pbb
+from oracle import oracle as orc
+from tests._helpers import clustered_points, oracle_ops_from, uniform_points, weights
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [(pbb.capi.PREC_MIXED, 1e-6), (pbb.capi.PREC_MIXED_F16X2, 1e-6),
+            (pbb.capi.PREC_MIXED_BF16X2, 1e-4)]
+
+
+def _mv(plan, w, ncols, prec):
+    import torch
+    wd = torch.tensor(w.reshape(ncols, plan.npoints), dtype=torch.float64, device="cuda:0")
+    out = plan.matvec(wd, precision=prec)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().reshape(-1)
+
+
+@pytest.mark.parametrize("prec,bar", VARIANTS, ids=["bf16x3", "f16x2", "bf16x2"])
+@pytest.mark.parametrize("kernel,n,eps,ncols", [(pbb.laplacian(), 5, 1e-7, 8),
+                                                (pbb.gaussian(0.3), 6, 1e-8, 32)],
+                         ids=["laplacian-n5", "gaussian-n6"])
+def test_mixed_vs_fp64(gpu, prec, bar, kernel, n, eps, ncols):
+    N = 30000
+    xyz = clustered_points(N, seed=21)
+    w = weights(N, ncols, seed=22, dist="normal")
+    print(f"case prec={prec} kernel={kernel.id} n={n} ncols={ncols}", flush=True)
+    ops = pbb.Operators(kernel, 1.0, 5, n, eps)
+    plan = pbb.Plan(ops, xyz)
+    f64 = _mv(plan, w, ncols, pbb.capi.PREC_FP64)
+    mix = _mv(plan, w, ncols, prec)
+    diff = orc.rel_err(mix, f64)
+    print(f"rel diff mixed vs FP64 GPU: {diff:.3e}")
+    assert diff <= bar
+    # the FP64 GPU path itself against the oracle
+    ref = orc.matvec(oracle_ops_from(ops, kernel), xyz, w, ncols)
+    assert orc.rel_err(f64, ref) <= 1e-10
+
+
+def test_mixed_switching_and_accumulate(gpu):
+    import torch
+    N = 8000
+    xyz = uniform_points(N, seed=23)
+    w = weights(N, 4, seed=24)
+    ops = pbb.Operators(pbb.laplacian(), 1.0, 4, 4, 1e-5)
+    plan = pbb.Plan(ops, xyz)
+    wd = torch.tensor(w.reshape(4, N), dtype=torch.float64, device="cuda:0")
+    out = torch.zeros_like(wd)
+    plan.matvec(wd, out, precision=pbb.capi.PREC_MIXED)
+    plan.matvec(wd, out, precision=pbb.capi.PREC_FP64)
+    plan.matvec(wd, out, precision=pbb.capi.PREC_MIXED)
+    torch.cuda.synchronize()
+    f64 = plan.matvec(wd, precision=pbb.capi.PREC_FP64)
+    torch.cuda.synchronize()
+    assert orc.rel_err(out.cpu().numpy() / 3, f64.cpu().numpy()) <= 1e-6

@@ -137,3 +137,17 @@ def test_drop_in_host_path(gpu):
     pbb.matvec_host(ops, xyz, w, 3, out)
     ref = orc.matvec(oracle_ops_from(ops, pbb.laplacian()), xyz, w, 3)
     assert orc.rel_err(out - 0.5, ref) <= PARITY
+
+
+def test_result_ordered_on_torch_stream(gpu):
+    # the matvec runs on torch's current stream (here the null stream):
+    # reading the result back needs no device-wide synchronisation
+    import torch
+    xyz = uniform_points(3000, seed=40)
+    w = weights(3000, 2, seed=41)
+    ops = pbb.Operators(pbb.gaussian(0.5), 1.0, 3, 4, 1e-6)
+    plan = pbb.Plan(ops, xyz)
+    wd = torch.tensor(w.reshape(2, 3000), dtype=torch.float64, device="cuda:0")
+    got = plan.matvec(wd).cpu().numpy().reshape(-1)
+    ref = orc.matvec(oracle_ops_from(ops, pbb.gaussian(0.5)), xyz, w, 2)
+    assert orc.rel_err(got, ref) <= PARITY
