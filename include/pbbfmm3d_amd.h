@@ -194,6 +194,18 @@ PBB_API pbb_status pbb_plan_matvec_transpose(pbb_plan *plan, const double *w_dev
                                              int32_t ncols, double *out_dev,
                                              int32_t precision, void *stream);
 
+/* Position gradients of L(out), out = A~(x) sigma, given psi = dL/dout:
+ * grad[3*i + d] += dL/dx_i[d] (AoS N x 3, caller's order) for the tree
+ * fixed (box membership and the bbox centre are piecewise constant in x).
+ * Target side: the derivative of the L2P interpolant and of the P2P kernel
+ * at x_i weighted by psi; source side: the same for the transposed FMM run
+ * on psi, weighted by sigma.  sigma, psi: device FP64, column-major N x
+ * ncols.  No reference counterpart; used by the autograd function.        */
+PBB_API pbb_status pbb_plan_grad_positions(pbb_plan *plan, const double *sigma_dev,
+                                           const double *psi_dev, int32_t ncols,
+                                           double *grad_dev, int32_t precision,
+                                           void *stream);
+
 /* Phase times of the last pbb_plan_matvec on this plan, in ms
  * (PBB_NUM_PHASES values).  Synchronises the plan's events.               */
 PBB_API pbb_status pbb_plan_timings(pbb_plan *plan, double *ms);
@@ -202,6 +214,27 @@ PBB_API pbb_status pbb_plan_timings(pbb_plan *plan, double *ms);
 PBB_API pbb_status pbb_plan_set_timing(pbb_plan *plan, int32_t enable);
 
 PBB_API void pbb_plan_destroy(pbb_plan *plan);
+
+/* ---- batches of clouds (one HIP graph) -------------------------------- */
+
+typedef struct pbb_batch pbb_batch;
+
+#define PBB_BATCH_TRANSPOSE 1  /* run the transposed FMM of every plan      */
+#define PBB_BATCH_OVERWRITE 2  /* clear each output first (else +=)         */
+
+/* Captures the matvecs of nplans plans (any sizes, orders and tree levels,
+ * one device) on weights w_dev[i] / outputs out_dev[i] (device FP64,
+ * column-major N_i x ncols) into one HIP graph.  The buffers are bound at
+ * capture: refill them in place between launches.  All workspaces are
+ * allocated here, none during a launch.  No reference counterpart: the
+ * reference runs one cloud per H2_3D_Compute.                            */
+PBB_API pbb_status pbb_batch_create(pbb_plan *const *plans, int32_t nplans,
+                                    const double *const *w_dev, double *const *out_dev,
+                                    int32_t ncols, int32_t precision, int32_t flags,
+                                    pbb_batch **out);
+/* Replays the graph on stream (NULL = the null stream). */
+PBB_API pbb_status pbb_batch_launch(pbb_batch *batch, void *stream);
+PBB_API void pbb_batch_destroy(pbb_batch *batch);
 
 /* ---- accuracy harness ------------------------------------------------- */
 

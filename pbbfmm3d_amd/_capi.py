@@ -44,6 +44,8 @@ PREC_MIXED_F16X2 = 2   # FP16 x2 split M2L
 PREC_MIXED_BF16X2 = 3  # BF16 x2 split M2L
 
 NUM_PHASES = 8
+BATCH_TRANSPOSE = 1
+BATCH_OVERWRITE = 2
 PHASE_NAMES = ("p2m", "m2m", "vt_proxy", "m2l", "u_expand", "l2l", "near", "total")
 
 
@@ -89,6 +91,11 @@ _SIGNATURES = {
     "pbb_plan_query": (C.c_int, [_vp, C.POINTER(PlanInfo)]),
     "pbb_plan_matvec": (C.c_int, [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp]),
     "pbb_plan_matvec_transpose": (C.c_int, [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp]),
+    "pbb_plan_grad_positions": (C.c_int, [_vp, _vp, _vp, C.c_int32, _vp, C.c_int32, _vp]),
+    "pbb_batch_create": (C.c_int, [C.POINTER(_vp), C.c_int32, C.POINTER(_vp), C.POINTER(_vp), C.c_int32,
+                                   C.c_int32, C.c_int32, C.POINTER(_vp)]),
+    "pbb_batch_launch": (C.c_int, [_vp, _vp]),
+    "pbb_batch_destroy": (None, [_vp]),
     "pbb_plan_timings": (C.c_int, [_vp, _dp]),
     "pbb_plan_set_timing": (C.c_int, [_vp, C.c_int32]),
     "pbb_plan_destroy": (None, [_vp]),

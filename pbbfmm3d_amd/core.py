@@ -260,6 +260,23 @@ class Plan:
         self.matvec_transpose_ptr(w.data_ptr(), ncols, out.data_ptr(), precision, stream)
         return out
 
+    def grad_positions(self, sigma, psi, grad=None, precision: int = capi.PREC_FP64):
+        """grad (N x 3, float64 CUDA) += dL/dx for out = A~(x) sigma and
+        psi = dL/dout (both (ncols, N) float64 CUDA tensors)."""
+        import torch
+        sigma = sigma.contiguous()
+        psi = psi.contiguous()
+        ncols = 1 if sigma.dim() == 1 else sigma.shape[0]
+        if sigma.numel() != ncols * self.npoints or psi.shape != sigma.shape:
+            raise ValueError("sigma and psi must both hold ncols * npoints values")
+        if grad is None:
+            grad = torch.zeros((self.npoints, 3), dtype=torch.float64, device=sigma.device)
+        stream = torch.cuda.current_stream(sigma.device).cuda_stream
+        capi.check(capi.lib().pbb_plan_grad_positions(
+            self._h, C.c_void_p(sigma.data_ptr()), C.c_void_p(psi.data_ptr()), int(ncols),
+            C.c_void_p(grad.data_ptr()), int(precision), C.c_void_p(stream)))
+        return grad
+
     def set_timing(self, enable: bool = True):
         capi.check(capi.lib().pbb_plan_set_timing(self._h, 1 if enable else 0))
 
@@ -272,6 +289,44 @@ class Plan:
         h = getattr(self, "_h", None)
         if h and capi._lib is not None:
             capi.lib().pbb_plan_destroy(h)
+            self._h = None
+
+
+class Batch:
+    """Many plans' matvecs (forward or transposed) captured into one HIP
+    graph.  weights[i] / outputs[i] are float64 CUDA tensors (ncols, N_i)
+    bound at capture: refill them in place between launches."""
+
+    def __init__(self, plans, weights, outputs, ncols: int, precision: int = capi.PREC_FP64,
+                 transpose: bool = False, overwrite: bool = True):
+        if not (len(plans) == len(weights) == len(outputs)) or not plans:
+            raise ValueError("plans, weights and outputs must have the same non-zero length")
+        for p, w, o in zip(plans, weights, outputs):
+            if w.numel() != ncols * p.npoints or o.numel() != ncols * p.npoints:
+                raise ValueError("each weight/output tensor must hold ncols * npoints values")
+            if not (w.is_contiguous() and o.is_contiguous()):
+                raise ValueError("weight/output tensors must be contiguous")
+        self.plans, self.weights, self.outputs = list(plans), list(weights), list(outputs)
+        n = len(plans)
+        P = (C.c_void_p * n)(*[p._h.value for p in plans])
+        W = (C.c_void_p * n)(*[w.data_ptr() for w in weights])
+        O = (C.c_void_p * n)(*[o.data_ptr() for o in outputs])
+        flags = (capi.BATCH_TRANSPOSE if transpose else 0) | (capi.BATCH_OVERWRITE if overwrite else 0)
+        h = C.c_void_p()
+        capi.check(capi.lib().pbb_batch_create(P, n, W, O, int(ncols), int(precision), flags,
+                                               C.byref(h)))
+        self._h = h
+
+    def launch(self, stream: Optional[int] = None):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        capi.check(capi.lib().pbb_batch_launch(self._h, C.c_void_p(stream)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and capi._lib is not None:
+            capi.lib().pbb_batch_destroy(h)
             self._h = None
 
 

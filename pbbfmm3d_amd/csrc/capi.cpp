@@ -234,6 +234,8 @@ struct pbb_plan {
     double *ws = nullptr, *S = nullptr, *P = nullptr, *Pf = nullptr;
     float *P32 = nullptr, *Pf32 = nullptr;
     unsigned *xmax = nullptr;
+    double *gF = nullptr, *gU = nullptr;  // position gradients: saved leaf F, sorted weights
+    int g_ncols = 0;
     int kpad = 0;
     bool timing = false;
     hipEvent_t ev[PBB_NUM_PHASES] = {};
@@ -251,7 +253,7 @@ struct pbb_plan {
         for (auto &L : lv) {
             dfree(L.coord); dfree(L.map); dfree(L.child); dfree(L.order); dfree(L.ctab);
         }
-        dfree(ws); dfree(S); dfree(P); dfree(Pf); dfree(P32); dfree(Pf32); dfree(xmax);
+        dfree(ws); dfree(S); dfree(P); dfree(Pf); dfree(P32); dfree(Pf32); dfree(xmax); dfree(gF); dfree(gU);
         if (events_made)
             for (auto &e : ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
@@ -540,6 +542,8 @@ PBB_API pbb_status pbb_plan_timings(pbb_plan *plan, double *ms)
     });
 }
 
+// One FMM application.  out == nullptr stops after the downward pass, leaving
+// the leaf local expansions in p->S and the sorted weights in p->ws.
 static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, double *out, hipStream_t st, int prec)
 {
     const bool mixed = prec != PBB_PREC_FP64;
@@ -596,12 +600,13 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
     // halve the E traffic per flop when the columns allow it
     const int cpt = std::min(ncols, 64);
     const int width = ncols >= 32 || ncols <= 16 ? 128 : 64;
-    const int TB = std::max(1, width / cpt);
+    const int TB = std::max(1, std::min(16, width / cpt));  // at most 16 targets per tile
     const int nchunks = (ncols + cpt - 1) / cpt;
     for (int l = 2; l <= Lv && mixed; l++) {
         const DevLevel &D = p->lv[l];
         dev::M2LMixArgs a{};
-        a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = std::max(1, 128 / cpt); a.cpt = cpt; a.side = 1 << l;
+        a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = std::max(1, std::min(16, 128 / cpt)); a.cpt = cpt;
+        a.side = 1 << l;
         a.kpad = ldP; a.nch = SE->nch; a.escale = SE->escale[h.ops_set(l)]; a.off = D.off;
         a.tile_prefix[0] = 0;
         a.class_start[0] = 0;
@@ -658,7 +663,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         dev::launch_l2l(a, (int)p->lv[l - 1].nb, st);
     }
     mark(6);
-    {
+    if (out) {
         const DevLevel &LL = p->lv[Lv];
         dev::NearArgs a{};
         a.n = n; a.ncols = ncols; a.side = 1 << Lv; a.N = p->N; a.off = LL.off; a.len = LL.len;
@@ -707,7 +712,154 @@ PBB_API pbb_status pbb_plan_matvec_transpose(pbb_plan *plan, const double *w_dev
     });
 }
 
+PBB_API pbb_status pbb_plan_grad_positions(pbb_plan *plan, const double *sigma_dev, const double *psi_dev,
+                                           int32_t ncols, double *grad_dev, int32_t precision, void *stream)
+{
+    return guarded([&] {
+        if (!plan || !sigma_dev || !psi_dev || !grad_dev) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
+        if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
+            fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
+        if (plan->ops->h.kernel.symmetry != 1)
+            fail(PBB_ERR_UNSUPPORTED, "position gradients need a symmetric kernel");
+        pbb_ops *adj = plan->ops->adjoint();
+        DeviceGuard g(plan->device);
+        adj->upload(plan->device);
+        hipStream_t st = (hipStream_t)stream;
+        pbb_plan *p = plan;
+        const HostOps &h = p->ops->h;
+        const int Lv = p->Lv, n3 = h.n3;
+        const DevLevel &LL = p->lv[Lv];
+        if (ncols > p->g_ncols) {
+            dfree(p->gF); dfree(p->gU);
+            p->gF = p->gU = nullptr;
+            p->g_ncols = 0;
+            if (Lv >= 2) p->gF = dalloc<double>((size_t)LL.nb * ncols * n3, "saved leaf expansions");
+            p->gU = dalloc<double>((size_t)p->N * ncols, "sorted weights (gradient)");
+            p->g_ncols = ncols;
+        }
+        const size_t leafF = (size_t)LL.nb * ncols * n3;
+        // forward FMM with sigma: leaf expansions F and sorted sigma
+        matvec_run(p, p->ops, sigma_dev, ncols, nullptr, st, precision);
+        if (Lv >= 2)
+            check(hipMemcpyAsync(p->gF, p->S + (size_t)LL.off * ncols * n3, leafF * sizeof(double),
+                                 hipMemcpyDeviceToDevice, st), "save leaf expansions");
+        check(hipMemcpyAsync(p->gU, p->ws, (size_t)p->N * ncols * sizeof(double), hipMemcpyDeviceToDevice, st),
+              "save sorted weights");
+        // transposed FMM with psi: its leaf expansions and sorted psi
+        matvec_run(p, adj, psi_dev, ncols, nullptr, st, precision);
+        dev::NearGradArgs a{};
+        a.n = h.n; a.ncols = ncols; a.side = 1 << Lv; a.N = p->N; a.len = LL.len;
+        std::memcpy(a.kp.p, h.kernel.param, sizeof(a.kp.p));
+        a.leaf_start = p->leaf_start; a.coord = LL.coord; a.map = LL.map; a.perm = p->perm;
+        a.ctab = LL.ctab; a.Tkz = p->ops->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z;
+        a.grad = grad_dev;
+        // target side: d/dx_i of (A sigma)_i weighted by psi_i
+        a.v = p->ws; a.u = p->gU; a.F = Lv >= 2 ? p->gF : nullptr;
+        dev::launch_near_grad(a, h.kernel.id, (int)LL.nb, st);
+        // source side: d/dx_j of (A^T psi)_j weighted by sigma_j
+        a.v = p->gU; a.u = p->ws; a.F = Lv >= 2 ? p->S + (size_t)LL.off * ncols * n3 : nullptr;
+        dev::launch_near_grad(a, h.kernel.id, (int)LL.nb, st);
+    });
+}
+
 PBB_API void pbb_plan_destroy(pbb_plan *plan) { delete plan; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: many plans' matvecs captured once into a HIP graph and replayed.
+struct pbb_batch {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~pbb_batch()
+    {
+        if (device < 0) return;
+        int prev = -1;
+        (void)hipGetDevice(&prev);
+        (void)hipSetDevice(device);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (stream) (void)hipStreamDestroy(stream);
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+extern "C" {
+
+PBB_API pbb_status pbb_batch_create(pbb_plan *const *plans, int32_t nplans, const double *const *w_dev,
+                                    double *const *out_dev, int32_t ncols, int32_t precision, int32_t flags,
+                                    pbb_batch **out)
+{
+    return guarded([&] {
+        if (!out) fail(PBB_ERR_INVALID_ARG, "out is NULL");
+        *out = nullptr;
+        if (!plans || !w_dev || !out_dev || nplans < 1) fail(PBB_ERR_INVALID_ARG, "empty batch");
+        if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
+        if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
+            fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
+        const bool transpose = (flags & PBB_BATCH_TRANSPOSE) != 0;
+        const bool overwrite = (flags & PBB_BATCH_OVERWRITE) != 0;
+        const bool mixed = precision != PBB_PREC_FP64;
+        const int dev = plans[0]->device;
+        for (int i = 0; i < nplans; i++) {
+            if (!plans[i] || !w_dev[i] || !out_dev[i]) fail(PBB_ERR_INVALID_ARG, "NULL plan or buffer in batch");
+            if (plans[i]->device != dev) fail(PBB_ERR_INVALID_ARG, "batch plans must share one device");
+            if (transpose && plans[i]->ops->h.kernel.symmetry != 1)
+                fail(PBB_ERR_UNSUPPORTED, "the transposed near field needs a symmetric kernel");
+        }
+        DeviceGuard g(dev);
+        auto b = std::make_unique<pbb_batch>();
+        b->device = dev;
+        check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        // everything that allocates or synchronises happens before the capture
+        std::vector<pbb_ops *> ops(nplans);
+        for (int i = 0; i < nplans; i++) {
+            pbb_plan *p = plans[i];
+            ops[i] = transpose ? p->ops->adjoint() : p->ops;
+            ops[i]->upload(dev);
+            if (mixed && ops[i]->h.rf > 224) fail(PBB_ERR_UNSUPPORTED, "mixed-precision M2L supports ranks up to 224");
+            p->ensure_workspace(ncols, mixed);
+            if (mixed && p->Lv >= 2) ops[i]->ensure_split(precision, b->stream);
+        }
+        check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        std::vector<bool> timing(nplans);
+        for (int i = 0; i < nplans; i++) {
+            timing[i] = plans[i]->timing;
+            plans[i]->timing = false;
+        }
+        check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            for (int i = 0; i < nplans; i++) {
+                if (overwrite) dev::launch_fill_zero(out_dev[i], plans[i]->N * (int64_t)ncols, b->stream);
+                matvec_run(plans[i], ops[i], w_dev[i], ncols, out_dev[i], b->stream, precision);
+            }
+        } catch (...) {
+            hipGraph_t junk = nullptr;
+            (void)hipStreamEndCapture(b->stream, &junk);
+            if (junk) (void)hipGraphDestroy(junk);
+            for (int i = 0; i < nplans; i++) plans[i]->timing = timing[i];
+            throw;
+        }
+        check(hipStreamEndCapture(b->stream, &b->graph), "hipStreamEndCapture");
+        for (int i = 0; i < nplans; i++) plans[i]->timing = timing[i];
+        check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        *out = b.release();
+    });
+}
+
+PBB_API pbb_status pbb_batch_launch(pbb_batch *batch, void *stream)
+{
+    return guarded([&] {
+        if (!batch) fail(PBB_ERR_INVALID_ARG, "batch is NULL");
+        DeviceGuard g(batch->device);
+        check(hipGraphLaunch(batch->exec, (hipStream_t)stream), "hipGraphLaunch");
+    });
+}
+
+PBB_API void pbb_batch_destroy(pbb_batch *batch) { delete batch; }
 
 PBB_API pbb_status pbb_direct_sum(const pbb_kernel_desc *kernel, const double *tgt_dev, int64_t ntargets,
                                   const double *src_dev, int64_t nsources, const double *w_dev, int32_t ncols,

@@ -62,6 +62,22 @@ __device__ __forceinline__ void sn_eval(int n, const double *__restrict__ Tkz, d
     }
 }
 
+// S_m(u) for m < n written to dst[m * stride]
+__device__ __forceinline__ void sn_eval_to(int n, const double *__restrict__ Tkz, double u, double *dst, int stride)
+{
+    const double pfac = 2.0 / n;
+    for (int m = 0; m < n; m++) {
+        const double *Tk = Tkz + m * n;
+        double d0 = 0, d1 = 0;
+        for (int j = n - 1; j > 0; j--) {
+            double d2 = d0;
+            d0 = 2.0 * u * d0 - d1 + Tk[j];
+            d1 = d2;
+        }
+        dst[m * stride] = pfac * (u * d0 - d1 + 0.5 * Tk[0]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // ws[c*N + k] = w[c*N + perm[k]]
 __global__ void gather_weights(const double *__restrict__ w, const int32_t *__restrict__ perm, int64_t N,
@@ -321,27 +337,33 @@ __global__ __launch_bounds__(256) void gemm_small_a(GemmArgs g)
 //   D[i][j]: j = l % 16, i = l / 16 + 4*v     (v = 0..3, one double each)
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
+constexpr int TB_MAX = 16;
+
 template <int RB, int NW>
 __global__ __launch_bounds__(NW * 64) void m2l_mfma_f64(M2LArgs a)
 {
     constexpr int NT = NW * 64, W = 16 * NW, R = 16 * RB, KC = 16;
     constexpr int AS = R + 2, BS = W + 2;  // padded LDS row strides
+    constexpr int AR = (KC * R + NT - 1) / NT, BR = KC * W / NT;
     __shared__ double As[KC * AS];
     __shared__ double Bs[KC * BS];
-    __shared__ int s_sid[W];
-    __shared__ int s_tgt[W];
-    __shared__ int s_xyz[W][3];
+    __shared__ int s_sid[189 * TB_MAX];
+    __shared__ short s_dir[189];
+    __shared__ int s_nd;
+    __shared__ int s_tgt[TB_MAX];
+    __shared__ int s_xyz[TB_MAX][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int rf = a.rf, rs = a.rs;
+    const int rf = a.rf, rs = a.rs, TB = a.TB;
+    const int nch = (rs + KC - 1) / KC;
     const int tile = blockIdx.x;
     int o = 0;
     while (o < 7 && tile >= a.tile_prefix[o + 1]) o++;
-    const int64_t first = a.class_start[o] + (int64_t)(tile - a.tile_prefix[o]) * a.TB;
+    const int64_t first = a.class_start[o] + (int64_t)(tile - a.tile_prefix[o]) * TB;
     const int64_t cend = a.class_start[o + 1];
     const int c0 = blockIdx.y * a.cpt;
     const int cpt = min(a.cpt, a.ncols - c0);
     const int S = a.side;
-    if (tid < a.TB) {
+    if (tid < TB) {
         int64_t q = first + tid;
         int t = q < cend ? a.order[q] : -1;
         s_tgt[tid] = t;
@@ -352,70 +374,94 @@ __global__ __launch_bounds__(NW * 64) void m2l_mfma_f64(M2LArgs a)
             s_xyz[tid][2] = (c >> 20) & 1023;
         }
     }
+    __syncthreads();
+    for (int q = tid; q < 189 * TB; q += NT) {
+        const int k = q / TB, tb = q - k * TB;
+        int sid = -1;
+        if (s_tgt[tb] >= 0) {
+            int sx = s_xyz[tb][0] + c_dirs[o][k][0], sy = s_xyz[tb][1] + c_dirs[o][k][1],
+                sz = s_xyz[tb][2] + c_dirs[o][k][2];
+            if (sx >= 0 && sy >= 0 && sz >= 0 && sx < S && sy < S && sz < S)
+                sid = a.map[((int64_t)sx * S + sy) * S + sz];
+        }
+        s_sid[k * TB_MAX + tb] = sid;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int nd = 0;
+        for (int k = 0; k < 189; k++) {
+            bool any = false;
+            for (int tb = 0; tb < TB; tb++) any |= s_sid[k * TB_MAX + tb] >= 0;
+            if (any) s_dir[nd++] = (short)k;
+        }
+        s_nd = nd;
+    }
+    __syncthreads();
+    const int nsteps = s_nd * nch;
+
+    double ra[AR], rb[BR];
+    auto load = [&](int step) {
+        const int k = s_dir[step / nch], k0 = (step - (step / nch) * nch) * KC;
+        const double *Ed = a.E + (size_t)c_dirs[o][k][3] * rf * rs;
+#pragma unroll
+        for (int r = 0; r < AR; r++) {
+            const int q = tid + r * NT, row = q % R, kk = q / R, gk = k0 + kk;
+            ra[r] = (q < KC * R && row < rf && gk < rs) ? Ed[row + (size_t)gk * rf] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < BR; r++) {
+            const int q = tid + r * NT, kk = q % KC, col = q / KC, gk = k0 + kk;
+            const int tb = col / cpt, cl = col - tb * cpt;
+            double v = 0.0;
+            if (tb < TB && gk < rs) {
+                const int sid = s_sid[k * TB_MAX + tb];
+                if (sid >= 0) v = a.P[((size_t)(a.off + sid) * a.ncols + c0 + cl) * rs + gk];
+            }
+            rb[r] = v;
+        }
+    };
     dbl4 acc[RB];
 #pragma unroll
     for (int i = 0; i < RB; i++) acc[i] = dbl4{0, 0, 0, 0};
-    __syncthreads();
 
     const int fr = lane % 16, fk = lane / 16;
-    for (int k = 0; k < 189; k++) {
-        const int dx = c_dirs[o][k][0], dy = c_dirs[o][k][1], dz = c_dirs[o][k][2];
-        const int eidx = c_dirs[o][k][3];
-        int any = 0;
-        if (tid < a.TB) {
-            int sid = -1;
-            if (s_tgt[tid] >= 0) {
-                int sx = s_xyz[tid][0] + dx, sy = s_xyz[tid][1] + dy, sz = s_xyz[tid][2] + dz;
-                if (sx >= 0 && sy >= 0 && sz >= 0 && sx < S && sy < S && sz < S)
-                    sid = a.map[((int64_t)sx * S + sy) * S + sz];
-            }
-            s_sid[tid] = sid;
-            any = sid >= 0;
+    if (nsteps > 0) load(0);
+    for (int step = 0; step < nsteps; step++) {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < AR; r++) {
+            const int q = tid + r * NT;
+            if (q < KC * R) As[(q / R) * AS + q % R] = ra[r];
         }
-        if (!__syncthreads_or(any)) continue;
-        const double *Ed = a.E + (size_t)eidx * rf * rs;
-        for (int k0 = 0; k0 < rs; k0 += KC) {
-            for (int q = tid; q < KC * R; q += NT) {
-                int row = q % R, kk = q / R;
-                int gk = k0 + kk;
-                As[kk * AS + row] = (row < rf && gk < rs) ? Ed[row + (size_t)gk * rf] : 0.0;
-            }
-            for (int q = tid; q < KC * W; q += NT) {
-                int kk = q % KC, col = q / KC;
-                int tb = col / cpt, cl = col - tb * cpt;
-                int gk = k0 + kk;
-                double v = 0.0;
-                if (tb < a.TB && gk < rs) {
-                    int sid = s_sid[tb];
-                    if (sid >= 0) v = a.P[((size_t)(a.off + sid) * a.ncols + c0 + cl) * rs + gk];
-                }
-                Bs[kk * BS + col] = v;
-            }
-            __syncthreads();
 #pragma unroll
-            for (int ks = 0; ks < KC / 4; ks++) {
-                const double b = Bs[(ks * 4 + fk) * BS + wave * 16 + fr];
+        for (int r = 0; r < BR; r++) {
+            const int q = tid + r * NT;
+            Bs[(q % KC) * BS + q / KC] = rb[r];
+        }
+        __syncthreads();
+        if (step + 1 < nsteps) load(step + 1);
 #pragma unroll
-                for (int rb = 0; rb < RB; rb++) {
-                    const double av = As[(ks * 4 + fk) * AS + rb * 16 + fr];
-                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b, acc[rb], 0, 0, 0);
-                }
+        for (int ks = 0; ks < KC / 4; ks++) {
+            const double b = Bs[(ks * 4 + fk) * BS + wave * 16 + fr];
+#pragma unroll
+            for (int rb_ = 0; rb_ < RB; rb_++) {
+                const double av = As[(ks * 4 + fk) * AS + rb_ * 16 + fr];
+                acc[rb_] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b, acc[rb_], 0, 0, 0);
             }
-            __syncthreads();
         }
     }
     const int col = wave * 16 + fr;
     const int tb = col / cpt, cl = col - tb * cpt;
-    if (tb < a.TB) {
+    if (tb < TB) {
         const int t = s_tgt[tb];
         if (t >= 0) {
             double *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + cl) * rf;
 #pragma unroll
-            for (int rb = 0; rb < RB; rb++)
+            for (int rb_ = 0; rb_ < RB; rb_++)
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    const int row = rb * 16 + fk + 4 * v;
-                    if (row < rf) dst[row] = acc[rb][v];
+                    const int row = rb_ * 16 + fk + 4 * v;
+                    if (row < rf) dst[row] = acc[rb_][v];
                 }
         }
     }
@@ -431,7 +477,7 @@ __global__ __launch_bounds__(NW * 64) void m2l_mfma_f64(M2LArgs a)
 // streamed in 64-point chunks; each chunk's kernel values K(t_i, s_j) are
 // evaluated once into LDS (targets in passes of up to 16), then lane
 // (i, g) accumulates columns g, g+G, ... of target i (G = 64 / TT).
-constexpr int NV_SRC = 64;  // source points per chunk
+constexpr int NV_SRC = 32;  // source points per chunk (LDS per wave bounds occupancy)
 constexpr int NV_TGT = 16;  // targets per pass
 constexpr int NV_COL = 32;  // columns per pass
 
@@ -442,6 +488,7 @@ __global__ __launch_bounds__(64) void near_fp64(NearArgs a)
     __shared__ double sw[NV_COL][NV_SRC + 1];
     __shared__ double sx[NV_SRC], sy[NV_SRC], sz[NV_SRC];
     __shared__ double tx[NV_TGT], ty[NV_TGT], tz[NV_TGT];
+    __shared__ double sS[3][16][NV_TGT];  // S_m of each target coordinate
     __shared__ int s_start[27], s_pre[28];
     const int b = blockIdx.x, lane = threadIdx.x;
     const int n = a.n, n2 = n * n, n3 = n2 * n;
@@ -480,15 +527,17 @@ __global__ __launch_bounds__(64) void near_fp64(NearArgs a)
                 tz[lane] = a.z[t0 + i0 + lane];
             }
             __syncthreads();
+            if (a.F != nullptr && lane < 3 * tn) {
+                const int ti = lane / 3, ax = lane % 3;
+                const double xv = ax == 0 ? tx[ti] : ax == 1 ? ty[ti] : tz[ti];
+                sn_eval_to(n, a.Tkz, ihalfL * (xv - cen[ax]), &sS[ax][0][ti], NV_TGT);
+            }
+            __syncthreads();
             double acc[NV_COL / 4];
 #pragma unroll
             for (int k = 0; k < NV_COL / 4; k++) acc[k] = 0;
             // L2P (compute.hpp:594-648)
             if (a.F != nullptr && active) {
-                double Sx[16], Sy[16], Sz[16];
-                sn_eval(n, a.Tkz, ihalfL * (tx[i] - cen[0]), Sx);
-                sn_eval(n, a.Tkz, ihalfL * (ty[i] - cen[1]), Sy);
-                sn_eval(n, a.Tkz, ihalfL * (tz[i] - cen[2]), Sz);
                 const double *Fb = a.F + ((size_t)(a.off + b) * a.ncols + c0) * n3;
 #pragma unroll
                 for (int k = 0; k < NV_COL / 4; k++) {
@@ -499,8 +548,8 @@ __global__ __launch_bounds__(64) void near_fp64(NearArgs a)
                     int l = 0;
                     for (int l1 = 0; l1 < n; l1++)
                         for (int l2 = 0; l2 < n; l2++) {
-                            const double t2 = Sx[l1] * Sy[l2];
-                            for (int l3 = 0; l3 < n; l3++) sum += F[l++] * t2 * Sz[l3];
+                            const double t2 = sS[0][l1][i] * sS[1][l2][i];
+                            for (int l3 = 0; l3 < n; l3++) sum += F[l++] * t2 * sS[2][l3][i];
                         }
                     acc[k] += sum;
                 }
@@ -549,6 +598,220 @@ __global__ __launch_bounds__(64) void near_fp64(NearArgs a)
                     if (cl < cn) a.out[(int64_t)(c0 + cl) * a.N + oi] += acc[k];
                 }
             }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Position gradients of the FMM matvec (no reference counterpart: the
+// reference has no derivatives).  For each target i of a leaf:
+//   g_i += sum_c v_ic [ sum_l F_c[l] grad S_l(x_i) + sum_{j near} grad_1 K(x_i, x_j) u_jc ]
+// where grad S_l is the derivative of the tensor-product Chebyshev
+// interpolant (S_m'(u) = (2/n) sum_k T_k(x_m) k U_{k-1}(u), times 2/len) and
+// grad_1 K the kernel's gradient in its first (target) argument.  Called
+// with (v, F, u) = (dL/dout, forward leaf expansions, weights) for the
+// target side and (weights, transposed-FMM leaf expansions, dL/dout) for the
+// source side (symmetric kernels: d/dy K(x, y) = grad_1 K(y, x)).
+template <int KID>
+__device__ __forceinline__ void kgrad(double dx, double dy, double dz, const KParams &p, double &gx, double &gy,
+                                      double &gz)
+{
+    // d = source - target; returns grad wrt the target position
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    double s = 0.0;  // grad = s * d
+    if (r2 == 0.0) {
+        gx = gy = gz = 0.0;
+        return;
+    }
+    if constexpr (KID == PBB_KERNEL_LAPLACIAN) { double ri = 1.0 / sqrt(r2); s = ri * ri * ri; }
+    else if constexpr (KID == PBB_KERNEL_ONEOVERR2) { s = 2.0 / (r2 * r2); }
+    else if constexpr (KID == PBB_KERNEL_GAUSSIAN) { double a2 = p.p[0] * p.p[0]; s = 2.0 / a2 * exp(-r2 / a2); }
+    else if constexpr (KID == PBB_KERNEL_EXPONENTIAL) { double r = sqrt(r2); s = p.p[1] * exp(-r / p.p[0]) / (p.p[0] * r); }
+    else if constexpr (KID == PBB_KERNEL_SOFT_LAPLACIAN) { double q = 1.0 / sqrt(r2 + p.p[0]); s = q * q * q; }
+    else if constexpr (KID == PBB_KERNEL_QUADRIC || KID == PBB_KERNEL_R2) { s = -2.0; }
+    else if constexpr (KID == PBB_KERNEL_INV_QUADRIC) { double q = 1.0 / (1.0 + r2); s = 2.0 * q * q; }
+    else if constexpr (KID == PBB_KERNEL_LOGARITHM) { s = -1.0 / r2; }
+    else if constexpr (KID == PBB_KERNEL_THIN_PLATE) { s = -(log(r2) + 1.0); }
+    else if constexpr (KID == PBB_KERNEL_EXP_ANISO) {
+        const double rho = sqrt(p.p[1] * dx * dx + p.p[2] * dy * dy + p.p[3] * dz * dz);
+        const double f = rho > 0 ? p.p[0] * exp(-rho) / rho : 0.0;
+        gx = f * p.p[1] * dx; gy = f * p.p[2] * dy; gz = f * p.p[3] * dz;
+        return;
+    }
+    gx = s * dx; gy = s * dy; gz = s * dz;
+}
+
+// S_m(u) and S_m'(u), m < n
+__device__ __forceinline__ void sn_eval_d(int n, const double *__restrict__ Tkz, double u, double *S, double *dS)
+{
+    double T[16], dT[16];
+    T[0] = 1.0; dT[0] = 0.0;
+    if (n > 1) { T[1] = u; dT[1] = 1.0; }
+    double Um2 = 1.0, Um1 = 2.0 * u;  // U_0, U_1
+    for (int k = 2; k < n; k++) {
+        T[k] = 2.0 * u * T[k - 1] - T[k - 2];
+        dT[k] = k * Um1;  // T_k' = k U_{k-1}
+        const double Uk = 2.0 * u * Um1 - Um2;
+        Um2 = Um1;
+        Um1 = Uk;
+    }
+    const double pfac = 2.0 / n;
+    for (int m = 0; m < n; m++) {
+        const double *Tk = Tkz + m * n;
+        double s = 0.5 * Tk[0], d = 0.0;
+        for (int k = 1; k < n; k++) {
+            s += Tk[k] * T[k];
+            d += Tk[k] * dT[k];
+        }
+        S[m] = pfac * s;
+        dS[m] = pfac * d;
+    }
+}
+
+template <int KID>
+__global__ __launch_bounds__(64) void near_grad_fp64(NearGradArgs a)
+{
+    __shared__ double sG[3][NV_TGT][NV_SRC + 1];
+    __shared__ double su[NV_COL][NV_SRC + 1];
+    __shared__ double sx[NV_SRC], sy[NV_SRC], sz[NV_SRC];
+    __shared__ double tx[NV_TGT], ty[NV_TGT], tz[NV_TGT];
+    __shared__ double red[3][64];
+    __shared__ double sS[6][16][NV_TGT];  // S_m and S_m' of each target coordinate
+    __shared__ int s_start[27], s_pre[28];
+    const int b = blockIdx.x, lane = threadIdx.x;
+    const int n = a.n, n2 = n * n, n3 = n2 * n;
+    const int t0 = a.leaf_start[b], nt = a.leaf_start[b + 1] - t0;
+    const uint32_t cc = (uint32_t)a.coord[b];
+    const int ix = cc & 1023, iy = (cc >> 10) & 1023, iz = (cc >> 20) & 1023;
+    const int S = a.side;
+    const double cen[3] = {a.ctab[ix], a.ctab[iy], a.ctab[iz]};
+    const double ihalfL = 2.0 / a.len;
+    if (lane < 27) {
+        int x = ix + lane / 9 - 1, y = iy + (lane / 3) % 3 - 1, z = iz + lane % 3 - 1;
+        int sb = (x >= 0 && y >= 0 && z >= 0 && x < S && y < S && z < S) ? a.map[((int64_t)x * S + y) * S + z] : -1;
+        s_start[lane] = sb >= 0 ? a.leaf_start[sb] : 0;
+        s_pre[lane + 1] = sb >= 0 ? a.leaf_start[sb + 1] - a.leaf_start[sb] : 0;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        s_pre[0] = 0;
+        for (int m = 0; m < 27; m++) s_pre[m + 1] += s_pre[m];
+    }
+    __syncthreads();
+    const int ns = s_pre[27];
+    for (int i0 = 0; i0 < nt; i0 += NV_TGT) {
+        const int tn = min(NV_TGT, nt - i0);
+        const int TT = tn <= 1 ? 1 : tn <= 2 ? 2 : tn <= 4 ? 4 : tn <= 8 ? 8 : 16;
+        const int G = 64 / TT;
+        const int i = lane / G, g = lane % G;
+        const bool active = i < tn;
+        __syncthreads();
+        if (lane < tn) {
+            tx[lane] = a.x[t0 + i0 + lane];
+            ty[lane] = a.y[t0 + i0 + lane];
+            tz[lane] = a.z[t0 + i0 + lane];
+        }
+        __syncthreads();
+        if (a.F != nullptr && lane < 3 * tn) {
+            const int ti = lane / 3, ax = lane % 3;
+            const double xv = ax == 0 ? tx[ti] : ax == 1 ? ty[ti] : tz[ti];
+            double Sv[16], dSv[16];
+            sn_eval_d(n, a.Tkz, ihalfL * (xv - cen[ax]), Sv, dSv);
+            for (int m = 0; m < n; m++) {
+                sS[ax][m][ti] = Sv[m];
+                sS[3 + ax][m][ti] = dSv[m];
+            }
+        }
+        __syncthreads();
+        double gx = 0, gy = 0, gz = 0;
+        for (int c0 = 0; c0 < a.ncols; c0 += NV_COL) {
+            const int cn = min(NV_COL, a.ncols - c0);
+            double vv[NV_COL / 4];
+#pragma unroll
+            for (int k = 0; k < NV_COL / 4; k++) {
+                const int cl = g + G * k;
+                vv[k] = (active && cl < cn) ? a.v[(int64_t)(c0 + cl) * a.N + t0 + i0 + i] : 0.0;
+            }
+            if (a.F != nullptr && active) {
+                const double *Sx = &sS[0][0][i], *Sy = &sS[1][0][i], *Sz = &sS[2][0][i];
+                const double *dSx = &sS[3][0][i], *dSy = &sS[4][0][i], *dSz = &sS[5][0][i];
+                const double *Fb = a.F + ((size_t)b * a.ncols + c0) * n3;
+#pragma unroll
+                for (int k = 0; k < NV_COL / 4; k++) {
+                    const int cl = g + G * k;
+                    if (cl >= cn) break;
+                    const double *F = Fb + (size_t)cl * n3;
+                    double ax = 0, ay = 0, az = 0;
+                    int l = 0;
+                    for (int l1 = 0; l1 < n; l1++)
+                        for (int l2 = 0; l2 < n; l2++)
+                            for (int l3 = 0; l3 < n; l3++) {
+                                const double f = F[l++];
+                                const int o1 = l1 * NV_TGT, o2 = l2 * NV_TGT, o3 = l3 * NV_TGT;
+                                ax += f * dSx[o1] * Sy[o2] * Sz[o3];
+                                ay += f * Sx[o1] * dSy[o2] * Sz[o3];
+                                az += f * Sx[o1] * Sy[o2] * dSz[o3];
+                            }
+                    gx += vv[k] * ax * ihalfL;
+                    gy += vv[k] * ay * ihalfL;
+                    gz += vv[k] * az * ihalfL;
+                }
+            }
+            for (int j0 = 0; j0 < ns; j0 += NV_SRC) {
+                const int jn = min(NV_SRC, ns - j0);
+                __syncthreads();
+                if (lane < jn) {
+                    const int q = j0 + lane;
+                    int m = 0;
+                    while (s_pre[m + 1] <= q) m++;
+                    const int gi = s_start[m] + q - s_pre[m];
+                    sx[lane] = a.x[gi];
+                    sy[lane] = a.y[gi];
+                    sz[lane] = a.z[gi];
+                    for (int c = 0; c < cn; c++) su[c][lane] = a.u[(int64_t)(c0 + c) * a.N + gi];
+                }
+                __syncthreads();
+                for (int q = lane; q < TT * NV_SRC; q += 64) {
+                    const int ii = q / NV_SRC, jj = q % NV_SRC;
+                    double kx = 0, ky = 0, kz = 0;
+                    if (ii < tn && jj < jn) kgrad<KID>(sx[jj] - tx[ii], sy[jj] - ty[ii], sz[jj] - tz[ii], a.kp, kx, ky, kz);
+                    sG[0][ii][jj] = kx;
+                    sG[1][ii][jj] = ky;
+                    sG[2][ii][jj] = kz;
+                }
+                __syncthreads();
+                if (active) {
+                    for (int jj = 0; jj < jn; jj++) {
+                        double w = 0;
+#pragma unroll
+                        for (int k = 0; k < NV_COL / 4; k++) {
+                            const int cl = g + G * k;
+                            if (cl < cn) w += vv[k] * su[cl][jj];
+                        }
+                        gx += w * sG[0][i][jj];
+                        gy += w * sG[1][i][jj];
+                        gz += w * sG[2][i][jj];
+                    }
+                }
+            }
+        }
+        // reduce the G lanes of each target
+        __syncthreads();
+        red[0][lane] = gx;
+        red[1][lane] = gy;
+        red[2][lane] = gz;
+        __syncthreads();
+        if (lane < tn) {
+            double sxg = 0, syg = 0, szg = 0;
+            for (int q = 0; q < G; q++) {
+                sxg += red[0][lane * G + q];
+                syg += red[1][lane * G + q];
+                szg += red[2][lane * G + q];
+            }
+            const int64_t oi = a.perm[t0 + i0 + lane];
+            a.grad[3 * oi] += sxg;
+            a.grad[3 * oi + 1] += syg;
+            a.grad[3 * oi + 2] += szg;
         }
     }
 }
@@ -737,6 +1000,40 @@ void launch_direct(const DirectArgs &a, int kid, hipStream_t st)
     int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
     direct_reduce<<<blocks, 256, 0, st>>>(a.part, a.nsplit, total, a.out);
     check(hipGetLastError(), "direct_reduce");
+}
+
+void launch_near_grad(const NearGradArgs &a, int kid, int nleaf, hipStream_t st)
+{
+    if (nleaf <= 0) return;
+#define PBB_GRAD_CASE(K)                                  \
+    case K:                                               \
+        near_grad_fp64<K><<<nleaf, 64, 0, st>>>(a);       \
+        break;
+    switch (kid) {
+        PBB_GRAD_CASE(PBB_KERNEL_LAPLACIAN) PBB_GRAD_CASE(PBB_KERNEL_ONEOVERR2) PBB_GRAD_CASE(PBB_KERNEL_GAUSSIAN)
+        PBB_GRAD_CASE(PBB_KERNEL_EXPONENTIAL) PBB_GRAD_CASE(PBB_KERNEL_SOFT_LAPLACIAN)
+        PBB_GRAD_CASE(PBB_KERNEL_QUADRIC) PBB_GRAD_CASE(PBB_KERNEL_INV_QUADRIC)
+        PBB_GRAD_CASE(PBB_KERNEL_LOGARITHM) PBB_GRAD_CASE(PBB_KERNEL_THIN_PLATE) PBB_GRAD_CASE(PBB_KERNEL_R2)
+        PBB_GRAD_CASE(PBB_KERNEL_CONSTANT) PBB_GRAD_CASE(PBB_KERNEL_EXP_ANISO)
+    default: fail(PBB_ERR_UNSUPPORTED, "no device functor for this kernel id");
+    }
+#undef PBB_GRAD_CASE
+    check(hipGetLastError(), "near_grad_fp64");
+}
+
+// out[0..n) = 0, as a kernel so that it can sit in a captured graph
+__global__ void fill_zero_f64(double *__restrict__ out, int64_t n)
+{
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        out[q] = 0.0;
+}
+
+void launch_fill_zero(double *out, int64_t n, hipStream_t st)
+{
+    if (n <= 0) return;
+    int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+    fill_zero_f64<<<blocks, 256, 0, st>>>(out, n);
+    check(hipGetLastError(), "fill_zero_f64");
 }
 
 }  // namespace dev

@@ -92,7 +92,22 @@ struct M2LMixArgs {
     float *Pf;
 };
 
+struct NearGradArgs {
+    int n, ncols, side;
+    int64_t N;
+    double len;
+    KParams kp;
+    const int32_t *leaf_start, *coord, *map, *perm;
+    const double *ctab, *Tkz;
+    const double *x, *y, *z;
+    const double *u, *v;        // sorted source weights / target coefficients (column-major N)
+    const double *F;            // leaf expansions [leaf][col][n3] (nullptr: none)
+    double *grad;               // AoS N x 3, caller's order, accumulated
+};
+
 void upload_dirs(const int dirs[8][189][4]);
+void launch_near_grad(const NearGradArgs &a, int kid, int nleaf, hipStream_t st);
+void launch_fill_zero(double *out, int64_t n, hipStream_t st);
 void launch_split_e(int variant, const double *E, int rf, int rs, int R, int nch, int escale, void *Es,
                     hipStream_t st);
 void launch_absmax(const float *x, int64_t n, unsigned *out, hipStream_t st);

@@ -90,23 +90,35 @@ __global__ void absmax_f32(const float *__restrict__ x, int64_t n, unsigned *__r
 }
 
 // ---------------------------------------------------------------------------
+// One workgroup = 4 waves owns TB (<= 16) target boxes of one octant class x
+// cpt columns (W = 64*NCB columns); each wave holds NCB 16-column blocks x
+// RB 16-row blocks of FP32 accumulators.  The (direction, 32-deep k chunk)
+// steps of the tile's non-empty directions form one software pipeline: the
+// next step's E chunk (pre-split) and gathered FP32 proxies are loaded into
+// registers while the current step's MFMAs run from LDS.
+constexpr int TB_MAX = 16;
+
 template <typename T, int NSPLIT, int RB, int NCB>
-__global__ __launch_bounds__(256) void m2l_mfma_split(M2LMixArgs a)
+__global__ __launch_bounds__(256, 2) void m2l_mfma_split(M2LMixArgs a)
 {
     typedef typename Vec8<T>::type V8;
-    constexpr int NW = 4, NT = 256, W = 16 * NW * NCB, R = 16 * RB, KS = 40;  // KS: LDS row stride
-    constexpr int NPROD = NSPLIT == 3 ? 6 : 3;
+    constexpr int NT = 256, W = 64 * NCB, R = 16 * RB, KS = 40, BS = 36;
+    constexpr int NA = NSPLIT * R * 4;        // 16-byte units of one E chunk
+    constexpr int AR = (NA + NT - 1) / NT;
+    constexpr int BR = W * 8 / NT;            // float4 units of one proxy chunk per thread
     __shared__ __attribute__((aligned(16))) T As[NSPLIT * R * KS];
-    __shared__ __attribute__((aligned(16))) T Bs[NSPLIT * W * KS];
-    __shared__ int s_sid[W];
-    __shared__ int s_tgt[W];
-    __shared__ int s_xyz[W][3];
+    __shared__ __attribute__((aligned(16))) float Bs[W * BS];
+    __shared__ int s_sid[189 * TB_MAX];
+    __shared__ short s_dir[189];
+    __shared__ int s_nd;
+    __shared__ int s_tgt[TB_MAX];
+    __shared__ int s_xyz[TB_MAX][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int rf = a.rf;
+    const int rf = a.rf, TB = a.TB, nch = a.nch;
     const int tile = blockIdx.x;
     int o = 0;
     while (o < 7 && tile >= a.tile_prefix[o + 1]) o++;
-    const int64_t first = a.class_start[o] + (int64_t)(tile - a.tile_prefix[o]) * a.TB;
+    const int64_t first = a.class_start[o] + (int64_t)(tile - a.tile_prefix[o]) * TB;
     const int64_t cend = a.class_start[o + 1];
     const int c0 = blockIdx.y * a.cpt;
     const int cpt = min(a.cpt, a.ncols - c0);
@@ -121,8 +133,9 @@ __global__ __launch_bounds__(256) void m2l_mfma_split(M2LMixArgs a)
             xs = 12 - e;
         }
     }
+    const float xscale = ldexpf(1.0f, xs);
     const float unscale = ldexpf(1.0f, -(xs + a.escale));
-    if (tid < a.TB) {
+    if (tid < TB) {
         int64_t q = first + tid;
         int t = q < cend ? a.order[q] : -1;
         s_tgt[tid] = t;
@@ -133,93 +146,121 @@ __global__ __launch_bounds__(256) void m2l_mfma_split(M2LMixArgs a)
             s_xyz[tid][2] = (c >> 20) & 1023;
         }
     }
+    __syncthreads();
+    // source box of every (direction, target) pair of the tile
+    for (int q = tid; q < 189 * TB; q += NT) {
+        const int k = q / TB, tb = q - k * TB;
+        int sid = -1;
+        if (s_tgt[tb] >= 0) {
+            int sx = s_xyz[tb][0] + c_dirs[o][k][0], sy = s_xyz[tb][1] + c_dirs[o][k][1],
+                sz = s_xyz[tb][2] + c_dirs[o][k][2];
+            if (sx >= 0 && sy >= 0 && sz >= 0 && sx < S && sy < S && sz < S)
+                sid = a.map[((int64_t)sx * S + sy) * S + sz];
+        }
+        s_sid[k * TB_MAX + tb] = sid;
+    }
+    __syncthreads();
+    if (tid == 0) {  // non-empty directions, in class order
+        int nd = 0;
+        for (int k = 0; k < 189; k++) {
+            bool any = false;
+            for (int tb = 0; tb < TB; tb++) any |= s_sid[k * TB_MAX + tb] >= 0;
+            if (any) s_dir[nd++] = (short)k;
+        }
+        s_nd = nd;
+    }
+    __syncthreads();
+    const int nsteps = s_nd * nch;
+
+    uint4 ra[AR];
+    float4 rb[BR];
+    auto load = [&](int step) {
+        const int k = s_dir[step / nch], ch = step - (step / nch) * nch;
+        const uint4 *src = (const uint4 *)((const T *)a.Es + ((size_t)c_dirs[o][k][3] * nch + ch) * NSPLIT * R * 32);
+#pragma unroll
+        for (int r = 0; r < AR; r++) {
+            const int u = tid + r * NT;
+            ra[r] = u < NA ? src[u] : uint4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int r = 0; r < BR; r++) {
+            const int u = tid + r * NT, col = u / 8, k4 = u % 8;
+            const int tb = col / cpt, cl = col - tb * cpt;
+            float4 v = {0, 0, 0, 0};
+            if (tb < TB) {
+                const int sid = s_sid[k * TB_MAX + tb];
+                if (sid >= 0)
+                    v = *(const float4 *)(a.P + ((size_t)(a.off + sid) * a.ncols + c0 + cl) * a.kpad + ch * 32 + 4 * k4);
+            }
+            rb[r] = v;
+        }
+    };
     f32x4 acc[RB][NCB];
 #pragma unroll
     for (int i = 0; i < RB; i++)
 #pragma unroll
         for (int j = 0; j < NCB; j++) acc[i][j] = f32x4{0, 0, 0, 0};
-    __syncthreads();
 
     const int fr = lane % 16, fq = lane / 16;
-    for (int k = 0; k < 189; k++) {
-        const int dx = c_dirs[o][k][0], dy = c_dirs[o][k][1], dz = c_dirs[o][k][2];
-        const int eidx = c_dirs[o][k][3];
-        int any = 0;
-        if (tid < a.TB) {
-            int sid = -1;
-            if (s_tgt[tid] >= 0) {
-                int sx = s_xyz[tid][0] + dx, sy = s_xyz[tid][1] + dy, sz = s_xyz[tid][2] + dz;
-                if (sx >= 0 && sy >= 0 && sz >= 0 && sx < S && sy < S && sz < S)
-                    sid = a.map[((int64_t)sx * S + sy) * S + sz];
+    if (nsteps > 0) load(0);
+    for (int step = 0; step < nsteps; step++) {
+        __syncthreads();  // every wave is done with the previous step's LDS
+#pragma unroll
+        for (int r = 0; r < AR; r++) {
+            const int u = tid + r * NT;
+            if (u < NA) {
+                const int p = u / (R * 4), row = (u / 4) % R, q = u % 4;
+                *(uint4 *)&As[(p * R + row) * KS + 8 * q] = ra[r];
             }
-            s_sid[tid] = sid;
-            any = sid >= 0;
         }
-        if (!__syncthreads_or(any)) continue;
-        const T *Ed = (const T *)a.Es + (size_t)eidx * a.nch * NSPLIT * R * 32;
-        for (int ch = 0; ch < a.nch; ch++) {
-            // A: the chunk's NSPLIT x R x 32 block, 16 bytes (8 elements) per load
-            const uint4 *src = (const uint4 *)(Ed + (size_t)ch * NSPLIT * R * 32);
-            for (int u = tid; u < NSPLIT * R * 4; u += NT) {
-                int p = u / (R * 4), row = (u / 4) % R, q = u % 4;
-                *(uint4 *)&As[(p * R + row) * KS + 8 * q] = src[u];
-            }
-            // B: gathered FP32 proxies, split on the fly
-            for (int u = tid; u < W * 8; u += NT) {
-                int col = u / 8, k4 = u % 8;
-                int tb = col / cpt, cl = col - tb * cpt;
-                float4 v = {0, 0, 0, 0};
-                if (tb < a.TB) {
-                    int sid = s_sid[tb];
-                    if (sid >= 0)
-                        v = *(const float4 *)(a.P + ((size_t)(a.off + sid) * a.ncols + c0 + cl) * a.kpad + ch * 32 + 4 * k4);
+#pragma unroll
+        for (int r = 0; r < BR; r++) {
+            const int u = tid + r * NT;
+            *(float4 *)&Bs[(u / 8) * BS + 4 * (u % 8)] = rb[r];
+        }
+        __syncthreads();
+        if (step + 1 < nsteps) load(step + 1);  // in flight during the MFMAs
+        V8 bfr[NCB][NSPLIT];
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++) {
+            const float *bp = &Bs[((wave * NCB + cb) * 16 + fr) * BS + 8 * fq];
+            const float4 v0 = *(const float4 *)bp, v1 = *(const float4 *)(bp + 4);
+            float x[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+            for (int e = 0; e < 8; e++) x[e] *= xscale;
+#pragma unroll
+            for (int p = 0; p < NSPLIT; p++)
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const T h = (T)x[e];
+                    bfr[cb][p][e] = h;
+                    x[e] -= (float)h;
                 }
-                float x[4] = {ldexpf(v.x, xs), ldexpf(v.y, xs), ldexpf(v.z, xs), ldexpf(v.w, xs)};
+        }
 #pragma unroll
-                for (int p = 0; p < NSPLIT; p++) {
-                    T h[4];
+        for (int rb = 0; rb < RB; rb++) {
+            V8 afr[NSPLIT];
 #pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        h[e] = (T)x[e];
-                        x[e] -= (float)h[e];
-                    }
-                    T *dst = &Bs[(p * W + col) * KS + 4 * k4];
-                    dst[0] = h[0]; dst[1] = h[1]; dst[2] = h[2]; dst[3] = h[3];
+            for (int p = 0; p < NSPLIT; p++) afr[p] = *(const V8 *)&As[(p * R + rb * 16 + fr) * KS + 8 * fq];
+#pragma unroll
+            for (int cb = 0; cb < NCB; cb++) {
+                // smallest terms first
+                if constexpr (NSPLIT == 3) {
+                    acc[rb][cb] = mfma16(afr[2], bfr[cb][0], acc[rb][cb]);
+                    acc[rb][cb] = mfma16(afr[1], bfr[cb][1], acc[rb][cb]);
+                    acc[rb][cb] = mfma16(afr[0], bfr[cb][2], acc[rb][cb]);
                 }
+                acc[rb][cb] = mfma16(afr[1], bfr[cb][0], acc[rb][cb]);
+                acc[rb][cb] = mfma16(afr[0], bfr[cb][1], acc[rb][cb]);
+                acc[rb][cb] = mfma16(afr[0], bfr[cb][0], acc[rb][cb]);
             }
-            __syncthreads();
-            V8 bfr[NCB][NSPLIT];
-#pragma unroll
-            for (int cb = 0; cb < NCB; cb++)
-#pragma unroll
-                for (int p = 0; p < NSPLIT; p++)
-                    bfr[cb][p] = *(const V8 *)&Bs[(p * W + (wave * NCB + cb) * 16 + fr) * KS + 8 * fq];
-#pragma unroll
-            for (int rb = 0; rb < RB; rb++) {
-                V8 afr[NSPLIT];
-#pragma unroll
-                for (int p = 0; p < NSPLIT; p++) afr[p] = *(const V8 *)&As[(p * R + rb * 16 + fr) * KS + 8 * fq];
-#pragma unroll
-                for (int cb = 0; cb < NCB; cb++) {
-                    // smallest terms first
-                    if constexpr (NPROD == 6) {
-                        acc[rb][cb] = mfma16(afr[2], bfr[cb][0], acc[rb][cb]);
-                        acc[rb][cb] = mfma16(afr[1], bfr[cb][1], acc[rb][cb]);
-                        acc[rb][cb] = mfma16(afr[0], bfr[cb][2], acc[rb][cb]);
-                    }
-                    acc[rb][cb] = mfma16(afr[1], bfr[cb][0], acc[rb][cb]);
-                    acc[rb][cb] = mfma16(afr[0], bfr[cb][1], acc[rb][cb]);
-                    acc[rb][cb] = mfma16(afr[0], bfr[cb][0], acc[rb][cb]);
-                }
-            }
-            __syncthreads();
         }
     }
 #pragma unroll
     for (int cb = 0; cb < NCB; cb++) {
         const int col = (wave * NCB + cb) * 16 + fr;
         const int tb = col / cpt, cl = col - tb * cpt;
-        if (tb >= a.TB) continue;
+        if (tb >= TB) continue;
         const int t = s_tgt[tb];
         if (t < 0) continue;
         float *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + cl) * rf;
