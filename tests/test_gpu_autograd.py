@@ -18,13 +18,17 @@ def _loss(xyz, sigma, b, ops):
     return 0.5 * ((out - b) ** 2).sum()
 
 
-def _safe_coords(xyz, level, h, k, rng):
-    """k (point, axis) pairs >= 2h from every leaf-box face and not a bbox
-    extreme (box membership and the centre stay fixed under +-h)."""
+def _safe_coords(xyz, level, h, k, rng, rmin=0.02):
+    """k (point, axis) pairs >= 2h from every leaf-box face, not a bbox
+    extreme (box membership and the centre stay fixed under +-h), and with
+    no other point within rmin (the finite difference resolves the kernel)."""
     ctr = 0.5 * (xyz.min(0) + xyz.max(0))
     rel = (xyz - ctr + 0.5) * 2 ** level
     dist = np.abs(rel - np.round(rel)) / 2 ** level
     ok = dist > 2 * h
+    from scipy.spatial import cKDTree
+    dnn, _ = cKDTree(xyz).query(xyz, k=2)
+    ok &= (dnn[:, 1] >= rmin)[:, None]
     for d in range(3):
         ok[np.argmin(xyz[:, d]), d] = False
         ok[np.argmax(xyz[:, d]), d] = False
@@ -52,17 +56,22 @@ def test_position_gradient_vs_finite_differences(gpu, ncols):
         r = plan.matvec(sigma) - b
         gsig = plan.matvec_transpose(r)
     assert float((sig.grad - gsig).norm() / gsig.norm()) < 1e-12
-    # positions: central differences of the FMM loss
+    # positions: central differences of the FMM loss (h = 1e-4), Richardson-
+    # extrapolated with h/2 so that the check resolves ~1e-8
     picks = _safe_coords(xyz0, level, h, 24, rng)
-    errs = []
-    for i, d in picks:
+
+    def central(i, d, hh):
         vals = []
         for sgn in (1, -1):
             xp = xyz0.copy()
-            xp[i, d] += sgn * h
+            xp[i, d] += sgn * hh
             with torch.no_grad():
                 vals.append(float(_loss(torch.tensor(xp), sigma, b, ops)))
-        fd = (vals[0] - vals[1]) / (2 * h)
+        return (vals[0] - vals[1]) / (2 * hh)
+
+    errs = []
+    for i, d in picks:
+        fd = (4 * central(i, d, h / 2) - central(i, d, h)) / 3
         errs.append(abs(fd - gx[i, d]) / max(abs(fd), 1e-3 * np.abs(gx).max()))
     print(f"max relative FD mismatch over {len(picks)} coordinates: {max(errs):.3e}")
     assert max(errs) < 1e-5
