@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--err-targets", type=int, default=10000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
+                    help="first run one device copy of this many GiB (a known-size kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters of profiled runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -108,6 +111,15 @@ def main():
     from pbbfmm3d_amd import _capi as capi
 
     dev = torch.device("cuda:0")
+    if args.calibrate_copy_gib > 0:
+        n = int(args.calibrate_copy_gib * (1 << 30)) // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy of {n * 8} bytes done")
+        del src, dst
     kernel = make_kernel(args.kernel)
     N, nc = args.npoints, args.ncols
     log(f"workload N={N} ncols={nc} level={args.level} order={args.order} eps={args.eps} "

@@ -214,6 +214,9 @@ struct pbb_ops {
 struct DevLevel {
     int64_t nb = 0, off = 0;
     int32_t *coord = nullptr, *map = nullptr, *child = nullptr, *order = nullptr;
+    int32_t *pstart = nullptr;                       // box -> first sorted point (levels >= 2)
+    int32_t *seg_box = nullptr, *seg_p0 = nullptr;   // L2P segments of at most L2P_SEG points
+    int nseg = 0;
     double *ctab = nullptr;
     double len = 0;
     int64_t class_count[8] = {0};
@@ -228,10 +231,13 @@ struct pbb_plan {
     pbb_plan_info info{};
     double *x = nullptr, *y = nullptr, *z = nullptr;
     int32_t *perm = nullptr, *leaf_start = nullptr;
+    int32_t *leaf_class = nullptr;              // leaves grouped by point count
+    int leaf_class_off[dev::NEAR_CLASSES + 1] = {0};
     std::vector<DevLevel> lv;
     int64_t Btot = 0;
     int ws_ncols = 0, ws_mixed = -1;
     double *ws = nullptr, *S = nullptr, *P = nullptr, *Pf = nullptr;
+    double *wsT = nullptr, *lp = nullptr;  // near field: point-major weights, L2P results
     float *P32 = nullptr, *Pf32 = nullptr;
     unsigned *xmax = nullptr;
     double *gF = nullptr, *gU = nullptr;  // position gradients: saved leaf F, sorted weights
@@ -249,11 +255,13 @@ struct pbb_plan {
         (void)hipGetDevice(&prev);
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
-        dfree(x); dfree(y); dfree(z); dfree(perm); dfree(leaf_start);
+        dfree(x); dfree(y); dfree(z); dfree(perm); dfree(leaf_start); dfree(leaf_class);
         for (auto &L : lv) {
             dfree(L.coord); dfree(L.map); dfree(L.child); dfree(L.order); dfree(L.ctab);
+            dfree(L.pstart); dfree(L.seg_box); dfree(L.seg_p0);
         }
         dfree(ws); dfree(S); dfree(P); dfree(Pf); dfree(P32); dfree(Pf32); dfree(xmax); dfree(gF); dfree(gU);
+        dfree(wsT); dfree(lp);
         if (events_made)
             for (auto &e : ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
@@ -267,12 +275,14 @@ struct pbb_plan {
     {
         const HostOps &h = ops->h;
         if (ncols > ws_ncols) {
-            dfree(ws); dfree(S); dfree(P); dfree(Pf); dfree(P32); dfree(Pf32);
-            ws = S = P = Pf = nullptr;
+            dfree(ws); dfree(S); dfree(P); dfree(Pf); dfree(P32); dfree(Pf32); dfree(wsT); dfree(lp);
+            ws = S = P = Pf = wsT = lp = nullptr;
             P32 = Pf32 = nullptr;
             ws_ncols = 0;
             ws_mixed = -1;
             ws = dalloc<double>((size_t)N * ncols, "sorted weights");
+            wsT = dalloc<double>((size_t)N * ncols, "sorted weights, point-major");
+            if (Lv >= 2) lp = dalloc<double>((size_t)N * ncols, "L2P results");
             if (Lv >= 2) S = dalloc<double>((size_t)Btot * ncols * h.n3, "expansions");
             ws_ncols = ncols;
         }
@@ -464,6 +474,23 @@ PBB_API pbb_status pbb_plan_create(pbb_ops *ops, const double *xyz, int64_t npoi
         plan->z = dupload(T.zs, "z");
         plan->perm = dupload(T.perm, "perm");
         plan->leaf_start = dupload(T.leaf_start, "leaf_start");
+        {  // class c: leaves of at most 2 << c points; the last class takes the rest
+            const int64_t nbl = T.lv[T.Lv].nb;
+            std::vector<int32_t> cls[dev::NEAR_CLASSES], all;
+            for (int64_t b = 0; b < nbl; b++) {
+                const int nt = T.leaf_start[b + 1] - T.leaf_start[b];
+                int c = 0;
+                while (c < dev::NEAR_CLASSES - 1 && nt > (2 << c)) c++;
+                cls[c].push_back((int32_t)b);
+            }
+            all.reserve(nbl);
+            for (int c = 0; c < dev::NEAR_CLASSES; c++) {
+                plan->leaf_class_off[c] = (int)all.size();
+                all.insert(all.end(), cls[c].begin(), cls[c].end());
+            }
+            plan->leaf_class_off[dev::NEAR_CLASSES] = (int)all.size();
+            plan->leaf_class = dupload(all, "leaf classes");
+        }
         plan->lv.resize(T.Lv + 1);
         int64_t off = 0;
         for (int l = 0; l <= T.Lv; l++) {
@@ -483,6 +510,33 @@ PBB_API pbb_status pbb_plan_create(pbb_ops *ops, const double *xyz, int64_t npoi
             }
         }
         plan->Btot = off;
+        // point ranges of the boxes of levels 2..L: boxes and points are both
+        // in Morton order, so every box owns a contiguous range
+        for (int l = 2; l <= T.Lv; l++) {
+            const Level &V = T.lv[l], &LL = T.lv[T.Lv];
+            const int sh = T.Lv - l, S = 1 << l;
+            std::vector<int32_t> ps(V.nb + 1, -1), sb, s0;
+            for (int64_t b = 0; b < LL.nb; b++) {
+                const uint32_t c = (uint32_t)LL.coord[b];
+                const int ix = (c & 1023) >> sh, iy = ((c >> 10) & 1023) >> sh, iz = ((c >> 20) & 1023) >> sh;
+                const int32_t a = V.map[((int64_t)ix * S + iy) * S + iz];
+                if (a < 0 || a >= V.nb) fail(PBB_ERR_INTERNAL, "leaf without an ancestor box");
+                if (ps[a] < 0) ps[a] = T.leaf_start[b];
+            }
+            ps[V.nb] = (int32_t)T.N;
+            for (int64_t b = 0; b < V.nb; b++) {
+                if (ps[b] < 0 || (b > 0 && ps[b] < ps[b - 1])) fail(PBB_ERR_INTERNAL, "box point ranges out of order");
+                for (int32_t q = ps[b]; q < ps[b + 1]; q += dev::L2P_SEG) {
+                    sb.push_back((int32_t)b);
+                    s0.push_back(q);
+                }
+            }
+            DevLevel &D = plan->lv[l];
+            D.pstart = dupload(ps, "box point ranges");
+            D.seg_box = dupload(sb, "L2P segments");
+            D.seg_p0 = dupload(s0, "L2P segments");
+            D.nseg = (int)sb.size();
+        }
         int dirs[8][189][4];
         for (int o = 0; o < 8; o++) class_directions(o, h.w.Ktable, dirs[o]);
         dev::upload_dirs(dirs);
@@ -544,6 +598,21 @@ PBB_API pbb_status pbb_plan_timings(pbb_plan *plan, double *ms)
 
 // One FMM application.  out == nullptr stops after the downward pass, leaving
 // the leaf local expansions in p->S and the sorted weights in p->ws.
+static void fill_levels(const pbb_plan *p, dev::LevelTab &t)
+{
+    t.nlev = p->Lv - 1;
+    if (t.nlev > dev::MAX_FAR_LEVELS) fail(PBB_ERR_UNSUPPORTED, "tree too deep for the direct passes");
+    for (int i = 0; i < t.nlev; i++) {
+        const DevLevel &D = p->lv[i + 2];
+        t.off[i] = D.off;
+        t.pstart[i] = D.pstart;
+        t.coord[i] = D.coord;
+        t.ctab[i] = D.ctab;
+        t.len[i] = D.len;
+    }
+    t.off[t.nlev] = p->Btot;
+}
+
 static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, double *out, hipStream_t st, int prec)
 {
     const bool mixed = prec != PBB_PREC_FP64;
@@ -557,10 +626,22 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
     const int ldP = mixed ? p->kpad : h.rs;
     if (mixed && h.rs > p->kpad) fail(PBB_ERR_INTERNAL, "proxy padding too small");
     mark(0);
+    // orders 4..8 run the direct passes: P2M at every level instead of
+    // P2M + M2M, and (with a near field to add into) L2P at every level
+    // instead of L2L + L2P; both are exact rewritings (see p2m_levels)
+    const bool direct = n >= 4 && n <= 8;
+    const bool direct_down = direct && out != nullptr && Lv >= 2;
     dev::launch_gather_weights(w, p->perm, p->N, ncols, p->ws, st);
+    if (direct) dev::launch_gather_weights_t(w, p->perm, p->N, ncols, p->wsT, st);
     const int cc_p2m = std::max(1, std::min({ncols, 64, 4096 / n3}));
     const int cc_m2m = std::max(1, std::min(ncols, 2048 / n3));
-    if (Lv >= 2) {
+    if (Lv >= 2 && direct) {
+        dev::UpArgs a{};
+        a.n = n; a.ncols = ncols;
+        fill_levels(p, a.lt);
+        a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.wsT = p->wsT; a.S = p->S;
+        dev::launch_p2m_levels(a, st);
+    } else if (Lv >= 2) {
         const DevLevel &LL = p->lv[Lv];
         dev::P2MArgs a{};
         a.n = n; a.ncols = ncols; a.cc = cc_p2m; a.N = p->N; a.off = LL.off; a.len = LL.len;
@@ -569,7 +650,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         dev::launch_p2m(a, (int)LL.nb, st);
     }
     mark(1);
-    for (int l = Lv - 1; l >= 2; l--) {
+    for (int l = Lv - 1; l >= 2 && !direct; l--) {
         dev::M2MArgs a{};
         a.n = n; a.ncols = ncols; a.cc = cc_m2m;
         a.off_parent = p->lv[l].off; a.off_child = p->lv[l + 1].off;
@@ -596,16 +677,22 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         }
     }
     mark(3);
-    // M2L tile: W columns = TB target boxes x cpt columns; 128-wide tiles
-    // halve the E traffic per flop when the columns allow it
+    // M2L tiles: W columns = TB target boxes x cpt columns.
+    // FP64: W = 128 (64 for 17..31 columns), matrix-core bound, so few boxes
+    // per tile keep the share of empty (box, direction) slots low.
+    // Mixed: W = 256 with at most 16 columns per box: tiles of 16
+    // neighbouring boxes (2x2x4 in Morton order), whose proxy gathers of
+    // consecutive directions overlap in L2.
     const int cpt = std::min(ncols, 64);
     const int width = ncols >= 32 || ncols <= 16 ? 128 : 64;
-    const int TB = std::max(1, std::min(16, width / cpt));  // at most 16 targets per tile
+    const int TB = std::max(1, std::min(16, width / cpt));
     const int nchunks = (ncols + cpt - 1) / cpt;
+    const int cptm = std::min(ncols, 16);
+    const int nchunks_m = (ncols + cptm - 1) / cptm;
     for (int l = 2; l <= Lv && mixed; l++) {
         const DevLevel &D = p->lv[l];
         dev::M2LMixArgs a{};
-        a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = std::max(1, std::min(16, 128 / cpt)); a.cpt = cpt;
+        a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = std::max(1, std::min(16, 256 / cptm)); a.cpt = cptm;
         a.side = 1 << l;
         a.kpad = ldP; a.nch = SE->nch; a.escale = SE->escale[h.ops_set(l)]; a.off = D.off;
         a.tile_prefix[0] = 0;
@@ -619,7 +706,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         a.Es = (const uint16_t *)SE->d + (size_t)h.ops_set(l) * 316 * SE->nch * nsplit * SE->R * 32;
         a.P = p->P32; a.Pf = p->Pf32;
         a.xmax = prec == PBB_PREC_MIXED_F16X2 ? p->xmax + l : nullptr;
-        dev::launch_m2l_mixed(prec, a, a.tile_prefix[8], nchunks, st);
+        dev::launch_m2l_mixed(prec, a, a.tile_prefix[8], nchunks_m, st);
     }
     for (int l = 2; l <= Lv && !mixed; l++) {
         const DevLevel &D = p->lv[l];
@@ -654,8 +741,17 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         dev::launch_gemm(g, st);
     }
     mark(5);
-    // downward pass, top-down: F_l += L2L(F_{l-1}), F_{l-1} already complete
-    for (int l = 3; l <= Lv; l++) {
+    // downward pass, top-down: F_l += L2L(F_{l-1}), F_{l-1} already complete;
+    // the direct pass evaluates every level's F at the points instead
+    for (int l = 2; l <= Lv && direct_down; l++) {
+        const DevLevel &D = p->lv[l];
+        dev::DownArgs a{};
+        a.n = n; a.ncols = ncols; a.accumulate = l > 2; a.off = D.off; a.len = D.len;
+        a.seg_box = D.seg_box; a.seg_p0 = D.seg_p0; a.pstart = D.pstart; a.coord = D.coord; a.ctab = D.ctab;
+        a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.F = p->S; a.lp = p->lp;
+        dev::launch_l2p_level(a, D.nseg, st);
+    }
+    for (int l = 3; l <= Lv && !direct_down; l++) {
         dev::M2MArgs a{};
         a.n = n; a.ncols = ncols; a.cc = cc_m2m;
         a.off_parent = p->lv[l - 1].off; a.off_child = p->lv[l].off;
@@ -672,7 +768,12 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         a.ctab = LL.ctab; a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.ws = p->ws;
         a.F = Lv >= 2 ? p->S : nullptr;
         a.out = out;
-        dev::launch_near(a, h.kernel.id, (int)LL.nb, st);
+        a.wsT = p->wsT;
+        a.lp = direct_down ? p->lp : nullptr;
+        a.leaf_class = p->leaf_class;
+        std::copy(std::begin(p->leaf_class_off), std::end(p->leaf_class_off), a.leaf_class_off);
+        if (!(direct && dev::launch_near_split(a, h.kernel.id, (int)LL.nb, st)))
+            dev::launch_near(a, h.kernel.id, (int)LL.nb, st);
     }
     mark(7);
 }

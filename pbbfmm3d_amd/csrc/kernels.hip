@@ -23,6 +23,8 @@ namespace dev {
 
 __constant__ int c_dirs[8][189][4];  // far directions of each octant class
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
 // ---------------------------------------------------------------------------
 // device kernel functors (formulas of pbb_kernel_id); r2 = |s - t|^2 with
 // the differences taken as source - target (kernel_Types.cpp:21-23)
@@ -88,6 +90,18 @@ __global__ void gather_weights(const double *__restrict__ w, const int32_t *__re
          q += (int64_t)gridDim.x * blockDim.x) {
         int64_t c = q / N, k = q - c * N;
         ws[q] = w[c * N + perm[k]];
+    }
+}
+
+// wsT[k*ncols + c] = w[c*N + perm[k]] (point-major copy for the near field)
+__global__ void gather_weights_t(const double *__restrict__ w, const int32_t *__restrict__ perm, int64_t N,
+                                 int ncols, double *__restrict__ wsT)
+{
+    int64_t total = N * ncols;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        int64_t k = q / ncols, c = q - k * ncols;
+        wsT[q] = w[c * N + perm[k]];
     }
 }
 
@@ -322,6 +336,80 @@ __global__ __launch_bounds__(256) void gemm_small_a(GemmArgs g)
 }
 
 // ---------------------------------------------------------------------------
+// The same GEMM on the FP64 matrix cores: a workgroup of 4 waves computes all
+// RB 16-row blocks of M x 64 columns (one 16-column block per wave), K in
+// chunks of 16 through LDS with the next chunk prefetched into registers.
+template <typename TB, typename TC, int RB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gemm_mfma_f64(GemmArgs g)
+{
+    constexpr int NT = 256, W = 64, R = 16 * RB, KC = 16, AS = R + 2, BS = W + 2;
+    constexpr int AR = (KC * R + NT - 1) / NT, BR = KC * W / NT;
+    __shared__ double As[KC * AS];
+    __shared__ double Bs[KC * BS];
+    const TB *B = (const TB *)g.B;
+    TC *C = (TC *)g.C;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t n0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * W;
+    const int nsteps = (g.K + KC - 1) / KC;
+    double ra[AR], rb[BR];
+    auto load = [&](int step) {
+        const int k0 = step * KC;
+#pragma unroll
+        for (int r = 0; r < AR; r++) {
+            const int q = tid + r * NT, row = q % R, kk = q / R, gk = k0 + kk;
+            ra[r] = (q < KC * R && row < g.M && gk < g.K) ? g.A[row + (int64_t)gk * g.M] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < BR; r++) {
+            const int q = tid + r * NT, kk = q % KC, col = q / KC, gk = k0 + kk;
+            const int64_t gc = n0 + col;
+            rb[r] = (gk < g.K && gc < g.ncol) ? (double)B[gk + gc * g.ldb] : 0.0;
+        }
+    };
+    dbl4 acc[RB];
+#pragma unroll
+    for (int i = 0; i < RB; i++) acc[i] = dbl4{0, 0, 0, 0};
+    const int fr = lane % 16, fk = lane / 16;
+    load(0);
+    for (int step = 0; step < nsteps; step++) {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < AR; r++) {
+            const int q = tid + r * NT;
+            if (q < KC * R) As[(q / R) * AS + q % R] = ra[r];
+        }
+#pragma unroll
+        for (int r = 0; r < BR; r++) {
+            const int q = tid + r * NT;
+            Bs[(q % KC) * BS + q / KC] = rb[r];
+        }
+        __syncthreads();
+        if (step + 1 < nsteps) load(step + 1);
+#pragma unroll 1
+        for (int ks = 0; ks < KC / 4; ks++) {
+            const double b = Bs[(ks * 4 + fk) * BS + wave * 16 + fr];
+#pragma unroll
+            for (int rb_ = 0; rb_ < RB; rb_++)
+                acc[rb_] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[(ks * 4 + fk) * AS + rb_ * 16 + fr], b, acc[rb_], 0, 0,
+                                                               0);
+        }
+    }
+    const int64_t gc = n0 + wave * 16 + fr;
+    if (gc < g.ncol) {
+#pragma unroll
+        for (int rb_ = 0; rb_ < RB; rb_++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = rb_ * 16 + fk + 4 * v;
+                if (row < g.M) C[row + gc * g.ldc] = (TC)(g.alpha * acc[rb_][v]);
+                else if (row < g.ldc) C[row + gc * g.ldc] = (TC)0;
+            }
+        for (int row = R; row < g.ldc; row++)  // padding rows beyond the MFMA tile
+            if (fk == 0) C[row + gc * g.ldc] = (TC)0;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // M2L, compute.hpp:401-493 + Moment2Local :880-923.  A workgroup owns TB
 // target boxes of one octant class (Morton-adjacent) and the same column
 // chunk of each: a W-wide output tile Pf (all r_f rows) that stays in
@@ -335,7 +423,6 @@ __global__ __launch_bounds__(256) void gemm_small_a(GemmArgs g)
 // Fragment layout of the 16x16x4 FP64 MFMA (lane l of 64):
 //   A[i][k]: i = l % 16, k = l / 16      B[k][j]: j = l % 16, k = l / 16
 //   D[i][j]: j = l % 16, i = l / 16 + 4*v     (v = 0..3, one double each)
-typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 constexpr int TB_MAX = 16;
 
@@ -355,12 +442,16 @@ __global__ __launch_bounds__(NW * 64) void m2l_mfma_f64(M2LArgs a)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int rf = a.rf, rs = a.rs, TB = a.TB;
     const int nch = (rs + KC - 1) / KC;
-    const int tile = blockIdx.x;
+    const int lin = xcd_tile(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    // the column chunks of one tile are adjacent in launch order: they run
+    // side by side on one XCD through the same (direction, k-chunk) steps and
+    // share each E chunk through its L2
+    const int tile = lin / gridDim.y;
     int o = 0;
     while (o < 7 && tile >= a.tile_prefix[o + 1]) o++;
     const int64_t first = a.class_start[o] + (int64_t)(tile - a.tile_prefix[o]) * TB;
     const int64_t cend = a.class_start[o + 1];
-    const int c0 = blockIdx.y * a.cpt;
+    const int c0 = (lin - tile * (int)gridDim.y) * a.cpt;
     const int cpt = min(a.cpt, a.ncols - c0);
     const int S = a.side;
     if (tid < TB) {
@@ -387,21 +478,27 @@ __global__ __launch_bounds__(NW * 64) void m2l_mfma_f64(M2LArgs a)
         s_sid[k * TB_MAX + tb] = sid;
     }
     __syncthreads();
-    if (tid == 0) {
+    for (int k = tid; k < 189; k += NT) {  // does direction k reach any source box?
+        bool any = false;
+        for (int tb = 0; tb < TB; tb++) any |= s_sid[k * TB_MAX + tb] >= 0;
+        s_dir[k] = any ? 1 : 0;
+    }
+    __syncthreads();
+    if (tid == 0) {  // compact the non-empty directions in class order
         int nd = 0;
-        for (int k = 0; k < 189; k++) {
-            bool any = false;
-            for (int tb = 0; tb < TB; tb++) any |= s_sid[k * TB_MAX + tb] >= 0;
-            if (any) s_dir[nd++] = (short)k;
-        }
+        for (int k = 0; k < 189; k++)
+            if (s_dir[k]) s_dir[nd++] = (short)k;
         s_nd = nd;
     }
     __syncthreads();
-    const int nsteps = s_nd * nch;
+    // steps run k-chunk-major: all directions of one chunk, then the next
+    // chunk, so that consecutive steps gather the same proxy rows of largely
+    // the same source boxes (see class_directions)
+    const int nd = s_nd, nsteps = nd * nch;
 
     double ra[AR], rb[BR];
     auto load = [&](int step) {
-        const int k = s_dir[step / nch], k0 = (step - (step / nch) * nch) * KC;
+        const int ch = step / nd, k = s_dir[step - ch * nd], k0 = ch * KC;
         const double *Ed = a.E + (size_t)c_dirs[o][k][3] * rf * rs;
 #pragma unroll
         for (int r = 0; r < AR; r++) {
@@ -440,7 +537,7 @@ __global__ __launch_bounds__(NW * 64) void m2l_mfma_f64(M2LArgs a)
         }
         __syncthreads();
         if (step + 1 < nsteps) load(step + 1);
-#pragma unroll
+#pragma unroll 1
         for (int ks = 0; ks < KC / 4; ks++) {
             const double b = Bs[(ks * 4 + fk) * BS + wave * 16 + fr];
 #pragma unroll
@@ -599,6 +696,328 @@ __global__ __launch_bounds__(64) void near_fp64(NearArgs a)
                 }
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Near field in two kernels, for Chebyshev orders known at compile time.
+//   l2p_fp64  L2P (compute.hpp:594-648): lp[k*ncols + c] = sum_l S_l(x_k)
+//             F[c][l] for the targets k of each leaf, point-major, sorted
+//             order.  8 columns of the leaf's F at a time are staged in LDS
+//             with coalesced loads; lanes (target, column, part) split the
+//             n^3 sum and reduce across the parts, so that leaves of one or
+//             two points keep every lane busy.
+//   p2p_fp64  P2P (compute.hpp:650-737) + scatter (:739-749) on top of lp.
+//             One wave per leaf; lane (i, g) holds columns g, g+G, ... of
+//             target i (G = 64 / targets per pass) in registers.  Source
+//             chunks of 32 points (positions, point-major weights) are staged
+//             in LDS while the next chunk is loaded.  Passes of up to 16
+//             targets evaluate each K(t, s) once into LDS; wider passes
+//             evaluate it per lane (at most 2 lanes per target).
+template <int N>
+__device__ __forceinline__ void sn_eval_n(const double *__restrict__ Tkz, double u, double *S)
+{
+    const double pfac = 2.0 / N;
+#pragma unroll
+    for (int m = 0; m < N; m++) {
+        double d0 = 0, d1 = 0;
+#pragma unroll
+        for (int j = N - 1; j > 0; j--) {
+            const double d2 = d0;
+            d0 = 2.0 * u * d0 - d1 + Tkz[m * N + j];
+            d1 = d2;
+        }
+        S[m] = pfac * (u * d0 - d1 + 0.5 * Tkz[m * N]);
+    }
+}
+
+constexpr int L2P_CC = 8;    // columns of F staged per step
+constexpr int L2P_TGT = 32;  // targets per pass
+
+// one level of the direct downward pass: one workgroup per segment of at
+// most L2P_SEG points of one box; writes (or adds to) lp[k*ncols + c]
+template <int N>
+__global__ __launch_bounds__(256) void l2p_fp64(DownArgs a)
+{
+    constexpr int N2 = N * N, N3 = N2 * N, FS = N3 + 1;
+    __shared__ double Fs[L2P_CC * FS];
+    __shared__ double sS[3][N][L2P_TGT];
+    const int tid = threadIdx.x;
+    const int b = a.seg_box[blockIdx.x], p0 = a.seg_p0[blockIdx.x];
+    const int p1 = min(p0 + L2P_SEG, a.pstart[b + 1]);
+    const uint32_t cc = (uint32_t)a.coord[b];
+    const double ihalfL = 2.0 / a.len;
+    const double *Fb = a.F + (size_t)(a.off + b) * a.ncols * N3;
+    for (int c0 = 0; c0 < a.ncols; c0 += L2P_CC) {
+        const int cn = min(L2P_CC, a.ncols - c0);
+        __syncthreads();  // the previous step's readers of Fs and sS are done
+        for (int q = tid; q < cn * N3; q += 256) {
+            const int qc = q / N3;
+            Fs[qc * FS + q - qc * N3] = Fb[(size_t)c0 * N3 + q];
+        }
+        for (int i0 = p0; i0 < p1; i0 += L2P_TGT) {
+            const int tn = min(L2P_TGT, p1 - i0);
+            const int tp = tn <= 1 ? 1 : tn <= 2 ? 2 : tn <= 4 ? 4 : tn <= 8 ? 8 : tn <= 16 ? 16 : 32;
+            const int P = 32 / tp;  // parts per (target, column): tp * 8 * P = 256
+            const int p = tid % P, c = (tid / P) % L2P_CC, i = tid / (P * L2P_CC);
+            if (i0 != p0) __syncthreads();  // the previous chunk's readers of sS are done
+            if (tid < 3 * L2P_TGT) {
+                const int ti = tid % L2P_TGT, ax = tid / L2P_TGT;
+                if (ti < tn) {
+                    const double *xa = ax == 0 ? a.x : ax == 1 ? a.y : a.z;
+                    const double cen = a.ctab[(cc >> (10 * ax)) & 1023];
+                    double S[N];
+                    sn_eval_n<N>(a.Tkz, ihalfL * (xa[i0 + ti] - cen), S);
+#pragma unroll
+                    for (int m = 0; m < N; m++) sS[ax][m][ti] = S[m];
+                }
+            }
+            __syncthreads();
+            const bool active = i < tn && c < cn;
+            double s = 0.0;
+            if (active) {
+                double Sz[N];
+#pragma unroll
+                for (int m = 0; m < N; m++) Sz[m] = sS[2][m][i];
+                for (int l12 = p; l12 < N2; l12 += P) {
+                    const int l1 = l12 / N, l2 = l12 - l1 * N;
+                    const double *f = &Fs[c * FS + l12 * N];
+                    double s3 = 0.0;
+#pragma unroll
+                    for (int l3 = 0; l3 < N; l3++) s3 += f[l3] * Sz[l3];
+                    s += sS[0][l1][i] * sS[1][l2][i] * s3;
+                }
+            }
+            for (int o = 1; o < P; o <<= 1) s += __shfl_xor(s, o);
+            if (active && p == 0) {
+                double *dst = a.lp + (int64_t)(i0 + i) * a.ncols + c0 + c;
+                *dst = a.accumulate ? *dst + s : s;
+            }
+        }
+    }
+}
+
+// Direct upward pass.  P2M at the leaves followed by M2M (compute.hpp:
+// 234-333, :956-1034) equals P2M at every level straight from the points:
+// the child-to-parent transfer interpolates the parent's Chebyshev
+// polynomials S_m, which have degree n-1, at the child's nodes, which is
+// exact.  One workgroup per box of levels 2..L (coarse levels first, so
+// their long point loops start early); thread pencils (column c, l1, l2)
+// hold the n values of l3 in registers.
+template <int N>
+__global__ __launch_bounds__(256) void p2m_levels(UpArgs a)
+{
+    constexpr int N2 = N * N, N3 = N2 * N, PTS = 64;
+    constexpr int CC = N <= 6 ? 32 : 16;        // columns per pass
+    constexpr int PR = (CC * N2 + 255) / 256;   // pencils per thread
+    __shared__ double sS[3][N][PTS];
+    __shared__ double sw[PTS][CC + 1];
+    const int tid = threadIdx.x;
+    const int64_t g = blockIdx.x;
+    int li = 0;
+    while (li + 1 < a.lt.nlev && g >= a.lt.off[li + 1]) li++;
+    const int b = (int)(g - a.lt.off[li]);
+    const int p0 = a.lt.pstart[li][b], p1 = a.lt.pstart[li][b + 1];
+    const uint32_t cc = (uint32_t)a.lt.coord[li][b];
+    const double *ctab = a.lt.ctab[li];
+    const double cen[3] = {ctab[cc & 1023], ctab[(cc >> 10) & 1023], ctab[(cc >> 20) & 1023]};
+    const double ihalfL = 2.0 / a.lt.len[li];
+    double *Sb = a.S + (size_t)g * a.ncols * N3;
+    for (int c0 = 0; c0 < a.ncols; c0 += CC) {
+        const int cn = min(CC, a.ncols - c0);
+        double acc[PR][N];
+#pragma unroll
+        for (int r = 0; r < PR; r++)
+#pragma unroll
+            for (int m = 0; m < N; m++) acc[r][m] = 0.0;
+        for (int j0 = p0; j0 < p1; j0 += PTS) {
+            const int jn = min(PTS, p1 - j0);
+            __syncthreads();  // the previous chunk is consumed
+            if (tid < 3 * PTS) {
+                const int pp = tid % PTS, ax = tid / PTS;
+                double S[N];
+                if (pp < jn) {
+                    const double *xa = ax == 0 ? a.x : ax == 1 ? a.y : a.z;
+                    sn_eval_n<N>(a.Tkz, ihalfL * (xa[j0 + pp] - cen[ax]), S);
+                } else {
+#pragma unroll
+                    for (int m = 0; m < N; m++) S[m] = 0.0;
+                }
+#pragma unroll
+                for (int m = 0; m < N; m++) sS[ax][m][pp] = S[m];
+            }
+            for (int q = tid; q < PTS * CC; q += 256) {
+                const int pp = q / CC, c = q - pp * CC;
+                sw[pp][c] = (pp < jn && c < cn) ? a.wsT[(int64_t)(j0 + pp) * a.ncols + c0 + c] : 0.0;
+            }
+            __syncthreads();
+            for (int pp = 0; pp < jn; pp++) {
+                double sz[N];
+#pragma unroll
+                for (int m = 0; m < N; m++) sz[m] = sS[2][m][pp];
+#pragma unroll
+                for (int r = 0; r < PR; r++) {
+                    const int q = tid + 256 * r;
+                    if (q < CC * N2) {
+                        const int c = q / N2, l12 = q - c * N2, l1 = l12 / N, l2 = l12 - l1 * N;
+                        const double t = sw[pp][c] * sS[0][l1][pp] * sS[1][l2][pp];
+#pragma unroll
+                        for (int m = 0; m < N; m++) acc[r][m] += t * sz[m];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < PR; r++) {
+            const int q = tid + 256 * r;
+            if (q < CC * N2) {
+                const int c = q / N2, l12 = q - c * N2;
+                if (c < cn) {
+                    double *dst = Sb + (size_t)(c0 + c) * N3 + l12 * N;
+#pragma unroll
+                    for (int m = 0; m < N; m++) dst[m] = acc[r][m];
+                }
+            }
+        }
+    }
+}
+
+constexpr int P2P_SC = 32;  // source points per LDS chunk
+constexpr int P2P_NC = 32;  // columns per pass
+constexpr int P2P_SK = 16;  // passes of at most this many targets share K through LDS
+
+struct P2PShared {
+    double sw[P2P_SC][P2P_NC + 1];
+    double sx[P2P_SC], sy[P2P_SC], sz[P2P_SC];
+    double sK[P2P_SK][P2P_SC + 1];
+    double tx[P2P_SK], ty[P2P_SK], tz[P2P_SK];
+    int s_start[27], s_pre[28];
+};
+
+template <int KID, int TT>
+__device__ __forceinline__ void p2p_pass(const NearArgs &a, P2PShared &sh, int t0, int i0, int tn, int c0,
+                                         int cn, int ns)
+{
+    constexpr int G = 64 / TT, CPL = P2P_NC / G, WR = P2P_SC * P2P_NC / 64;
+    constexpr bool SHARED_K = TT <= P2P_SK;
+    const int lane = threadIdx.x, i = lane / G, g = lane % G;
+    const bool active = i < tn;
+    const int64_t ti = t0 + i0 + (active ? i : 0);
+    const double tx = a.x[ti], ty = a.y[ti], tz = a.z[ti];
+    if constexpr (SHARED_K) {
+        __syncthreads();
+        if (lane < tn) {
+            sh.tx[lane] = a.x[t0 + i0 + lane];
+            sh.ty[lane] = a.y[t0 + i0 + lane];
+            sh.tz[lane] = a.z[t0 + i0 + lane];
+        }
+    }
+    double acc[CPL];
+#pragma unroll
+    for (int k = 0; k < CPL; k++) {
+        const int cl = g + G * k;
+        acc[k] = (a.lp && active && cl < cn) ? a.lp[ti * a.ncols + c0 + cl] : 0.0;
+    }
+    double pw[WR], px = 0, py = 0, pz = 0;
+    auto load = [&](int j0) {
+        const int jn = min(P2P_SC, ns - j0);
+        int gi = 0;
+        if (lane < jn) {
+            const int q = j0 + lane;
+            int m = 0;
+            while (sh.s_pre[m + 1] <= q) m++;
+            gi = sh.s_start[m] + q - sh.s_pre[m];
+            px = a.x[gi];
+            py = a.y[gi];
+            pz = a.z[gi];
+        }
+#pragma unroll
+        for (int r = 0; r < WR; r++) {
+            const int u = lane + 64 * r, j = u / P2P_NC, c = u % P2P_NC;
+            const int gj = __shfl(gi, j);
+            pw[r] = (j < jn && c < cn) ? a.wsT[(int64_t)gj * a.ncols + c0 + c] : 0.0;
+        }
+    };
+    if (ns > 0) load(0);
+    for (int j0 = 0; j0 < ns; j0 += P2P_SC) {
+        const int jn = min(P2P_SC, ns - j0);
+        __syncthreads();  // the previous chunk is consumed
+#pragma unroll
+        for (int r = 0; r < WR; r++) {
+            const int u = lane + 64 * r;
+            sh.sw[u / P2P_NC][u % P2P_NC] = pw[r];
+        }
+        if (lane < P2P_SC) {
+            sh.sx[lane] = px;
+            sh.sy[lane] = py;
+            sh.sz[lane] = pz;
+        }
+        __syncthreads();
+        if (j0 + P2P_SC < ns) load(j0 + P2P_SC);  // in flight during the sums
+        if constexpr (SHARED_K) {
+            for (int q = lane; q < TT * P2P_SC; q += 64) {
+                const int ii = q / P2P_SC, jj = q % P2P_SC;
+                double kv = 0.0;
+                if (ii < tn && jj < jn) {
+                    kv = keval<KID>(sh.sx[jj] - sh.tx[ii], sh.sy[jj] - sh.ty[ii], sh.sz[jj] - sh.tz[ii], a.kp);
+                    if (isinf(kv)) kv = 0.0;
+                }
+                sh.sK[ii][jj] = kv;
+            }
+            __syncthreads();
+            if (active)
+                for (int jj = 0; jj < jn; jj++) {
+                    const double kv = sh.sK[i][jj];
+#pragma unroll
+                    for (int k = 0; k < CPL; k++) acc[k] += kv * sh.sw[jj][g + G * k];
+                }
+        } else {
+            if (active)
+                for (int jj = 0; jj < jn; jj++) {
+                    double kv = keval<KID>(sh.sx[jj] - tx, sh.sy[jj] - ty, sh.sz[jj] - tz, a.kp);
+                    if (isinf(kv)) kv = 0.0;
+#pragma unroll
+                    for (int k = 0; k < CPL; k++) acc[k] += kv * sh.sw[jj][g + G * k];
+                }
+        }
+    }
+    if (active) {
+        const int64_t oi = a.perm[ti];
+#pragma unroll
+        for (int k = 0; k < CPL; k++) {
+            const int cl = g + G * k;
+            if (cl < cn) a.out[(int64_t)(c0 + cl) * a.N + oi] += acc[k];
+        }
+    }
+}
+
+// one workgroup per leaf of a list whose leaves hold at most TT targets (or
+// more, in passes of TT, for the widest class)
+template <int KID, int TT>
+__global__ __launch_bounds__(64) void p2p_fp64(NearArgs a, const int32_t *__restrict__ leaves)
+{
+    __shared__ P2PShared sh;
+    const int b = leaves[blockIdx.x], lane = threadIdx.x;
+    const int t0 = a.leaf_start[b], nt = a.leaf_start[b + 1] - t0;
+    const uint32_t cc = (uint32_t)a.coord[b];
+    const int ix = cc & 1023, iy = (cc >> 10) & 1023, iz = (cc >> 20) & 1023;
+    const int S = a.side;
+    if (lane < 27) {
+        const int x = ix + lane / 9 - 1, y = iy + (lane / 3) % 3 - 1, z = iz + lane % 3 - 1;
+        const int sb = (x >= 0 && y >= 0 && z >= 0 && x < S && y < S && z < S) ? a.map[((int64_t)x * S + y) * S + z] : -1;
+        sh.s_start[lane] = sb >= 0 ? a.leaf_start[sb] : 0;
+        sh.s_pre[lane + 1] = sb >= 0 ? a.leaf_start[sb + 1] - a.leaf_start[sb] : 0;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        sh.s_pre[0] = 0;
+        for (int m = 0; m < 27; m++) sh.s_pre[m + 1] += sh.s_pre[m];
+    }
+    __syncthreads();
+    const int ns = sh.s_pre[27];
+    for (int c0 = 0; c0 < a.ncols; c0 += P2P_NC) {
+        const int cn = min(P2P_NC, a.ncols - c0);
+        for (int i0 = 0; i0 < nt; i0 += TT) p2p_pass<KID, TT>(a, sh, t0, i0, min(TT, nt - i0), c0, cn, ns);
     }
 }
 
@@ -902,6 +1321,16 @@ void launch_gather_weights(const double *w, const int32_t *perm, int64_t N, int 
     check(hipGetLastError(), "gather_weights");
 }
 
+void launch_gather_weights_t(const double *w, const int32_t *perm, int64_t N, int ncols, double *wsT,
+                             hipStream_t st)
+{
+    int64_t total = N * ncols;
+    if (total <= 0) return;
+    int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    gather_weights_t<<<blocks, 256, 0, st>>>(w, perm, N, ncols, wsT);
+    check(hipGetLastError(), "gather_weights_t");
+}
+
 void launch_p2m(const P2MArgs &a, int nleaf, hipStream_t st)
 {
     if (nleaf <= 0) return;
@@ -929,6 +1358,24 @@ void launch_gemm(const GemmArgs &g, hipStream_t st)
 {
     if (g.ncol <= 0) return;
     if (g.ldb < g.K || g.ldc < g.M) fail(PBB_ERR_INTERNAL, "gemm: leading dimension too small");
+    const int rb = (g.M + 15) / 16;
+    if (rb <= 16) {
+        dim3 grid((unsigned)((g.ncol + 63) / 64));
+#define PBB_G(R)                                                                                   \
+    case R:                                                                                        \
+        if (!g.b_f32 && !g.c_f32) gemm_mfma_f64<double, double, R><<<grid, 256, 0, st>>>(g);       \
+        else if (!g.b_f32 && g.c_f32) gemm_mfma_f64<double, float, R><<<grid, 256, 0, st>>>(g);    \
+        else if (g.b_f32 && !g.c_f32) gemm_mfma_f64<float, double, R><<<grid, 256, 0, st>>>(g);    \
+        else gemm_mfma_f64<float, float, R><<<grid, 256, 0, st>>>(g);                              \
+        break;
+        switch (rb) {
+            PBB_G(1) PBB_G(2) PBB_G(3) PBB_G(4) PBB_G(5) PBB_G(6) PBB_G(7) PBB_G(8) PBB_G(9) PBB_G(10) PBB_G(11)
+            PBB_G(12) PBB_G(13) PBB_G(14) PBB_G(15) PBB_G(16)
+        }
+#undef PBB_G
+        check(hipGetLastError(), "gemm_mfma_f64");
+        return;
+    }
     dim3 grid((unsigned)((g.ncol + 63) / 64), (unsigned)((std::max(g.M, g.ldc) + 63) / 64));
     if (!g.b_f32 && !g.c_f32) gemm_small_a<double, double><<<grid, 256, 0, st>>>(g);
     else if (!g.b_f32 && g.c_f32) gemm_small_a<double, float><<<grid, 256, 0, st>>>(g);
@@ -975,6 +1422,71 @@ void launch_near(const NearArgs &a, int kid, int nleaf, hipStream_t st)
     }
 #undef PBB_NEAR_CASE
     check(hipGetLastError(), "near_fp64");
+}
+
+bool launch_near_split(const NearArgs &a, int kid, int nleaf, hipStream_t st)
+{
+    if (a.n < 4 || a.n > 8) return false;
+    if (nleaf <= 0) return true;
+    const NearArgs &b = a;
+    for (int c = 0; c < NEAR_CLASSES; c++) {
+        const int cnt = a.leaf_class_off[c + 1] - a.leaf_class_off[c];
+        if (cnt <= 0) continue;
+        const int32_t *lst = a.leaf_class + a.leaf_class_off[c];
+#define PBB_P2P_TT(K, T)                                                    \
+    case T:                                                                 \
+        p2p_fp64<K, T><<<cnt, 64, 0, st>>>(b, lst);                         \
+        break;
+#define PBB_P2P_CASE(K)                                                     \
+    case K:                                                                 \
+        switch (2 << c) { PBB_P2P_TT(K, 2) PBB_P2P_TT(K, 4) PBB_P2P_TT(K, 8) \
+                          PBB_P2P_TT(K, 16) PBB_P2P_TT(K, 32) PBB_P2P_TT(K, 64) } \
+        break;
+        switch (kid) {
+            PBB_P2P_CASE(PBB_KERNEL_LAPLACIAN) PBB_P2P_CASE(PBB_KERNEL_ONEOVERR2) PBB_P2P_CASE(PBB_KERNEL_GAUSSIAN)
+            PBB_P2P_CASE(PBB_KERNEL_EXPONENTIAL) PBB_P2P_CASE(PBB_KERNEL_SOFT_LAPLACIAN)
+            PBB_P2P_CASE(PBB_KERNEL_QUADRIC) PBB_P2P_CASE(PBB_KERNEL_INV_QUADRIC)
+            PBB_P2P_CASE(PBB_KERNEL_LOGARITHM) PBB_P2P_CASE(PBB_KERNEL_THIN_PLATE) PBB_P2P_CASE(PBB_KERNEL_R2)
+            PBB_P2P_CASE(PBB_KERNEL_CONSTANT) PBB_P2P_CASE(PBB_KERNEL_EXP_ANISO)
+        default: fail(PBB_ERR_UNSUPPORTED, "no device functor for this kernel id");
+        }
+#undef PBB_P2P_CASE
+#undef PBB_P2P_TT
+    }
+    check(hipGetLastError(), "p2p_fp64");
+    return true;
+}
+
+bool launch_p2m_levels(const UpArgs &a, hipStream_t st)
+{
+    if (a.n < 4 || a.n > 8) return false;
+    const int64_t nb = a.lt.off[a.lt.nlev];
+    if (nb <= 0) return true;
+    if (nb > 0x7fffffff) fail(PBB_ERR_UNSUPPORTED, "too many boxes");
+    switch (a.n) {
+    case 4: p2m_levels<4><<<(unsigned)nb, 256, 0, st>>>(a); break;
+    case 5: p2m_levels<5><<<(unsigned)nb, 256, 0, st>>>(a); break;
+    case 6: p2m_levels<6><<<(unsigned)nb, 256, 0, st>>>(a); break;
+    case 7: p2m_levels<7><<<(unsigned)nb, 256, 0, st>>>(a); break;
+    default: p2m_levels<8><<<(unsigned)nb, 256, 0, st>>>(a); break;
+    }
+    check(hipGetLastError(), "p2m_levels");
+    return true;
+}
+
+bool launch_l2p_level(const DownArgs &a, int nseg, hipStream_t st)
+{
+    if (a.n < 4 || a.n > 8) return false;
+    if (nseg <= 0) return true;
+    switch (a.n) {
+    case 4: l2p_fp64<4><<<nseg, 256, 0, st>>>(a); break;
+    case 5: l2p_fp64<5><<<nseg, 256, 0, st>>>(a); break;
+    case 6: l2p_fp64<6><<<nseg, 256, 0, st>>>(a); break;
+    case 7: l2p_fp64<7><<<nseg, 256, 0, st>>>(a); break;
+    default: l2p_fp64<8><<<nseg, 256, 0, st>>>(a); break;
+    }
+    check(hipGetLastError(), "l2p_fp64");
+    return true;
 }
 
 void launch_direct(const DirectArgs &a, int kid, hipStream_t st)

@@ -17,6 +17,16 @@ inline void check(hipError_t e, const char *what)
 
 namespace dev {
 
+// Workgroups are dispatched to the 8 XCDs round-robin by linear id; this
+// maps linear id b of G so that each XCD receives one contiguous range of
+// logical tiles (Morton-adjacent targets share sources and operators in
+// that XCD's L2).
+__device__ __forceinline__ int xcd_tile(int b, int G)
+{
+    const int x = b & 7, q = b >> 3, base = G >> 3, rem = G & 7;
+    return x * base + (x < rem ? x : rem) + q;
+}
+
 struct KParams {
     double p[4];
 };
@@ -40,6 +50,36 @@ struct M2MArgs {
     double *Schild;             // M2M: read;    L2L: accumulated
 };
 
+// per-level tables of the direct upward / downward passes (levels 2..L)
+constexpr int MAX_FAR_LEVELS = 9;
+struct LevelTab {
+    int nlev;                                  // levels 2 .. 1 + nlev
+    int64_t off[MAX_FAR_LEVELS + 1];           // global box offsets; off[nlev] = total boxes
+    const int32_t *pstart[MAX_FAR_LEVELS];     // box -> first sorted point (nb + 1 entries)
+    const int32_t *coord[MAX_FAR_LEVELS];
+    const double *ctab[MAX_FAR_LEVELS];
+    double len[MAX_FAR_LEVELS];
+};
+
+struct UpArgs {
+    int n, ncols;
+    LevelTab lt;
+    const double *Tkz, *x, *y, *z, *wsT;
+    double *S;
+};
+
+constexpr int L2P_SEG = 256;  // targets per workgroup of the per-level L2P
+
+struct DownArgs {
+    int n, ncols, accumulate;
+    int64_t off;                               // global box offset of the level
+    double len;
+    const int32_t *seg_box, *seg_p0, *pstart, *coord;
+    const double *ctab, *Tkz, *x, *y, *z;
+    const double *F;
+    double *lp;                                // point-major L2P results
+};
+
 struct GemmArgs {
     int M, K, ldb, ldc;
     int b_f32, c_f32;          // B / C element type: 0 FP64, 1 FP32
@@ -60,6 +100,8 @@ struct M2LArgs {
     double *Pf;
 };
 
+constexpr int NEAR_CLASSES = 6;
+
 struct NearArgs {
     int n, ncols, side;
     int64_t N, off;
@@ -70,6 +112,12 @@ struct NearArgs {
     const double *x, *y, *z, *ws;
     const double *F;            // leaf-level local expansions (nullptr: no far field)
     double *out;
+    const double *wsT;          // sorted weights, point-major wsT[k*ncols + c]
+    double *lp;                 // L2P results, point-major (nullptr: no far field)
+    // leaves grouped by size: class c holds leaves of at most 2 << c points
+    // (the last class also the larger ones)
+    const int32_t *leaf_class;
+    int leaf_class_off[NEAR_CLASSES + 1];
 };
 
 struct DirectArgs {
@@ -121,6 +169,14 @@ void launch_l2l(const M2MArgs &a, int nparent, hipStream_t st);
 void launch_gemm(const GemmArgs &g, hipStream_t st);
 void launch_m2l(const M2LArgs &a, int ntiles, int ncolchunks, hipStream_t st);
 void launch_near(const NearArgs &a, int kid, int nleaf, hipStream_t st);
+// the split near field for orders 4..8 (false: order not covered)
+bool launch_near_split(const NearArgs &a, int kid, int nleaf, hipStream_t st);
+// direct upward pass: S of every box of levels 2..L from its points
+bool launch_p2m_levels(const UpArgs &a, hipStream_t st);
+// direct downward pass, one level: lp (+)= L2P of the level's F at its points
+bool launch_l2p_level(const DownArgs &a, int nseg, hipStream_t st);
+void launch_gather_weights_t(const double *w, const int32_t *perm, int64_t N, int ncols, double *wsT,
+                             hipStream_t st);
 void launch_direct(const DirectArgs &a, int kid, hipStream_t st);
 
 }  // namespace dev

@@ -90,37 +90,44 @@ __global__ void absmax_f32(const float *__restrict__ x, int64_t n, unsigned *__r
 }
 
 // ---------------------------------------------------------------------------
-// One workgroup = 4 waves owns TB (<= 16) target boxes of one octant class x
-// cpt columns (W = 64*NCB columns); each wave holds NCB 16-column blocks x
-// RB 16-row blocks of FP32 accumulators.  The (direction, 32-deep k chunk)
-// steps of the tile's non-empty directions form one software pipeline: the
-// next step's E chunk (pre-split) and gathered FP32 proxies are loaded into
-// registers while the current step's MFMAs run from LDS.
+// One workgroup owns TB (<= 16) target boxes of one octant class x cpt
+// columns (W = 64*NWC columns).  Its 2*NWC waves form a 2 x NWC grid: wave
+// (wr, wc) holds row blocks [wr*RBW, wr*RBW + RBW) x the 4 16-column blocks
+// of column group wc in FP32 accumulators.  The (direction, 32-deep k chunk)
+// steps of the tile's non-empty directions form one software pipeline with
+// one barrier per step:
+//   E chunks (pre-split) are staged in a double-buffered LDS tile, the next
+//   one written while the current one feeds the MFMAs;
+//   each lane loads the proxies of its own B fragments straight from global
+//   memory into registers one step ahead, and splits them into 16-bit parts.
 constexpr int TB_MAX = 16;
 
-template <typename T, int NSPLIT, int RB, int NCB>
-__global__ __launch_bounds__(256, 2) void m2l_mfma_split(M2LMixArgs a)
+template <typename T, int NSPLIT, int RB, int NWC>
+__global__ __launch_bounds__(NWC * 128) void m2l_mfma_split(M2LMixArgs a)
 {
     typedef typename Vec8<T>::type V8;
-    constexpr int NT = 256, W = 64 * NCB, R = 16 * RB, KS = 40, BS = 36;
-    constexpr int NA = NSPLIT * R * 4;        // 16-byte units of one E chunk
+    constexpr int NW = 2 * NWC, NT = NW * 64, R = 16 * RB, KS = 40, RBW = (RB + 1) / 2;
+    constexpr int NA = NSPLIT * R * 4;  // 16-byte units of one E chunk
     constexpr int AR = (NA + NT - 1) / NT;
-    constexpr int BR = W * 8 / NT;            // float4 units of one proxy chunk per thread
-    __shared__ __attribute__((aligned(16))) T As[NSPLIT * R * KS];
-    __shared__ __attribute__((aligned(16))) float Bs[W * BS];
+    __shared__ __attribute__((aligned(16))) T As[2][NSPLIT * R * KS];
     __shared__ int s_sid[189 * TB_MAX];
     __shared__ short s_dir[189];
     __shared__ int s_nd;
     __shared__ int s_tgt[TB_MAX];
     __shared__ int s_xyz[TB_MAX][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave & 1, wc = wave >> 1;
     const int rf = a.rf, TB = a.TB, nch = a.nch;
-    const int tile = blockIdx.x;
+    const int lin = xcd_tile(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    // the column chunks of one tile are adjacent in launch order: they run
+    // side by side on one XCD through the same (direction, k-chunk) steps and
+    // share each E chunk through its L2
+    const int tile = lin / gridDim.y;
     int o = 0;
     while (o < 7 && tile >= a.tile_prefix[o + 1]) o++;
     const int64_t first = a.class_start[o] + (int64_t)(tile - a.tile_prefix[o]) * TB;
     const int64_t cend = a.class_start[o + 1];
-    const int c0 = blockIdx.y * a.cpt;
+    const int c0 = (lin - tile * (int)gridDim.y) * a.cpt;
     const int cpt = min(a.cpt, a.ncols - c0);
     const int S = a.side;
     // FP16 operands are scaled into range by 2^xs (P) and 2^escale (E)
@@ -160,72 +167,90 @@ __global__ __launch_bounds__(256, 2) void m2l_mfma_split(M2LMixArgs a)
         s_sid[k * TB_MAX + tb] = sid;
     }
     __syncthreads();
-    if (tid == 0) {  // non-empty directions, in class order
+    for (int k = tid; k < 189; k += NT) {  // does direction k reach any source box?
+        bool any = false;
+        for (int tb = 0; tb < TB; tb++) any |= s_sid[k * TB_MAX + tb] >= 0;
+        s_dir[k] = any ? 1 : 0;
+    }
+    __syncthreads();
+    if (tid == 0) {  // compact the non-empty directions in class order
         int nd = 0;
-        for (int k = 0; k < 189; k++) {
-            bool any = false;
-            for (int tb = 0; tb < TB; tb++) any |= s_sid[k * TB_MAX + tb] >= 0;
-            if (any) s_dir[nd++] = (short)k;
-        }
+        for (int k = 0; k < 189; k++)
+            if (s_dir[k]) s_dir[nd++] = (short)k;
         s_nd = nd;
     }
     __syncthreads();
-    const int nsteps = s_nd * nch;
+    // steps run k-chunk-major: all directions of one chunk, then the next
+    // chunk, so that consecutive steps gather the same proxy rows of largely
+    // the same source boxes (see class_directions)
+    const int nd = s_nd, nsteps = nd * nch;
 
+    const int fr = lane % 16, fq = lane / 16;
+    int btb[4], bcl[4];  // box slot and column of this lane's B column in each block
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++) {
+        const int col = wc * 64 + cb * 16 + fr;
+        const int tb = col / cpt;
+        btb[cb] = tb < TB ? tb : -1;
+        bcl[cb] = col - tb * cpt;
+    }
     uint4 ra[AR];
-    float4 rb[BR];
-    auto load = [&](int step) {
-        const int k = s_dir[step / nch], ch = step - (step / nch) * nch;
+    float4 bn[4][2];
+    auto loadA = [&](int step) {
+        const int ch = step / nd, k = s_dir[step - ch * nd];
         const uint4 *src = (const uint4 *)((const T *)a.Es + ((size_t)c_dirs[o][k][3] * nch + ch) * NSPLIT * R * 32);
 #pragma unroll
         for (int r = 0; r < AR; r++) {
             const int u = tid + r * NT;
             ra[r] = u < NA ? src[u] : uint4{0, 0, 0, 0};
         }
-#pragma unroll
-        for (int r = 0; r < BR; r++) {
-            const int u = tid + r * NT, col = u / 8, k4 = u % 8;
-            const int tb = col / cpt, cl = col - tb * cpt;
-            float4 v = {0, 0, 0, 0};
-            if (tb < TB) {
-                const int sid = s_sid[k * TB_MAX + tb];
-                if (sid >= 0)
-                    v = *(const float4 *)(a.P + ((size_t)(a.off + sid) * a.ncols + c0 + cl) * a.kpad + ch * 32 + 4 * k4);
-            }
-            rb[r] = v;
-        }
     };
-    f32x4 acc[RB][NCB];
-#pragma unroll
-    for (int i = 0; i < RB; i++)
-#pragma unroll
-        for (int j = 0; j < NCB; j++) acc[i][j] = f32x4{0, 0, 0, 0};
-
-    const int fr = lane % 16, fq = lane / 16;
-    if (nsteps > 0) load(0);
-    for (int step = 0; step < nsteps; step++) {
-        __syncthreads();  // every wave is done with the previous step's LDS
+    auto storeA = [&](int buf) {
 #pragma unroll
         for (int r = 0; r < AR; r++) {
             const int u = tid + r * NT;
             if (u < NA) {
                 const int p = u / (R * 4), row = (u / 4) % R, q = u % 4;
-                *(uint4 *)&As[(p * R + row) * KS + 8 * q] = ra[r];
+                *(uint4 *)&As[buf][(p * R + row) * KS + 8 * q] = ra[r];
             }
         }
+    };
+    auto loadB = [&](int step) {
+        const int ch = step / nd, k = s_dir[step - ch * nd];
 #pragma unroll
-        for (int r = 0; r < BR; r++) {
-            const int u = tid + r * NT;
-            *(float4 *)&Bs[(u / 8) * BS + 4 * (u % 8)] = rb[r];
+        for (int cb = 0; cb < 4; cb++) {
+            float4 v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0};
+            const int sid = btb[cb] >= 0 ? s_sid[k * TB_MAX + btb[cb]] : -1;
+            if (sid >= 0) {
+                const float4 *src = (const float4 *)(a.P + ((size_t)(a.off + sid) * a.ncols + c0 + bcl[cb]) * a.kpad +
+                                                     ch * 32 + 8 * fq);
+                v0 = src[0];
+                v1 = src[1];
+            }
+            bn[cb][0] = v0;
+            bn[cb][1] = v1;
         }
-        __syncthreads();
-        if (step + 1 < nsteps) load(step + 1);  // in flight during the MFMAs
-        V8 bfr[NCB][NSPLIT];
+    };
+    f32x4 acc[RBW][4];
 #pragma unroll
-        for (int cb = 0; cb < NCB; cb++) {
-            const float *bp = &Bs[((wave * NCB + cb) * 16 + fr) * BS + 8 * fq];
-            const float4 v0 = *(const float4 *)bp, v1 = *(const float4 *)(bp + 4);
-            float x[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+    for (int i = 0; i < RBW; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0, 0, 0, 0};
+
+    if (nsteps > 0) {
+        loadA(0);
+        storeA(0);
+        if (nsteps > 1) loadA(1);
+        loadB(0);
+    }
+    __syncthreads();
+    for (int step = 0; step < nsteps; step++) {
+        const int buf = step & 1;
+        V8 bfr[4][NSPLIT];
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++) {
+            float x[8] = {bn[cb][0].x, bn[cb][0].y, bn[cb][0].z, bn[cb][0].w,
+                          bn[cb][1].x, bn[cb][1].y, bn[cb][1].z, bn[cb][1].w};
 #pragma unroll
             for (int e = 0; e < 8; e++) x[e] *= xscale;
 #pragma unroll
@@ -237,40 +262,48 @@ __global__ __launch_bounds__(256, 2) void m2l_mfma_split(M2LMixArgs a)
                     x[e] -= (float)h;
                 }
         }
+        // the other LDS buffer was last read in step - 1, before the barrier
+        if (step + 1 < nsteps) storeA(buf ^ 1);
+        if (step + 2 < nsteps) loadA(step + 2);  // in flight during the MFMAs
+        if (step + 1 < nsteps) loadB(step + 1);
+        const T *Ab = As[buf];
 #pragma unroll
-        for (int rb = 0; rb < RB; rb++) {
+        for (int i = 0; i < RBW; i++) {
+            const int rb = wr * RBW + i;
+            if (rb >= RB) break;
             V8 afr[NSPLIT];
 #pragma unroll
-            for (int p = 0; p < NSPLIT; p++) afr[p] = *(const V8 *)&As[(p * R + rb * 16 + fr) * KS + 8 * fq];
+            for (int p = 0; p < NSPLIT; p++) afr[p] = *(const V8 *)&Ab[(p * R + rb * 16 + fr) * KS + 8 * fq];
 #pragma unroll
-            for (int cb = 0; cb < NCB; cb++) {
+            for (int cb = 0; cb < 4; cb++) {
                 // smallest terms first
                 if constexpr (NSPLIT == 3) {
-                    acc[rb][cb] = mfma16(afr[2], bfr[cb][0], acc[rb][cb]);
-                    acc[rb][cb] = mfma16(afr[1], bfr[cb][1], acc[rb][cb]);
-                    acc[rb][cb] = mfma16(afr[0], bfr[cb][2], acc[rb][cb]);
+                    acc[i][cb] = mfma16(afr[2], bfr[cb][0], acc[i][cb]);
+                    acc[i][cb] = mfma16(afr[1], bfr[cb][1], acc[i][cb]);
+                    acc[i][cb] = mfma16(afr[0], bfr[cb][2], acc[i][cb]);
                 }
-                acc[rb][cb] = mfma16(afr[1], bfr[cb][0], acc[rb][cb]);
-                acc[rb][cb] = mfma16(afr[0], bfr[cb][1], acc[rb][cb]);
-                acc[rb][cb] = mfma16(afr[0], bfr[cb][0], acc[rb][cb]);
+                acc[i][cb] = mfma16(afr[1], bfr[cb][0], acc[i][cb]);
+                acc[i][cb] = mfma16(afr[0], bfr[cb][1], acc[i][cb]);
+                acc[i][cb] = mfma16(afr[0], bfr[cb][0], acc[i][cb]);
             }
         }
+        __syncthreads();
     }
 #pragma unroll
-    for (int cb = 0; cb < NCB; cb++) {
-        const int col = (wave * NCB + cb) * 16 + fr;
-        const int tb = col / cpt, cl = col - tb * cpt;
-        if (tb >= TB) continue;
-        const int t = s_tgt[tb];
-        if (t < 0) continue;
-        float *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + cl) * rf;
+    for (int cb = 0; cb < 4; cb++) {
+        if (btb[cb] < 0) continue;
+        const int t = s_tgt[btb[cb]];
+        if (t < 0 || bcl[cb] >= cpt) continue;
+        float *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + bcl[cb]) * rf;
 #pragma unroll
-        for (int rb = 0; rb < RB; rb++)
+        for (int i = 0; i < RBW; i++) {
+            const int rb = wr * RBW + i;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int row = rb * 16 + 4 * fq + v;
-                if (row < rf) dst[row] = acc[rb][cb][v] * unscale;
+                if (rb < RB && row < rf) dst[row] = acc[i][cb][v] * unscale;
             }
+        }
     }
 }
 
@@ -325,9 +358,13 @@ template <typename T, int NSPLIT>
 static void m2l_split_t(const M2LMixArgs &a, dim3 grid, hipStream_t st)
 {
     const int rb = (a.rf + 15) / 16;
-#define PBB_CASE(RB)                                                      \
-    case RB:                                                              \
-        m2l_mfma_split<T, NSPLIT, RB, 2><<<grid, 256, 0, st>>>(a);        \
+    const int w = a.TB * a.cpt;  // tile width: 64, 128 or 256 columns
+    if (w > 256) fail(PBB_ERR_INTERNAL, "mixed-precision M2L tile wider than 256 columns");
+#define PBB_CASE(RB)                                                                   \
+    case RB:                                                                           \
+        if (w > 128) m2l_mfma_split<T, NSPLIT, RB, 4><<<grid, 512, 0, st>>>(a);        \
+        else if (w > 64) m2l_mfma_split<T, NSPLIT, RB, 2><<<grid, 256, 0, st>>>(a);    \
+        else m2l_mfma_split<T, NSPLIT, RB, 1><<<grid, 128, 0, st>>>(a);                \
         break;
     switch (rb) {
         PBB_CASE(1) PBB_CASE(2) PBB_CASE(3) PBB_CASE(4) PBB_CASE(5) PBB_CASE(6) PBB_CASE(7) PBB_CASE(8)

@@ -275,19 +275,32 @@ void precompute_ops(HostOps &ops, const HostEval &eval)
 
 // Far directions of an octant class: the children of the parent's 27
 // neighbours that are not among the target's own 27 neighbours
-// (InteractionList, compute.hpp:1062-1100, on a uniform grid).
+// (InteractionList, compute.hpp:1062-1100, on a uniform grid).  They are
+// listed in the order the M2L kernels visit them: grouped by the parity of
+// the offset and, inside a group, as a snake over its 3x3x3 grid of step 2
+// with z fastest.  Boxes of one class sit 2 apart, so consecutive directions
+// of a group reach largely the same source boxes for a tile of neighbouring
+// targets.
 void class_directions(int octant, const std::vector<int> &Ktable, int dirs[189][4])
 {
-    int o[3] = {(octant >> 2) & 1, (octant >> 1) & 1, octant & 1};
+    const int o[3] = {(octant >> 2) & 1, (octant >> 1) & 1, octant & 1};
     int k = 0;
-    for (int dx = -2 - o[0]; dx <= 3 - o[0]; dx++)
-        for (int dy = -2 - o[1]; dy <= 3 - o[1]; dy++)
-            for (int dz = -2 - o[2]; dz <= 3 - o[2]; dz++) {
-                if (std::abs(dx) <= 1 && std::abs(dy) <= 1 && std::abs(dz) <= 1) continue;
-                dirs[k][0] = dx; dirs[k][1] = dy; dirs[k][2] = dz;
-                dirs[k][3] = Ktable[(dx + 3) * 49 + (dy + 3) * 7 + (dz + 3)];
-                k++;
+    for (int par = 0; par < 8; par++) {
+        const int s0[3] = {-2 - o[0] + ((par >> 2) & 1), -2 - o[1] + ((par >> 1) & 1), -2 - o[2] + (par & 1)};
+        for (int i = 0; i < 3; i++)
+            for (int jj = 0; jj < 3; jj++) {
+                const int j = (i & 1) ? 2 - jj : jj;
+                for (int ll = 0; ll < 3; ll++) {
+                    const int l = ((i * 3 + jj) & 1) ? 2 - ll : ll;
+                    const int dx = s0[0] + 2 * i, dy = s0[1] + 2 * j, dz = s0[2] + 2 * l;
+                    if (std::abs(dx) <= 1 && std::abs(dy) <= 1 && std::abs(dz) <= 1) continue;
+                    dirs[k][0] = dx; dirs[k][1] = dy; dirs[k][2] = dz;
+                    dirs[k][3] = Ktable[(dx + 3) * 49 + (dy + 3) * 7 + (dz + 3)];
+                    k++;
+                }
             }
+    }
+    if (k != 189) fail(PBB_ERR_INTERNAL, "far-direction count is not 189");
 }
 
 }  // namespace pbb
