@@ -143,6 +143,13 @@ def main():
     prec = precs[args.precision]
     w = torch.tensor(w_host, dtype=torch.float64, device=dev)
     out = torch.zeros_like(w)
+    # M2L tile widths picked by the plan's autotuner (untimed, like the tree)
+    t0 = time.perf_counter()
+    widths = {"m2l_width": plan.tune_m2l(w, prec)}
+    if prec != capi.PREC_FP64 and args.fp64_steps > 0:
+        widths["m2l_width_fp64"] = plan.tune_m2l(w, capi.PREC_FP64)
+    widths["tune_s"] = round(time.perf_counter() - t0, 3)
+    log(f"m2l tiles {widths}")
     plan.set_timing(True)
 
     def timed(p, warmup, steps, tag):
@@ -239,6 +246,10 @@ def main():
         "ranks": [ops.rank_f, ops.rank_s],
         "m2l_pairs": info["m2l_pairs"], "near_pairs": info["near_pairs"],
         "precompute_s": round(t_pre, 3), "plan_s": round(t_plan, 3),
+        # as the reference times its constructor (distribution + matvec,
+        # compute.hpp:140-146): host tree build and upload plus one matvec
+        "with_plan_targets_per_s": N / (t_step + t_plan),
+        "tuning": widths,
     }
     print(json.dumps(line), flush=True)
 

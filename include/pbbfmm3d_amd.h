@@ -213,6 +213,20 @@ PBB_API pbb_status pbb_plan_timings(pbb_plan *plan, double *ms);
 /* Enables (1) or disables (0, default) per-phase event timing.            */
 PBB_API pbb_status pbb_plan_set_timing(pbb_plan *plan, int32_t enable);
 
+/* M2L tile width (columns of target boxes x weight columns that share each
+ * E_d load) for the FP64 mode (precision PBB_PREC_FP64: 32, 64 or 128) or
+ * the mixed modes (any mixed precision: 64, 128 or 256); 0 restores the
+ * built-in choice.  No reference counterpart: tuning of this build's M2L.  */
+PBB_API pbb_status pbb_plan_set_m2l_width(pbb_plan *plan, int32_t precision, int32_t width);
+
+/* Autotuner: times the M2L of one matvec on w_dev (device FP64, N x ncols)
+ * for every candidate width of the precision's class, keeps the fastest on
+ * the plan and returns it in *best_width (may be NULL).  Runs on `stream`,
+ * waits for it, and uses a scratch output of its own.  No reference
+ * counterpart.                                                            */
+PBB_API pbb_status pbb_plan_tune_m2l(pbb_plan *plan, const double *w_dev, int32_t ncols,
+                                     int32_t precision, void *stream, int32_t *best_width);
+
 PBB_API void pbb_plan_destroy(pbb_plan *plan);
 
 /* ---- batches of clouds (one HIP graph) -------------------------------- */

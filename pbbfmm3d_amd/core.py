@@ -280,6 +280,24 @@ class Plan:
     def set_timing(self, enable: bool = True):
         capi.check(capi.lib().pbb_plan_set_timing(self._h, 1 if enable else 0))
 
+    def set_m2l_width(self, width: int, precision: int = capi.PREC_FP64):
+        """Columns per M2L tile for the precision's class (0: built-in)."""
+        capi.check(capi.lib().pbb_plan_set_m2l_width(self._h, int(precision), int(width)))
+
+    def tune_m2l(self, weights, precision: int = capi.PREC_FP64) -> int:
+        """Times the M2L tile widths on these weights (float64 CUDA tensor,
+        (ncols, N) or (N,)), keeps the fastest and returns it."""
+        import torch
+        w = weights.contiguous()
+        ncols = w.numel() // self.npoints
+        if ncols * self.npoints != w.numel():
+            raise ValueError("weights must hold ncols * npoints values")
+        best = C.c_int32(0)
+        stream = torch.cuda.current_stream(w.device).cuda_stream
+        capi.check(capi.lib().pbb_plan_tune_m2l(self._h, C.c_void_p(w.data_ptr()), int(ncols), int(precision),
+                                                C.c_void_p(stream), C.byref(best)))
+        return best.value
+
     def timings(self) -> dict:
         ms = np.zeros(capi.NUM_PHASES)
         capi.check(capi.lib().pbb_plan_timings(self._h, ms.ctypes.data_as(C.POINTER(C.c_double))))

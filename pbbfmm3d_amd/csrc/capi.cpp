@@ -232,6 +232,9 @@ struct pbb_plan {
     double *x = nullptr, *y = nullptr, *z = nullptr;
     int32_t *perm = nullptr, *leaf_start = nullptr;
     int32_t *leaf_class = nullptr;              // leaves grouped by point count
+    int4 *up_items = nullptr, *up_reds = nullptr;  // direct upward pass work list
+    int n_up_items = 0, n_up_reds = 0, n_up_slots = 0;
+    double *up_scratch = nullptr;               // partial expansions of split boxes
     int leaf_class_off[dev::NEAR_CLASSES + 1] = {0};
     std::vector<DevLevel> lv;
     int64_t Btot = 0;
@@ -244,6 +247,7 @@ struct pbb_plan {
     int g_ncols = 0;
     int kpad = 0;
     bool timing = false;
+    int m2l_width[2] = {0, 0};  // FP64, mixed; 0 = built-in choice
     hipEvent_t ev[PBB_NUM_PHASES] = {};
     bool events_made = false;
     double last_ms[PBB_NUM_PHASES] = {0};
@@ -256,6 +260,7 @@ struct pbb_plan {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
         dfree(x); dfree(y); dfree(z); dfree(perm); dfree(leaf_start); dfree(leaf_class);
+        dfree(up_items); dfree(up_reds); dfree(up_scratch);
         for (auto &L : lv) {
             dfree(L.coord); dfree(L.map); dfree(L.child); dfree(L.order); dfree(L.ctab);
             dfree(L.pstart); dfree(L.seg_box); dfree(L.seg_p0);
@@ -276,13 +281,16 @@ struct pbb_plan {
         const HostOps &h = ops->h;
         if (ncols > ws_ncols) {
             dfree(ws); dfree(S); dfree(P); dfree(Pf); dfree(P32); dfree(Pf32); dfree(wsT); dfree(lp);
-            ws = S = P = Pf = wsT = lp = nullptr;
+            dfree(up_scratch);
+            ws = S = P = Pf = wsT = lp = up_scratch = nullptr;
             P32 = Pf32 = nullptr;
             ws_ncols = 0;
             ws_mixed = -1;
             ws = dalloc<double>((size_t)N * ncols, "sorted weights");
             wsT = dalloc<double>((size_t)N * ncols, "sorted weights, point-major");
             if (Lv >= 2) lp = dalloc<double>((size_t)N * ncols, "L2P results");
+            if (n_up_slots > 0)
+                up_scratch = dalloc<double>((size_t)n_up_slots * ncols * h.n3, "partial expansions");
             if (Lv >= 2) S = dalloc<double>((size_t)Btot * ncols * h.n3, "expansions");
             ws_ncols = ncols;
         }
@@ -512,6 +520,9 @@ PBB_API pbb_status pbb_plan_create(pbb_ops *ops, const double *xyz, int64_t npoi
         plan->Btot = off;
         // point ranges of the boxes of levels 2..L: boxes and points are both
         // in Morton order, so every box owns a contiguous range
+        if (off >= (1LL << 31)) fail(PBB_ERR_UNSUPPORTED, "too many boxes");
+        std::vector<int4> up_items, up_reds;
+        int n_slots = 0;
         for (int l = 2; l <= T.Lv; l++) {
             const Level &V = T.lv[l], &LL = T.lv[T.Lv];
             const int sh = T.Lv - l, S = 1 << l;
@@ -531,12 +542,27 @@ PBB_API pbb_status pbb_plan_create(pbb_ops *ops, const double *xyz, int64_t npoi
                     s0.push_back(q);
                 }
             }
+            for (int64_t b = 0; b < V.nb; b++) {  // direct upward pass work items
+                const int32_t np = ps[b + 1] - ps[b];
+                if (np <= dev::P2M_SEG) {
+                    up_items.push_back(int4{l - 2, (int)b, ps[b], -1});
+                    continue;
+                }
+                const int first = n_slots;
+                for (int32_t q = ps[b]; q < ps[b + 1]; q += dev::P2M_SEG) up_items.push_back(int4{l - 2, (int)b, q, n_slots++});
+                up_reds.push_back(int4{(int)(plan->lv[l].off + b), first, n_slots - first, 0});
+            }
             DevLevel &D = plan->lv[l];
             D.pstart = dupload(ps, "box point ranges");
             D.seg_box = dupload(sb, "L2P segments");
             D.seg_p0 = dupload(s0, "L2P segments");
             D.nseg = (int)sb.size();
         }
+        plan->up_items = dupload(up_items, "upward work items");
+        plan->up_reds = dupload(up_reds, "upward reductions");
+        plan->n_up_items = (int)up_items.size();
+        plan->n_up_reds = (int)up_reds.size();
+        plan->n_up_slots = n_slots;
         int dirs[8][189][4];
         for (int o = 0; o < 8; o++) class_directions(o, h.w.Ktable, dirs[o]);
         dev::upload_dirs(dirs);
@@ -578,6 +604,20 @@ PBB_API pbb_status pbb_plan_set_timing(pbb_plan *plan, int32_t enable)
     });
 }
 
+PBB_API pbb_status pbb_plan_set_m2l_width(pbb_plan *plan, int32_t precision, int32_t width)
+{
+    return guarded([&] {
+        if (!plan) fail(PBB_ERR_INVALID_ARG, "plan is NULL");
+        if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
+            fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
+        const bool mixed = precision != PBB_PREC_FP64;
+        const bool ok = width == 0 || (mixed ? (width == 64 || width == 128 || width == 256)
+                                             : (width == 32 || width == 64 || width == 128));
+        if (!ok) fail(PBB_ERR_INVALID_ARG, "M2L width must be 0, 64, 128 or 256 (mixed) / 32, 64 or 128 (FP64)");
+        plan->m2l_width[mixed ? 1 : 0] = width;
+    });
+}
+
 PBB_API pbb_status pbb_plan_timings(pbb_plan *plan, double *ms)
 {
     return guarded([&] {
@@ -593,6 +633,51 @@ PBB_API pbb_status pbb_plan_timings(pbb_plan *plan, double *ms)
         float tot = 0;
         check(hipEventElapsedTime(&tot, plan->ev[0], plan->ev[PBB_NUM_PHASES - 1]), "hipEventElapsedTime");
         ms[PBB_NUM_PHASES - 1] = tot;
+    });
+}
+
+static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, double *out, hipStream_t st, int prec);
+
+PBB_API pbb_status pbb_plan_tune_m2l(pbb_plan *plan, const double *w_dev, int32_t ncols, int32_t precision,
+                                     void *stream, int32_t *best_width)
+{
+    return guarded([&] {
+        if (!plan || !w_dev) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
+        if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
+            fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
+        const bool mixed = precision != PBB_PREC_FP64;
+        DeviceGuard g(plan->device);
+        hipStream_t st = (hipStream_t)stream;
+        const bool was_timing = plan->timing;
+        check(pbb_plan_set_timing(plan, 1) == PBB_OK ? hipSuccess : hipErrorUnknown, "enable timing");
+        double *scratch = dalloc<double>((size_t)plan->N * ncols, "tuning output");
+        const int cands[2][3] = {{32, 64, 128}, {64, 128, 256}};
+        int best = 0;
+        double best_ms = 0;
+        try {
+            // one untimed matvec builds the operator splits and workspaces
+            matvec_run(plan, plan->ops, w_dev, ncols, scratch, st, precision);
+            for (int c : cands[mixed ? 1 : 0]) {
+                plan->m2l_width[mixed ? 1 : 0] = c;
+                matvec_run(plan, plan->ops, w_dev, ncols, scratch, st, precision);
+                check(hipStreamSynchronize(st), "hipStreamSynchronize");
+                float t = 0;
+                check(hipEventElapsedTime(&t, plan->ev[3], plan->ev[4]), "hipEventElapsedTime");
+                if (best == 0 || t < best_ms) {
+                    best = c;
+                    best_ms = t;
+                }
+            }
+        } catch (...) {
+            dfree(scratch);
+            plan->timing = was_timing;
+            throw;
+        }
+        dfree(scratch);
+        plan->timing = was_timing;
+        plan->m2l_width[mixed ? 1 : 0] = best;
+        if (best_width) *best_width = best;
     });
 }
 
@@ -639,8 +724,13 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         dev::UpArgs a{};
         a.n = n; a.ncols = ncols;
         fill_levels(p, a.lt);
+        a.items = p->up_items;
         a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.wsT = p->wsT; a.S = p->S;
-        dev::launch_p2m_levels(a, st);
+        a.scratch = p->up_scratch;
+        dev::UpReduce r{};
+        r.ncols = ncols; r.n3 = n3; r.nred = p->n_up_reds; r.reds = p->up_reds;
+        r.scratch = p->up_scratch; r.S = p->S;
+        dev::launch_p2m_levels(a, p->n_up_items, r, st);
     } else if (Lv >= 2) {
         const DevLevel &LL = p->lv[Lv];
         dev::P2MArgs a{};
@@ -675,26 +765,28 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
             dev::launch_zero_u32(p->xmax + l, 1, st);
             dev::launch_absmax(p->P32 + (size_t)D.off * ncols * ldP, D.nb * ncols * ldP, p->xmax + l, st);
         }
+        if (prec == PBB_PREC_MIXED_F16X2 || prec == PBB_PREC_MIXED_BF16X2)
+            dev::launch_split_p2(prec, p->P32 + (size_t)D.off * ncols * ldP, D.nb * ncols * (ldP / 32),
+                                 prec == PBB_PREC_MIXED_F16X2 ? p->xmax + l : nullptr, st);
     }
     mark(3);
     // M2L tiles: W columns = TB target boxes x cpt columns.
     // FP64: W = 128 (64 for 17..31 columns), matrix-core bound, so few boxes
     // per tile keep the share of empty (box, direction) slots low.
-    // Mixed: W = 256 with at most 16 columns per box: tiles of 16
-    // neighbouring boxes (2x2x4 in Morton order), whose proxy gathers of
-    // consecutive directions overlap in L2.
+    // Mixed: W = 128 (4 boxes x 32 columns), two workgroups per CU.
     const int cpt = std::min(ncols, 64);
-    const int width = ncols >= 32 || ncols <= 16 ? 128 : 64;
+    const int width = p->m2l_width[0] ? p->m2l_width[0] : ncols >= 32 || ncols <= 16 ? 128 : 64;
     const int TB = std::max(1, std::min(16, width / cpt));
     const int nchunks = (ncols + cpt - 1) / cpt;
-    const int cptm = std::min(ncols, 16);
+    const int cptm = std::min(ncols, 32);
     const int nchunks_m = (ncols + cptm - 1) / cptm;
+    const int width_m = p->m2l_width[1] ? p->m2l_width[1] : 128;
     for (int l = 2; l <= Lv && mixed; l++) {
         const DevLevel &D = p->lv[l];
         dev::M2LMixArgs a{};
-        a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = std::max(1, std::min(16, 256 / cptm)); a.cpt = cptm;
+        a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = std::max(1, std::min(16, width_m / cptm)); a.cpt = cptm;
         a.side = 1 << l;
-        a.kpad = ldP; a.nch = SE->nch; a.escale = SE->escale[h.ops_set(l)]; a.off = D.off;
+        a.kpad = ldP; a.nch = SE->nch; a.escale = SE->escale[h.ops_set(l)]; a.off = D.off; a.R = SE->R;
         a.tile_prefix[0] = 0;
         a.class_start[0] = 0;
         for (int c = 0; c < 8; c++) {

@@ -338,63 +338,79 @@ __global__ __launch_bounds__(256) void gemm_small_a(GemmArgs g)
 // ---------------------------------------------------------------------------
 // The same GEMM on the FP64 matrix cores: a workgroup of 4 waves computes all
 // RB 16-row blocks of M x 64 columns (one 16-column block per wave), K in
-// chunks of 16 through LDS with the next chunk prefetched into registers.
+// chunks of 16.  A (small, shared by every workgroup) goes through a
+// double-buffered LDS tile; each lane loads its own B fragment straight from
+// memory, one chunk ahead, into two register sets used in turn.  Within a
+// chunk, lane group fk supplies k = 4 fk + ks at MFMA sub-step ks, so its
+// four B values are consecutive in memory.  Loads are branch-free (clamped
+// addresses, masked values) so that the waits on loads in flight are exact.
 template <typename TB, typename TC, int RB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gemm_mfma_f64(GemmArgs g)
 {
-    constexpr int NT = 256, W = 64, R = 16 * RB, KC = 16, AS = R + 2, BS = W + 2;
-    constexpr int AR = (KC * R + NT - 1) / NT, BR = KC * W / NT;
-    __shared__ double As[KC * AS];
-    __shared__ double Bs[KC * BS];
+    constexpr int NT = 256, W = 64, R = 16 * RB, KC = 16, AS = R + 2;
+    constexpr int AR = (KC * R + NT - 1) / NT;
+    __shared__ double As[2][KC * AS];
     const TB *B = (const TB *)g.B;
     TC *C = (TC *)g.C;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t n0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * W;
     const int nsteps = (g.K + KC - 1) / KC;
-    double ra[AR], rb[BR];
-    auto load = [&](int step) {
+    const int fr = lane % 16, fk = lane / 16;
+    const int64_t gc = n0 + wave * 16 + fr;  // this lane's B and C column
+    const TB *bcol = B + (gc < g.ncol ? gc : 0) * g.ldb;
+    double ra[AR];
+    auto loadA = [&](int step) {
         const int k0 = step * KC;
 #pragma unroll
         for (int r = 0; r < AR; r++) {
-            const int q = tid + r * NT, row = q % R, kk = q / R, gk = k0 + kk;
-            ra[r] = (q < KC * R && row < g.M && gk < g.K) ? g.A[row + (int64_t)gk * g.M] : 0.0;
+            const int q = tid + r * NT, row = q % R, gk = k0 + q / R;
+            const bool ok = q < KC * R && row < g.M && gk < g.K;
+            const double v = g.A[ok ? row + (int64_t)gk * g.M : 0];
+            ra[r] = ok ? v : 0.0;
         }
+    };
+    auto storeA = [&](int buf) {
 #pragma unroll
-        for (int r = 0; r < BR; r++) {
-            const int q = tid + r * NT, kk = q % KC, col = q / KC, gk = k0 + kk;
-            const int64_t gc = n0 + col;
-            rb[r] = (gk < g.K && gc < g.ncol) ? (double)B[gk + gc * g.ldb] : 0.0;
+        for (int r = 0; r < AR; r++) {
+            const int q = tid + r * NT;
+            if (AR * NT == KC * R || q < KC * R) As[buf][(q / R) * AS + q % R] = ra[r];
+        }
+    };
+    auto loadB = [&](int step, double (&b)[4]) {
+        const int k = step * KC + 4 * fk;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const double v = (double)bcol[min(k + e, g.K - 1)];
+            b[e] = k + e < g.K ? v : 0.0;
         }
     };
     dbl4 acc[RB];
 #pragma unroll
     for (int i = 0; i < RB; i++) acc[i] = dbl4{0, 0, 0, 0};
-    const int fr = lane % 16, fk = lane / 16;
-    load(0);
-    for (int step = 0; step < nsteps; step++) {
-        __syncthreads();
+    auto run_step = [&](int step, const double (&bcur)[4], double (&bnext)[4]) {
+        const int buf = step & 1;
+        storeA(buf ^ 1);  // last read in step - 1, before the barrier
+        loadA(min(step + 2, nsteps - 1));
+        loadB(min(step + 1, nsteps - 1), bnext);
 #pragma unroll
-        for (int r = 0; r < AR; r++) {
-            const int q = tid + r * NT;
-            if (q < KC * R) As[(q / R) * AS + q % R] = ra[r];
-        }
-#pragma unroll
-        for (int r = 0; r < BR; r++) {
-            const int q = tid + r * NT;
-            Bs[(q % KC) * BS + q / KC] = rb[r];
-        }
-        __syncthreads();
-        if (step + 1 < nsteps) load(step + 1);
-#pragma unroll 1
-        for (int ks = 0; ks < KC / 4; ks++) {
-            const double b = Bs[(ks * 4 + fk) * BS + wave * 16 + fr];
+        for (int ks = 0; ks < 4; ks++) {
+            const double *a = &As[buf][(4 * fk + ks) * AS + fr];
 #pragma unroll
             for (int rb_ = 0; rb_ < RB; rb_++)
-                acc[rb_] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[(ks * 4 + fk) * AS + rb_ * 16 + fr], b, acc[rb_], 0, 0,
-                                                               0);
+                acc[rb_] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb_ * 16], bcur[ks], acc[rb_], 0, 0, 0);
         }
+        __syncthreads();
+    };
+    double bA[4], bB[4];
+    loadA(0);
+    storeA(0);
+    loadA(min(1, nsteps - 1));
+    loadB(0, bA);
+    __syncthreads();
+    for (int step = 0; step < nsteps; step += 2) {
+        run_step(step, bA, bB);
+        if (step + 1 < nsteps) run_step(step + 1, bB, bA);
     }
-    const int64_t gc = n0 + wave * 16 + fr;
     if (gc < g.ncol) {
 #pragma unroll
         for (int rb_ = 0; rb_ < RB; rb_++)
@@ -416,30 +432,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
 // registers while the 189 far directions of the class stream through:
 //   Pf[:, tile] += E_d P[:, tile + d]     (empty source boxes give zeros)
 // ---------------------------------------------------------------------------
-// On the FP64 matrix cores (v_mfma_f64_16x16x4_f64): each wave owns one
-// 16-column block of the W = 16*NW column tile and all RB 16-row blocks of
-// r_f; per 4-deep k-step a wave reads one
-// B fragment and RB A fragments from LDS and issues RB independent MFMAs.
+// On the FP64 matrix cores (v_mfma_f64_16x16x4_f64).  The tile is W = 32*NWC
+// columns; its 2*NWC waves form a 2 x NWC grid: wave (wr, wc) holds row
+// blocks [wr*RBW, wr*RBW + RBW) x the two 16-column blocks of column group
+// wc.  The (direction, 16-deep k chunk) steps form one software pipeline
+// with one barrier per step: E chunks go through a double-buffered LDS tile;
+// each lane loads the proxies of its own B fragments straight from memory,
+// one step ahead, into two register sets used in turn.  Within a chunk, lane
+// group fk supplies k = 4 fk + ks at MFMA sub-step ks, so its four proxy
+// values are consecutive in memory.  Loads are branch-free (clamped
+// addresses, masked values) so that the waits on loads in flight are exact.
 // Fragment layout of the 16x16x4 FP64 MFMA (lane l of 64):
 //   A[i][k]: i = l % 16, k = l / 16      B[k][j]: j = l % 16, k = l / 16
 //   D[i][j]: j = l % 16, i = l / 16 + 4*v     (v = 0..3, one double each)
 
 constexpr int TB_MAX = 16;
 
-template <int RB, int NW>
-__global__ __launch_bounds__(NW * 64) void m2l_mfma_f64(M2LArgs a)
+template <int RB, int NWC>
+__global__ __launch_bounds__(NWC * 128) void m2l_mfma_f64(M2LArgs a)
 {
-    constexpr int NT = NW * 64, W = 16 * NW, R = 16 * RB, KC = 16;
-    constexpr int AS = R + 2, BS = W + 2;  // padded LDS row strides
-    constexpr int AR = (KC * R + NT - 1) / NT, BR = KC * W / NT;
-    __shared__ double As[KC * AS];
-    __shared__ double Bs[KC * BS];
+    constexpr int NT = NWC * 128, R = 16 * RB, KC = 16, AS = R + 2, RBW = (RB + 1) / 2;
+    constexpr int AR = (KC * R + NT - 1) / NT;
+    __shared__ double As[2][KC * AS];
     __shared__ int s_sid[189 * TB_MAX];
     __shared__ short s_dir[189];
     __shared__ int s_nd;
     __shared__ int s_tgt[TB_MAX];
     __shared__ int s_xyz[TB_MAX][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave & 1, wc = wave >> 1;
     const int rf = a.rf, rs = a.rs, TB = a.TB;
     const int nch = (rs + KC - 1) / KC;
     const int lin = xcd_tile(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
@@ -496,70 +517,110 @@ __global__ __launch_bounds__(NW * 64) void m2l_mfma_f64(M2LArgs a)
     // the same source boxes (see class_directions)
     const int nd = s_nd, nsteps = nd * nch;
 
-    double ra[AR], rb[BR];
-    auto load = [&](int step) {
+    const int fr = lane % 16, fk = lane / 16;
+    int btb[2], bcl[2];  // box slot and column of this lane's B column in each block
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) {
+        const int col = wc * 32 + cb * 16 + fr;
+        const int tb = col / cpt;
+        btb[cb] = tb < TB ? tb : -1;
+        bcl[cb] = col - tb * cpt;
+    }
+    double ra[AR];
+    auto loadA = [&](int step) {
         const int ch = step / nd, k = s_dir[step - ch * nd], k0 = ch * KC;
         const double *Ed = a.E + (size_t)c_dirs[o][k][3] * rf * rs;
 #pragma unroll
         for (int r = 0; r < AR; r++) {
-            const int q = tid + r * NT, row = q % R, kk = q / R, gk = k0 + kk;
-            ra[r] = (q < KC * R && row < rf && gk < rs) ? Ed[row + (size_t)gk * rf] : 0.0;
-        }
-#pragma unroll
-        for (int r = 0; r < BR; r++) {
-            const int q = tid + r * NT, kk = q % KC, col = q / KC, gk = k0 + kk;
-            const int tb = col / cpt, cl = col - tb * cpt;
-            double v = 0.0;
-            if (tb < TB && gk < rs) {
-                const int sid = s_sid[k * TB_MAX + tb];
-                if (sid >= 0) v = a.P[((size_t)(a.off + sid) * a.ncols + c0 + cl) * rs + gk];
-            }
-            rb[r] = v;
+            const int q = tid + r * NT, row = q % R, gk = k0 + q / R;
+            const bool ok = q < KC * R && row < rf && gk < rs;
+            const double v = Ed[ok ? row + (size_t)gk * rf : 0];
+            ra[r] = ok ? v : 0.0;
         }
     };
-    dbl4 acc[RB];
-#pragma unroll
-    for (int i = 0; i < RB; i++) acc[i] = dbl4{0, 0, 0, 0};
-
-    const int fr = lane % 16, fk = lane / 16;
-    if (nsteps > 0) load(0);
-    for (int step = 0; step < nsteps; step++) {
-        __syncthreads();
+    auto storeA = [&](int buf) {
 #pragma unroll
         for (int r = 0; r < AR; r++) {
             const int q = tid + r * NT;
-            if (q < KC * R) As[(q / R) * AS + q % R] = ra[r];
+            if (AR * NT == KC * R || q < KC * R) As[buf][(q / R) * AS + q % R] = ra[r];
         }
+    };
+    // a column without a source box loads box 0 of the level (always valid)
+    // and is zeroed through the mask at its use
+    auto loadB = [&](int step, double (&bn)[2][4], int &okm) {
+        const int ch = step / nd, k = s_dir[step - ch * nd], k0 = ch * KC + 4 * fk;
+        okm = 0;
 #pragma unroll
-        for (int r = 0; r < BR; r++) {
-            const int q = tid + r * NT;
-            Bs[(q % KC) * BS + q / KC] = rb[r];
-        }
-        __syncthreads();
-        if (step + 1 < nsteps) load(step + 1);
-#pragma unroll 1
-        for (int ks = 0; ks < KC / 4; ks++) {
-            const double b = Bs[(ks * 4 + fk) * BS + wave * 16 + fr];
+        for (int cb = 0; cb < 2; cb++) {
+            const int sid0 = btb[cb] >= 0 ? s_sid[k * TB_MAX + btb[cb]] : -1;
+            okm |= (sid0 >= 0) << cb;
+            const double *col = a.P + ((size_t)(a.off + max(sid0, 0)) * a.ncols + c0 + bcl[cb]) * rs;
 #pragma unroll
-            for (int rb_ = 0; rb_ < RB; rb_++) {
-                const double av = As[(ks * 4 + fk) * AS + rb_ * 16 + fr];
-                acc[rb_] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b, acc[rb_], 0, 0, 0);
+            for (int e = 0; e < 4; e++) {
+                const double v = col[min(k0 + e, rs - 1)];
+                bn[cb][e] = k0 + e < rs ? v : 0.0;
             }
         }
+    };
+    dbl4 acc[RBW][2];
+#pragma unroll
+    for (int i = 0; i < RBW; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = dbl4{0, 0, 0, 0};
+    auto run_step = [&](int step, const double (&bcur)[2][4], int okcur, double (&bnext)[2][4], int &oknext) {
+        const int buf = step & 1;
+        double b[2][4];
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            const bool ok = (okcur >> cb) & 1;
+#pragma unroll
+            for (int e = 0; e < 4; e++) b[cb][e] = ok ? bcur[cb][e] : 0.0;
+        }
+        storeA(buf ^ 1);  // last read in step - 1, before the barrier
+        loadA(min(step + 2, nsteps - 1));
+        loadB(min(step + 1, nsteps - 1), bnext, oknext);
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+            const double *ap = &As[buf][(4 * fk + ks) * AS + fr];
+#pragma unroll
+            for (int i = 0; i < RBW; i++) {
+                const int rb_ = wr * RBW + i;
+                if (rb_ >= RB) break;
+                const double av = ap[rb_ * 16];
+#pragma unroll
+                for (int cb = 0; cb < 2; cb++)
+                    acc[i][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[cb][ks], acc[i][cb], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    };
+    double bA[2][4], bB[2][4];
+    int okA = 0, okB = 0;
+    if (nsteps > 0) {
+        loadA(0);
+        storeA(0);
+        loadA(min(1, nsteps - 1));
+        loadB(0, bA, okA);
     }
-    const int col = wave * 16 + fr;
-    const int tb = col / cpt, cl = col - tb * cpt;
-    if (tb < TB) {
-        const int t = s_tgt[tb];
-        if (t >= 0) {
-            double *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + cl) * rf;
+    __syncthreads();
+    for (int step = 0; step < nsteps; step += 2) {
+        run_step(step, bA, okA, bB, okB);
+        if (step + 1 < nsteps) run_step(step + 1, bB, okB, bA, okA);
+    }
 #pragma unroll
-            for (int rb_ = 0; rb_ < RB; rb_++)
+    for (int cb = 0; cb < 2; cb++) {
+        if (btb[cb] < 0) continue;
+        const int t = s_tgt[btb[cb]];
+        if (t < 0) continue;
+        double *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + bcl[cb]) * rf;
 #pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const int row = rb_ * 16 + fk + 4 * v;
-                    if (row < rf) dst[row] = acc[rb_][v];
-                }
+        for (int i = 0; i < RBW; i++) {
+            const int rb_ = wr * RBW + i;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = rb_ * 16 + fk + 4 * v;
+                if (rb_ < RB && row < rf) dst[row] = acc[i][cb][v];
+            }
         }
     }
 }
@@ -801,28 +862,27 @@ __global__ __launch_bounds__(256) void l2p_fp64(DownArgs a)
 // 234-333, :956-1034) equals P2M at every level straight from the points:
 // the child-to-parent transfer interpolates the parent's Chebyshev
 // polynomials S_m, which have degree n-1, at the child's nodes, which is
-// exact.  One workgroup per box of levels 2..L (coarse levels first, so
-// their long point loops start early); thread pencils (column c, l1, l2)
-// hold the n values of l3 in registers.
+// exact.  One workgroup per work item (a box of levels 2..L, or a segment of
+// P2M_SEG points of a larger box); thread pencils (column c, l1, l2) hold
+// the n values of l3 in registers.
 template <int N>
 __global__ __launch_bounds__(256) void p2m_levels(UpArgs a)
 {
-    constexpr int N2 = N * N, N3 = N2 * N, PTS = 64;
+    constexpr int N2 = N * N, N3 = N2 * N, PTS = 64, PP = PTS + 1;
     constexpr int CC = N <= 6 ? 32 : 16;        // columns per pass
     constexpr int PR = (CC * N2 + 255) / 256;   // pencils per thread
-    __shared__ double sS[3][N][PTS];
+    __shared__ double sS[3][N][PP];
     __shared__ double sw[PTS][CC + 1];
     const int tid = threadIdx.x;
-    const int64_t g = blockIdx.x;
-    int li = 0;
-    while (li + 1 < a.lt.nlev && g >= a.lt.off[li + 1]) li++;
-    const int b = (int)(g - a.lt.off[li]);
-    const int p0 = a.lt.pstart[li][b], p1 = a.lt.pstart[li][b + 1];
+    const int4 it = a.items[blockIdx.x];
+    const int li = it.x, b = it.y, p0 = it.z;
+    const int p1 = min(p0 + P2M_SEG, a.lt.pstart[li][b + 1]);
     const uint32_t cc = (uint32_t)a.lt.coord[li][b];
     const double *ctab = a.lt.ctab[li];
     const double cen[3] = {ctab[cc & 1023], ctab[(cc >> 10) & 1023], ctab[(cc >> 20) & 1023]};
     const double ihalfL = 2.0 / a.lt.len[li];
-    double *Sb = a.S + (size_t)g * a.ncols * N3;
+    double *Sb = it.w >= 0 ? a.scratch + (size_t)it.w * a.ncols * N3
+                           : a.S + (size_t)(a.lt.off[li] + b) * a.ncols * N3;
     for (int c0 = 0; c0 < a.ncols; c0 += CC) {
         const int cn = min(CC, a.ncols - c0);
         double acc[PR][N];
@@ -879,6 +939,22 @@ __global__ __launch_bounds__(256) void p2m_levels(UpArgs a)
                 }
             }
         }
+    }
+}
+
+// S of the split boxes: the sum of their segments' partial expansions, in
+// segment order
+__global__ void p2m_reduce(UpReduce r)
+{
+    const int64_t len = (int64_t)r.ncols * r.n3;
+    const int64_t total = (int64_t)r.nred * len;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = q / len, e = q - k * len;
+        const int4 rd = r.reds[k];
+        double s = 0.0;
+        for (int i = 0; i < rd.z; i++) s += r.scratch[(int64_t)(rd.y + i) * len + e];
+        r.S[(int64_t)rd.x * len + e] = s;
     }
 }
 
@@ -1389,11 +1465,13 @@ void launch_m2l(const M2LArgs &a, int ntiles, int ncolchunks, hipStream_t st)
     if (ntiles <= 0) return;
     dim3 grid((unsigned)ntiles, (unsigned)ncolchunks);
     const int rb = (a.rf + 15) / 16;
-    const int width = a.TB * a.cpt <= 64 ? 64 : 128;  // tile width chosen by the caller
+    const int w = a.TB * a.cpt;  // tile width chosen by the caller: at most 128 columns
+    if (w > 128) fail(PBB_ERR_INTERNAL, "FP64 M2L tile wider than 128 columns");
 #define PBB_M2L_CASE(R)                                                        \
     case R:                                                                    \
-        if (width == 64) m2l_mfma_f64<R, 4><<<grid, 256, 0, st>>>(a);          \
-        else m2l_mfma_f64<R, 8><<<grid, 512, 0, st>>>(a);                      \
+        if (w > 64) m2l_mfma_f64<R, 4><<<grid, 512, 0, st>>>(a);               \
+        else if (w > 32) m2l_mfma_f64<R, 2><<<grid, 256, 0, st>>>(a);          \
+        else m2l_mfma_f64<R, 1><<<grid, 128, 0, st>>>(a);                      \
         break;
     switch (rb) {
         PBB_M2L_CASE(1) PBB_M2L_CASE(2) PBB_M2L_CASE(3) PBB_M2L_CASE(4) PBB_M2L_CASE(5) PBB_M2L_CASE(6)
@@ -1457,20 +1535,24 @@ bool launch_near_split(const NearArgs &a, int kid, int nleaf, hipStream_t st)
     return true;
 }
 
-bool launch_p2m_levels(const UpArgs &a, hipStream_t st)
+bool launch_p2m_levels(const UpArgs &a, int nitems, const UpReduce &r, hipStream_t st)
 {
     if (a.n < 4 || a.n > 8) return false;
-    const int64_t nb = a.lt.off[a.lt.nlev];
-    if (nb <= 0) return true;
-    if (nb > 0x7fffffff) fail(PBB_ERR_UNSUPPORTED, "too many boxes");
+    if (nitems <= 0) return true;
     switch (a.n) {
-    case 4: p2m_levels<4><<<(unsigned)nb, 256, 0, st>>>(a); break;
-    case 5: p2m_levels<5><<<(unsigned)nb, 256, 0, st>>>(a); break;
-    case 6: p2m_levels<6><<<(unsigned)nb, 256, 0, st>>>(a); break;
-    case 7: p2m_levels<7><<<(unsigned)nb, 256, 0, st>>>(a); break;
-    default: p2m_levels<8><<<(unsigned)nb, 256, 0, st>>>(a); break;
+    case 4: p2m_levels<4><<<nitems, 256, 0, st>>>(a); break;
+    case 5: p2m_levels<5><<<nitems, 256, 0, st>>>(a); break;
+    case 6: p2m_levels<6><<<nitems, 256, 0, st>>>(a); break;
+    case 7: p2m_levels<7><<<nitems, 256, 0, st>>>(a); break;
+    default: p2m_levels<8><<<nitems, 256, 0, st>>>(a); break;
     }
     check(hipGetLastError(), "p2m_levels");
+    if (r.nred > 0) {
+        const int64_t total = (int64_t)r.nred * r.ncols * r.n3;
+        const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
+        p2m_reduce<<<blocks, 256, 0, st>>>(r);
+        check(hipGetLastError(), "p2m_reduce");
+    }
     return true;
 }
 

@@ -61,10 +61,23 @@ struct LevelTab {
     double len[MAX_FAR_LEVELS];
 };
 
+constexpr int P2M_SEG = 2048;  // points per workgroup of the direct upward pass
+
 struct UpArgs {
     int n, ncols;
     LevelTab lt;
+    // work items (level index, box, first point, scratch slot or -1), coarse
+    // levels first; boxes of more than P2M_SEG points are split into items
+    // whose partial expansions go to scratch slots and are summed afterwards
+    const int4 *items;
     const double *Tkz, *x, *y, *z, *wsT;
+    double *S, *scratch;
+};
+
+struct UpReduce {
+    int ncols, n3, nred;
+    const int4 *reds;          // (global box id, first slot, slot count, 0)
+    const double *scratch;
     double *S;
 };
 
@@ -129,7 +142,7 @@ struct DirectArgs {
 };
 
 struct M2LMixArgs {
-    int rf, rs, ncols, TB, cpt, side, kpad, nch, escale;
+    int rf, rs, ncols, TB, cpt, side, kpad, nch, escale, R;  // R: rows of the split E (16-multiple)
     int64_t off;
     int tile_prefix[9];
     int64_t class_start[9];
@@ -159,6 +172,8 @@ void launch_fill_zero(double *out, int64_t n, hipStream_t st);
 void launch_split_e(int variant, const double *E, int rf, int rs, int R, int nch, int escale, void *Es,
                     hipStream_t st);
 void launch_absmax(const float *x, int64_t n, unsigned *out, hipStream_t st);
+// two-part modes: rewrite nchunks 32-float k-chunks of P as scaled 16-bit parts
+void launch_split_p2(int variant, float *P, int64_t nchunks, const unsigned *xmax, hipStream_t st);
 void upload_dirs_mixed(const int dirs[8][189][4]);
 void launch_zero_u32(unsigned *p, int n, hipStream_t st);
 void launch_m2l_mixed(int variant, const M2LMixArgs &a, int ntiles, int ncolchunks, hipStream_t st);
@@ -172,7 +187,7 @@ void launch_near(const NearArgs &a, int kid, int nleaf, hipStream_t st);
 // the split near field for orders 4..8 (false: order not covered)
 bool launch_near_split(const NearArgs &a, int kid, int nleaf, hipStream_t st);
 // direct upward pass: S of every box of levels 2..L from its points
-bool launch_p2m_levels(const UpArgs &a, hipStream_t st);
+bool launch_p2m_levels(const UpArgs &a, int nitems, const UpReduce &r, hipStream_t st);
 // direct downward pass, one level: lp (+)= L2P of the level's F at its points
 bool launch_l2p_level(const DownArgs &a, int nseg, hipStream_t st);
 void launch_gather_weights_t(const double *w, const int32_t *perm, int64_t N, int ncols, double *wsT,

@@ -89,39 +89,100 @@ __global__ void absmax_f32(const float *__restrict__ x, int64_t n, unsigned *__r
     if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
 }
 
+// FP16 operands are scaled into range: P by 2^xs with xs from the level's
+// max |P| (its FP32 bits in *xmax), E by 2^escale per operator set
+__device__ __forceinline__ int p_scale_exp(const unsigned *xmax)
+{
+    int xs = 0;
+    if (xmax) {
+        const float mx = __uint_as_float(*xmax);
+        if (mx > 0.0f) {
+            int e;
+            frexpf(mx, &e);
+            xs = 12 - e;
+        }
+    }
+    return xs;
+}
+
+// Two-part proxies, in place: every 32-float k-chunk of P (128 bytes) is
+// rewritten as 32 high then 32 low 16-bit parts of x * 2^xs, so that the
+// M2L loads its B fragments ready-made.  One thread per chunk.
+template <typename T>
+__global__ void split_p2(float *__restrict__ P, int64_t nchunks, const unsigned *__restrict__ xmax)
+{
+    const float xscale = ldexpf(1.0f, p_scale_exp(xmax));
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nchunks;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        float4 *c = (float4 *)(P + q * 32);
+        float x[32];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const float4 v = c[i];
+            x[4 * i] = v.x * xscale;
+            x[4 * i + 1] = v.y * xscale;
+            x[4 * i + 2] = v.z * xscale;
+            x[4 * i + 3] = v.w * xscale;
+        }
+        T h[32], l[32];
+#pragma unroll
+        for (int e = 0; e < 32; e++) {
+            h[e] = (T)x[e];
+            l[e] = (T)(x[e] - (float)h[e]);
+        }
+        uint4 *o = (uint4 *)c;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            o[i] = *(const uint4 *)&h[8 * i];
+            o[4 + i] = *(const uint4 *)&l[8 * i];
+        }
+    }
+}
+
+void launch_split_p2(int variant, float *P, int64_t nchunks, const unsigned *xmax, hipStream_t st)
+{
+    if (nchunks <= 0) return;
+    const int blocks = (int)std::min<int64_t>((nchunks + 255) / 256, 65536);
+    if (variant == PBB_PREC_MIXED_F16X2) split_p2<_Float16><<<blocks, 256, 0, st>>>(P, nchunks, xmax);
+    else if (variant == PBB_PREC_MIXED_BF16X2) split_p2<__bf16><<<blocks, 256, 0, st>>>(P, nchunks, xmax);
+    else fail(PBB_ERR_INVALID_ARG, "proxies are pre-split for the two-part modes only");
+    check(hipGetLastError(), "split_p2");
+}
+
 // ---------------------------------------------------------------------------
-// One workgroup owns TB (<= 16) target boxes of one octant class x cpt
-// columns (W = 64*NWC columns).  Its 2*NWC waves form a 2 x NWC grid: wave
-// (wr, wc) holds row blocks [wr*RBW, wr*RBW + RBW) x the 4 16-column blocks
-// of column group wc in FP32 accumulators.  The (direction, 32-deep k chunk)
-// steps of the tile's non-empty directions form one software pipeline with
-// one barrier per step:
-//   E chunks (pre-split) are staged in a double-buffered LDS tile, the next
-//   one written while the current one feeds the MFMAs;
-//   each lane loads the proxies of its own B fragments straight from global
-//   memory into registers one step ahead, and splits them into 16-bit parts.
+// One workgroup of 4 waves owns TB (<= 16) target boxes of one octant class
+// x cpt columns (W = 64*NCB columns); each wave holds NCB 16-column blocks x
+// RB 16-row blocks of FP32 accumulators.  The (direction, 32-deep k chunk)
+// steps of the tile's non-empty directions form one software pipeline: the
+// next step's E chunk (pre-split) and proxy chunk are loaded into registers
+// while the current step's MFMAs run from LDS.  Steps run direction-major
+// (the k chunks of one E_d, then the next direction), so the proxy gathers of
+// consecutive steps continue along the same columns.  Two workgroups share a
+// CU and cover each other's stalls.  Two-part modes read proxies pre-split by
+// split_p2 ([32 high][32 low] 16-bit parts per k chunk, the same 128 bytes
+// as 32 floats); the three-part mode splits FP32 proxies in registers.
 constexpr int TB_MAX = 16;
 
-template <typename T, int NSPLIT, int RB, int NWC>
-__global__ __launch_bounds__(NWC * 128) void m2l_mfma_split(M2LMixArgs a)
+template <typename T, int NSPLIT, int RB, int NCB>
+__global__ __launch_bounds__(256, NCB <= 2 ? 2 : 1) void m2l_mfma_split(M2LMixArgs a)
 {
     typedef typename Vec8<T>::type V8;
-    constexpr int NW = 2 * NWC, NT = NW * 64, R = 16 * RB, KS = 40, RBW = (RB + 1) / 2;
+    constexpr int NT = 256, W = 64 * NCB, R = 16 * RB, KS = 40, BS = 36;
     constexpr int NA = NSPLIT * R * 4;  // 16-byte units of one E chunk
     constexpr int AR = (NA + NT - 1) / NT;
-    __shared__ __attribute__((aligned(16))) T As[2][NSPLIT * R * KS];
+    constexpr int BR = W * 8 / NT;      // 16-byte units of one proxy chunk per thread
+    constexpr bool PRESPLIT = NSPLIT == 2;
+    __shared__ __attribute__((aligned(16))) T As[NSPLIT * R * KS];
+    __shared__ __attribute__((aligned(16))) float Bs[W * BS];
     __shared__ int s_sid[189 * TB_MAX];
     __shared__ short s_dir[189];
     __shared__ int s_nd;
     __shared__ int s_tgt[TB_MAX];
     __shared__ int s_xyz[TB_MAX][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = wave & 1, wc = wave >> 1;
     const int rf = a.rf, TB = a.TB, nch = a.nch;
     const int lin = xcd_tile(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
-    // the column chunks of one tile are adjacent in launch order: they run
-    // side by side on one XCD through the same (direction, k-chunk) steps and
-    // share each E chunk through its L2
+    // the column chunks of one tile are adjacent in launch order
     const int tile = lin / gridDim.y;
     int o = 0;
     while (o < 7 && tile >= a.tile_prefix[o + 1]) o++;
@@ -130,16 +191,7 @@ __global__ __launch_bounds__(NWC * 128) void m2l_mfma_split(M2LMixArgs a)
     const int c0 = (lin - tile * (int)gridDim.y) * a.cpt;
     const int cpt = min(a.cpt, a.ncols - c0);
     const int S = a.side;
-    // FP16 operands are scaled into range by 2^xs (P) and 2^escale (E)
-    int xs = 0;
-    if (a.xmax) {
-        float mx = __uint_as_float(*a.xmax);
-        if (mx > 0.0f) {
-            int e;
-            frexpf(mx, &e);
-            xs = 12 - e;
-        }
-    }
+    const int xs = p_scale_exp(a.xmax);
     const float xscale = ldexpf(1.0f, xs);
     const float unscale = ldexpf(1.0f, -(xs + a.escale));
     if (tid < TB) {
@@ -180,130 +232,112 @@ __global__ __launch_bounds__(NWC * 128) void m2l_mfma_split(M2LMixArgs a)
         s_nd = nd;
     }
     __syncthreads();
-    // steps run k-chunk-major: all directions of one chunk, then the next
-    // chunk, so that consecutive steps gather the same proxy rows of largely
-    // the same source boxes (see class_directions)
-    const int nd = s_nd, nsteps = nd * nch;
+    const int nsteps = s_nd * nch;
 
-    const int fr = lane % 16, fq = lane / 16;
-    int btb[4], bcl[4];  // box slot and column of this lane's B column in each block
-#pragma unroll
-    for (int cb = 0; cb < 4; cb++) {
-        const int col = wc * 64 + cb * 16 + fr;
-        const int tb = col / cpt;
-        btb[cb] = tb < TB ? tb : -1;
-        bcl[cb] = col - tb * cpt;
-    }
     uint4 ra[AR];
-    float4 bn[4][2];
-    auto loadA = [&](int step) {
-        const int ch = step / nd, k = s_dir[step - ch * nd];
+    float4 rb[BR];
+    auto load = [&](int step) {
+        const int di = step / nch, ch = step - di * nch, k = s_dir[di];
         const uint4 *src = (const uint4 *)((const T *)a.Es + ((size_t)c_dirs[o][k][3] * nch + ch) * NSPLIT * R * 32);
 #pragma unroll
         for (int r = 0; r < AR; r++) {
             const int u = tid + r * NT;
             ra[r] = u < NA ? src[u] : uint4{0, 0, 0, 0};
         }
+#pragma unroll
+        for (int r = 0; r < BR; r++) {
+            const int u = tid + r * NT, col = u / 8, k4 = u % 8;
+            const int tb = col / cpt, cl = col - tb * cpt;
+            float4 v = {0, 0, 0, 0};
+            if (tb < TB) {
+                const int sid = s_sid[k * TB_MAX + tb];
+                if (sid >= 0)
+                    v = *(const float4 *)(a.P + ((size_t)(a.off + sid) * a.ncols + c0 + cl) * a.kpad + ch * 32 + 4 * k4);
+            }
+            rb[r] = v;
+        }
     };
-    auto storeA = [&](int buf) {
+    f32x4 acc[RB][NCB];
+#pragma unroll
+    for (int i = 0; i < RB; i++)
+#pragma unroll
+        for (int j = 0; j < NCB; j++) acc[i][j] = f32x4{0, 0, 0, 0};
+
+    const int fr = lane % 16, fq = lane / 16;
+    if (nsteps > 0) load(0);
+    for (int step = 0; step < nsteps; step++) {
+        __syncthreads();  // every wave is done with the previous step's LDS
 #pragma unroll
         for (int r = 0; r < AR; r++) {
             const int u = tid + r * NT;
             if (u < NA) {
                 const int p = u / (R * 4), row = (u / 4) % R, q = u % 4;
-                *(uint4 *)&As[buf][(p * R + row) * KS + 8 * q] = ra[r];
+                *(uint4 *)&As[(p * R + row) * KS + 8 * q] = ra[r];
             }
         }
-    };
-    auto loadB = [&](int step) {
-        const int ch = step / nd, k = s_dir[step - ch * nd];
 #pragma unroll
-        for (int cb = 0; cb < 4; cb++) {
-            float4 v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0};
-            const int sid = btb[cb] >= 0 ? s_sid[k * TB_MAX + btb[cb]] : -1;
-            if (sid >= 0) {
-                const float4 *src = (const float4 *)(a.P + ((size_t)(a.off + sid) * a.ncols + c0 + bcl[cb]) * a.kpad +
-                                                     ch * 32 + 8 * fq);
-                v0 = src[0];
-                v1 = src[1];
-            }
-            bn[cb][0] = v0;
-            bn[cb][1] = v1;
-        }
-    };
-    f32x4 acc[RBW][4];
-#pragma unroll
-    for (int i = 0; i < RBW; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0, 0, 0, 0};
-
-    if (nsteps > 0) {
-        loadA(0);
-        storeA(0);
-        if (nsteps > 1) loadA(1);
-        loadB(0);
-    }
-    __syncthreads();
-    for (int step = 0; step < nsteps; step++) {
-        const int buf = step & 1;
-        V8 bfr[4][NSPLIT];
-#pragma unroll
-        for (int cb = 0; cb < 4; cb++) {
-            float x[8] = {bn[cb][0].x, bn[cb][0].y, bn[cb][0].z, bn[cb][0].w,
-                          bn[cb][1].x, bn[cb][1].y, bn[cb][1].z, bn[cb][1].w};
-#pragma unroll
-            for (int e = 0; e < 8; e++) x[e] *= xscale;
-#pragma unroll
-            for (int p = 0; p < NSPLIT; p++)
-#pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    const T h = (T)x[e];
-                    bfr[cb][p][e] = h;
-                    x[e] -= (float)h;
-                }
-        }
-        // the other LDS buffer was last read in step - 1, before the barrier
-        if (step + 1 < nsteps) storeA(buf ^ 1);
-        if (step + 2 < nsteps) loadA(step + 2);  // in flight during the MFMAs
-        if (step + 1 < nsteps) loadB(step + 1);
-        const T *Ab = As[buf];
-#pragma unroll
-        for (int i = 0; i < RBW; i++) {
-            const int rb = wr * RBW + i;
-            if (rb >= RB) break;
-            V8 afr[NSPLIT];
-#pragma unroll
-            for (int p = 0; p < NSPLIT; p++) afr[p] = *(const V8 *)&Ab[(p * R + rb * 16 + fr) * KS + 8 * fq];
-#pragma unroll
-            for (int cb = 0; cb < 4; cb++) {
-                // smallest terms first
-                if constexpr (NSPLIT == 3) {
-                    acc[i][cb] = mfma16(afr[2], bfr[cb][0], acc[i][cb]);
-                    acc[i][cb] = mfma16(afr[1], bfr[cb][1], acc[i][cb]);
-                    acc[i][cb] = mfma16(afr[0], bfr[cb][2], acc[i][cb]);
-                }
-                acc[i][cb] = mfma16(afr[1], bfr[cb][0], acc[i][cb]);
-                acc[i][cb] = mfma16(afr[0], bfr[cb][1], acc[i][cb]);
-                acc[i][cb] = mfma16(afr[0], bfr[cb][0], acc[i][cb]);
-            }
+        for (int r = 0; r < BR; r++) {
+            const int u = tid + r * NT;
+            *(float4 *)&Bs[(u / 8) * BS + 4 * (u % 8)] = rb[r];
         }
         __syncthreads();
-    }
+        if (step + 1 < nsteps) load(step + 1);  // in flight during the MFMAs
+        V8 bfr[NCB][NSPLIT];
 #pragma unroll
-    for (int cb = 0; cb < 4; cb++) {
-        if (btb[cb] < 0) continue;
-        const int t = s_tgt[btb[cb]];
-        if (t < 0 || bcl[cb] >= cpt) continue;
-        float *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + bcl[cb]) * rf;
+        for (int cb = 0; cb < NCB; cb++) {
+            const float *bp = &Bs[((wave * NCB + cb) * 16 + fr) * BS];
+            if constexpr (PRESPLIT) {  // plane p of k 8fq..8fq+7 at floats 16p + 4fq
+                bfr[cb][0] = *(const V8 *)(bp + 4 * fq);
+                bfr[cb][1] = *(const V8 *)(bp + 16 + 4 * fq);
+            } else {
+                const float4 v0 = *(const float4 *)(bp + 8 * fq), v1 = *(const float4 *)(bp + 8 * fq + 4);
+                float x[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
 #pragma unroll
-        for (int i = 0; i < RBW; i++) {
-            const int rb = wr * RBW + i;
+                for (int e = 0; e < 8; e++) x[e] *= xscale;
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int row = rb * 16 + 4 * fq + v;
-                if (rb < RB && row < rf) dst[row] = acc[i][cb][v] * unscale;
+                for (int p = 0; p < NSPLIT; p++)
+#pragma unroll
+                    for (int e = 0; e < 8; e++) {
+                        const T h = (T)x[e];
+                        bfr[cb][p][e] = h;
+                        x[e] -= (float)h;
+                    }
             }
         }
+#pragma unroll
+        for (int rb_ = 0; rb_ < RB; rb_++) {
+            V8 afr[NSPLIT];
+#pragma unroll
+            for (int p = 0; p < NSPLIT; p++) afr[p] = *(const V8 *)&As[(p * R + rb_ * 16 + fr) * KS + 8 * fq];
+#pragma unroll
+            for (int cb = 0; cb < NCB; cb++) {
+                // smallest terms first
+                if constexpr (NSPLIT == 3) {
+                    acc[rb_][cb] = mfma16(afr[2], bfr[cb][0], acc[rb_][cb]);
+                    acc[rb_][cb] = mfma16(afr[1], bfr[cb][1], acc[rb_][cb]);
+                    acc[rb_][cb] = mfma16(afr[0], bfr[cb][2], acc[rb_][cb]);
+                }
+                acc[rb_][cb] = mfma16(afr[1], bfr[cb][0], acc[rb_][cb]);
+                acc[rb_][cb] = mfma16(afr[0], bfr[cb][1], acc[rb_][cb]);
+                acc[rb_][cb] = mfma16(afr[0], bfr[cb][0], acc[rb_][cb]);
+            }
+        }
+    }
+#pragma unroll
+    for (int cb = 0; cb < NCB; cb++) {
+        const int col = (wave * NCB + cb) * 16 + fr;
+        const int tb = col / cpt, cl = col - tb * cpt;
+        if (tb >= TB) continue;
+        const int t = s_tgt[tb];
+        if (t < 0) continue;
+        float *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + cl) * rf;
+#pragma unroll
+        for (int rb_ = 0; rb_ < RB; rb_++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = rb_ * 16 + 4 * fq + v;
+                if (row < rf) dst[row] = acc[rb_][cb][v] * unscale;
+            }
     }
 }
 
@@ -355,16 +389,17 @@ void launch_absmax(const float *x, int64_t n, unsigned *out, hipStream_t st)
 }
 
 template <typename T, int NSPLIT>
-static void m2l_split_t(const M2LMixArgs &a, dim3 grid, hipStream_t st)
+static void m2l_split_t(const M2LMixArgs &a, int ntiles, int ncolchunks, hipStream_t st)
 {
     const int rb = (a.rf + 15) / 16;
     const int w = a.TB * a.cpt;  // tile width: 64, 128 or 256 columns
     if (w > 256) fail(PBB_ERR_INTERNAL, "mixed-precision M2L tile wider than 256 columns");
+    dim3 grid((unsigned)ntiles, (unsigned)ncolchunks);
 #define PBB_CASE(RB)                                                                   \
     case RB:                                                                           \
-        if (w > 128) m2l_mfma_split<T, NSPLIT, RB, 4><<<grid, 512, 0, st>>>(a);        \
+        if (w > 128) m2l_mfma_split<T, NSPLIT, RB, 4><<<grid, 256, 0, st>>>(a);        \
         else if (w > 64) m2l_mfma_split<T, NSPLIT, RB, 2><<<grid, 256, 0, st>>>(a);    \
-        else m2l_mfma_split<T, NSPLIT, RB, 1><<<grid, 128, 0, st>>>(a);                \
+        else m2l_mfma_split<T, NSPLIT, RB, 1><<<grid, 256, 0, st>>>(a);                \
         break;
     switch (rb) {
         PBB_CASE(1) PBB_CASE(2) PBB_CASE(3) PBB_CASE(4) PBB_CASE(5) PBB_CASE(6) PBB_CASE(7) PBB_CASE(8)
@@ -377,11 +412,10 @@ static void m2l_split_t(const M2LMixArgs &a, dim3 grid, hipStream_t st)
 void launch_m2l_mixed(int variant, const M2LMixArgs &a, int ntiles, int ncolchunks, hipStream_t st)
 {
     if (ntiles <= 0) return;
-    dim3 grid((unsigned)ntiles, (unsigned)ncolchunks);
     switch (variant) {
-    case PBB_PREC_MIXED: m2l_split_t<__bf16, 3>(a, grid, st); break;
-    case PBB_PREC_MIXED_F16X2: m2l_split_t<_Float16, 2>(a, grid, st); break;
-    case PBB_PREC_MIXED_BF16X2: m2l_split_t<__bf16, 2>(a, grid, st); break;
+    case PBB_PREC_MIXED: m2l_split_t<__bf16, 3>(a, ntiles, ncolchunks, st); break;
+    case PBB_PREC_MIXED_F16X2: m2l_split_t<_Float16, 2>(a, ntiles, ncolchunks, st); break;
+    case PBB_PREC_MIXED_BF16X2: m2l_split_t<__bf16, 2>(a, ntiles, ncolchunks, st); break;
     default: fail(PBB_ERR_INVALID_ARG, "unknown mixed-precision variant");
     }
     check(hipGetLastError(), "m2l_mfma_split");

@@ -62,3 +62,29 @@ def test_mixed_switching_and_accumulate(gpu):
     f64 = plan.matvec(wd, precision=pbb.capi.PREC_FP64)
     torch.cuda.synchronize()
     assert orc.rel_err(out.cpu().numpy() / 3, f64.cpu().numpy()) <= 1e-6
+
+
+def test_m2l_widths_and_tuner(gpu):
+    # every M2L tile width gives the same operator: FP64 widths to rounding,
+    # mixed widths within the mixed bar; the tuner keeps one of its candidates
+    import torch
+    N, nc = 12000, 32
+    xyz = clustered_points(N, seed=60)
+    w = weights(N, nc, seed=61, dist="normal")
+    ops = pbb.Operators(pbb.gaussian(0.3), 1.0, 4, 5, 1e-7)
+    plan = pbb.Plan(ops, xyz)
+    wd = torch.tensor(w.reshape(nc, N), dtype=torch.float64, device="cuda:0")
+    ref = _mv(plan, w, nc, pbb.capi.PREC_FP64)
+    for width in (32, 64, 128):
+        print(f"case fp64 width={width}", flush=True)
+        plan.set_m2l_width(width, pbb.capi.PREC_FP64)
+        assert orc.rel_err(_mv(plan, w, nc, pbb.capi.PREC_FP64), ref) <= 1e-12
+    for prec, bar in VARIANTS:
+        for width in (64, 128, 256):
+            print(f"case prec={prec} width={width}", flush=True)
+            plan.set_m2l_width(width, prec)
+            assert orc.rel_err(_mv(plan, w, nc, prec), ref) <= bar
+    assert plan.tune_m2l(wd, pbb.capi.PREC_MIXED_F16X2) in (64, 128, 256)
+    assert plan.tune_m2l(wd, pbb.capi.PREC_FP64) in (32, 64, 128)
+    torch.cuda.synchronize()
+    assert orc.rel_err(_mv(plan, w, nc, pbb.capi.PREC_FP64), ref) <= 1e-12

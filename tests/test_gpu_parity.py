@@ -84,6 +84,17 @@ def test_levels(gpu, level):
     assert orc.rel_err(got, ref) <= PARITY
 
 
+@pytest.mark.parametrize("n,kernel", [(3, pbb.gaussian(0.4)), (8, pbb.laplacian())], ids=["n3", "n8"])
+def test_transfer_paths_by_order(gpu, n, kernel):
+    # orders 4..8 run the direct per-level P2M / L2P passes and the split
+    # near field; order 3 runs P2M + M2M + L2L and the one-kernel near
+    # field; order 8 (n^3 = 512) also takes the generic GEMM
+    xyz = clustered_points(4000, seed=50 + n)
+    w = weights(4000, 3, seed=51)
+    ops, plan, got, ref = _run(kernel, 1.0, 4, n, 1e-6, xyz, w, 3)
+    assert orc.rel_err(got, ref) <= PARITY
+
+
 def test_duplicates_and_tiny(gpu):
     # coincident points: 1/r at r = 0 is inf and is zeroed (compute.hpp:767)
     base = uniform_points(40, seed=12)

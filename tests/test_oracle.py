@@ -62,6 +62,47 @@ def test_cweights(n):
                                        atol=1e-12)
 
 
+def _p2m(n, u, w):
+    """S[l1, l2, l3] = sum_j w_j S_l1(u_j0) S_l2(u_j1) S_l3(u_j2) (one column)."""
+    S = np.zeros((n, n, n))
+    for uj, wj in zip(u, w):
+        sx, sy, sz = (orc.sn(n, c) for c in uj)
+        S += wj * np.einsum("a,b,c->abc", sx, sy, sz)
+    return S
+
+
+@pytest.mark.parametrize("n", [4, 6])
+def test_direct_passes_equal_m2m_and_l2l(n):
+    """The GPU path's direct upward pass (P2M at every level) and downward
+    pass (L2P from every level) rest on two identities of the reference's
+    transfers (compute.hpp:956-1034 M2M, :798-878 L2L, Cweights
+    H2_3D_Tree.cpp:805-819):
+      M2M(P2M of the 8 children) == P2M at the parent, and
+      L2P(L2L(F_parent)) at a child point == L2P(F_parent) at that point,
+    because the parent's Chebyshev polynomials have degree n-1 and are
+    interpolated exactly at the child's nodes."""
+    rng = np.random.default_rng(n)
+    C = orc.cweights(n).reshape(2, n, n)  # C[half][m][j] = S_m(x_j / 2 -+ 1/2)
+    x = rng.uniform(-1, 1, (60, 3))       # points in the parent frame
+    w = rng.standard_normal(60)
+    Sp = np.zeros((n, n, n))
+    Fp = rng.standard_normal((n, n, n))
+    for o in range(8):
+        bits = np.array([(o >> 2) & 1, (o >> 1) & 1, o & 1])
+        sel = np.all((x >= 0) == bits.astype(bool), axis=1)
+        centre = bits - 0.5
+        uc = 2.0 * (x[sel] - centre)      # child frame
+        Sc = _p2m(n, uc, w[sel])
+        Cx, Cy, Cz = C[bits[0]], C[bits[1]], C[bits[2]]
+        Sp += np.einsum("ai,bj,ck,ijk->abc", Cx, Cy, Cz, Sc)
+        Fc = np.einsum("ai,bj,ck,abc->ijk", Cx, Cy, Cz, Fp)  # L2L = M2M^T
+        for xj, ucj in zip(x[sel], uc):
+            via_child = np.einsum("a,b,c,abc->", *(orc.sn(n, c) for c in ucj), Fc)
+            direct = np.einsum("a,b,c,abc->", *(orc.sn(n, c) for c in xj), Fp)
+            assert abs(via_child - direct) <= 1e-12 * (1 + np.abs(Fp).sum())
+    np.testing.assert_allclose(Sp, _p2m(n, x, w), atol=1e-12 * np.abs(w).sum())
+
+
 def test_ktable():
     K = orc.ktable()
     offs = [(a, b, c) for a in range(-3, 4) for b in range(-3, 4) for c in range(-3, 4)]
