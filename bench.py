@@ -145,9 +145,9 @@ def main():
     out = torch.zeros_like(w)
     # M2L tile widths picked by the plan's autotuner (untimed, like the tree)
     t0 = time.perf_counter()
-    widths = {"m2l_width": plan.tune_m2l(w, prec)}
+    widths = {"m2l_tile": plan.tune_m2l(w, prec)}
     if prec != capi.PREC_FP64 and args.fp64_steps > 0:
-        widths["m2l_width_fp64"] = plan.tune_m2l(w, capi.PREC_FP64)
+        widths["m2l_tile_fp64"] = plan.tune_m2l(w, capi.PREC_FP64)
     widths["tune_s"] = round(time.perf_counter() - t0, 3)
     log(f"m2l tiles {widths}")
     plan.set_timing(True)

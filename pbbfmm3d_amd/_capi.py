@@ -98,7 +98,7 @@ _SIGNATURES = {
     "pbb_batch_destroy": (None, [_vp]),
     "pbb_plan_timings": (C.c_int, [_vp, _dp]),
     "pbb_plan_set_timing": (C.c_int, [_vp, C.c_int32]),
-    "pbb_plan_set_m2l_width": (C.c_int, [_vp, C.c_int32, C.c_int32]),
+    "pbb_plan_set_m2l_tile": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "pbb_plan_tune_m2l": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, _vp, C.POINTER(C.c_int32)]),
     "pbb_plan_destroy": (None, [_vp]),
     "pbb_matvec": (C.c_int, [_vp, _dp, _dp, C.c_int64, _dp, C.c_int32, _dp]),

@@ -280,23 +280,28 @@ class Plan:
     def set_timing(self, enable: bool = True):
         capi.check(capi.lib().pbb_plan_set_timing(self._h, 1 if enable else 0))
 
-    def set_m2l_width(self, width: int, precision: int = capi.PREC_FP64):
-        """Columns per M2L tile for the precision's class (0: built-in)."""
-        capi.check(capi.lib().pbb_plan_set_m2l_width(self._h, int(precision), int(width)))
+    def set_m2l_tile(self, width: int, cols_per_box: int = 32, order: int = 0,
+                     precision: int = capi.PREC_FP64):
+        """M2L tile of the precision's class: width columns, cols_per_box
+        weight columns per target box, step order (0 direction-major, 1
+        chunk-major); width 0 restores the built-in choice."""
+        capi.check(capi.lib().pbb_plan_set_m2l_tile(self._h, int(precision), int(width), int(cols_per_box),
+                                                    int(order)))
 
-    def tune_m2l(self, weights, precision: int = capi.PREC_FP64) -> int:
-        """Times the M2L tile widths on these weights (float64 CUDA tensor,
-        (ncols, N) or (N,)), keeps the fastest and returns it."""
+    def tune_m2l(self, weights, precision: int = capi.PREC_FP64) -> dict:
+        """Times the candidate M2L tiles on these weights (float64 CUDA
+        tensor, (ncols, N) or (N,)), keeps the fastest and returns it as
+        {"width", "cols_per_box", "order"}."""
         import torch
         w = weights.contiguous()
         ncols = w.numel() // self.npoints
         if ncols * self.npoints != w.numel():
             raise ValueError("weights must hold ncols * npoints values")
-        best = C.c_int32(0)
+        best = (C.c_int32 * 3)()
         stream = torch.cuda.current_stream(w.device).cuda_stream
         capi.check(capi.lib().pbb_plan_tune_m2l(self._h, C.c_void_p(w.data_ptr()), int(ncols), int(precision),
-                                                C.c_void_p(stream), C.byref(best)))
-        return best.value
+                                                C.c_void_p(stream), best))
+        return {"width": best[0], "cols_per_box": best[1], "order": best[2]}
 
     def timings(self) -> dict:
         ms = np.zeros(capi.NUM_PHASES)

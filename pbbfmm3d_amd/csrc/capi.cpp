@@ -247,7 +247,9 @@ struct pbb_plan {
     int g_ncols = 0;
     int kpad = 0;
     bool timing = false;
-    int m2l_width[2] = {0, 0};  // FP64, mixed; 0 = built-in choice
+    struct M2LTile {
+        int width = 0, cpt = 0, order = 0;  // width 0 = built-in choice
+    } m2l_tile[2];                          // FP64, mixed
     hipEvent_t ev[PBB_NUM_PHASES] = {};
     bool events_made = false;
     double last_ms[PBB_NUM_PHASES] = {0};
@@ -604,17 +606,21 @@ PBB_API pbb_status pbb_plan_set_timing(pbb_plan *plan, int32_t enable)
     });
 }
 
-PBB_API pbb_status pbb_plan_set_m2l_width(pbb_plan *plan, int32_t precision, int32_t width)
+PBB_API pbb_status pbb_plan_set_m2l_tile(pbb_plan *plan, int32_t precision, int32_t width, int32_t cols_per_box,
+                                         int32_t order)
 {
     return guarded([&] {
         if (!plan) fail(PBB_ERR_INVALID_ARG, "plan is NULL");
         if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
             fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
         const bool mixed = precision != PBB_PREC_FP64;
-        const bool ok = width == 0 || (mixed ? (width == 64 || width == 128 || width == 256)
-                                             : (width == 32 || width == 64 || width == 128));
-        if (!ok) fail(PBB_ERR_INVALID_ARG, "M2L width must be 0, 64, 128 or 256 (mixed) / 32, 64 or 128 (FP64)");
-        plan->m2l_width[mixed ? 1 : 0] = width;
+        const bool wok = width == 0 || (mixed ? (width == 64 || width == 128 || width == 256)
+                                              : (width == 32 || width == 64 || width == 128));
+        if (!wok) fail(PBB_ERR_INVALID_ARG, "M2L width must be 0, 64, 128 or 256 (mixed) / 32, 64 or 128 (FP64)");
+        if (width && (cols_per_box < 1 || cols_per_box > 64)) fail(PBB_ERR_INVALID_ARG, "cols_per_box must be 1..64");
+        if (order != PBB_M2L_DIRECTION_MAJOR && order != PBB_M2L_CHUNK_MAJOR)
+            fail(PBB_ERR_INVALID_ARG, "unknown M2L step order");
+        plan->m2l_tile[mixed ? 1 : 0] = {width, width ? cols_per_box : 0, order};
     });
 }
 
@@ -639,7 +645,7 @@ PBB_API pbb_status pbb_plan_timings(pbb_plan *plan, double *ms)
 static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, double *out, hipStream_t st, int prec);
 
 PBB_API pbb_status pbb_plan_tune_m2l(pbb_plan *plan, const double *w_dev, int32_t ncols, int32_t precision,
-                                     void *stream, int32_t *best_width)
+                                     void *stream, int32_t *best)
 {
     return guarded([&] {
         if (!plan || !w_dev) fail(PBB_ERR_INVALID_ARG, "NULL argument");
@@ -652,20 +658,25 @@ PBB_API pbb_status pbb_plan_tune_m2l(pbb_plan *plan, const double *w_dev, int32_
         const bool was_timing = plan->timing;
         check(pbb_plan_set_timing(plan, 1) == PBB_OK ? hipSuccess : hipErrorUnknown, "enable timing");
         double *scratch = dalloc<double>((size_t)plan->N * ncols, "tuning output");
-        const int cands[2][3] = {{32, 64, 128}, {64, 128, 256}};
-        int best = 0;
+        // candidates {width, columns per box, order}
+        static const int c64[][3] = {{32, 32, 0}, {64, 32, 0}, {128, 32, 0}, {64, 32, 1}, {128, 32, 1}};
+        static const int cmx[][3] = {{64, 16, 0}, {128, 32, 0}, {128, 16, 0}, {256, 32, 0},
+                                     {128, 32, 1}, {128, 16, 1}, {256, 16, 1}};
+        const int(*cands)[3] = mixed ? cmx : c64;
+        const int ncand = mixed ? (int)(sizeof(cmx) / sizeof(cmx[0])) : (int)(sizeof(c64) / sizeof(c64[0]));
+        pbb_plan::M2LTile bestt{};
         double best_ms = 0;
         try {
             // one untimed matvec builds the operator splits and workspaces
             matvec_run(plan, plan->ops, w_dev, ncols, scratch, st, precision);
-            for (int c : cands[mixed ? 1 : 0]) {
-                plan->m2l_width[mixed ? 1 : 0] = c;
+            for (int c = 0; c < ncand; c++) {
+                plan->m2l_tile[mixed ? 1 : 0] = {cands[c][0], std::min(cands[c][1], ncols), cands[c][2]};
                 matvec_run(plan, plan->ops, w_dev, ncols, scratch, st, precision);
                 check(hipStreamSynchronize(st), "hipStreamSynchronize");
                 float t = 0;
                 check(hipEventElapsedTime(&t, plan->ev[3], plan->ev[4]), "hipEventElapsedTime");
-                if (best == 0 || t < best_ms) {
-                    best = c;
+                if (bestt.width == 0 || t < best_ms) {
+                    bestt = plan->m2l_tile[mixed ? 1 : 0];
                     best_ms = t;
                 }
             }
@@ -676,8 +687,12 @@ PBB_API pbb_status pbb_plan_tune_m2l(pbb_plan *plan, const double *w_dev, int32_
         }
         dfree(scratch);
         plan->timing = was_timing;
-        plan->m2l_width[mixed ? 1 : 0] = best;
-        if (best_width) *best_width = best;
+        plan->m2l_tile[mixed ? 1 : 0] = bestt;
+        if (best) {
+            best[0] = bestt.width;
+            best[1] = bestt.cpt;
+            best[2] = bestt.order;
+        }
     });
 }
 
@@ -774,17 +789,19 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
     // FP64: W = 128 (64 for 17..31 columns), matrix-core bound, so few boxes
     // per tile keep the share of empty (box, direction) slots low.
     // Mixed: W = 128 (4 boxes x 32 columns), two workgroups per CU.
-    const int cpt = std::min(ncols, 64);
-    const int width = p->m2l_width[0] ? p->m2l_width[0] : ncols >= 32 || ncols <= 16 ? 128 : 64;
+    const pbb_plan::M2LTile &t64 = p->m2l_tile[0], &tmx = p->m2l_tile[1];
+    const int cpt = t64.width ? std::min(ncols, t64.cpt) : std::min(ncols, 64);
+    const int width = t64.width ? t64.width : ncols >= 32 || ncols <= 16 ? 128 : 64;
     const int TB = std::max(1, std::min(16, width / cpt));
     const int nchunks = (ncols + cpt - 1) / cpt;
-    const int cptm = std::min(ncols, 32);
+    const int cptm = tmx.width ? std::min(ncols, tmx.cpt) : std::min(ncols, 32);
     const int nchunks_m = (ncols + cptm - 1) / cptm;
-    const int width_m = p->m2l_width[1] ? p->m2l_width[1] : 128;
+    const int width_m = tmx.width ? tmx.width : 128;
     for (int l = 2; l <= Lv && mixed; l++) {
         const DevLevel &D = p->lv[l];
         dev::M2LMixArgs a{};
         a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = std::max(1, std::min(16, width_m / cptm)); a.cpt = cptm;
+        a.chunk_major = tmx.width ? tmx.order : PBB_M2L_DIRECTION_MAJOR;
         a.side = 1 << l;
         a.kpad = ldP; a.nch = SE->nch; a.escale = SE->escale[h.ops_set(l)]; a.off = D.off; a.R = SE->R;
         a.tile_prefix[0] = 0;
@@ -804,6 +821,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         const DevLevel &D = p->lv[l];
         dev::M2LArgs a{};
         a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = TB; a.cpt = cpt; a.side = 1 << l; a.off = D.off;
+        a.chunk_major = t64.width ? t64.order : PBB_M2L_CHUNK_MAJOR;
         a.tile_prefix[0] = 0;
         a.class_start[0] = 0;
         for (int c = 0; c < 8; c++) {

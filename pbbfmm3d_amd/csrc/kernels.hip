@@ -512,9 +512,9 @@ __global__ __launch_bounds__(NWC * 128) void m2l_mfma_f64(M2LArgs a)
         s_nd = nd;
     }
     __syncthreads();
-    // steps run k-chunk-major: all directions of one chunk, then the next
-    // chunk, so that consecutive steps gather the same proxy rows of largely
-    // the same source boxes (see class_directions)
+    // step order: a.chunk_major (see m2l_step); in chunk-major order
+    // consecutive steps gather the same proxy rows of largely the same source
+    // boxes (see class_directions)
     const int nd = s_nd, nsteps = nd * nch;
 
     const int fr = lane % 16, fk = lane / 16;
@@ -528,7 +528,9 @@ __global__ __launch_bounds__(NWC * 128) void m2l_mfma_f64(M2LArgs a)
     }
     double ra[AR];
     auto loadA = [&](int step) {
-        const int ch = step / nd, k = s_dir[step - ch * nd], k0 = ch * KC;
+        int di, ch;
+        m2l_step(step, nd, nch, a.chunk_major, di, ch);
+        const int k = s_dir[di], k0 = ch * KC;
         const double *Ed = a.E + (size_t)c_dirs[o][k][3] * rf * rs;
 #pragma unroll
         for (int r = 0; r < AR; r++) {
@@ -548,7 +550,9 @@ __global__ __launch_bounds__(NWC * 128) void m2l_mfma_f64(M2LArgs a)
     // a column without a source box loads box 0 of the level (always valid)
     // and is zeroed through the mask at its use
     auto loadB = [&](int step, double (&bn)[2][4], int &okm) {
-        const int ch = step / nd, k = s_dir[step - ch * nd], k0 = ch * KC + 4 * fk;
+        int di, ch;
+        m2l_step(step, nd, nch, a.chunk_major, di, ch);
+        const int k = s_dir[di], k0 = ch * KC + 4 * fk;
         okm = 0;
 #pragma unroll
         for (int cb = 0; cb < 2; cb++) {

@@ -104,7 +104,7 @@ struct GemmArgs {
 };
 
 struct M2LArgs {
-    int rf, rs, ncols, TB, cpt, side;
+    int rf, rs, ncols, TB, cpt, side, chunk_major;  // step order: see m2l_step
     int64_t off;
     int tile_prefix[9];
     int64_t class_start[9];
@@ -114,6 +114,20 @@ struct M2LArgs {
 };
 
 constexpr int NEAR_CLASSES = 6;
+
+// Step s of an M2L tile with nd non-empty directions and nch k chunks:
+// direction-major (the chunks of one E_d, then the next direction) or
+// chunk-major (one chunk of every direction, then the next chunk).
+__device__ __forceinline__ void m2l_step(int s, int nd, int nch, int chunk_major, int &di, int &ch)
+{
+    if (chunk_major) {
+        ch = s / nd;
+        di = s - ch * nd;
+    } else {
+        di = s / nch;
+        ch = s - di * nch;
+    }
+}
 
 struct NearArgs {
     int n, ncols, side;
@@ -143,6 +157,7 @@ struct DirectArgs {
 
 struct M2LMixArgs {
     int rf, rs, ncols, TB, cpt, side, kpad, nch, escale, R;  // R: rows of the split E (16-multiple)
+    int chunk_major;                                         // step order: see m2l_step
     int64_t off;
     int tile_prefix[9];
     int64_t class_start[9];

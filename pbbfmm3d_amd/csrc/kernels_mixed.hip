@@ -155,9 +155,10 @@ void launch_split_p2(int variant, float *P, int64_t nchunks, const unsigned *xma
 // RB 16-row blocks of FP32 accumulators.  The (direction, 32-deep k chunk)
 // steps of the tile's non-empty directions form one software pipeline: the
 // next step's E chunk (pre-split) and proxy chunk are loaded into registers
-// while the current step's MFMAs run from LDS.  Steps run direction-major
-// (the k chunks of one E_d, then the next direction), so the proxy gathers of
-// consecutive steps continue along the same columns.  Two workgroups share a
+// while the current step's MFMAs run from LDS.  Steps run in the order of
+// a.chunk_major (see m2l_step; direction-major by default, so that the proxy
+// gathers of consecutive steps continue along the same columns).  Two
+// workgroups share a
 // CU and cover each other's stalls.  Two-part modes read proxies pre-split by
 // split_p2 ([32 high][32 low] 16-bit parts per k chunk, the same 128 bytes
 // as 32 floats); the three-part mode splits FP32 proxies in registers.
@@ -237,7 +238,9 @@ __global__ __launch_bounds__(256, NCB <= 2 ? 2 : 1) void m2l_mfma_split(M2LMixAr
     uint4 ra[AR];
     float4 rb[BR];
     auto load = [&](int step) {
-        const int di = step / nch, ch = step - di * nch, k = s_dir[di];
+        int di, ch;
+        m2l_step(step, s_nd, nch, a.chunk_major, di, ch);
+        const int k = s_dir[di];
         const uint4 *src = (const uint4 *)((const T *)a.Es + ((size_t)c_dirs[o][k][3] * nch + ch) * NSPLIT * R * 32);
 #pragma unroll
         for (int r = 0; r < AR; r++) {

@@ -58,7 +58,7 @@ def test_invalid_arguments_return_status():
     assert L.pbb_ops_create(C.byref(bad), none, None, 1.0, 3, 4, 1e-5, 1, C.byref(h)) == \
         capi.PBB_ERR_INVALID_ARG
     assert L.pbb_plan_matvec(None, None, 1, None, 0, None) == capi.PBB_ERR_INVALID_ARG
-    assert L.pbb_plan_set_m2l_width(None, 0, 128) == capi.PBB_ERR_INVALID_ARG
+    assert L.pbb_plan_set_m2l_tile(None, 0, 128, 32, 0) == capi.PBB_ERR_INVALID_ARG
     assert L.pbb_plan_tune_m2l(None, None, 1, 0, None, None) == capi.PBB_ERR_INVALID_ARG
     assert L.pbb_ops_query(None, None) == capi.PBB_ERR_INVALID_ARG
 

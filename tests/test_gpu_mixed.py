@@ -64,9 +64,10 @@ def test_mixed_switching_and_accumulate(gpu):
     assert orc.rel_err(out.cpu().numpy() / 3, f64.cpu().numpy()) <= 1e-6
 
 
-def test_m2l_widths_and_tuner(gpu):
-    # every M2L tile width gives the same operator: FP64 widths to rounding,
-    # mixed widths within the mixed bar; the tuner keeps one of its candidates
+def test_m2l_tiles_and_tuner(gpu):
+    # every M2L tile (width, columns per box, step order) gives the same
+    # operator: FP64 tiles to rounding, mixed tiles within the mixed bar; the
+    # tuner keeps one of its candidates
     import torch
     N, nc = 12000, 32
     xyz = clustered_points(N, seed=60)
@@ -75,16 +76,18 @@ def test_m2l_widths_and_tuner(gpu):
     plan = pbb.Plan(ops, xyz)
     wd = torch.tensor(w.reshape(nc, N), dtype=torch.float64, device="cuda:0")
     ref = _mv(plan, w, nc, pbb.capi.PREC_FP64)
-    for width in (32, 64, 128):
-        print(f"case fp64 width={width}", flush=True)
-        plan.set_m2l_width(width, pbb.capi.PREC_FP64)
+    for width, cpb, order in ((32, 32, 0), (64, 32, 1), (128, 32, 0), (128, 16, 1), (64, 8, 0)):
+        print(f"case fp64 width={width} cpb={cpb} order={order}", flush=True)
+        plan.set_m2l_tile(width, cpb, order, pbb.capi.PREC_FP64)
         assert orc.rel_err(_mv(plan, w, nc, pbb.capi.PREC_FP64), ref) <= 1e-12
     for prec, bar in VARIANTS:
-        for width in (64, 128, 256):
-            print(f"case prec={prec} width={width}", flush=True)
-            plan.set_m2l_width(width, prec)
+        for width, cpb, order in ((64, 16, 0), (128, 32, 1), (256, 16, 0), (256, 32, 1), (128, 8, 0)):
+            print(f"case prec={prec} width={width} cpb={cpb} order={order}", flush=True)
+            plan.set_m2l_tile(width, cpb, order, prec)
             assert orc.rel_err(_mv(plan, w, nc, prec), ref) <= bar
-    assert plan.tune_m2l(wd, pbb.capi.PREC_MIXED_F16X2) in (64, 128, 256)
-    assert plan.tune_m2l(wd, pbb.capi.PREC_FP64) in (32, 64, 128)
+    t = plan.tune_m2l(wd, pbb.capi.PREC_MIXED_F16X2)
+    assert t["width"] in (64, 128, 256) and t["order"] in (0, 1)
+    t = plan.tune_m2l(wd, pbb.capi.PREC_FP64)
+    assert t["width"] in (32, 64, 128)
     torch.cuda.synchronize()
     assert orc.rel_err(_mv(plan, w, nc, pbb.capi.PREC_FP64), ref) <= 1e-12
