@@ -28,6 +28,12 @@ DTYPE = {"fp64": "f64", "mixed": "bf16x3->f32 (M2L), f64 elsewhere",
          "f16x2": "f16x2->f32 (M2L), f64 elsewhere", "bf16x2": "bf16x2->f32 (M2L), f64 elsewhere"}
 
 
+# bytes per M2L launch, averaged over the 6 launches of one f16x2 matvec of the
+# default workload (8439 GB read + 28 GB written per matvec)
+M2L_TRAFFIC_F16X2_C3 = 1.411e12
+TRAFFIC_SOURCE = "profiles/m2l_f16x2_c3_gaussian_traffic.txt"
+
+
 def clustered_points(n, seed):
     rng = np.random.default_rng(seed)
     nb = int(round(0.2 * n))
@@ -216,8 +222,15 @@ def main():
     flops = 2.0 * ops.rank_f * ops.rank_s * nc * info["m2l_pairs"]
     achieved = flops / (phase["m2l"] * 1e-3) / 1e12
     bound, peak, unit = PEAK[args.precision]
+    # HBM bytes per M2L launch (FETCH_SIZE x 2 + WRITE_SIZE x 1, calibrated
+    # on a plain copy), measured by rocprofv3 on this workload and kept in
+    # profiles/; null for other workloads
+    traffic = None
+    if (args.precision, N, nc, args.level, args.order, args.kernel) == ("f16x2", 4_000_000, 32, 7, 6, "gaussian"):
+        traffic = M2L_TRAFFIC_F16X2_C3
     roofline = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-                "frac": round(achieved / peak, 4), "traffic": None,
+                "frac": round(achieved / peak, 4), "traffic": traffic,
+                "traffic_source": TRAFFIC_SOURCE if traffic else None,
                 "kernel": "m2l", "launches_per_step": args.level - 1,
                 "algorithmic_flops_per_step": flops}
 
