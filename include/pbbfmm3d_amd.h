@@ -216,18 +216,20 @@ PBB_API pbb_status pbb_plan_set_timing(pbb_plan *plan, int32_t enable);
 /* M2L tile of the FP64 mode (precision PBB_PREC_FP64) or of the mixed
  * modes (any mixed precision): `width` columns per workgroup tile (FP64 32,
  * 64 or 128; mixed 64, 128 or 256), `cols_per_box` weight columns of each
- * target box in a tile (1..64), `order` 0 = direction-major steps (the k
- * chunks of one E_d, then the next direction), 1 = chunk-major (one k chunk
- * of every direction, then the next chunk).  width 0 restores the built-in
- * choice.  No reference counterpart: tuning of this build's M2L.          */
-#define PBB_M2L_DIRECTION_MAJOR 0
+ * target box in a tile (1..64), `flags`: PBB_M2L_CHUNK_MAJOR for chunk-major
+ * steps (one k chunk of every direction, then the next chunk; otherwise the
+ * k chunks of one E_d, then the next direction), PBB_M2L_ROW_SPLIT (mixed
+ * modes) to split the rows of r_f over two workgroups per tile.  width 0
+ * restores the built-in choice.  No reference counterpart: tuning of this
+ * build's M2L.                                                            */
 #define PBB_M2L_CHUNK_MAJOR 1
+#define PBB_M2L_ROW_SPLIT 2
 PBB_API pbb_status pbb_plan_set_m2l_tile(pbb_plan *plan, int32_t precision, int32_t width,
-                                         int32_t cols_per_box, int32_t order);
+                                         int32_t cols_per_box, int32_t flags);
 
 /* Autotuner: times the M2L of one matvec on w_dev (device FP64, N x ncols)
  * for every candidate tile of the precision's class, keeps the fastest on
- * the plan and returns it in best[3] = {width, cols_per_box, order} (may be
+ * the plan and returns it in best[3] = {width, cols_per_box, flags} (may be
  * NULL).  Runs on `stream`, waits for it, and uses a scratch output of its
  * own.  No reference counterpart.                                         */
 PBB_API pbb_status pbb_plan_tune_m2l(pbb_plan *plan, const double *w_dev, int32_t ncols,

@@ -42,6 +42,8 @@ PREC_FP64 = 0
 PREC_MIXED = 1          # BF16 x3 split M2L
 PREC_MIXED_F16X2 = 2   # FP16 x2 split M2L
 PREC_MIXED_BF16X2 = 3  # BF16 x2 split M2L
+M2L_CHUNK_MAJOR = 1
+M2L_ROW_SPLIT = 2
 
 NUM_PHASES = 8
 BATCH_TRANSPOSE = 1

@@ -280,18 +280,18 @@ class Plan:
     def set_timing(self, enable: bool = True):
         capi.check(capi.lib().pbb_plan_set_timing(self._h, 1 if enable else 0))
 
-    def set_m2l_tile(self, width: int, cols_per_box: int = 32, order: int = 0,
+    def set_m2l_tile(self, width: int, cols_per_box: int = 32, flags: int = 0,
                      precision: int = capi.PREC_FP64):
         """M2L tile of the precision's class: width columns, cols_per_box
-        weight columns per target box, step order (0 direction-major, 1
-        chunk-major); width 0 restores the built-in choice."""
+        weight columns per target box, flags (capi.M2L_CHUNK_MAJOR,
+        capi.M2L_ROW_SPLIT); width 0 restores the built-in choice."""
         capi.check(capi.lib().pbb_plan_set_m2l_tile(self._h, int(precision), int(width), int(cols_per_box),
-                                                    int(order)))
+                                                    int(flags)))
 
     def tune_m2l(self, weights, precision: int = capi.PREC_FP64) -> dict:
         """Times the candidate M2L tiles on these weights (float64 CUDA
         tensor, (ncols, N) or (N,)), keeps the fastest and returns it as
-        {"width", "cols_per_box", "order"}."""
+        {"width", "cols_per_box", "flags"}."""
         import torch
         w = weights.contiguous()
         ncols = w.numel() // self.npoints
@@ -301,7 +301,7 @@ class Plan:
         stream = torch.cuda.current_stream(w.device).cuda_stream
         capi.check(capi.lib().pbb_plan_tune_m2l(self._h, C.c_void_p(w.data_ptr()), int(ncols), int(precision),
                                                 C.c_void_p(stream), best))
-        return {"width": best[0], "cols_per_box": best[1], "order": best[2]}
+        return {"width": best[0], "cols_per_box": best[1], "flags": best[2]}
 
     def timings(self) -> dict:
         ms = np.zeros(capi.NUM_PHASES)

@@ -108,6 +108,31 @@ struct pbb_ops {
     std::unique_ptr<pbb_ops> adj;  // transposed operator set, built on demand
     double *d_Tkz = nullptr, *d_Cw = nullptr, *d_VTw = nullptr, *d_Uw = nullptr, *d_E = nullptr;
     SplitE split[4];
+    // VTw and Uw of every operator set split into three BF16 parts for the
+    // mixed modes' proxy / expansion products ([set][3][R][32*nch])
+    struct SplitA {
+        void *d = nullptr;
+        int R = 0, nch = 0;
+        size_t per_set = 0;
+    } vt3, u3;
+
+    const SplitA *ensure_split3(hipStream_t st)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (vt3.d) return &vt3;
+        auto make = [&](SplitA &S, const double *A, int M, int K, const char *what) {
+            S.R = 16 * ((M + 15) / 16);
+            S.nch = (K + 31) / 32;
+            S.per_set = (size_t)3 * S.R * S.nch * 32;
+            S.d = dalloc<uint16_t>(S.per_set * h.nsets, what);
+            for (int s = 0; s < h.nsets; s++)
+                dev::launch_split_a3(A + (size_t)s * M * K, M, K, S.R, S.nch * 32, (uint16_t *)S.d + s * S.per_set, st);
+        };
+        make(vt3, d_VTw, h.rs, h.n3, "split VT");
+        make(u3, d_Uw, h.n3, h.rf, "split U");
+        check(hipStreamSynchronize(st), "split VT / U");
+        return &vt3;
+    }
 
     // E_d split into 16-bit parts for a mixed-precision variant (built once)
     const SplitE &ensure_split(int variant, hipStream_t st)
@@ -206,6 +231,7 @@ struct pbb_ops {
             (void)hipSetDevice(device);
             dfree(d_Tkz); dfree(d_Cw); dfree(d_VTw); dfree(d_Uw); dfree(d_E);
             for (auto &x : split) dfree(x.d);
+            dfree(vt3.d); dfree(u3.d);
             if (prev >= 0) (void)hipSetDevice(prev);
         }
     }
@@ -248,7 +274,7 @@ struct pbb_plan {
     int kpad = 0;
     bool timing = false;
     struct M2LTile {
-        int width = 0, cpt = 0, order = 0;  // width 0 = built-in choice
+        int width = 0, cpt = 0, flags = 0;  // width 0 = built-in choice
     } m2l_tile[2];                          // FP64, mixed
     hipEvent_t ev[PBB_NUM_PHASES] = {};
     bool events_made = false;
@@ -607,7 +633,7 @@ PBB_API pbb_status pbb_plan_set_timing(pbb_plan *plan, int32_t enable)
 }
 
 PBB_API pbb_status pbb_plan_set_m2l_tile(pbb_plan *plan, int32_t precision, int32_t width, int32_t cols_per_box,
-                                         int32_t order)
+                                         int32_t flags)
 {
     return guarded([&] {
         if (!plan) fail(PBB_ERR_INVALID_ARG, "plan is NULL");
@@ -618,9 +644,9 @@ PBB_API pbb_status pbb_plan_set_m2l_tile(pbb_plan *plan, int32_t precision, int3
                                               : (width == 32 || width == 64 || width == 128));
         if (!wok) fail(PBB_ERR_INVALID_ARG, "M2L width must be 0, 64, 128 or 256 (mixed) / 32, 64 or 128 (FP64)");
         if (width && (cols_per_box < 1 || cols_per_box > 64)) fail(PBB_ERR_INVALID_ARG, "cols_per_box must be 1..64");
-        if (order != PBB_M2L_DIRECTION_MAJOR && order != PBB_M2L_CHUNK_MAJOR)
-            fail(PBB_ERR_INVALID_ARG, "unknown M2L step order");
-        plan->m2l_tile[mixed ? 1 : 0] = {width, width ? cols_per_box : 0, order};
+        const int known = PBB_M2L_CHUNK_MAJOR | (mixed ? PBB_M2L_ROW_SPLIT : 0);
+        if (flags & ~known) fail(PBB_ERR_INVALID_ARG, "unknown M2L tile flags");
+        plan->m2l_tile[mixed ? 1 : 0] = {width, width ? cols_per_box : 0, flags};
     });
 }
 
@@ -660,8 +686,8 @@ PBB_API pbb_status pbb_plan_tune_m2l(pbb_plan *plan, const double *w_dev, int32_
         double *scratch = dalloc<double>((size_t)plan->N * ncols, "tuning output");
         // candidates {width, columns per box, order}
         static const int c64[][3] = {{32, 32, 0}, {64, 32, 0}, {128, 32, 0}, {64, 32, 1}, {128, 32, 1}};
-        static const int cmx[][3] = {{64, 16, 0}, {128, 32, 0}, {128, 16, 0}, {256, 32, 0},
-                                     {128, 32, 1}, {128, 16, 1}, {256, 16, 1}};
+        static const int cmx[][3] = {{64, 16, 0}, {128, 32, 0}, {128, 32, 1}, {256, 32, 2}, {256, 16, 2},
+                                     {256, 16, 3}, {256, 32, 3}, {128, 32, 2}};
         const int(*cands)[3] = mixed ? cmx : c64;
         const int ncand = mixed ? (int)(sizeof(cmx) / sizeof(cmx[0])) : (int)(sizeof(c64) / sizeof(c64[0]));
         pbb_plan::M2LTile bestt{};
@@ -691,7 +717,7 @@ PBB_API pbb_status pbb_plan_tune_m2l(pbb_plan *plan, const double *w_dev, int32_
         if (best) {
             best[0] = bestt.width;
             best[1] = bestt.cpt;
-            best[2] = bestt.order;
+            best[2] = bestt.flags;
         }
     });
 }
@@ -723,6 +749,10 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
     };
     p->ensure_workspace(ncols, mixed);
     const SplitE *SE = mixed && Lv >= 2 ? &o->ensure_split(prec, st) : nullptr;
+    // the mixed modes' VT / U products on the BF16 matrix cores (3 parts),
+    // when both operators have at most 224 rows
+    const pbb_ops::SplitA *SA =
+        mixed && Lv >= 2 && h.rs <= 224 && h.n3 <= 224 ? o->ensure_split3(st) : nullptr;
     const int ldP = mixed ? p->kpad : h.rs;
     if (mixed && h.rs > p->kpad) fail(PBB_ERR_INTERNAL, "proxy padding too small");
     mark(0);
@@ -775,7 +805,15 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         } else {
             g.C = p->P + (size_t)D.off * ncols * h.rs;
         }
-        dev::launch_gemm(g, st);
+        if (SA) {  // three-part BF16 matrix cores (P is FP32 in the mixed modes)
+            dev::GemmSplitArgs q{};
+            q.M = g.M; q.K = g.K; q.ldb = g.ldb; q.ldc = g.ldc; q.b_f32 = 0; q.c_f32 = 1; q.nch = SA->nch;
+            q.ncol = g.ncol; q.alpha = 1.0; q.As = (const uint16_t *)SA->d + h.ops_set(l) * SA->per_set;
+            q.B = g.B; q.C = g.C;
+            dev::launch_gemm_split3(q, st);
+        } else {
+            dev::launch_gemm(g, st);
+        }
         if (prec == PBB_PREC_MIXED_F16X2) {
             dev::launch_zero_u32(p->xmax + l, 1, st);
             dev::launch_absmax(p->P32 + (size_t)D.off * ncols * ldP, D.nb * ncols * ldP, p->xmax + l, st);
@@ -801,7 +839,8 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         const DevLevel &D = p->lv[l];
         dev::M2LMixArgs a{};
         a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = std::max(1, std::min(16, width_m / cptm)); a.cpt = cptm;
-        a.chunk_major = tmx.width ? tmx.order : PBB_M2L_DIRECTION_MAJOR;
+        a.chunk_major = tmx.width ? (tmx.flags & PBB_M2L_CHUNK_MAJOR) : 0;
+        a.row_split = tmx.width ? (tmx.flags & PBB_M2L_ROW_SPLIT) != 0 : 0;
         a.side = 1 << l;
         a.kpad = ldP; a.nch = SE->nch; a.escale = SE->escale[h.ops_set(l)]; a.off = D.off; a.R = SE->R;
         a.tile_prefix[0] = 0;
@@ -821,7 +860,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         const DevLevel &D = p->lv[l];
         dev::M2LArgs a{};
         a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = TB; a.cpt = cpt; a.side = 1 << l; a.off = D.off;
-        a.chunk_major = t64.width ? t64.order : PBB_M2L_CHUNK_MAJOR;
+        a.chunk_major = t64.width ? (t64.flags & PBB_M2L_CHUNK_MAJOR) : 1;
         a.tile_prefix[0] = 0;
         a.class_start[0] = 0;
         for (int c = 0; c < 8; c++) {
@@ -848,7 +887,16 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
             g.B = p->Pf + (size_t)D.off * ncols * h.rf;
         }
         g.C = p->S + (size_t)D.off * ncols * n3;
-        dev::launch_gemm(g, st);
+        if (SA) {  // three-part BF16 matrix cores (Pf is FP32 in the mixed modes)
+            const auto &U3 = o->u3;
+            dev::GemmSplitArgs q{};
+            q.M = g.M; q.K = g.K; q.ldb = g.ldb; q.ldc = g.ldc; q.b_f32 = 1; q.c_f32 = 0; q.nch = U3.nch;
+            q.ncol = g.ncol; q.alpha = g.alpha; q.As = (const uint16_t *)U3.d + h.ops_set(l) * U3.per_set;
+            q.B = g.B; q.C = g.C;
+            dev::launch_gemm_split3(q, st);
+        } else {
+            dev::launch_gemm(g, st);
+        }
     }
     mark(5);
     // downward pass, top-down: F_l += L2L(F_{l-1}), F_{l-1} already complete;
@@ -1033,7 +1081,10 @@ PBB_API pbb_status pbb_batch_create(pbb_plan *const *plans, int32_t nplans, cons
             ops[i]->upload(dev);
             if (mixed && ops[i]->h.rf > 224) fail(PBB_ERR_UNSUPPORTED, "mixed-precision M2L supports ranks up to 224");
             p->ensure_workspace(ncols, mixed);
-            if (mixed && p->Lv >= 2) ops[i]->ensure_split(precision, b->stream);
+            if (mixed && p->Lv >= 2) {
+                ops[i]->ensure_split(precision, b->stream);
+                if (ops[i]->h.rs <= 224 && ops[i]->h.n3 <= 224) ops[i]->ensure_split3(b->stream);
+            }
         }
         check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
         std::vector<bool> timing(nplans);

@@ -158,6 +158,7 @@ struct DirectArgs {
 struct M2LMixArgs {
     int rf, rs, ncols, TB, cpt, side, kpad, nch, escale, R;  // R: rows of the split E (16-multiple)
     int chunk_major;                                         // step order: see m2l_step
+    int row_split;                                           // rows of r_f split over two workgroups
     int64_t off;
     int tile_prefix[9];
     int64_t class_start[9];
@@ -197,6 +198,18 @@ void launch_p2m(const P2MArgs &a, int nleaf, hipStream_t st);
 void launch_m2m(const M2MArgs &a, int nparent, hipStream_t st);
 void launch_l2l(const M2MArgs &a, int nparent, hipStream_t st);
 void launch_gemm(const GemmArgs &g, hipStream_t st);
+
+// C = alpha * A B on the BF16 matrix cores with three-part operands (mixed
+// modes): A pre-split by launch_split_a3 into [3][R][32*nch] BF16
+struct GemmSplitArgs {
+    int M, K, ldb, ldc, b_f32, c_f32, nch;
+    int64_t ncol;
+    double alpha;
+    const void *As, *B;
+    void *C;
+};
+void launch_split_a3(const double *A, int M, int K, int R, int Kp, void *As, hipStream_t st);
+void launch_gemm_split3(const GemmSplitArgs &g, hipStream_t st);
 void launch_m2l(const M2LArgs &a, int ntiles, int ncolchunks, hipStream_t st);
 void launch_near(const NearArgs &a, int kid, int nleaf, hipStream_t st);
 // the split near field for orders 4..8 (false: order not covered)

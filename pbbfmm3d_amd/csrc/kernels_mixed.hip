@@ -162,13 +162,17 @@ void launch_split_p2(int variant, float *P, int64_t nchunks, const unsigned *xma
 // CU and cover each other's stalls.  Two-part modes read proxies pre-split by
 // split_p2 ([32 high][32 low] 16-bit parts per k chunk, the same 128 bytes
 // as 32 floats); the three-part mode splits FP32 proxies in registers.
+// With the row split (gridDim.z = 2) a workgroup holds RB of the row blocks
+// of r_f, starting at row r0: the two halves of a tile run as two
+// workgroups, adjacent in launch order so that they share the proxy gathers
+// through L2, and wide tiles keep two workgroups per CU.
 constexpr int TB_MAX = 16;
 
 template <typename T, int NSPLIT, int RB, int NCB>
-__global__ __launch_bounds__(256, NCB <= 2 ? 2 : 1) void m2l_mfma_split(M2LMixArgs a)
+__global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2LMixArgs a)
 {
     typedef typename Vec8<T>::type V8;
-    constexpr int NT = 256, W = 64 * NCB, R = 16 * RB, KS = 40, BS = 36;
+    constexpr int NT = 256, W = 64 * NCB, R = 16 * RB, KS = 40, BS = 36;  // R: rows of this workgroup
     constexpr int NA = NSPLIT * R * 4;  // 16-byte units of one E chunk
     constexpr int AR = (NA + NT - 1) / NT;
     constexpr int BR = W * 8 / NT;      // 16-byte units of one proxy chunk per thread
@@ -181,9 +185,11 @@ __global__ __launch_bounds__(256, NCB <= 2 ? 2 : 1) void m2l_mfma_split(M2LMixAr
     __shared__ int s_tgt[TB_MAX];
     __shared__ int s_xyz[TB_MAX][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int rf = a.rf, TB = a.TB, nch = a.nch;
-    const int lin = xcd_tile(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
-    // the column chunks of one tile are adjacent in launch order
+    const int rf = a.rf, TB = a.TB, nch = a.nch, RE = a.R;
+    const int lin0 = xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                              gridDim.x * gridDim.y * gridDim.z);
+    // the row halves and column chunks of one tile are adjacent in launch order
+    const int r0 = (lin0 % gridDim.z) * R, lin = lin0 / gridDim.z;
     const int tile = lin / gridDim.y;
     int o = 0;
     while (o < 7 && tile >= a.tile_prefix[o + 1]) o++;
@@ -241,11 +247,11 @@ __global__ __launch_bounds__(256, NCB <= 2 ? 2 : 1) void m2l_mfma_split(M2LMixAr
         int di, ch;
         m2l_step(step, s_nd, nch, a.chunk_major, di, ch);
         const int k = s_dir[di];
-        const uint4 *src = (const uint4 *)((const T *)a.Es + ((size_t)c_dirs[o][k][3] * nch + ch) * NSPLIT * R * 32);
+        const uint4 *src = (const uint4 *)((const T *)a.Es + ((size_t)c_dirs[o][k][3] * nch + ch) * NSPLIT * RE * 32);
 #pragma unroll
         for (int r = 0; r < AR; r++) {
-            const int u = tid + r * NT;
-            ra[r] = u < NA ? src[u] : uint4{0, 0, 0, 0};
+            const int u = tid + r * NT, p = u / (R * 4), row = r0 + (u / 4) % R, q = u % 4;
+            ra[r] = u < NA && row < RE ? src[(p * RE + row) * 4 + q] : uint4{0, 0, 0, 0};
         }
 #pragma unroll
         for (int r = 0; r < BR; r++) {
@@ -338,10 +344,180 @@ __global__ __launch_bounds__(256, NCB <= 2 ? 2 : 1) void m2l_mfma_split(M2LMixAr
         for (int rb_ = 0; rb_ < RB; rb_++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int row = rb_ * 16 + 4 * fq + v;
+                const int row = r0 + rb_ * 16 + 4 * fq + v;
                 if (row < rf) dst[row] = acc[rb_][cb][v] * unscale;
             }
     }
+}
+
+// ---------------------------------------------------------------------------
+// The proxy (VT) and expansion (U) products of the mixed modes on the BF16
+// matrix cores with three-part operands (6 products, about FP32 accuracy;
+// P and Pf are FP32 in these modes):  C[M x ncol] = alpha * A B.
+//   A: M x K operator, pre-split once per operator set (split_a3) into
+//      [3 planes][R rows][Kp] BF16 (row-major, zero-padded to R = 16*RB rows
+//      and Kp = 32*nch columns);
+//   B: K x ncol, column-major (ld ldb), FP64 or FP32, split in registers.
+// A workgroup of 8 waves owns 256 columns (two 16-column blocks per wave)
+// and all RB row blocks; A goes through LDS one 32-deep k chunk at a time
+// (double-buffered), the next chunk's A and B loaded one step ahead.
+__global__ void split_a3(const double *__restrict__ A, int M, int K, int R, int Kp, __bf16 *__restrict__ As)
+{
+    const int64_t per = (int64_t)R * Kp, total = 3 * per;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int p = (int)(q / per), rem = (int)(q - p * per), row = rem / Kp, k = rem - row * Kp;
+        double x = (row < M && k < K) ? A[row + (int64_t)k * M] : 0.0;
+        __bf16 part = (__bf16)0.0f;
+        for (int i = 0; i <= p; i++) {
+            part = (__bf16)(float)x;
+            x -= (double)(float)part;
+        }
+        As[q] = part;
+    }
+}
+
+void launch_split_a3(const double *A, int M, int K, int R, int Kp, void *As, hipStream_t st)
+{
+    const int64_t total = 3LL * R * Kp;
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    split_a3<<<blocks, 256, 0, st>>>(A, M, K, R, Kp, (__bf16 *)As);
+    check(hipGetLastError(), "split_a3");
+}
+
+template <typename TBt, typename TC, int RB>
+__global__ __launch_bounds__(512) void gemm_split3(GemmSplitArgs g)
+{
+    constexpr int NT = 512, W = 256, R = 16 * RB, KS = 40;
+    constexpr int NA = 3 * R * 4;           // 16-byte units of one A chunk
+    constexpr int AR = (NA + NT - 1) / NT;
+    __shared__ __attribute__((aligned(16))) __bf16 As[2][3 * R * KS];
+    const TBt *B = (const TBt *)g.B;
+    TC *C = (TC *)g.C;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t n0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * W;
+    const int nch = g.nch, fr = lane % 16, fq = lane / 16;
+    int64_t gcol[2];
+    const TBt *bcol[2];
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) {
+        gcol[cb] = n0 + wave * 32 + cb * 16 + fr;
+        bcol[cb] = B + (gcol[cb] < g.ncol ? gcol[cb] : 0) * g.ldb;
+    }
+    uint4 ra[AR];
+    auto loadA = [&](int ch) {
+        const uint4 *src = (const uint4 *)g.As;  // [3][R][Kp] bf16: 16-byte unit (p, row, q) at (p*R + row)*nch*4 + ch*4 + q
+#pragma unroll
+        for (int r = 0; r < AR; r++) {
+            const int u = min(tid + r * NT, NA - 1), pr = u / 4, q = u % 4;
+            ra[r] = src[(int64_t)pr * nch * 4 + ch * 4 + q];
+        }
+    };
+    auto storeA = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < AR; r++) {
+            const int u = tid + r * NT;
+            if (AR * NT == NA || u < NA) *(uint4 *)&As[buf][(u / 4) * KS + 8 * (u % 4)] = ra[r];
+        }
+    };
+    float bv[2][2][8];  // [set][cb][8 k values]
+    auto loadB = [&](int ch, float (&b)[2][8]) {
+        const int k0 = ch * 32 + 8 * fq;
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const float v = (float)bcol[cb][min(k0 + e, g.K - 1)];
+                b[cb][e] = k0 + e < g.K ? v : 0.0f;
+            }
+    };
+    f32x4 acc[RB][2];
+#pragma unroll
+    for (int i = 0; i < RB; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0, 0, 0, 0};
+    auto run = [&](int ch, const float (&bcur)[2][8], float (&bnext)[2][8]) {
+        const int buf = ch & 1;
+        bf16x8 bfr[2][3];
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            float x[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) x[e] = bcur[cb][e];
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const __bf16 h = (__bf16)x[e];
+                    bfr[cb][p][e] = h;
+                    x[e] -= (float)h;
+                }
+        }
+        storeA(buf ^ 1);  // last read in step ch - 1, before the barrier
+        loadA(min(ch + 2, nch - 1));
+        loadB(min(ch + 1, nch - 1), bnext);
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+            bf16x8 afr[3];
+#pragma unroll
+            for (int p = 0; p < 3; p++) afr[p] = *(const bf16x8 *)&As[buf][((p * R) + rb * 16 + fr) * KS + 8 * fq];
+#pragma unroll
+            for (int cb = 0; cb < 2; cb++) {  // smallest terms first
+                acc[rb][cb] = mfma16(afr[2], bfr[cb][0], acc[rb][cb]);
+                acc[rb][cb] = mfma16(afr[1], bfr[cb][1], acc[rb][cb]);
+                acc[rb][cb] = mfma16(afr[0], bfr[cb][2], acc[rb][cb]);
+                acc[rb][cb] = mfma16(afr[1], bfr[cb][0], acc[rb][cb]);
+                acc[rb][cb] = mfma16(afr[0], bfr[cb][1], acc[rb][cb]);
+                acc[rb][cb] = mfma16(afr[0], bfr[cb][0], acc[rb][cb]);
+            }
+        }
+        __syncthreads();
+    };
+    loadA(0);
+    storeA(0);
+    loadA(min(1, nch - 1));
+    loadB(0, bv[0]);
+    __syncthreads();
+    for (int ch = 0; ch < nch; ch += 2) {
+        run(ch, bv[0], bv[1]);
+        if (ch + 1 < nch) run(ch + 1, bv[1], bv[0]);
+    }
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) {
+        if (gcol[cb] >= g.ncol) continue;
+        TC *dst = C + gcol[cb] * g.ldc;
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = rb * 16 + 4 * fq + v;
+                if (row < g.M) dst[row] = (TC)(g.alpha * (double)acc[rb][cb][v]);
+                else if (row < g.ldc) dst[row] = (TC)0;
+            }
+        for (int row = R; row < g.ldc; row++)  // padding rows beyond the MFMA tile
+            if (fq == 0) dst[row] = (TC)0;
+    }
+}
+
+void launch_gemm_split3(const GemmSplitArgs &g, hipStream_t st)
+{
+    if (g.ncol <= 0) return;
+    if (g.ldb < g.K || g.ldc < g.M) fail(PBB_ERR_INTERNAL, "gemm: leading dimension too small");
+    const int rb = (g.M + 15) / 16;
+    dim3 grid((unsigned)((g.ncol + 255) / 256));
+#define PBB_G3(RB)                                                                                   \
+    case RB:                                                                                         \
+        if (!g.b_f32 && g.c_f32) gemm_split3<double, float, RB><<<grid, 512, 0, st>>>(g);           \
+        else if (g.b_f32 && !g.c_f32) gemm_split3<float, double, RB><<<grid, 512, 0, st>>>(g);      \
+        else fail(PBB_ERR_INTERNAL, "gemm_split3: unsupported operand types");                       \
+        break;
+    switch (rb) {
+        PBB_G3(1) PBB_G3(2) PBB_G3(3) PBB_G3(4) PBB_G3(5) PBB_G3(6) PBB_G3(7) PBB_G3(8) PBB_G3(9) PBB_G3(10)
+        PBB_G3(11) PBB_G3(12) PBB_G3(13) PBB_G3(14)
+    default: fail(PBB_ERR_UNSUPPORTED, "split GEMM supports at most 224 rows");
+    }
+#undef PBB_G3
+    check(hipGetLastError(), "gemm_split3");
 }
 
 // ---------------------------------------------------------------------------
@@ -394,10 +570,10 @@ void launch_absmax(const float *x, int64_t n, unsigned *out, hipStream_t st)
 template <typename T, int NSPLIT>
 static void m2l_split_t(const M2LMixArgs &a, int ntiles, int ncolchunks, hipStream_t st)
 {
-    const int rb = (a.rf + 15) / 16;
+    const int rball = (a.rf + 15) / 16, halves = a.row_split ? 2 : 1, rb = (rball + halves - 1) / halves;
     const int w = a.TB * a.cpt;  // tile width: 64, 128 or 256 columns
     if (w > 256) fail(PBB_ERR_INTERNAL, "mixed-precision M2L tile wider than 256 columns");
-    dim3 grid((unsigned)ntiles, (unsigned)ncolchunks);
+    dim3 grid((unsigned)ntiles, (unsigned)ncolchunks, (unsigned)((rball + rb - 1) / rb));
 #define PBB_CASE(RB)                                                                   \
     case RB:                                                                           \
         if (w > 128) m2l_mfma_split<T, NSPLIT, RB, 4><<<grid, 256, 0, st>>>(a);        \

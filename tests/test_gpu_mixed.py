@@ -81,12 +81,12 @@ def test_m2l_tiles_and_tuner(gpu):
         plan.set_m2l_tile(width, cpb, order, pbb.capi.PREC_FP64)
         assert orc.rel_err(_mv(plan, w, nc, pbb.capi.PREC_FP64), ref) <= 1e-12
     for prec, bar in VARIANTS:
-        for width, cpb, order in ((64, 16, 0), (128, 32, 1), (256, 16, 0), (256, 32, 1), (128, 8, 0)):
-            print(f"case prec={prec} width={width} cpb={cpb} order={order}", flush=True)
-            plan.set_m2l_tile(width, cpb, order, prec)
+        for width, cpb, flags in ((64, 16, 0), (128, 32, 1), (256, 16, 2), (256, 32, 3), (128, 8, 0), (128, 32, 2)):
+            print(f"case prec={prec} width={width} cpb={cpb} flags={flags}", flush=True)
+            plan.set_m2l_tile(width, cpb, flags, prec)
             assert orc.rel_err(_mv(plan, w, nc, prec), ref) <= bar
     t = plan.tune_m2l(wd, pbb.capi.PREC_MIXED_F16X2)
-    assert t["width"] in (64, 128, 256) and t["order"] in (0, 1)
+    assert t["width"] in (64, 128, 256) and t["flags"] in (0, 1, 2, 3)
     t = plan.tune_m2l(wd, pbb.capi.PREC_FP64)
     assert t["width"] in (32, 64, 128)
     torch.cuda.synchronize()
