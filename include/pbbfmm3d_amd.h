@@ -219,11 +219,19 @@ PBB_API pbb_status pbb_plan_set_timing(pbb_plan *plan, int32_t enable);
  * target box in a tile (1..64), `flags`: PBB_M2L_CHUNK_MAJOR for chunk-major
  * steps (one k chunk of every direction, then the next chunk; otherwise the
  * k chunks of one E_d, then the next direction), PBB_M2L_ROW_SPLIT (mixed
- * modes) to split the rows of r_f over two workgroups per tile.  width 0
- * restores the built-in choice.  No reference counterpart: tuning of this
- * build's M2L.                                                            */
+ * modes) to split the rows of r_f over two workgroups per tile,
+ * PBB_M2L_WAVE_ROWS (mixed modes, no row split) for the kernel whose
+ * waves split the rows of r_f and load E_d into registers (ranks above 192,
+ * or above 128 in PBB_PREC_MIXED, cap its width at 128),
+ * PBB_M2L_DIR_GROUPS(g) (mixed modes, g <= 64) to split the 316 M2L
+ * operators into g groups of consecutive index, one launch per group on the
+ * large levels, so that a launch's operators stay in L2.  width 0 restores
+ * the built-in choice.  No reference counterpart: tuning of this build's
+ * M2L.                                                                    */
 #define PBB_M2L_CHUNK_MAJOR 1
 #define PBB_M2L_ROW_SPLIT 2
+#define PBB_M2L_WAVE_ROWS 4
+#define PBB_M2L_DIR_GROUPS(g) ((g) << 8)
 PBB_API pbb_status pbb_plan_set_m2l_tile(pbb_plan *plan, int32_t precision, int32_t width,
                                          int32_t cols_per_box, int32_t flags);
 

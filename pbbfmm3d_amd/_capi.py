@@ -44,6 +44,13 @@ PREC_MIXED_F16X2 = 2   # FP16 x2 split M2L
 PREC_MIXED_BF16X2 = 3  # BF16 x2 split M2L
 M2L_CHUNK_MAJOR = 1
 M2L_ROW_SPLIT = 2
+M2L_WAVE_ROWS = 4
+
+
+def M2L_DIR_GROUPS(g):
+    """M2L flag: g groups of operators, one launch per group (mixed modes)."""
+    return int(g) << 8
+
 
 NUM_PHASES = 8
 BATCH_TRANSPOSE = 1

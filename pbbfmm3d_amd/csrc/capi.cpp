@@ -644,8 +644,12 @@ PBB_API pbb_status pbb_plan_set_m2l_tile(pbb_plan *plan, int32_t precision, int3
                                               : (width == 32 || width == 64 || width == 128));
         if (!wok) fail(PBB_ERR_INVALID_ARG, "M2L width must be 0, 64, 128 or 256 (mixed) / 32, 64 or 128 (FP64)");
         if (width && (cols_per_box < 1 || cols_per_box > 64)) fail(PBB_ERR_INVALID_ARG, "cols_per_box must be 1..64");
-        const int known = PBB_M2L_CHUNK_MAJOR | (mixed ? PBB_M2L_ROW_SPLIT : 0);
+        const int known =
+            PBB_M2L_CHUNK_MAJOR | (mixed ? PBB_M2L_ROW_SPLIT | PBB_M2L_WAVE_ROWS | PBB_M2L_DIR_GROUPS(0xff) : 0);
         if (flags & ~known) fail(PBB_ERR_INVALID_ARG, "unknown M2L tile flags");
+        if ((flags & PBB_M2L_WAVE_ROWS) && (flags & PBB_M2L_ROW_SPLIT))
+            fail(PBB_ERR_INVALID_ARG, "PBB_M2L_WAVE_ROWS takes no row split");
+        if (((flags >> 8) & 0xff) > 64) fail(PBB_ERR_INVALID_ARG, "at most 64 M2L direction groups");
         plan->m2l_tile[mixed ? 1 : 0] = {width, width ? cols_per_box : 0, flags};
     });
 }
@@ -684,10 +688,10 @@ PBB_API pbb_status pbb_plan_tune_m2l(pbb_plan *plan, const double *w_dev, int32_
         const bool was_timing = plan->timing;
         check(pbb_plan_set_timing(plan, 1) == PBB_OK ? hipSuccess : hipErrorUnknown, "enable timing");
         double *scratch = dalloc<double>((size_t)plan->N * ncols, "tuning output");
-        // candidates {width, columns per box, order}
+        // candidates {width, columns per box, flags}
         static const int c64[][3] = {{32, 32, 0}, {64, 32, 0}, {128, 32, 0}, {64, 32, 1}, {128, 32, 1}};
-        static const int cmx[][3] = {{64, 16, 0}, {128, 32, 0}, {128, 32, 1}, {256, 32, 2}, {256, 16, 2},
-                                     {256, 16, 3}, {256, 32, 3}, {128, 32, 2}};
+        static const int cmx[][3] = {{128, 16, 5}, {128, 32, 5}, {128, 8, 5}, {256, 8, 5}, {128, 16, 4},
+                                     {64, 16, 5}, {128, 32, 1}, {64, 16, 0}, {128, 16, 5 | PBB_M2L_DIR_GROUPS(4)}};
         const int(*cands)[3] = mixed ? cmx : c64;
         const int ncand = mixed ? (int)(sizeof(cmx) / sizeof(cmx[0])) : (int)(sizeof(c64) / sizeof(c64[0]));
         pbb_plan::M2LTile bestt{};
@@ -826,21 +830,25 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
     // M2L tiles: W columns = TB target boxes x cpt columns.
     // FP64: W = 128 (64 for 17..31 columns), matrix-core bound, so few boxes
     // per tile keep the share of empty (box, direction) slots low.
-    // Mixed: W = 128 (4 boxes x 32 columns), two workgroups per CU.
+    // Mixed: the row-wave kernel, W = 128 (8 boxes x 16 columns), chunk-major
+    // steps, two workgroups per CU.
     const pbb_plan::M2LTile &t64 = p->m2l_tile[0], &tmx = p->m2l_tile[1];
     const int cpt = t64.width ? std::min(ncols, t64.cpt) : std::min(ncols, 64);
     const int width = t64.width ? t64.width : ncols >= 32 || ncols <= 16 ? 128 : 64;
     const int TB = std::max(1, std::min(16, width / cpt));
     const int nchunks = (ncols + cpt - 1) / cpt;
-    const int cptm = tmx.width ? std::min(ncols, tmx.cpt) : std::min(ncols, 32);
+    const int cptm = tmx.width ? std::min(ncols, tmx.cpt) : std::min(ncols, 16);
     const int nchunks_m = (ncols + cptm - 1) / cptm;
     const int width_m = tmx.width ? tmx.width : 128;
+    const int groups_m = tmx.width ? std::max(1, (tmx.flags >> 8) & 0xff) : 1;
     for (int l = 2; l <= Lv && mixed; l++) {
         const DevLevel &D = p->lv[l];
         dev::M2LMixArgs a{};
         a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = std::max(1, std::min(16, width_m / cptm)); a.cpt = cptm;
-        a.chunk_major = tmx.width ? (tmx.flags & PBB_M2L_CHUNK_MAJOR) : 0;
+        a.chunk_major = tmx.width ? (tmx.flags & PBB_M2L_CHUNK_MAJOR) : 1;
         a.row_split = tmx.width ? (tmx.flags & PBB_M2L_ROW_SPLIT) != 0 : 0;
+        a.wave_rows = tmx.width ? (tmx.flags & PBB_M2L_WAVE_ROWS) != 0 : 1;
+        if (a.wave_rows) a.TB = std::max(1, std::min(a.TB, dev::m2l_rows_max_width(prec, SE->R) / cptm));
         a.side = 1 << l;
         a.kpad = ldP; a.nch = SE->nch; a.escale = SE->escale[h.ops_set(l)]; a.off = D.off; a.R = SE->R;
         a.tile_prefix[0] = 0;
@@ -854,7 +862,15 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         a.Es = (const uint16_t *)SE->d + (size_t)h.ops_set(l) * 316 * SE->nch * nsplit * SE->R * 32;
         a.P = p->P32; a.Pf = p->Pf32;
         a.xmax = prec == PBB_PREC_MIXED_F16X2 ? p->xmax + l : nullptr;
-        dev::launch_m2l_mixed(prec, a, a.tile_prefix[8], nchunks_m, st);
+        // direction groups only where the launch fills the GPU many times over
+        const int64_t wgs = (int64_t)a.tile_prefix[8] * nchunks_m * (a.row_split ? 2 : 1);
+        const int ng = wgs >= 2048 ? groups_m : 1, per = (316 + ng - 1) / ng;
+        for (int g = 0; g < ng; g++) {
+            a.dlo = g * per;
+            a.dhi = std::min(316, (g + 1) * per);
+            a.accumulate = g > 0;
+            dev::launch_m2l_mixed(prec, a, a.tile_prefix[8], nchunks_m, st);
+        }
     }
     for (int l = 2; l <= Lv && !mixed; l++) {
         const DevLevel &D = p->lv[l];

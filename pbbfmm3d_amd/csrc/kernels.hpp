@@ -159,6 +159,9 @@ struct M2LMixArgs {
     int rf, rs, ncols, TB, cpt, side, kpad, nch, escale, R;  // R: rows of the split E (16-multiple)
     int chunk_major;                                         // step order: see m2l_step
     int row_split;                                           // rows of r_f split over two workgroups
+    int wave_rows;    // row-wave kernel: the waves split the rows, E straight to registers
+    int dlo, dhi;     // only directions whose operator index (0..315) lies in [dlo, dhi)
+    int accumulate;   // add to Pf (an earlier direction group's partial sums) instead of overwriting
     int64_t off;
     int tile_prefix[9];
     int64_t class_start[9];
@@ -192,6 +195,8 @@ void launch_absmax(const float *x, int64_t n, unsigned *out, hipStream_t st);
 void launch_split_p2(int variant, float *P, int64_t nchunks, const unsigned *xmax, hipStream_t st);
 void upload_dirs_mixed(const int dirs[8][189][4]);
 void launch_zero_u32(unsigned *p, int n, hipStream_t st);
+// widest row-wave M2L tile (PBB_M2L_WAVE_ROWS) for a variant and split-E row count
+int m2l_rows_max_width(int variant, int R);
 void launch_m2l_mixed(int variant, const M2LMixArgs &a, int ntiles, int ncolchunks, hipStream_t st);
 void launch_gather_weights(const double *w, const int32_t *perm, int64_t N, int ncols, double *ws, hipStream_t st);
 void launch_p2m(const P2MArgs &a, int nleaf, hipStream_t st);

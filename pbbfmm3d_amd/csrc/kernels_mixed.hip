@@ -31,6 +31,7 @@ __constant__ int c_dirs[8][189][4];  // far directions of each octant class (thi
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 template <typename T>
 struct Vec8;
@@ -166,7 +167,12 @@ void launch_split_p2(int variant, float *P, int64_t nchunks, const unsigned *xma
 // of r_f, starting at row r0: the two halves of a tile run as two
 // workgroups, adjacent in launch order so that they share the proxy gathers
 // through L2, and wide tiles keep two workgroups per CU.
+// Direction groups: a launch covers only the directions whose operator index
+// lies in [a.dlo, a.dhi), so every workgroup of the launch reads the same few
+// E_d and they stay in L2; later groups add to the partial sums in Pf
+// (a.accumulate), and a tile with no direction in the group is skipped.
 constexpr int TB_MAX = 16;
+__device__ __attribute__((aligned(16))) float g_zero32[32];  // a zero k chunk (empty slots), never written
 
 template <typename T, int NSPLIT, int RB, int NCB>
 __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2LMixArgs a)
@@ -181,7 +187,7 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
     __shared__ __attribute__((aligned(16))) float Bs[W * BS];
     __shared__ int s_sid[189 * TB_MAX];
     __shared__ short s_dir[189];
-    __shared__ int s_nd;
+    __shared__ int s_nd, s_wcnt[3];
     __shared__ int s_tgt[TB_MAX];
     __shared__ int s_xyz[TB_MAX][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -217,7 +223,8 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
     for (int q = tid; q < 189 * TB; q += NT) {
         const int k = q / TB, tb = q - k * TB;
         int sid = -1;
-        if (s_tgt[tb] >= 0) {
+        const int e = c_dirs[o][k][3];
+        if (s_tgt[tb] >= 0 && e >= a.dlo && e < a.dhi) {
             int sx = s_xyz[tb][0] + c_dirs[o][k][0], sy = s_xyz[tb][1] + c_dirs[o][k][1],
                 sz = s_xyz[tb][2] + c_dirs[o][k][2];
             if (sx >= 0 && sy >= 0 && sz >= 0 && sx < S && sy < S && sz < S)
@@ -226,44 +233,52 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
         s_sid[k * TB_MAX + tb] = sid;
     }
     __syncthreads();
-    for (int k = tid; k < 189; k += NT) {  // does direction k reach any source box?
-        bool any = false;
-        for (int tb = 0; tb < TB; tb++) any |= s_sid[k * TB_MAX + tb] >= 0;
-        s_dir[k] = any ? 1 : 0;
+    // compact the directions that reach a source box, in class order
+    // (waves 0..2 cover k = 0..191: one ballot each, then the prefix)
+    bool live = false;
+    int pre = 0;
+    if (tid < 192) {
+        if (tid < 189)
+            for (int tb = 0; tb < TB; tb++) live |= s_sid[tid * TB_MAX + tb] >= 0;
+        const unsigned long long m = __ballot(live);
+        pre = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) s_wcnt[wave] = __popcll(m);
     }
     __syncthreads();
-    if (tid == 0) {  // compact the non-empty directions in class order
-        int nd = 0;
-        for (int k = 0; k < 189; k++)
-            if (s_dir[k]) s_dir[nd++] = (short)k;
-        s_nd = nd;
-    }
+    if (live) s_dir[pre + (wave > 0 ? s_wcnt[0] : 0) + (wave > 1 ? s_wcnt[1] : 0)] = (short)tid;
+    if (tid == 0) s_nd = s_wcnt[0] + s_wcnt[1] + s_wcnt[2];
     __syncthreads();
     const int nsteps = s_nd * nch;
+    if (a.accumulate && nsteps == 0) return;  // nothing to add for this tile
 
-    uint4 ra[AR];
-    float4 rb[BR];
-    auto load = [&](int step) {
+    // Loads are unconditional, so that the compiler's wait counts stay exact:
+    // rows past RE repeat the last row (their results are never stored) and
+    // empty (box, direction) slots read a line of zeros.
+    u32x4 ra[AR];
+    f32x4 pb0[BR], pb1[BR];  // proxy chunks of two consecutive steps
+    const int last = nsteps - 1;
+    auto loadE = [&](int step) __attribute__((always_inline)) {
         int di, ch;
-        m2l_step(step, s_nd, nch, a.chunk_major, di, ch);
+        m2l_step(min(step, last), s_nd, nch, a.chunk_major, di, ch);
         const int k = s_dir[di];
-        const uint4 *src = (const uint4 *)((const T *)a.Es + ((size_t)c_dirs[o][k][3] * nch + ch) * NSPLIT * RE * 32);
+        const u32x4 *src = (const u32x4 *)((const T *)a.Es + ((size_t)c_dirs[o][k][3] * nch + ch) * NSPLIT * RE * 32);
 #pragma unroll
         for (int r = 0; r < AR; r++) {
-            const int u = tid + r * NT, p = u / (R * 4), row = r0 + (u / 4) % R, q = u % 4;
-            ra[r] = u < NA && row < RE ? src[(p * RE + row) * 4 + q] : uint4{0, 0, 0, 0};
+            const int u = min(tid + r * NT, NA - 1), p = u / (R * 4), row = min(r0 + (u / 4) % R, RE - 1), q = u % 4;
+            ra[r] = src[(p * RE + row) * 4 + q];
         }
+    };
+    auto loadP = [&](int step, f32x4 (&pb)[BR]) __attribute__((always_inline)) {
+        int di, ch;
+        m2l_step(min(step, last), s_nd, nch, a.chunk_major, di, ch);
+        const int k = s_dir[di];
 #pragma unroll
         for (int r = 0; r < BR; r++) {
             const int u = tid + r * NT, col = u / 8, k4 = u % 8;
             const int tb = col / cpt, cl = col - tb * cpt;
-            float4 v = {0, 0, 0, 0};
-            if (tb < TB) {
-                const int sid = s_sid[k * TB_MAX + tb];
-                if (sid >= 0)
-                    v = *(const float4 *)(a.P + ((size_t)(a.off + sid) * a.ncols + c0 + cl) * a.kpad + ch * 32 + 4 * k4);
-            }
-            rb[r] = v;
+            const int sid = tb < TB && step <= last ? s_sid[k * TB_MAX + tb] : -1;  // zeros past the last step
+            const float *src = sid >= 0 ? a.P + ((size_t)(a.off + sid) * a.ncols + c0 + cl) * a.kpad + ch * 32 : g_zero32;
+            pb[r] = *(const f32x4 *)(src + 4 * k4);
         }
     };
     f32x4 acc[RB][NCB];
@@ -273,24 +288,50 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
         for (int j = 0; j < NCB; j++) acc[i][j] = f32x4{0, 0, 0, 0};
 
     const int fr = lane % 16, fq = lane / 16;
-    if (nsteps > 0) load(0);
-    for (int step = 0; step < nsteps; step++) {
+    if (nsteps > 0) {
+        loadE(0);
+        loadP(0, pb0);
+        loadP(1, pb1);
+    }
+    if (a.accumulate) {  // continue from the earlier groups' sums (exact power-of-two rescale)
+        // unconditional loads (clamped to the tile's first box and row r_f - 1),
+        // so that they overlap instead of waiting one by one
+        const float rescale = ldexpf(1.0f, xs + a.escale);
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++) {
+            const int col = (wave * NCB + cb) * 16 + fr;
+            const int tb = col / cpt, cl = min(col - tb * cpt, cpt - 1);
+            const int t = tb < TB ? s_tgt[tb] : -1;
+            const float *src = a.Pf + ((size_t)(a.off + (t >= 0 ? t : s_tgt[0])) * a.ncols + c0 + cl) * rf;
+#pragma unroll
+            for (int rb_ = 0; rb_ < RB; rb_++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int row = r0 + rb_ * 16 + 4 * fq + v;
+                    const float x = src[min(row, rf - 1)];
+                    acc[rb_][cb][v] = t >= 0 && row < rf ? x * rescale : 0.0f;
+                }
+        }
+    }
+    // one step: E chunk loaded one step ahead, proxy chunk two steps ahead
+    auto run = [&](int step, f32x4 (&pb)[BR]) __attribute__((always_inline)) {
         __syncthreads();  // every wave is done with the previous step's LDS
 #pragma unroll
         for (int r = 0; r < AR; r++) {
             const int u = tid + r * NT;
-            if (u < NA) {
+            if (AR * NT == NA || u < NA) {
                 const int p = u / (R * 4), row = (u / 4) % R, q = u % 4;
-                *(uint4 *)&As[(p * R + row) * KS + 8 * q] = ra[r];
+                *(u32x4 *)&As[(p * R + row) * KS + 8 * q] = ra[r];
             }
         }
 #pragma unroll
         for (int r = 0; r < BR; r++) {
             const int u = tid + r * NT;
-            *(float4 *)&Bs[(u / 8) * BS + 4 * (u % 8)] = rb[r];
+            *(f32x4 *)&Bs[(u / 8) * BS + 4 * (u % 8)] = pb[r];
         }
         __syncthreads();
-        if (step + 1 < nsteps) load(step + 1);  // in flight during the MFMAs
+        loadE(step + 1);  // in flight during the MFMAs
+        loadP(step + 2, pb);
         V8 bfr[NCB][NSPLIT];
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++) {
@@ -331,6 +372,11 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
                 acc[rb_][cb] = mfma16(afr[0], bfr[cb][0], acc[rb_][cb]);
             }
         }
+    };
+    // an even number of steps: a padding step (zero proxies) adds nothing
+    for (int step = 0; step < nsteps; step += 2) {
+        run(step, pb0);
+        run(step + 1, pb1);
     }
 #pragma unroll
     for (int cb = 0; cb < NCB; cb++) {
@@ -346,6 +392,282 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
             for (int v = 0; v < 4; v++) {
                 const int row = r0 + rb_ * 16 + 4 * fq + v;
                 if (row < rf) dst[row] = acc[rb_][cb][v] * unscale;
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row-wave variant: 4 waves split the rows of r_f (RBW 16-row blocks each)
+// and share 16*NCB columns; WCOL = 2 puts two such groups of 4 waves side by
+// side in one workgroup (W = 32*NCB columns), so that each E_d chunk serves
+// twice the columns (the second wave's load of the same rows hits L1/L2
+// right after the first).  A wave loads
+// its rows of the E_d chunk straight into MFMA operand registers (one step
+// ahead, no LDS); the proxy chunk goes through LDS once for all waves
+// (loaded three steps ahead, written to the other half of a double buffer
+// one step ahead, one barrier per step).  Per step a wave reads NCB B
+// fragments from LDS, half the LDS traffic of the column-wave layout, each
+// one block ahead of its MFMAs.
+// A column block whose boxes have no source in the step's direction is
+// skipped: its proxies are zero, so the sums are unchanged bit for bit.
+// Step bookkeeping (direction, k chunk) runs in scalar registers; the
+// compacted per-direction tables (E offset, box mask, source boxes) are in
+// LDS, one read per address.
+template <typename T, int NSPLIT, int RBW, int NCB, int WCOL>
+__global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M2LMixArgs a)
+{
+    typedef typename Vec8<T>::type V8;
+    constexpr int NT = 256 * WCOL, W = 16 * NCB * WCOL, BS = 36;
+    constexpr int BR = W * 8 / NT;  // 16-byte proxy units per thread and step
+    constexpr bool PRESPLIT = NSPLIT == 2;
+    static_assert(BR >= 1 && BR * NT == W * 8, "tile width");
+    __shared__ __attribute__((aligned(16))) float Bs[2][W * BS];
+    __shared__ int s_sid[189 * TB_MAX];   // by class direction
+    __shared__ int s_sidc[189 * TB_MAX];  // by compacted direction
+    __shared__ int s_eoff[189];           // operator index * nch, by compacted direction
+    __shared__ unsigned short s_mask[189];
+    __shared__ int s_nd, s_wcnt[3];
+    __shared__ int s_tgt[TB_MAX];
+    __shared__ int s_xyz[TB_MAX][3];
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int rf = a.rf, TB = a.TB, nch = a.nch, RE = a.R;
+    const int lin = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int tile = lin / gridDim.y;  // the column chunks of one tile are adjacent in launch order
+    int o = 0;
+    while (o < 7 && tile >= a.tile_prefix[o + 1]) o++;
+    const int64_t first = a.class_start[o] + (int64_t)(tile - a.tile_prefix[o]) * TB;
+    const int64_t cend = a.class_start[o + 1];
+    const int c0 = (lin - tile * (int)gridDim.y) * a.cpt;
+    const int cpt = min(a.cpt, a.ncols - c0);
+    const int S = a.side;
+    const int xs = p_scale_exp(a.xmax);
+    const float xscale = ldexpf(1.0f, xs);
+    const float unscale = ldexpf(1.0f, -(xs + a.escale));
+    if (tid < TB) {
+        int64_t q = first + tid;
+        int t = q < cend ? a.order[q] : -1;
+        s_tgt[tid] = t;
+        if (t >= 0) {
+            uint32_t c = (uint32_t)a.coord[t];
+            s_xyz[tid][0] = c & 1023;
+            s_xyz[tid][1] = (c >> 10) & 1023;
+            s_xyz[tid][2] = (c >> 20) & 1023;
+        }
+    }
+    __syncthreads();
+    for (int q = tid; q < 189 * TB; q += NT) {  // source box of every (direction, target) pair
+        const int k = q / TB, tb = q - k * TB;
+        int sid = -1;
+        const int e = c_dirs[o][k][3];
+        if (s_tgt[tb] >= 0 && e >= a.dlo && e < a.dhi) {
+            int sx = s_xyz[tb][0] + c_dirs[o][k][0], sy = s_xyz[tb][1] + c_dirs[o][k][1],
+                sz = s_xyz[tb][2] + c_dirs[o][k][2];
+            if (sx >= 0 && sy >= 0 && sz >= 0 && sx < S && sy < S && sz < S)
+                sid = a.map[((int64_t)sx * S + sy) * S + sz];
+        }
+        s_sid[k * TB_MAX + tb] = sid;
+    }
+    __syncthreads();
+    // compact the directions that reach a source box, in class order, with
+    // the mask of the boxes they reach
+    unsigned bm = 0;
+    int pre = 0;
+    if (tid < 192) {
+        if (tid < 189)
+            for (int tb = 0; tb < TB; tb++) bm |= (s_sid[tid * TB_MAX + tb] >= 0 ? 1u : 0u) << tb;
+        const unsigned long long m = __ballot(bm != 0);
+        pre = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) s_wcnt[wave] = __popcll(m);
+    }
+    __syncthreads();
+    if (bm) {
+        const int pos = pre + (wave > 0 ? s_wcnt[0] : 0) + (wave > 1 ? s_wcnt[1] : 0);
+        s_eoff[pos] = c_dirs[o][tid][3] * nch;
+        s_mask[pos] = (unsigned short)bm;
+        for (int tb = 0; tb < TB; tb++) s_sidc[pos * TB_MAX + tb] = s_sid[tid * TB_MAX + tb];
+    }
+    if (tid == 0) s_nd = s_wcnt[0] + s_wcnt[1] + s_wcnt[2];
+    __syncthreads();
+    const int nd = __builtin_amdgcn_readfirstlane(s_nd);
+    const int nsteps = nd * nch;
+    if (a.accumulate && nsteps == 0) return;  // nothing to add for this tile
+
+    const int fr = lane % 16, fq = lane / 16, rb0 = (wave & 3) * RBW;
+    const int cbase = (wave >> 2) * 16 * NCB;  // first column of this wave's group
+    // boxes covered by each column block (wave-uniform)
+    unsigned cbm[NCB];
+#pragma unroll
+    for (int cb = 0; cb < NCB; cb++) {
+        const int lo = (cbase + cb * 16) / cpt, hi = min(TB - 1, (cbase + cb * 16 + 15) / cpt);
+        cbm[cb] = lo < TB ? ((2u << hi) - (1u << lo)) : 0u;
+    }
+    const int last = nsteps - 1;
+    const bool cmaj = a.chunk_major != 0;
+    // (direction, chunk) of a step, advanced one step at a time
+    struct Cur {
+        int step, di, ch;
+    };
+    auto adv = [&](Cur &c) __attribute__((always_inline)) {
+        c.step++;
+        if (cmaj) {
+            if (++c.di == nd) {
+                c.di = 0;
+                c.ch++;
+            }
+        } else if (++c.ch == nch) {
+            c.ch = 0;
+            c.di++;
+        }
+    };
+    // unconditional loads, so that the compiler's wait counts stay exact:
+    // rows past RE repeat row RE - 1 (never stored), empty slots and the
+    // padding step read a chunk of zeros
+    auto loadE = [&](const Cur &c, V8 (&ea)[RBW][NSPLIT]) __attribute__((always_inline)) {
+        const int eo = c.step <= last ? s_eoff[c.di] + c.ch : 0;
+        const T *base = (const T *)a.Es + (size_t)eo * NSPLIT * RE * 32;
+#pragma unroll
+        for (int i = 0; i < RBW; i++) {
+            const int row = min((rb0 + i) * 16 + fr, RE - 1);
+#pragma unroll
+            for (int p = 0; p < NSPLIT; p++) ea[i][p] = *(const V8 *)(base + ((size_t)p * RE + row) * 32 + 8 * fq);
+        }
+    };
+    auto loadP = [&](const Cur &c, f32x4 (&pb)[BR]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int r = 0; r < BR; r++) {
+            const int u = tid + r * NT, col = u / 8, k4 = u % 8;
+            const int tb = col / cpt, cl = col - tb * cpt;
+            const int sid = tb < TB && c.step <= last ? s_sidc[c.di * TB_MAX + tb] : -1;
+            const float *src =
+                sid >= 0 ? a.P + ((size_t)(a.off + sid) * a.ncols + c0 + cl) * a.kpad + c.ch * 32 : g_zero32;
+            pb[r] = *(const f32x4 *)(src + 4 * k4);
+        }
+    };
+    f32x4 acc[RBW][NCB];
+#pragma unroll
+    for (int i = 0; i < RBW; i++)
+#pragma unroll
+        for (int j = 0; j < NCB; j++) acc[i][j] = f32x4{0, 0, 0, 0};
+    V8 eA[RBW][NSPLIT], eB[RBW][NSPLIT];
+    f32x4 pb0[BR], pb1[BR];
+    Cur cc{0, 0, 0};          // the step being computed
+    Cur ce = cc, cp = cc;     // the next E / proxy load
+    auto storeP = [&](int buf, const f32x4 (&pb)[BR]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int r = 0; r < BR; r++) {
+            const int u = tid + r * NT;
+            *(f32x4 *)&Bs[buf][(u / 8) * BS + 4 * (u % 8)] = pb[r];
+        }
+    };
+    if (nsteps > 0) {  // Bs[0] = P(0); registers: E(0), P(1), P(2)
+        loadE(ce, eA);
+        adv(ce);
+        loadP(cp, pb0);
+        adv(cp);
+        loadP(cp, pb1);
+        adv(cp);
+        storeP(0, pb0);
+        loadP(cp, pb0);
+        adv(cp);
+    }
+    if (a.accumulate) {  // continue from the earlier groups' sums (exact power-of-two rescale)
+        const float rescale = ldexpf(1.0f, xs + a.escale);
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++) {
+            const int col = cbase + cb * 16 + fr;
+            const int tb = col / cpt, cl = min(col - tb * cpt, cpt - 1);
+            const int t = tb < TB ? s_tgt[tb] : -1;
+            const float *src = a.Pf + ((size_t)(a.off + (t >= 0 ? t : s_tgt[0])) * a.ncols + c0 + cl) * rf;
+#pragma unroll
+            for (int i = 0; i < RBW; i++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int row = (rb0 + i) * 16 + 4 * fq + v;
+                    const float x = src[min(row, rf - 1)];
+                    acc[i][cb][v] = t >= 0 && row < rf ? x * rescale : 0.0f;
+                }
+        }
+    }
+    // B fragment of column block cb: 32 bytes per lane from LDS
+    auto readB = [&](const float *Bb, int cb, u32x4 (&r)[2]) __attribute__((always_inline)) {
+        const float *bp = &Bb[(cbase + cb * 16 + fr) * BS];
+        if constexpr (PRESPLIT) {  // plane p of k 8fq..8fq+7 at floats 16p + 4fq
+            r[0] = *(const u32x4 *)(bp + 4 * fq);
+            r[1] = *(const u32x4 *)(bp + 16 + 4 * fq);
+        } else {
+            r[0] = *(const u32x4 *)(bp + 8 * fq);
+            r[1] = *(const u32x4 *)(bp + 8 * fq + 4);
+        }
+    };
+    // step s (buf = s & 1): Bs[buf] holds P(s), pb holds P(s + 1)
+    auto run = [&](int buf, f32x4 (&pb)[BR], V8 (&ea)[RBW][NSPLIT], V8 (&en)[RBW][NSPLIT])
+        __attribute__((always_inline)) {
+        __syncthreads();  // Bs[buf] is complete and every wave is done with Bs[buf ^ 1]
+        const unsigned live = cc.step <= last ? (unsigned)s_mask[cc.di] : 0u;
+        loadE(ce, en);  // the next step's E, in flight during this step
+        adv(ce);
+        storeP(buf ^ 1, pb);
+        loadP(cp, pb);  // three steps ahead
+        adv(cp);
+        const unsigned lv = __builtin_amdgcn_readfirstlane(live);
+        const float *Bb = Bs[buf];
+        u32x4 rq[2][2];
+        readB(Bb, 0, rq[0]);
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++) {
+            if (cb + 1 < NCB) readB(Bb, cb + 1, rq[(cb + 1) & 1]);
+            if (!(lv & cbm[cb])) continue;
+            V8 bf[NSPLIT];
+            if constexpr (PRESPLIT) {
+                bf[0] = __builtin_bit_cast(V8, rq[cb & 1][0]);
+                bf[1] = __builtin_bit_cast(V8, rq[cb & 1][1]);
+            } else {
+                const f32x4 v0 = __builtin_bit_cast(f32x4, rq[cb & 1][0]), v1 = __builtin_bit_cast(f32x4, rq[cb & 1][1]);
+                float x[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+                for (int e = 0; e < 8; e++) x[e] *= xscale;
+#pragma unroll
+                for (int p = 0; p < NSPLIT; p++)
+#pragma unroll
+                    for (int e = 0; e < 8; e++) {
+                        const T h = (T)x[e];
+                        bf[p][e] = h;
+                        x[e] -= (float)h;
+                    }
+            }
+#pragma unroll
+            for (int i = 0; i < RBW; i++) {  // smallest terms first
+                if constexpr (NSPLIT == 3) {
+                    acc[i][cb] = mfma16(ea[i][2], bf[0], acc[i][cb]);
+                    acc[i][cb] = mfma16(ea[i][1], bf[1], acc[i][cb]);
+                    acc[i][cb] = mfma16(ea[i][0], bf[2], acc[i][cb]);
+                }
+                acc[i][cb] = mfma16(ea[i][1], bf[0], acc[i][cb]);
+                acc[i][cb] = mfma16(ea[i][0], bf[1], acc[i][cb]);
+                acc[i][cb] = mfma16(ea[i][0], bf[0], acc[i][cb]);
+            }
+        }
+        adv(cc);
+    };
+    // an even number of steps: the padding step (zero proxies, empty mask) adds nothing
+    for (int step = 0; step < nsteps; step += 2) {
+        run(0, pb1, eA, eB);
+        run(1, pb0, eB, eA);
+    }
+#pragma unroll
+    for (int cb = 0; cb < NCB; cb++) {
+        const int col = cbase + cb * 16 + fr;
+        const int tb = col / cpt, cl = col - tb * cpt;
+        if (tb >= TB) continue;
+        const int t = s_tgt[tb];
+        if (t < 0) continue;
+        float *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + cl) * rf;
+#pragma unroll
+        for (int i = 0; i < RBW; i++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = (rb0 + i) * 16 + 4 * fq + v;
+                if (row < rf) dst[row] = acc[i][cb][v] * unscale;
             }
     }
 }
@@ -567,9 +889,57 @@ void launch_absmax(const float *x, int64_t n, unsigned *out, hipStream_t st)
     check(hipGetLastError(), "absmax_f32");
 }
 
+// 16-block row-wave groups spill registers for the widest row slices
+template <int NSPLIT, int RBW>
+constexpr bool rows_wide_ok()
+{
+    return !(RBW == 4 || (NSPLIT == 3 && RBW >= 3));
+}
+
+int m2l_rows_max_width(int variant, int R)
+{
+    const int rbw = (R / 16 + 3) / 4;
+    const bool three = variant == PBB_PREC_MIXED;
+    return rbw >= 4 || (three && rbw >= 3) ? 128 : 256;
+}
+
+template <typename T, int NSPLIT, int RBW>
+static void m2l_rows_launch(const M2LMixArgs &a, dim3 grid, int w, hipStream_t st)
+{
+    if constexpr (rows_wide_ok<NSPLIT, RBW>()) {
+        if (w > 128) m2l_mfma_rows<T, NSPLIT, RBW, 8, 2><<<grid, 512, 0, st>>>(a);
+        else if (w > 64) m2l_mfma_rows<T, NSPLIT, RBW, 8, 1><<<grid, 256, 0, st>>>(a);
+        else m2l_mfma_rows<T, NSPLIT, RBW, 4, 1><<<grid, 256, 0, st>>>(a);
+    } else {
+        if (w > 128) fail(PBB_ERR_INTERNAL, "row-wave M2L tile too wide for this rank");
+        if (w > 64) m2l_mfma_rows<T, NSPLIT, RBW, 4, 2><<<grid, 512, 0, st>>>(a);
+        else m2l_mfma_rows<T, NSPLIT, RBW, 4, 1><<<grid, 256, 0, st>>>(a);
+    }
+}
+
+template <typename T, int NSPLIT>
+static void m2l_rows_t(const M2LMixArgs &a, int ntiles, int ncolchunks, hipStream_t st)
+{
+    const int rbw = (a.R / 16 + 3) / 4;
+    const int w = a.TB * a.cpt;
+    if (w > 256) fail(PBB_ERR_INTERNAL, "row-wave M2L tiles are at most 256 columns wide");
+    dim3 grid((unsigned)ntiles, (unsigned)ncolchunks);
+    switch (rbw) {
+    case 1: m2l_rows_launch<T, NSPLIT, 1>(a, grid, w, st); break;
+    case 2: m2l_rows_launch<T, NSPLIT, 2>(a, grid, w, st); break;
+    case 3: m2l_rows_launch<T, NSPLIT, 3>(a, grid, w, st); break;
+    case 4: m2l_rows_launch<T, NSPLIT, 4>(a, grid, w, st); break;
+    default: fail(PBB_ERR_UNSUPPORTED, "mixed-precision M2L supports ranks up to 224");
+    }
+}
+
 template <typename T, int NSPLIT>
 static void m2l_split_t(const M2LMixArgs &a, int ntiles, int ncolchunks, hipStream_t st)
 {
+    if (a.wave_rows) {
+        m2l_rows_t<T, NSPLIT>(a, ntiles, ncolchunks, st);
+        return;
+    }
     const int rball = (a.rf + 15) / 16, halves = a.row_split ? 2 : 1, rb = (rball + halves - 1) / halves;
     const int w = a.TB * a.cpt;  // tile width: 64, 128 or 256 columns
     if (w > 256) fail(PBB_ERR_INTERNAL, "mixed-precision M2L tile wider than 256 columns");

@@ -81,13 +81,43 @@ def test_m2l_tiles_and_tuner(gpu):
         plan.set_m2l_tile(width, cpb, order, pbb.capi.PREC_FP64)
         assert orc.rel_err(_mv(plan, w, nc, pbb.capi.PREC_FP64), ref) <= 1e-12
     for prec, bar in VARIANTS:
-        for width, cpb, flags in ((64, 16, 0), (128, 32, 1), (256, 16, 2), (256, 32, 3), (128, 8, 0), (128, 32, 2)):
+        for width, cpb, flags in ((64, 16, 0), (128, 32, 1), (256, 16, 2), (256, 32, 3), (128, 8, 0), (128, 32, 2),
+                                  (128, 32, 4), (64, 16, 5), (128, 8, 4), (128, 20, 5), (256, 16, 5), (256, 32, 4)):
             print(f"case prec={prec} width={width} cpb={cpb} flags={flags}", flush=True)
             plan.set_m2l_tile(width, cpb, flags, prec)
             assert orc.rel_err(_mv(plan, w, nc, prec), ref) <= bar
     t = plan.tune_m2l(wd, pbb.capi.PREC_MIXED_F16X2)
-    assert t["width"] in (64, 128, 256) and t["flags"] in (0, 1, 2, 3)
+    assert t["width"] in (64, 128, 256) and (t["flags"] & 0xff) in range(8)
     t = plan.tune_m2l(wd, pbb.capi.PREC_FP64)
     assert t["width"] in (32, 64, 128)
     torch.cuda.synchronize()
     assert orc.rel_err(_mv(plan, w, nc, pbb.capi.PREC_FP64), ref) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_m2l_direction_groups_and_wave_rows(gpu):
+    # a level with enough tiles runs the M2L in groups of operators, later
+    # groups adding to the earlier partial sums (group counts that do not
+    # divide 316 included): within the mixed bar of the FP64 path.  The
+    # row-wave kernel (PBB_M2L_WAVE_ROWS) runs the same MFMAs in the same
+    # order as the column-wave kernel and skips only products with zero
+    # proxies, so the two agree bit for bit.
+    N, nc = 60000, 16
+    xyz = np.random.default_rng(70).random((N, 3)) - 0.5
+    w = weights(N, nc, seed=71, dist="normal")
+    ops = pbb.Operators(pbb.gaussian(0.3), 1.0, 5, 4, 1e-6)
+    plan = pbb.Plan(ops, xyz)
+    assert plan.info()["boxes"][-1] > 20000  # leaf boxes: enough tiles for the groups
+    ref = _mv(plan, w, nc, pbb.capi.PREC_FP64)
+    G = pbb.capi.M2L_DIR_GROUPS
+    for prec, bar in VARIANTS:
+        outs = {}
+        for width, cpb, flags in ((64, 16, 0), (64, 16, 4), (64, 16, 1), (64, 16, 5), (64, 16, 0 | G(3)),
+                                  (64, 16, 4 | G(3)), (64, 16, 5 | G(16)), (128, 16, 4 | G(64)), (128, 16, 1 | G(5)),
+                                  (256, 16, 5 | G(4))):
+            print(f"case prec={prec} width={width} cpb={cpb} flags={flags}", flush=True)
+            plan.set_m2l_tile(width, cpb, flags, prec)
+            outs[(width, flags)] = out = _mv(plan, w, nc, prec)
+            assert orc.rel_err(out, ref) <= bar
+        for f in (0, 1, G(3)):
+            assert np.array_equal(outs[(64, f)], outs[(64, f | 4)])
