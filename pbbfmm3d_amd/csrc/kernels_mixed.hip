@@ -522,8 +522,21 @@ __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M
     // unconditional loads, so that the compiler's wait counts stay exact:
     // rows past RE repeat row RE - 1 (never stored), empty slots and the
     // padding step read a chunk of zeros
-    auto loadE = [&](const Cur &c, V8 (&ea)[RBW][NSPLIT]) __attribute__((always_inline)) {
-        const int eo = c.step <= last ? s_eoff[c.di] + c.ch : 0;
+    // table reads for a step (issued one step before the loads that use them)
+    auto readM = [&](const Cur &c) __attribute__((always_inline)) {
+        return c.step <= last ? (unsigned)s_mask[c.di] : 0u;
+    };
+    auto readE = [&](const Cur &c) __attribute__((always_inline)) {
+        return c.step <= last ? s_eoff[c.di] + c.ch : 0;
+    };
+    auto readP = [&](const Cur &c, int (&sid)[BR]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int r = 0; r < BR; r++) {
+            const int tb = ((tid + r * NT) / 8) / cpt;
+            sid[r] = tb < TB && c.step <= last ? s_sidc[c.di * TB_MAX + tb] : -1;
+        }
+    };
+    auto loadE = [&](int eo, V8 (&ea)[RBW][NSPLIT]) __attribute__((always_inline)) {
         const T *base = (const T *)a.Es + (size_t)eo * NSPLIT * RE * 32;
 #pragma unroll
         for (int i = 0; i < RBW; i++) {
@@ -532,14 +545,13 @@ __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M
             for (int p = 0; p < NSPLIT; p++) ea[i][p] = *(const V8 *)(base + ((size_t)p * RE + row) * 32 + 8 * fq);
         }
     };
-    auto loadP = [&](const Cur &c, f32x4 (&pb)[BR]) __attribute__((always_inline)) {
+    auto loadP = [&](const int (&sid)[BR], int ch, f32x4 (&pb)[BR]) __attribute__((always_inline)) {
 #pragma unroll
         for (int r = 0; r < BR; r++) {
             const int u = tid + r * NT, col = u / 8, k4 = u % 8;
-            const int tb = col / cpt, cl = col - tb * cpt;
-            const int sid = tb < TB && c.step <= last ? s_sidc[c.di * TB_MAX + tb] : -1;
+            const int cl = col % cpt;
             const float *src =
-                sid >= 0 ? a.P + ((size_t)(a.off + sid) * a.ncols + c0 + cl) * a.kpad + c.ch * 32 : g_zero32;
+                sid[r] >= 0 ? a.P + ((size_t)(a.off + sid[r]) * a.ncols + c0 + cl) * a.kpad + ch * 32 : g_zero32;
             pb[r] = *(const f32x4 *)(src + 4 * k4);
         }
     };
@@ -550,8 +562,8 @@ __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M
         for (int j = 0; j < NCB; j++) acc[i][j] = f32x4{0, 0, 0, 0};
     V8 eA[RBW][NSPLIT], eB[RBW][NSPLIT];
     f32x4 pb0[BR], pb1[BR];
-    Cur cc{0, 0, 0};          // the step being computed
-    Cur ce = cc, cp = cc;     // the next E / proxy load
+    Cur cm{0, 0, 0};  // next table reads: mask, E offset, proxy sources
+    Cur ce = cm, cp = cm;
     auto storeP = [&](int buf, const f32x4 (&pb)[BR]) __attribute__((always_inline)) {
 #pragma unroll
         for (int r = 0; r < BR; r++) {
@@ -560,15 +572,16 @@ __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M
         }
     };
     if (nsteps > 0) {  // Bs[0] = P(0); registers: E(0), P(1), P(2)
-        loadE(ce, eA);
+        int sid[BR];
+        loadE(readE(ce), eA);
         adv(ce);
-        loadP(cp, pb0);
-        adv(cp);
-        loadP(cp, pb1);
-        adv(cp);
-        storeP(0, pb0);
-        loadP(cp, pb0);
-        adv(cp);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            readP(cp, sid);
+            loadP(sid, cp.ch, q == 1 ? pb1 : pb0);
+            adv(cp);
+            if (q == 1) storeP(0, pb0);
+        }
     }
     if (a.accumulate) {  // continue from the earlier groups' sums (exact power-of-two rescale)
         const float rescale = ldexpf(1.0f, xs + a.escale);
@@ -602,14 +615,20 @@ __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M
     // step s (buf = s & 1): Bs[buf] holds P(s), pb holds P(s + 1)
     auto run = [&](int buf, f32x4 (&pb)[BR], V8 (&ea)[RBW][NSPLIT], V8 (&en)[RBW][NSPLIT])
         __attribute__((always_inline)) {
-        __syncthreads();  // Bs[buf] is complete and every wave is done with Bs[buf ^ 1]
-        const unsigned live = cc.step <= last ? (unsigned)s_mask[cc.di] : 0u;
-        loadE(ce, en);  // the next step's E, in flight during this step
+        // this step's table entries, read before the barrier (the tables do
+        // not change) so that they arrive while the waves synchronise
+        const unsigned live = readM(cm);
+        adv(cm);
+        const int eo = readE(ce);
         adv(ce);
-        storeP(buf ^ 1, pb);
-        loadP(cp, pb);  // three steps ahead
-        adv(cp);
+        __syncthreads();  // Bs[buf] is complete and every wave is done with Bs[buf ^ 1]
         const unsigned lv = __builtin_amdgcn_readfirstlane(live);
+        loadE(eo, en);  // the next step's E, in flight during this step
+        int sid[BR];
+        readP(cp, sid);
+        storeP(buf ^ 1, pb);
+        loadP(sid, cp.ch, pb);  // three steps ahead
+        adv(cp);
         const float *Bb = Bs[buf];
         u32x4 rq[2][2];
         readB(Bb, 0, rq[0]);
@@ -647,7 +666,6 @@ __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M
                 acc[i][cb] = mfma16(ea[i][0], bf[0], acc[i][cb]);
             }
         }
-        adv(cc);
     };
     // an even number of steps: the padding step (zero proxies, empty mask) adds nothing
     for (int step = 0; step < nsteps; step += 2) {
