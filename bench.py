@@ -29,8 +29,8 @@ DTYPE = {"fp64": "f64", "mixed": "bf16x3->f32 (M2L), f64 elsewhere",
 
 
 # bytes per M2L launch, averaged over the 6 launches of one f16x2 matvec of the
-# default workload (8439 GB read + 28 GB written per matvec)
-M2L_TRAFFIC_F16X2_C3 = 1.411e12
+# default workload (7101 GB read + 28 GB written per matvec)
+M2L_TRAFFIC_F16X2_C3 = 1.188e12
 TRAFFIC_SOURCE = "profiles/m2l_f16x2_c3_gaussian_traffic.txt"
 
 
@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--err-targets", type=int, default=10000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the built-in M2L tiles instead of running the plan's autotuner first")
     ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
                     help="first run one device copy of this many GiB (a known-size kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of profiled runs)")
@@ -151,11 +153,13 @@ def main():
     out = torch.zeros_like(w)
     # M2L tile widths picked by the plan's autotuner (untimed, like the tree)
     t0 = time.perf_counter()
-    widths = {"m2l_tile": plan.tune_m2l(w, prec)}
-    if prec != capi.PREC_FP64 and args.fp64_steps > 0:
-        widths["m2l_tile_fp64"] = plan.tune_m2l(w, capi.PREC_FP64)
-    widths["tune_s"] = round(time.perf_counter() - t0, 3)
-    log(f"m2l tiles {widths}")
+    widths = None
+    if not args.no_tune:
+        widths = {"m2l_tile": plan.tune_m2l(w, prec)}
+        if prec != capi.PREC_FP64 and args.fp64_steps > 0:
+            widths["m2l_tile_fp64"] = plan.tune_m2l(w, capi.PREC_FP64)
+        widths["tune_s"] = round(time.perf_counter() - t0, 3)
+    log(f"m2l tiles {widths or 'built-in'}")
     plan.set_timing(True)
 
     def timed(p, warmup, steps, tag):
