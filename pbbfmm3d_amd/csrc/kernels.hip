@@ -583,17 +583,21 @@ __global__ __launch_bounds__(NWC * 128) void m2l_mfma_f64(M2LArgs a)
         storeA(buf ^ 1);  // last read in step - 1, before the barrier
         loadA(min(step + 2, nsteps - 1));
         loadB(min(step + 1, nsteps - 1), bnext, oknext);
+        // a wave whose two column blocks have no source box in this direction
+        // (zero proxies) skips the step's MFMAs; the sums are unchanged
+        if (__ballot(okcur != 0) != 0) {
 #pragma unroll
-        for (int ks = 0; ks < 4; ks++) {
-            const double *ap = &As[buf][(4 * fk + ks) * AS + fr];
+            for (int ks = 0; ks < 4; ks++) {
+                const double *ap = &As[buf][(4 * fk + ks) * AS + fr];
 #pragma unroll
-            for (int i = 0; i < RBW; i++) {
-                const int rb_ = wr * RBW + i;
-                if (rb_ >= RB) break;
-                const double av = ap[rb_ * 16];
+                for (int i = 0; i < RBW; i++) {
+                    const int rb_ = wr * RBW + i;
+                    if (rb_ >= RB) break;
+                    const double av = ap[rb_ * 16];
 #pragma unroll
-                for (int cb = 0; cb < 2; cb++)
-                    acc[i][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[cb][ks], acc[i][cb], 0, 0, 0);
+                    for (int cb = 0; cb < 2; cb++)
+                        acc[i][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[cb][ks], acc[i][cb], 0, 0, 0);
+                }
             }
         }
         __syncthreads();
