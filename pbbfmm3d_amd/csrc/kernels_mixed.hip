@@ -698,9 +698,13 @@ __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M
 //      [3 planes][R rows][Kp] BF16 (row-major, zero-padded to R = 16*RB rows
 //      and Kp = 32*nch columns);
 //   B: K x ncol, column-major (ld ldb), FP64 or FP32, split in registers.
-// A workgroup of 8 waves owns 256 columns (two 16-column blocks per wave)
-// and all RB row blocks; A goes through LDS one 32-deep k chunk at a time
-// (double-buffered), the next chunk's A and B loaded one step ahead.
+// A workgroup of 8 waves owns 128*NCBW columns (NCBW 16-column blocks per
+// wave; one for more than 12 row blocks, to stay within the registers) and
+// all RB row blocks.  A goes through LDS one 32-deep k chunk at a time
+// (double-buffered); B is gathered coalesced (32 consecutive k of two
+// columns per load instruction), converted to FP32 into LDS, and each wave
+// splits its fragments in registers.  The next chunks of A and B are loaded
+// one chunk ahead.
 __global__ void split_a3(const double *__restrict__ A, int M, int K, int R, int Kp, __bf16 *__restrict__ As)
 {
     const int64_t per = (int64_t)R * Kp, total = 3 * per;
@@ -725,65 +729,77 @@ void launch_split_a3(const double *A, int M, int K, int R, int Kp, void *As, hip
     check(hipGetLastError(), "split_a3");
 }
 
-template <typename TBt, typename TC, int RB>
+template <typename TBt, typename TC, int RB, int NCBW>
 __global__ __launch_bounds__(512) void gemm_split3(GemmSplitArgs g)
 {
-    constexpr int NT = 512, W = 256, R = 16 * RB, KS = 40;
+    constexpr int NT = 512, W = 128 * NCBW, R = 16 * RB, KS = 40, BST = 36;
     constexpr int NA = 3 * R * 4;           // 16-byte units of one A chunk
     constexpr int AR = (NA + NT - 1) / NT;
+    constexpr int BRW = W * 32 / NT;        // B elements per thread and chunk
     __shared__ __attribute__((aligned(16))) __bf16 As[2][3 * R * KS];
+    __shared__ __attribute__((aligned(16))) float Bs[W * BST];  // [column][32 k + pad], FP32
     const TBt *B = (const TBt *)g.B;
     TC *C = (TC *)g.C;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t n0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * W;
     const int nch = g.nch, fr = lane % 16, fq = lane / 16;
-    int64_t gcol[2];
-    const TBt *bcol[2];
-#pragma unroll
-    for (int cb = 0; cb < 2; cb++) {
-        gcol[cb] = n0 + wave * 32 + cb * 16 + fr;
-        bcol[cb] = B + (gcol[cb] < g.ncol ? gcol[cb] : 0) * g.ldb;
-    }
-    uint4 ra[AR];
-    auto loadA = [&](int ch) {
-        const uint4 *src = (const uint4 *)g.As;  // [3][R][Kp] bf16: 16-byte unit (p, row, q) at (p*R + row)*nch*4 + ch*4 + q
+    u32x4 ra[AR];
+    auto loadA = [&](int ch) __attribute__((always_inline)) {
+        const u32x4 *src = (const u32x4 *)g.As;  // [3][R][Kp] bf16: 16-byte unit (p, row, q) at (p*R + row)*nch*4 + ch*4 + q
 #pragma unroll
         for (int r = 0; r < AR; r++) {
             const int u = min(tid + r * NT, NA - 1), pr = u / 4, q = u % 4;
             ra[r] = src[(int64_t)pr * nch * 4 + ch * 4 + q];
         }
     };
-    auto storeA = [&](int buf) {
+    auto storeA = [&](int buf) __attribute__((always_inline)) {
 #pragma unroll
         for (int r = 0; r < AR; r++) {
             const int u = tid + r * NT;
-            if (AR * NT == NA || u < NA) *(uint4 *)&As[buf][(u / 4) * KS + 8 * (u % 4)] = ra[r];
+            if (AR * NT == NA || u < NA) *(u32x4 *)&As[buf][(u / 4) * KS + 8 * (u % 4)] = ra[r];
         }
     };
-    float bv[2][2][8];  // [set][cb][8 k values]
-    auto loadB = [&](int ch, float (&b)[2][8]) {
-        const int k0 = ch * 32 + 8 * fq;
+    // B chunk (32 k x W columns), coalesced: element e = tid + r*NT is k = e % 32
+    // of tile column e / 32, so one load instruction covers two columns' 32
+    // consecutive k values.  Addresses are clamped, values outside zeroed.
+    TBt rb[BRW];
+    auto loadB = [&](int ch) __attribute__((always_inline)) {
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++)
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const float v = (float)bcol[cb][min(k0 + e, g.K - 1)];
-                b[cb][e] = k0 + e < g.K ? v : 0.0f;
-            }
+        for (int r = 0; r < BRW; r++) {
+            const int e = tid + r * NT, k = ch * 32 + (e & 31);
+            const int64_t col = n0 + (e >> 5);
+            rb[r] = B[min(col, g.ncol - 1) * g.ldb + min(k, g.K - 1)];
+        }
     };
-    f32x4 acc[RB][2];
+    auto storeB = [&](int ch) __attribute__((always_inline)) {
+#pragma unroll
+        for (int r = 0; r < BRW; r++) {
+            const int e = tid + r * NT, k = ch * 32 + (e & 31);
+            const bool ok = n0 + (e >> 5) < g.ncol && k < g.K;
+            Bs[(e >> 5) * BST + (e & 31)] = ok ? (float)rb[r] : 0.0f;
+        }
+    };
+    f32x4 acc[RB][NCBW];
 #pragma unroll
     for (int i = 0; i < RB; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0, 0, 0, 0};
-    auto run = [&](int ch, const float (&bcur)[2][8], float (&bnext)[2][8]) {
+        for (int j = 0; j < NCBW; j++) acc[i][j] = f32x4{0, 0, 0, 0};
+    loadA(0);
+    storeA(0);
+    loadA(min(1, nch - 1));
+    loadB(0);
+    storeB(0);
+    loadB(min(1, nch - 1));
+    __syncthreads();
+    for (int ch = 0; ch < nch; ch++) {
         const int buf = ch & 1;
-        bf16x8 bfr[2][3];
+        // this wave's B fragments (NCBW 16-column blocks), split into 3 BF16 parts
+        bf16x8 bfr[NCBW][3];
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++) {
-            float x[8];
-#pragma unroll
-            for (int e = 0; e < 8; e++) x[e] = bcur[cb][e];
+        for (int cb = 0; cb < NCBW; cb++) {
+            const float *bp = &Bs[((wave * NCBW + cb) * 16 + fr) * BST + 8 * fq];
+            const f32x4 v0 = *(const f32x4 *)bp, v1 = *(const f32x4 *)(bp + 4);
+            float x[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
 #pragma unroll
             for (int p = 0; p < 3; p++)
 #pragma unroll
@@ -793,45 +809,39 @@ __global__ __launch_bounds__(512) void gemm_split3(GemmSplitArgs g)
                     x[e] -= (float)h;
                 }
         }
-        storeA(buf ^ 1);  // last read in step ch - 1, before the barrier
+        storeA(buf ^ 1);  // chunk ch + 1 (buffer last read in chunk ch - 1)
         loadA(min(ch + 2, nch - 1));
-        loadB(min(ch + 1, nch - 1), bnext);
+        __syncthreads();  // every wave holds its B fragments of chunk ch
+        storeB(min(ch + 1, nch - 1));
+        loadB(min(ch + 2, nch - 1));
 #pragma unroll
-        for (int rb = 0; rb < RB; rb++) {
+        for (int rb_ = 0; rb_ < RB; rb_++) {
             bf16x8 afr[3];
 #pragma unroll
-            for (int p = 0; p < 3; p++) afr[p] = *(const bf16x8 *)&As[buf][((p * R) + rb * 16 + fr) * KS + 8 * fq];
+            for (int p = 0; p < 3; p++) afr[p] = *(const bf16x8 *)&As[buf][((p * R) + rb_ * 16 + fr) * KS + 8 * fq];
 #pragma unroll
-            for (int cb = 0; cb < 2; cb++) {  // smallest terms first
-                acc[rb][cb] = mfma16(afr[2], bfr[cb][0], acc[rb][cb]);
-                acc[rb][cb] = mfma16(afr[1], bfr[cb][1], acc[rb][cb]);
-                acc[rb][cb] = mfma16(afr[0], bfr[cb][2], acc[rb][cb]);
-                acc[rb][cb] = mfma16(afr[1], bfr[cb][0], acc[rb][cb]);
-                acc[rb][cb] = mfma16(afr[0], bfr[cb][1], acc[rb][cb]);
-                acc[rb][cb] = mfma16(afr[0], bfr[cb][0], acc[rb][cb]);
+            for (int cb = 0; cb < NCBW; cb++) {  // smallest terms first
+                acc[rb_][cb] = mfma16(afr[2], bfr[cb][0], acc[rb_][cb]);
+                acc[rb_][cb] = mfma16(afr[1], bfr[cb][1], acc[rb_][cb]);
+                acc[rb_][cb] = mfma16(afr[0], bfr[cb][2], acc[rb_][cb]);
+                acc[rb_][cb] = mfma16(afr[1], bfr[cb][0], acc[rb_][cb]);
+                acc[rb_][cb] = mfma16(afr[0], bfr[cb][1], acc[rb_][cb]);
+                acc[rb_][cb] = mfma16(afr[0], bfr[cb][0], acc[rb_][cb]);
             }
         }
-        __syncthreads();
-    };
-    loadA(0);
-    storeA(0);
-    loadA(min(1, nch - 1));
-    loadB(0, bv[0]);
-    __syncthreads();
-    for (int ch = 0; ch < nch; ch += 2) {
-        run(ch, bv[0], bv[1]);
-        if (ch + 1 < nch) run(ch + 1, bv[1], bv[0]);
+        __syncthreads();  // B of chunk ch + 1 and A buffer buf ^ 1 complete; As[buf] free
     }
 #pragma unroll
-    for (int cb = 0; cb < 2; cb++) {
-        if (gcol[cb] >= g.ncol) continue;
-        TC *dst = C + gcol[cb] * g.ldc;
+    for (int cb = 0; cb < NCBW; cb++) {
+        const int64_t gc = n0 + (wave * NCBW + cb) * 16 + fr;
+        if (gc >= g.ncol) continue;
+        TC *dst = C + gc * g.ldc;
 #pragma unroll
-        for (int rb = 0; rb < RB; rb++)
+        for (int rb_ = 0; rb_ < RB; rb_++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int row = rb * 16 + 4 * fq + v;
-                if (row < g.M) dst[row] = (TC)(g.alpha * (double)acc[rb][cb][v]);
+                const int row = rb_ * 16 + 4 * fq + v;
+                if (row < g.M) dst[row] = (TC)(g.alpha * (double)acc[rb_][cb][v]);
                 else if (row < g.ldc) dst[row] = (TC)0;
             }
         for (int row = R; row < g.ldc; row++)  // padding rows beyond the MFMA tile
@@ -843,12 +853,12 @@ void launch_gemm_split3(const GemmSplitArgs &g, hipStream_t st)
 {
     if (g.ncol <= 0) return;
     if (g.ldb < g.K || g.ldc < g.M) fail(PBB_ERR_INTERNAL, "gemm: leading dimension too small");
-    const int rb = (g.M + 15) / 16;
-    dim3 grid((unsigned)((g.ncol + 255) / 256));
+    const int rb = (g.M + 15) / 16, w = rb > 12 ? 128 : 256;
+    dim3 grid((unsigned)((g.ncol + w - 1) / w));
 #define PBB_G3(RB)                                                                                   \
     case RB:                                                                                         \
-        if (!g.b_f32 && g.c_f32) gemm_split3<double, float, RB><<<grid, 512, 0, st>>>(g);           \
-        else if (g.b_f32 && !g.c_f32) gemm_split3<float, double, RB><<<grid, 512, 0, st>>>(g);      \
+        if (!g.b_f32 && g.c_f32) gemm_split3<double, float, RB, RB <= 12 ? 2 : 1><<<grid, 512, 0, st>>>(g); \
+        else if (g.b_f32 && !g.c_f32) gemm_split3<float, double, RB, RB <= 12 ? 2 : 1><<<grid, 512, 0, st>>>(g); \
         else fail(PBB_ERR_INTERNAL, "gemm_split3: unsupported operand types");                       \
         break;
     switch (rb) {
