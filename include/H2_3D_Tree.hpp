@@ -13,6 +13,12 @@
  * pbb_kernel_id in pbbfmm3d_amd.h).  A class that sets none can still build
  * its operators, but H2_3D_Compute rejects it.  Trees are reusable: any
  * number of H2_3D_Compute calls may use one built tree.
+ *
+ * Operator cache: with cache_dir set (empty by default: no files),
+ * buildFMMTree() works as the reference's PrecomputeM2L
+ * (H2_3D_Tree.cpp:74-99): it reads {cache_dir}/{kernelType}Cheb{K,U,V}{n}.bin
+ * when they fit (PrecomputeAvailable, :102-131), and otherwise precomputes
+ * and writes them, in the reference's file format.
  */
 #ifndef PBBFMM3D_AMD_H2_3D_TREE_HPP
 #define PBBFMM3D_AMD_H2_3D_TREE_HPP
@@ -57,9 +63,26 @@ class H2_3D_Tree {
             pbb_ops_destroy(ops_);
             ops_ = nullptr;
         }
-        pbb_status st = pbb_ops_create(&d, &H2_3D_Tree::eval_trampoline, this, L, tree_level,
-                                       interpolation_order, epsilon, use_chebyshev, &ops_);
-        if (st != PBB_OK) throw std::runtime_error(std::string("buildFMMTree: ") + pbb_last_error());
+        pbb_status st = PBB_ERR_NOT_FOUND;
+        std::string fk, fu, fv;
+        cache_hit = false;
+        if (!cache_dir.empty() && use_chebyshev == 1) {
+            const std::string base = cache_dir + "/" + kernelType + "Cheb";
+            const std::string n = std::to_string(interpolation_order) + ".bin";
+            fk = base + "K" + n;
+            fu = base + "U" + n;
+            fv = base + "V" + n;
+            st = pbb_ops_read_cache(&d, L, tree_level, interpolation_order, epsilon, fk.c_str(), fu.c_str(),
+                                    fv.c_str(), &ops_);
+        }
+        cache_hit = st == PBB_OK;
+        if (st != PBB_OK) {  // no usable cache: precompute (and write it)
+            st = pbb_ops_create(&d, &H2_3D_Tree::eval_trampoline, this, L, tree_level, interpolation_order,
+                                epsilon, use_chebyshev, &ops_);
+            if (st != PBB_OK) throw std::runtime_error(std::string("buildFMMTree: ") + pbb_last_error());
+            if (!fk.empty() && pbb_ops_write_cache(ops_, fk.c_str(), fu.c_str(), fv.c_str()) != PBB_OK)
+                throw std::runtime_error(std::string("buildFMMTree: ") + pbb_last_error());
+        }
         pbb_ops_info info{};
         pbb_ops_query(ops_, &info);
         cutoff.f = info.rank_f;
@@ -98,6 +121,8 @@ class H2_3D_Tree {
     int use_chebyshev;
     doft cutoff;
     std::string kernelType;
+    std::string cache_dir;  /* operator cache directory; empty: no cache files */
+    bool cache_hit = false; /* the last buildFMMTree() read its operators from cache_dir */
     bool computed = false;
     int device_kernel = PBB_KERNEL_NONE;
     double device_param[4];

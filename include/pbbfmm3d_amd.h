@@ -37,7 +37,8 @@ typedef enum pbb_status {
     PBB_ERR_HIP = 3,          /* a HIP runtime call failed               */
     PBB_ERR_OUT_OF_MEMORY = 4,
     PBB_ERR_NO_DEVICE = 5,    /* no usable gfx950 device                 */
-    PBB_ERR_INTERNAL = 6
+    PBB_ERR_INTERNAL = 6,
+    PBB_ERR_NOT_FOUND = 7     /* a file to read does not exist           */
 } pbb_status;
 
 /* Device kernel functors (the P2P near field runs these on the GPU; the
@@ -152,6 +153,26 @@ PBB_API pbb_status pbb_ops_create_from_arrays(
     const pbb_kernel_desc *kernel, double L, int32_t tree_level, int32_t order,
     double eps, int32_t rank_f, int32_t rank_s, int32_t nsets, const double *U,
     const double *VT, const double *E, pbb_ops **out);
+
+/* Operator cache files in the reference's format (PrecomputeM2L /
+ * StartPrecompute / ComputeKernelCheb / FMMReadMatrices,
+ * H2_3D_Tree.cpp:74-99, :138-162, :424-441, :496-515, :528-558, :186-239):
+ *   K file ({kernelType}ChebK{n}.bin): double L, int32 tree_level, then
+ *     per operator set 316 x (rank_f x rank_s) column-major E blocks;
+ *   U file ({kernelType}ChebU{n}.bin): int32 rank_f, then per set U;
+ *   V file ({kernelType}ChebV{n}.bin): int32 rank_s, then per set VT.
+ * One operator set for homogeneous kernels, else one per level 2..L.
+ * pbb_ops_read_cache returns PBB_ERR_NOT_FOUND when a file is missing and
+ * PBB_ERR_INVALID_ARG when the files do not fit the order, or (per-level
+ * operators) were written for another L or tree_level, as
+ * PrecomputeAvailable (:102-131) decides.                                 */
+PBB_API pbb_status pbb_ops_write_cache(const pbb_ops *ops, const char *k_path,
+                                       const char *u_path, const char *v_path);
+PBB_API pbb_status pbb_ops_read_cache(const pbb_kernel_desc *kernel, double L,
+                                      int32_t tree_level, int32_t order,
+                                      double eps, const char *k_path,
+                                      const char *u_path, const char *v_path,
+                                      pbb_ops **out);
 
 PBB_API pbb_status pbb_ops_query(const pbb_ops *ops, pbb_ops_info *info);
 

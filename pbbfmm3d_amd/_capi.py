@@ -21,6 +21,7 @@ PBB_ERR_HIP = 3
 PBB_ERR_OUT_OF_MEMORY = 4
 PBB_ERR_NO_DEVICE = 5
 PBB_ERR_INTERNAL = 6
+PBB_ERR_NOT_FOUND = 7
 
 # pbb_kernel_id
 KERNEL_NONE = 0
@@ -93,6 +94,9 @@ _SIGNATURES = {
     "pbb_ops_create_from_arrays": (C.c_int, [C.POINTER(KernelDesc), C.c_double, C.c_int32,
                                              C.c_int32, C.c_double, C.c_int32, C.c_int32,
                                              C.c_int32, _dp, _dp, _dp, C.POINTER(_vp)]),
+    "pbb_ops_write_cache": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.c_char_p]),
+    "pbb_ops_read_cache": (C.c_int, [C.POINTER(KernelDesc), C.c_double, C.c_int32, C.c_int32, C.c_double,
+                                     C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(_vp)]),
     "pbb_ops_query": (C.c_int, [_vp, C.POINTER(OpsInfo)]),
     "pbb_ops_export": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
     "pbb_ops_destroy": (None, [_vp]),

@@ -9,6 +9,7 @@ column-major ``w[c*N + i]``, outputs accumulated, as in compute.hpp:747.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 from typing import Callable, Optional, Sequence
 
@@ -152,6 +153,36 @@ class Operators:
         self._h = h
         self._info()
         return self
+
+    @staticmethod
+    def cache_paths(directory: str, kernel_type: str, order: int):
+        """The reference's cache file names, {kernelType}Cheb{K,U,V}{n}.bin
+        (H2_3D_Tree.cpp:81-83)."""
+        return tuple(os.path.join(directory, f"{kernel_type}Cheb{m}{int(order)}.bin") for m in "KUV")
+
+    @classmethod
+    def from_cache(cls, kernel: Kernel, L, tree_level, order, eps, directory: str,
+                   kernel_type: str) -> "Operators":
+        """Operators read from cache files in the reference's format
+        (FMMReadMatrices, H2_3D_Tree.cpp:186-239).  Raises PBBError with
+        status PBB_ERR_NOT_FOUND when a file is missing."""
+        self = cls.__new__(cls)
+        self.kernel = kernel
+        self._cb = None
+        h = C.c_void_p()
+        d = kernel.desc()
+        k, u, v = (p.encode() for p in cls.cache_paths(directory, kernel_type, order))
+        capi.check(capi.lib().pbb_ops_read_cache(C.byref(d), float(L), int(tree_level), int(order),
+                                                 float(eps), k, u, v, C.byref(h)))
+        self._h = h
+        self._info()
+        return self
+
+    def write_cache(self, directory: str, kernel_type: str) -> None:
+        """Writes the operators as the reference's cache files
+        (StartPrecompute / ComputeKernelCheb, H2_3D_Tree.cpp:138-162, :424-558)."""
+        k, u, v = (p.encode() for p in self.cache_paths(directory, kernel_type, self.order))
+        capi.check(capi.lib().pbb_ops_write_cache(self._h, k, u, v))
 
     def _info(self):
         info = capi.OpsInfo()

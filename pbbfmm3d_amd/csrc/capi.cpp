@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -410,6 +411,37 @@ PBB_API pbb_status pbb_ops_create(const pbb_kernel_desc *kernel, pbb_host_kernel
     });
 }
 
+// operators of a kernel and tree shape from given U, VT, E (any source);
+// the weights come from the order, the operator sets from the homogeneity
+static std::unique_ptr<pbb_ops> ops_from_arrays(const pbb_kernel_desc *kernel, double L, int tree_level, int order,
+                                                double eps, int rank_f, int rank_s, int nsets, const double *U,
+                                                const double *VT, const double *E)
+{
+    validate_params(kernel, L, tree_level, order, eps, 1);
+    if (!U || !VT || !E) fail(PBB_ERR_INVALID_ARG, "operator arrays must not be NULL");
+    auto ops = std::make_unique<pbb_ops>();
+    HostOps &h = ops->h;
+    h.kernel = *kernel;
+    h.L = L;
+    h.tree_level = tree_level;
+    h.n = order;
+    h.n3 = order * order * order;
+    h.eps = eps;
+    h.w.compute(order);
+    h.homogen_flipped = -kernel->homogen;
+    h.homog = std::fabs(h.homogen_flipped) > 0.1;
+    int want = h.homog ? 1 : std::max(1, tree_level - 1);
+    if (nsets != want) fail(PBB_ERR_INVALID_ARG, "nsets does not match the kernel's homogeneity and tree_level");
+    if (rank_f < 1 || rank_f > h.n3 || rank_s < 1 || rank_s > h.n3) fail(PBB_ERR_INVALID_ARG, "bad ranks");
+    h.rf = rank_f;
+    h.rs = rank_s;
+    h.nsets = nsets;
+    h.U.assign(U, U + (size_t)nsets * h.n3 * rank_f);
+    h.VT.assign(VT, VT + (size_t)nsets * rank_s * h.n3);
+    h.E.assign(E, E + (size_t)nsets * 316 * rank_f * rank_s);
+    return ops;
+}
+
 PBB_API pbb_status pbb_ops_create_from_arrays(const pbb_kernel_desc *kernel, double L, int32_t tree_level,
                                               int32_t order, double eps, int32_t rank_f, int32_t rank_s,
                                               int32_t nsets, const double *U, const double *VT, const double *E,
@@ -418,29 +450,99 @@ PBB_API pbb_status pbb_ops_create_from_arrays(const pbb_kernel_desc *kernel, dou
     return guarded([&] {
         if (!out) fail(PBB_ERR_INVALID_ARG, "out is NULL");
         *out = nullptr;
+        *out = ops_from_arrays(kernel, L, tree_level, order, eps, rank_f, rank_s, nsets, U, VT, E).release();
+    });
+}
+
+// ---- operator cache files in the reference's format ----------------------
+//   K file: double box length L, int32 tree level (StartPrecompute,
+//           H2_3D_Tree.cpp:141-143), then per operator set the 316 E blocks,
+//           rank_f x rank_s column-major each, in Ktable order (:528-558)
+//   U file: int32 rank_f (:424-427), then per set U, n3 x rank_f
+//           column-major (:440-441)
+//   V file: int32 rank_s (:496-498), then per set VT, rank_s x n3
+//           column-major (:510-515)
+// Operator sets: one for homogeneous kernels (unit box), else levels 2..L
+// (:146-160); read back as FMMReadMatrices does (:186-239).
+namespace {
+struct File {
+    FILE *f = nullptr;
+    File(const char *path, const char *mode) : f(path ? std::fopen(path, mode) : nullptr) {}
+    ~File()
+    {
+        if (f) std::fclose(f);
+    }
+};
+long file_size(FILE *f)
+{
+    std::fseek(f, 0, SEEK_END);
+    long n = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    return n;
+}
+void write_all(FILE *f, const void *p, size_t size, size_t n, const char *what)
+{
+    if (std::fwrite(p, size, n, f) != n) fail(PBB_ERR_INVALID_ARG, std::string("cannot write the ") + what);
+}
+void read_all(FILE *f, void *p, size_t size, size_t n, const char *what)
+{
+    if (std::fread(p, size, n, f) != n) fail(PBB_ERR_INVALID_ARG, std::string("cannot read the ") + what);
+}
+}  // namespace
+
+PBB_API pbb_status pbb_ops_write_cache(const pbb_ops *ops, const char *k_path, const char *u_path, const char *v_path)
+{
+    return guarded([&] {
+        if (!ops || !k_path || !u_path || !v_path) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        const HostOps &h = ops->h;
+        File fk(k_path, "wb"), fu(u_path, "wb"), fv(v_path, "wb");
+        if (!fk.f || !fu.f || !fv.f) fail(PBB_ERR_INVALID_ARG, "cannot open the cache files for writing");
+        const double L = h.L;
+        const int32_t lvl = h.tree_level, rf = h.rf, rs = h.rs;
+        write_all(fk.f, &L, sizeof(double), 1, "K file");
+        write_all(fk.f, &lvl, sizeof(int32_t), 1, "K file");
+        write_all(fk.f, h.E.data(), sizeof(double), h.E.size(), "K file");
+        write_all(fu.f, &rf, sizeof(int32_t), 1, "U file");
+        write_all(fu.f, h.U.data(), sizeof(double), h.U.size(), "U file");
+        write_all(fv.f, &rs, sizeof(int32_t), 1, "V file");
+        write_all(fv.f, h.VT.data(), sizeof(double), h.VT.size(), "V file");
+    });
+}
+
+PBB_API pbb_status pbb_ops_read_cache(const pbb_kernel_desc *kernel, double L, int32_t tree_level, int32_t order,
+                                      double eps, const char *k_path, const char *u_path, const char *v_path,
+                                      pbb_ops **out)
+{
+    return guarded([&] {
+        if (!out) fail(PBB_ERR_INVALID_ARG, "out is NULL");
+        *out = nullptr;
         validate_params(kernel, L, tree_level, order, eps, 1);
-        if (!U || !VT || !E) fail(PBB_ERR_INVALID_ARG, "operator arrays must not be NULL");
-        auto ops = std::make_unique<pbb_ops>();
-        HostOps &h = ops->h;
-        h.kernel = *kernel;
-        h.L = L;
-        h.tree_level = tree_level;
-        h.n = order;
-        h.n3 = order * order * order;
-        h.eps = eps;
-        h.w.compute(order);
-        h.homogen_flipped = -kernel->homogen;
-        h.homog = std::fabs(h.homogen_flipped) > 0.1;
-        int want = h.homog ? 1 : std::max(1, tree_level - 1);
-        if (nsets != want) fail(PBB_ERR_INVALID_ARG, "nsets does not match the kernel's homogeneity and tree_level");
-        if (rank_f < 1 || rank_f > h.n3 || rank_s < 1 || rank_s > h.n3) fail(PBB_ERR_INVALID_ARG, "bad ranks");
-        h.rf = rank_f;
-        h.rs = rank_s;
-        h.nsets = nsets;
-        h.U.assign(U, U + (size_t)nsets * h.n3 * rank_f);
-        h.VT.assign(VT, VT + (size_t)nsets * rank_s * h.n3);
-        h.E.assign(E, E + (size_t)nsets * 316 * rank_f * rank_s);
-        *out = ops.release();
+        File fk(k_path, "rb"), fu(u_path, "rb"), fv(v_path, "rb");
+        if (!fk.f || !fu.f || !fv.f) fail(PBB_ERR_NOT_FOUND, "cache files not found");
+        const long nk = file_size(fk.f), nu = file_size(fu.f), nv = file_size(fv.f);
+        double len = 0;
+        int32_t lvl = 0, rf = 0, rs = 0;
+        read_all(fk.f, &len, sizeof(double), 1, "K file header");
+        read_all(fk.f, &lvl, sizeof(int32_t), 1, "K file header");
+        read_all(fu.f, &rf, sizeof(int32_t), 1, "U file header");
+        read_all(fv.f, &rs, sizeof(int32_t), 1, "V file header");
+        const int n3 = order * order * order;
+        if (rf < 1 || rf > n3 || rs < 1 || rs > n3) fail(PBB_ERR_INVALID_ARG, "cache ranks do not fit the order");
+        const bool homog = std::fabs(kernel->homogen) > 0.1;
+        // PrecomputeAvailable (H2_3D_Tree.cpp:114-122): per-level operators
+        // belong to one root length and depth
+        if (!homog && (lvl != tree_level || std::fabs(len - L) >= 1e-6))
+            fail(PBB_ERR_INVALID_ARG, "cache was written for another root length or tree level");
+        const int nsets = homog ? 1 : std::max(1, tree_level - 1);
+        const size_t su = (size_t)nsets * n3 * rf, sv = (size_t)nsets * rs * n3, se = (size_t)nsets * 316 * rf * rs;
+        if ((size_t)nu != 4 + 8 * su || (size_t)nv != 4 + 8 * sv || (size_t)nk != 12 + 8 * se)
+            fail(PBB_ERR_INVALID_ARG, "cache file sizes do not match the order, ranks and operator sets");
+        std::vector<double> U(su), VT(sv), E(se);
+        read_all(fu.f, U.data(), sizeof(double), su, "U file");
+        read_all(fv.f, VT.data(), sizeof(double), sv, "V file");
+        read_all(fk.f, E.data(), sizeof(double), se, "K file");
+        *out = ops_from_arrays(kernel, L, tree_level, order, eps, rf, rs, nsets, U.data(), VT.data(), E.data())
+                   .release();
     });
 }
 

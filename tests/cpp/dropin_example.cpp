@@ -3,7 +3,8 @@
 // examples/get_input_through_routine_{standard_kernel,mykernel}.cpp flow:
 // build the tree, run H2_3D_Compute, compare with DirectCalc3D.
 // Flags as the reference's examples: -n order -N points -L length -m nCols
-// -level tree_level -eps eps -cheb use_chebyshev.
+// -level tree_level -eps eps -cheb use_chebyshev; -cache DIR keeps the
+// operators in DIR as the reference's cache files.
 // Exit: 0 ok, 1 accuracy failure, 3 no GPU (after a successful precompute).
 #include <cstdio>
 #include <cstring>
@@ -34,10 +35,12 @@ class myKernel : public H2_3D_Tree {
 
 template <typename T>
 static int run(const char *name, T &tree, std::vector<vector3> &pts, std::vector<double> &w, int nCols,
-               double tol)
+               double tol, const char *cache)
 {
+    if (cache) tree.cache_dir = cache;
     tree.buildFMMTree();
     std::printf("%s: ranks %d/%d\n", name, tree.cutoff.f, tree.cutoff.s);
+    if (cache) std::printf("%s: operator cache %s\n", name, tree.cache_hit ? "read" : "written");
     const int N = (int)pts.size();
     std::vector<double> out((size_t)N * nCols, 0.0);
     try {
@@ -60,6 +63,7 @@ int main(int argc, char **argv)
 {
     double L = 1.0, eps = 1e-7;
     int order = 5, N = 20000, nCols = 2, level = 4, cheb = 1;
+    const char *cache = nullptr;
     for (int i = 1; i + 1 < argc; i++) {
         if (!std::strcmp(argv[i], "-n")) order = std::atoi(argv[++i]);
         else if (!std::strcmp(argv[i], "-N")) N = std::atoi(argv[++i]);
@@ -68,6 +72,7 @@ int main(int argc, char **argv)
         else if (!std::strcmp(argv[i], "-level")) level = std::atoi(argv[++i]);
         else if (!std::strcmp(argv[i], "-eps")) eps = std::atof(argv[++i]);
         else if (!std::strcmp(argv[i], "-cheb")) cheb = std::atoi(argv[++i]);
+        else if (!std::strcmp(argv[i], "-cache")) cache = argv[++i];
     }
     std::vector<vector3> pts(N);
     std::vector<double> w((size_t)N * nCols);
@@ -75,8 +80,8 @@ int main(int argc, char **argv)
     for (auto &p : pts) p = vector3(frand(-0.5, 0.5) * L, frand(-0.5, 0.5) * L, frand(-0.5, 0.5) * L);
 
     kernel_Laplacian lap(L, level, order, eps, cheb);
-    int rc = run("kernel_Laplacian", lap, pts, w, nCols, 1e-5);
+    int rc = run("kernel_Laplacian", lap, pts, w, nCols, 1e-5, cache);
     if (rc) return rc;
     myKernel mk(L, level, order, eps, cheb);
-    return run("myKernel", mk, pts, w, nCols, 1e-6);
+    return run("myKernel", mk, pts, w, nCols, 1e-6, cache);
 }

@@ -162,3 +162,16 @@ def test_result_ordered_on_torch_stream(gpu):
     got = plan.matvec(wd).cpu().numpy().reshape(-1)
     ref = orc.matvec(oracle_ops_from(ops, pbb.gaussian(0.5)), xyz, w, 2)
     assert orc.rel_err(got, ref) <= PARITY
+
+
+def test_operators_from_reference_cache_files(gpu, tmp_path):
+    # operators read back from the reference-format cache give the same
+    # matvec bit for bit
+    xyz = clustered_points(6000, seed=30)
+    w = weights(6000, 2, seed=31)
+    ops = pbb.Operators(pbb.gaussian(0.3), 1.0, 4, 5, 1e-7)
+    ops.write_cache(str(tmp_path), "Gaussian")
+    back = pbb.Operators.from_cache(pbb.gaussian(0.3), 1.0, 4, 5, 1e-7, str(tmp_path), "Gaussian")
+    a = gpu_matvec(pbb.Plan(ops, xyz), w, 2)
+    b = gpu_matvec(pbb.Plan(back, xyz), w, 2)
+    assert np.array_equal(a, b)
