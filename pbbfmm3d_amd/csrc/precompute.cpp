@@ -9,6 +9,7 @@
 // Jacobi sweep on that triangle gives singular values and right singular
 // vectors to full FP64 accuracy (svd.cpp).  No LAPACK is needed.
 #include <algorithm>
+#include <cstdlib>
 #include <atomic>
 #include <cmath>
 #include <exception>
@@ -24,7 +25,13 @@ void fail(pbb_status st, const std::string &msg) { throw Error{st, msg}; }
 void parallel_for(int64_t n, const std::function<void(int64_t)> &body)
 {
     if (n <= 0) return;
+    // at most the hardware threads, and at most OMP_NUM_THREADS when it is set
+    // (a host's share of its cores for this process)
     int nt = (int)std::max(1u, std::thread::hardware_concurrency());
+    if (const char *e = std::getenv("OMP_NUM_THREADS")) {
+        const int cap = std::atoi(e);
+        if (cap > 0) nt = std::min(nt, cap);
+    }
     nt = (int)std::min<int64_t>(nt, n);
     if (nt == 1) {
         for (int64_t i = 0; i < n; i++) body(i);

@@ -31,6 +31,40 @@ static inline uint64_t morton(uint32_t ix, uint32_t iy, uint32_t iz)
     return (spread3(ix) << 2) | (spread3(iy) << 1) | spread3(iz);
 }
 
+// Stable LSD radix sort of v = key << 32 | index by its key bits (the low
+// 3*levels bits of the upper word), 11-bit digits, parallel over contiguous
+// blocks: equal keys keep their index order, as a sort of the full 64-bit
+// values would give.
+static void radix_sort_keys(std::vector<uint64_t> &v, int key_bits)
+{
+    const int64_t n = (int64_t)v.size();
+    constexpr int D = 11, R = 1 << D;
+    const int64_t nblk = std::min<int64_t>(64, std::max<int64_t>(1, n / 65536));
+    const int64_t per = (n + nblk - 1) / nblk;
+    std::vector<uint64_t> tmp(n);
+    std::vector<int64_t> cnt((size_t)nblk * R);
+    for (int shift = 32; shift < 32 + key_bits; shift += D) {
+        std::fill(cnt.begin(), cnt.end(), 0);
+        parallel_for(nblk, [&](int64_t b) {
+            int64_t *c = &cnt[(size_t)b * R];
+            for (int64_t i = b * per, e = std::min(n, i + per); i < e; i++) c[(v[i] >> shift) & (R - 1)]++;
+        });
+        int64_t run = 0;  // digit-major, block-minor offsets: stable
+        for (int d = 0; d < R; d++)
+            for (int64_t b = 0; b < nblk; b++) {
+                int64_t &c = cnt[(size_t)b * R + d];
+                const int64_t k = c;
+                c = run;
+                run += k;
+            }
+        parallel_for(nblk, [&](int64_t b) {
+            int64_t *c = &cnt[(size_t)b * R];
+            for (int64_t i = b * per, e = std::min(n, i + per); i < e; i++) tmp[c[(v[i] >> shift) & (R - 1)]++] = v[i];
+        });
+        v.swap(tmp);
+    }
+}
+
 void build_tree(HostTree &T, const double *xyz, int64_t N, double L, int levels)
 {
     if (levels < 1 || levels > 9) fail(PBB_ERR_INVALID_ARG, "tree_level must be in [1, 9]");
@@ -39,14 +73,26 @@ void build_tree(HostTree &T, const double *xyz, int64_t N, double L, int levels)
     T.Lv = levels;
     T.N = N;
     // bbox of the targets, centre, shift (compute.hpp:92-127)
+    const int64_t BLK = 65536, nblk = (N + BLK - 1) / BLK;
+    std::vector<double> bmn((size_t)nblk * 3, 1e32), bmx((size_t)nblk * 3, -1e32);
+    std::vector<char> bad(nblk, 0);
+    parallel_for(nblk, [&](int64_t b) {
+        for (int64_t i = b * BLK, e = std::min(N, i + BLK); i < e; i++)
+            for (int d = 0; d < 3; d++) {
+                double v = xyz[3 * i + d];
+                if (!std::isfinite(v)) bad[b] = 1;
+                bmn[3 * b + d] = std::min(bmn[3 * b + d], v);
+                bmx[3 * b + d] = std::max(bmx[3 * b + d], v);
+            }
+    });
     double mn[3] = {1e32, 1e32, 1e32}, mx[3] = {-1e32, -1e32, -1e32};
-    for (int64_t i = 0; i < N; i++)
+    for (int64_t b = 0; b < nblk; b++) {
+        if (bad[b]) fail(PBB_ERR_INVALID_ARG, "point coordinates must be finite");
         for (int d = 0; d < 3; d++) {
-            double v = xyz[3 * i + d];
-            if (!std::isfinite(v)) fail(PBB_ERR_INVALID_ARG, "point coordinates must be finite");
-            mn[d] = std::min(mn[d], v);
-            mx[d] = std::max(mx[d], v);
+            mn[d] = std::min(mn[d], bmn[3 * b + d]);
+            mx[d] = std::max(mx[d], bmx[3 * b + d]);
         }
+    }
     for (int d = 0; d < 3; d++) T.ctr[d] = 0.5 * (mn[d] + mx[d]);
 
     // per-axis node centres of every level, with the arithmetic of
@@ -69,13 +115,15 @@ void build_tree(HostTree &T, const double *xyz, int64_t N, double L, int levels)
     // leaf coordinates by the reference's comparison chain (strict '<')
     std::vector<uint64_t> keyidx(N);
     const double half = 0.5 * L;
-    int64_t outside = 0;
-    parallel_for((N + 65535) / 65536, [&](int64_t blk) {
-        int64_t i0 = blk * 65536, i1 = std::min(N, i0 + 65536);
+    std::vector<int64_t> bout(nblk, 0);
+    parallel_for(nblk, [&](int64_t blk) {
+        int64_t i0 = blk * BLK, i1 = std::min(N, i0 + BLK);
         for (int64_t i = i0; i < i1; i++) {
             uint32_t c[3];
+            bool out = false;
             for (int d = 0; d < 3; d++) {
                 double x = xyz[3 * i + d] - T.ctr[d];
+                out |= x < -half || x > half;
                 uint32_t ix = 0;
                 for (int l = 0; l < levels; l++) {
                     double cen = T.lv[l].ctab[ix];
@@ -83,25 +131,24 @@ void build_tree(HostTree &T, const double *xyz, int64_t N, double L, int levels)
                 }
                 c[d] = ix;
             }
+            bout[blk] += out ? 1 : 0;
             keyidx[i] = (morton(c[0], c[1], c[2]) << 32) | (uint64_t)i;
         }
     });
-    for (int64_t i = 0; i < N; i++)
-        for (int d = 0; d < 3; d++) {
-            double x = xyz[3 * i + d] - T.ctr[d];
-            if (x < -half || x > half) { outside++; break; }
-        }
-    T.outside = outside;
-    std::sort(keyidx.begin(), keyidx.end());  // key, then input index: stable
+    T.outside = 0;
+    for (int64_t o : bout) T.outside += o;
+    radix_sort_keys(keyidx, 3 * levels);  // key, then input index: stable
 
     T.xs.resize(N); T.ys.resize(N); T.zs.resize(N); T.perm.resize(N);
-    for (int64_t k = 0; k < N; k++) {
-        int64_t i = (int64_t)(keyidx[k] & 0xffffffffULL);
-        T.perm[k] = (int32_t)i;
-        T.xs[k] = xyz[3 * i] - T.ctr[0];
-        T.ys[k] = xyz[3 * i + 1] - T.ctr[1];
-        T.zs[k] = xyz[3 * i + 2] - T.ctr[2];
-    }
+    parallel_for(nblk, [&](int64_t blk) {
+        for (int64_t k = blk * BLK, e = std::min(N, k + BLK); k < e; k++) {
+            int64_t i = (int64_t)(keyidx[k] & 0xffffffffULL);
+            T.perm[k] = (int32_t)i;
+            T.xs[k] = xyz[3 * i] - T.ctr[0];
+            T.ys[k] = xyz[3 * i + 1] - T.ctr[1];
+            T.zs[k] = xyz[3 * i + 2] - T.ctr[2];
+        }
+    });
 
     // non-empty boxes per level (Morton order), leaf point ranges
     std::vector<uint64_t> keys;  // leaf keys of the non-empty leaves
@@ -181,16 +228,26 @@ void build_tree(HostTree &T, const double *xyz, int64_t N, double L, int levels)
         const int64_t S = 1LL << l;
         for (int64_t b = 0; b < V.nb; b++) {
             uint32_t c = (uint32_t)V.coord[b];
-            int ix = c & 1023, iy = (c >> 10) & 1023, iz = (c >> 20) & 1023;
-            int o = ((ix & 1) << 2) | ((iy & 1) << 1) | (iz & 1);
+            int o = (int)(((c & 1) << 2) | (((c >> 10) & 1) << 1) | ((c >> 20) & 1));
             V.order[fillp[o]++] = (int32_t)b;
-            const int(*dirs)[4] = cdirs[o];
-            for (int k = 0; k < 189; k++) {
-                int64_t sx = ix + dirs[k][0], sy = iy + dirs[k][1], sz = iz + dirs[k][2];
-                if (sx < 0 || sy < 0 || sz < 0 || sx >= S || sy >= S || sz >= S) continue;
-                if (V.map[((size_t)sx * S + sy) * S + sz] >= 0) T.m2l_pairs++;
-            }
         }
+        const int64_t nbb = (V.nb + 4095) / 4096;
+        std::vector<int64_t> part(nbb, 0);
+        parallel_for(nbb, [&](int64_t blk) {
+            int64_t q = 0;
+            for (int64_t b = blk * 4096, e = std::min(V.nb, b + 4096); b < e; b++) {
+                uint32_t c = (uint32_t)V.coord[b];
+                int ix = c & 1023, iy = (c >> 10) & 1023, iz = (c >> 20) & 1023;
+                const int(*dirs)[4] = cdirs[((ix & 1) << 2) | ((iy & 1) << 1) | (iz & 1)];
+                for (int k = 0; k < 189; k++) {
+                    int64_t sx = ix + dirs[k][0], sy = iy + dirs[k][1], sz = iz + dirs[k][2];
+                    if (sx < 0 || sy < 0 || sz < 0 || sx >= S || sy >= S || sz >= S) continue;
+                    if (V.map[((size_t)sx * S + sy) * S + sz] >= 0) q++;
+                }
+            }
+            part[blk] = q;
+        });
+        for (int64_t q : part) T.m2l_pairs += q;
     }
     // ordered near-field point pairs (27 neighbours incl. self)
     {
