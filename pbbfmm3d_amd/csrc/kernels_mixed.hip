@@ -623,18 +623,23 @@ __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M
         adv(ce);
         __syncthreads();  // Bs[buf] is complete and every wave is done with Bs[buf ^ 1]
         const unsigned lv = __builtin_amdgcn_readfirstlane(live);
-        loadE(eo, en);  // the next step's E, in flight during this step
-        int sid[BR];
-        readP(cp, sid);
-        storeP(buf ^ 1, pb);
-        loadP(sid, cp.ch, pb);  // three steps ahead
-        adv(cp);
         const float *Bb = Bs[buf];
         u32x4 rq[2][2];
         readB(Bb, 0, rq[0]);
+        int sid[BR];
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++) {
             if (cb + 1 < NCB) readB(Bb, cb + 1, rq[(cb + 1) & 1]);
+            // the loads and stores for later steps go between the first MFMAs
+            if (cb == 1) {
+                loadE(eo, en);  // the next step's E, in flight during this step
+                readP(cp, sid);
+            }
+            if (cb == NCB / 2) {
+                storeP(buf ^ 1, pb);
+                loadP(sid, cp.ch, pb);  // three steps ahead
+                adv(cp);
+            }
             if (!(lv & cbm[cb])) continue;
             V8 bf[NSPLIT];
             if constexpr (PRESPLIT) {
