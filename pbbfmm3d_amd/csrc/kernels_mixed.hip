@@ -412,7 +412,9 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
 // skipped: its proxies are zero, so the sums are unchanged bit for bit.
 // Step bookkeeping (direction, k chunk) runs in scalar registers; the
 // compacted per-direction tables (E offset, box mask, source boxes) are in
-// LDS, one read per address.
+// LDS, read before the step's barrier.  A step's loads and proxy stores are
+// issued between its first MFMA blocks, so that they overlap the matrix
+// cores instead of delaying the first MFMA after the barrier.
 template <typename T, int NSPLIT, int RBW, int NCB, int WCOL>
 __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M2LMixArgs a)
 {
@@ -621,20 +623,18 @@ __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M
         adv(cm);
         const int eo = readE(ce);
         adv(ce);
+        int sid[BR];
+        readP(cp, sid);
         __syncthreads();  // Bs[buf] is complete and every wave is done with Bs[buf ^ 1]
         const unsigned lv = __builtin_amdgcn_readfirstlane(live);
         const float *Bb = Bs[buf];
         u32x4 rq[2][2];
         readB(Bb, 0, rq[0]);
-        int sid[BR];
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++) {
             if (cb + 1 < NCB) readB(Bb, cb + 1, rq[(cb + 1) & 1]);
             // the loads and stores for later steps go between the first MFMAs
-            if (cb == 1) {
-                loadE(eo, en);  // the next step's E, in flight during this step
-                readP(cp, sid);
-            }
+            if (cb == 1) loadE(eo, en);  // the next step's E, in flight during this step
             if (cb == NCB / 2) {
                 storeP(buf ^ 1, pb);
                 loadP(sid, cp.ch, pb);  // three steps ahead
