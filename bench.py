@@ -29,8 +29,8 @@ DTYPE = {"fp64": "f64", "mixed": "bf16x3->f32 (M2L), f64 elsewhere",
 
 
 # bytes per M2L launch, averaged over the 6 launches of one f16x2 matvec of the
-# default workload (7110 GB read + 28 GB written per matvec)
-M2L_TRAFFIC_F16X2_C3 = 1.190e12
+# default workload (6758 GB read + 27 GB written per matvec)
+M2L_TRAFFIC_F16X2_C3 = 1.131e12
 TRAFFIC_SOURCE = "profiles/m2l_f16x2_c3_gaussian_traffic.txt"
 
 
