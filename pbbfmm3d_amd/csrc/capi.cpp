@@ -1149,6 +1149,8 @@ PBB_API void pbb_plan_destroy(pbb_plan *plan) { delete plan; }
 struct pbb_batch {
     int device = -1;
     hipStream_t stream = nullptr;
+    std::vector<hipStream_t> side;  // capture branches
+    std::vector<hipEvent_t> ev;     // fork and join points
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~pbb_batch()
@@ -1159,6 +1161,8 @@ struct pbb_batch {
         (void)hipSetDevice(device);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        for (hipStream_t s : side) (void)hipStreamDestroy(s);
         if (stream) (void)hipStreamDestroy(stream);
         if (prev >= 0) (void)hipSetDevice(prev);
     }
@@ -1205,6 +1209,38 @@ PBB_API pbb_status pbb_batch_create(pbb_plan *const *plans, int32_t nplans, cons
             }
         }
         check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        // independent entries run as concurrent graph branches: entries that
+        // share a plan (its workspace) or an output go to the same branch,
+        // in batch order; if that cannot be kept, everything runs in one
+        std::vector<int> branch(nplans, 0);
+        int nbr = 1;
+        {
+            const int maxbr = std::min(nplans, 4);
+            std::vector<std::pair<const void *, int>> seen;  // plan or output -> branch
+            auto find = [&](const void *k) {
+                for (auto &q : seen)
+                    if (q.first == k) return q.second;
+                return -1;
+            };
+            int next = 0;
+            bool ok = true;
+            for (int i = 0; i < nplans && ok; i++) {
+                const int bp = find(plans[i]), bo = find(out_dev[i]);
+                int br = bp >= 0 ? bp : bo;
+                if (bp >= 0 && bo >= 0 && bp != bo) ok = false;
+                if (br < 0) br = next++ % maxbr;
+                branch[i] = br;
+                if (bp < 0) seen.push_back({plans[i], br});
+                if (bo < 0) seen.push_back({out_dev[i], br});
+            }
+            if (!ok) std::fill(branch.begin(), branch.end(), 0);
+            else nbr = std::min(next, maxbr);
+            nbr = std::max(nbr, 1);
+        }
+        b->side.resize(nbr, nullptr);
+        b->ev.resize(nbr + 1, nullptr);
+        for (int k = 0; k < nbr; k++) check(hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking), "hipStreamCreate");
+        for (auto &e : b->ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
         std::vector<bool> timing(nplans);
         for (int i = 0; i < nplans; i++) {
             timing[i] = plans[i]->timing;
@@ -1212,9 +1248,16 @@ PBB_API pbb_status pbb_batch_create(pbb_plan *const *plans, int32_t nplans, cons
         }
         check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         try {
+            check(hipEventRecord(b->ev[nbr], b->stream), "fork");
+            for (int k = 0; k < nbr; k++) check(hipStreamWaitEvent(b->side[k], b->ev[nbr], 0), "fork");
             for (int i = 0; i < nplans; i++) {
-                if (overwrite) dev::launch_fill_zero(out_dev[i], plans[i]->N * (int64_t)ncols, b->stream);
-                matvec_run(plans[i], ops[i], w_dev[i], ncols, out_dev[i], b->stream, precision);
+                hipStream_t s = b->side[branch[i]];
+                if (overwrite) dev::launch_fill_zero(out_dev[i], plans[i]->N * (int64_t)ncols, s);
+                matvec_run(plans[i], ops[i], w_dev[i], ncols, out_dev[i], s, precision);
+            }
+            for (int k = 0; k < nbr; k++) {
+                check(hipEventRecord(b->ev[k], b->side[k]), "join");
+                check(hipStreamWaitEvent(b->stream, b->ev[k], 0), "join");
             }
         } catch (...) {
             hipGraph_t junk = nullptr;
