@@ -277,8 +277,10 @@ typedef struct pbb_batch pbb_batch;
  * one device) on weights w_dev[i] / outputs out_dev[i] (device FP64,
  * column-major N_i x ncols) into one HIP graph.  The buffers are bound at
  * capture: refill them in place between launches.  All workspaces are
- * allocated here, none during a launch.  No reference counterpart: the
- * reference runs one cloud per H2_3D_Compute.                            */
+ * allocated here, none during a launch.  The entries run as up to four
+ * concurrent graph branches; entries that share a plan or an output stay
+ * in one branch, in batch order.  No reference counterpart: the reference
+ * runs one cloud per H2_3D_Compute.                                      */
 PBB_API pbb_status pbb_batch_create(pbb_plan *const *plans, int32_t nplans,
                                     const double *const *w_dev, double *const *out_dev,
                                     int32_t ncols, int32_t precision, int32_t flags,
