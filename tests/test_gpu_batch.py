@@ -57,3 +57,39 @@ def test_batch_replays_match_eager(gpu, prec, transpose):
         xyz, level, order = clouds[0]
         ref = orc.matvec(oracle_ops_from(plans[0].ops, kernel), xyz, ws[0].cpu().numpy().reshape(-1), ncols)
         assert orc.rel_err(outs[0].cpu().numpy().reshape(-1), ref) <= 1e-10
+
+
+def test_batch_shared_plans_and_outputs(gpu):
+    # entries that share a plan (its workspace) or an accumulated output run
+    # in one graph branch, in batch order: a plan used twice gives both
+    # results, and two plans adding into one output give the sum
+    import torch
+    ncols = 4
+    kernel = pbb.exponential(0.5)
+    ops = pbb.Operators(kernel, 1.0, 4, 4, 1e-7)
+    pa = pbb.Plan(ops, uniform_points(6000, seed=80))
+    pb = pbb.Plan(ops, uniform_points(6000, seed=81))
+    pc = pbb.Plan(ops, clustered_points(5000, seed=82))
+    g = torch.Generator(device="cuda").manual_seed(83)
+    w = [torch.randn(ncols, n, dtype=torch.float64, device="cuda", generator=g) for n in (6000, 6000, 6000, 5000)]
+    o_a1, o_a2 = (torch.zeros(ncols, 6000, dtype=torch.float64, device="cuda") for _ in range(2))
+    o_sum = torch.zeros(ncols, 6000, dtype=torch.float64, device="cuda")
+    o_c = torch.zeros(ncols, 5000, dtype=torch.float64, device="cuda")
+    plans = [pa, pa, pa, pb, pc]
+    ws = [w[0], w[1], w[2], w[1], w[3]]
+    outs = [o_a1, o_a2, o_sum, o_sum, o_c]
+    batch = pbb.Batch(plans, ws, outs, ncols, overwrite=False)
+    e_a1, e_a2, e_a3 = pa.matvec(w[0]), pa.matvec(w[1]), pa.matvec(w[2])
+    e_b, e_c = pb.matvec(w[1]), pc.matvec(w[3])
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for o in (o_a1, o_a2, o_sum, o_c):
+            o.zero_()
+        torch.cuda.synchronize()
+        batch.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(o_a1, e_a1) and torch.equal(o_a2, e_a2) and torch.equal(o_c, e_c)
+        # the batch adds each result's parts into o_sum in turn: equal to the
+        # sum to rounding (a race between branches would lose whole updates)
+        err = float(torch.linalg.norm(o_sum - (e_a3 + e_b)) / torch.linalg.norm(e_a3 + e_b))
+        assert err <= 1e-13, f"replay {rep}: the shared output is not the sum ({err:.2e})"
