@@ -106,6 +106,9 @@ def main():
                          "HIP-graph batch (a separate line, not the headline metric)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the built-in M2L tiles instead of running the plan's autotuner first")
+    ap.add_argument("--m2l-split-acc", action="store_true",
+                    help="two-part modes: run the M2L with the split's cross products summed apart and "
+                         "VT / U in FP64 (64-column tiles; closer to the FP64 path, slower)")
     ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
                     help="first run one device copy of this many GiB (a known-size kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of profiled runs)")
@@ -166,6 +169,10 @@ def main():
         if prec != capi.PREC_FP64 and args.fp64_steps > 0:
             widths["m2l_tile_fp64"] = plan.tune_m2l(w, capi.PREC_FP64)
         widths["tune_s"] = round(time.perf_counter() - t0, 3)
+    if args.m2l_split_acc:  # a fixed tile with the cross products summed apart (two-part modes)
+        flags = capi.M2L_CHUNK_MAJOR | capi.M2L_WAVE_ROWS | capi.M2L_SPLIT_ACC
+        plan.set_m2l_tile(64, 16, flags, prec)
+        widths = dict(widths or {}, m2l_tile={"width": 64, "cols_per_box": 16, "flags": flags})
     log(f"m2l tiles {widths or 'built-in'}")
     plan.set_timing(True)
 

@@ -244,6 +244,11 @@ PBB_API pbb_status pbb_plan_set_timing(pbb_plan *plan, int32_t enable);
  * PBB_M2L_WAVE_ROWS (mixed modes, no row split) for the kernel whose
  * waves split the rows of r_f and load E_d into registers (ranks above 192,
  * or above 128 in PBB_PREC_MIXED, cap its width at 128),
+ * PBB_M2L_SPLIT_ACC (with PBB_M2L_WAVE_ROWS; PBB_PREC_MIXED_F16X2 and
+ * PBB_PREC_MIXED_BF16X2, ignored in PBB_PREC_MIXED) to sum the split's
+ * cross products apart from the leading products, so that the FP32 sums
+ * round a third as often, with the VT and U products in FP64: a smaller
+ * difference from the FP64 mode at a cost (tiles capped at 64 columns),
  * PBB_M2L_DIR_GROUPS(g) (mixed modes, g <= 64) to split the 316 M2L
  * operators into g groups of consecutive index, one launch per group on the
  * large levels, so that a launch's operators stay in L2.  width 0 restores
@@ -252,6 +257,7 @@ PBB_API pbb_status pbb_plan_set_timing(pbb_plan *plan, int32_t enable);
 #define PBB_M2L_CHUNK_MAJOR 1
 #define PBB_M2L_ROW_SPLIT 2
 #define PBB_M2L_WAVE_ROWS 4
+#define PBB_M2L_SPLIT_ACC 8
 #define PBB_M2L_DIR_GROUPS(g) ((g) << 8)
 PBB_API pbb_status pbb_plan_set_m2l_tile(pbb_plan *plan, int32_t precision, int32_t width,
                                          int32_t cols_per_box, int32_t flags);

@@ -46,6 +46,7 @@ PREC_MIXED_BF16X2 = 3  # BF16 x2 split M2L
 M2L_CHUNK_MAJOR = 1
 M2L_ROW_SPLIT = 2
 M2L_WAVE_ROWS = 4
+M2L_SPLIT_ACC = 8
 
 
 def M2L_DIR_GROUPS(g):

@@ -315,8 +315,8 @@ class Plan:
                      precision: int = capi.PREC_FP64):
         """M2L tile of the precision's class: width columns, cols_per_box
         weight columns per target box, flags (capi.M2L_CHUNK_MAJOR,
-        capi.M2L_ROW_SPLIT, capi.M2L_DIR_GROUPS(g)); width 0 restores the
-        built-in choice."""
+        capi.M2L_ROW_SPLIT, capi.M2L_WAVE_ROWS, capi.M2L_SPLIT_ACC,
+        capi.M2L_DIR_GROUPS(g)); width 0 restores the built-in choice."""
         capi.check(capi.lib().pbb_plan_set_m2l_tile(self._h, int(precision), int(width), int(cols_per_box),
                                                     int(flags)))
 

@@ -746,11 +746,14 @@ PBB_API pbb_status pbb_plan_set_m2l_tile(pbb_plan *plan, int32_t precision, int3
                                               : (width == 32 || width == 64 || width == 128));
         if (!wok) fail(PBB_ERR_INVALID_ARG, "M2L width must be 0, 64, 128 or 256 (mixed) / 32, 64 or 128 (FP64)");
         if (width && (cols_per_box < 1 || cols_per_box > 64)) fail(PBB_ERR_INVALID_ARG, "cols_per_box must be 1..64");
-        const int known =
-            PBB_M2L_CHUNK_MAJOR | (mixed ? PBB_M2L_ROW_SPLIT | PBB_M2L_WAVE_ROWS | PBB_M2L_DIR_GROUPS(0xff) : 0);
+        const int known = PBB_M2L_CHUNK_MAJOR | (mixed ? PBB_M2L_ROW_SPLIT | PBB_M2L_WAVE_ROWS | PBB_M2L_SPLIT_ACC |
+                                                             PBB_M2L_DIR_GROUPS(0xff)
+                                                       : 0);
         if (flags & ~known) fail(PBB_ERR_INVALID_ARG, "unknown M2L tile flags");
         if ((flags & PBB_M2L_WAVE_ROWS) && (flags & PBB_M2L_ROW_SPLIT))
             fail(PBB_ERR_INVALID_ARG, "PBB_M2L_WAVE_ROWS takes no row split");
+        if ((flags & PBB_M2L_SPLIT_ACC) && !(flags & PBB_M2L_WAVE_ROWS))
+            fail(PBB_ERR_INVALID_ARG, "PBB_M2L_SPLIT_ACC needs PBB_M2L_WAVE_ROWS");
         if (((flags >> 8) & 0xff) > 64) fail(PBB_ERR_INVALID_ARG, "at most 64 M2L direction groups");
         plan->m2l_tile[mixed ? 1 : 0] = {width, width ? cols_per_box : 0, flags};
     });
@@ -857,8 +860,12 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
     const SplitE *SE = mixed && Lv >= 2 ? &o->ensure_split(prec, st) : nullptr;
     // the mixed modes' VT / U products on the BF16 matrix cores (3 parts),
     // when both operators have at most 224 rows
+    // PBB_M2L_SPLIT_ACC (two-part modes) keeps them in FP64 (P and Pf FP32)
+    const pbb_plan::M2LTile &tmx = p->m2l_tile[1];
+    const bool split_acc = (prec == PBB_PREC_MIXED_F16X2 || prec == PBB_PREC_MIXED_BF16X2) && tmx.width &&
+                           (tmx.flags & PBB_M2L_WAVE_ROWS) && (tmx.flags & PBB_M2L_SPLIT_ACC);
     const pbb_ops::SplitA *SA =
-        mixed && Lv >= 2 && h.rs <= 224 && h.n3 <= 224 ? o->ensure_split3(st) : nullptr;
+        mixed && !split_acc && Lv >= 2 && h.rs <= 224 && h.n3 <= 224 ? o->ensure_split3(st) : nullptr;
     const int ldP = mixed ? p->kpad : h.rs;
     if (mixed && h.rs > p->kpad) fail(PBB_ERR_INTERNAL, "proxy padding too small");
     mark(0);
@@ -934,7 +941,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
     // per tile keep the share of empty (box, direction) slots low.
     // Mixed: the row-wave kernel, W = 128 (8 boxes x 16 columns), chunk-major
     // steps, two workgroups per CU.
-    const pbb_plan::M2LTile &t64 = p->m2l_tile[0], &tmx = p->m2l_tile[1];
+    const pbb_plan::M2LTile &t64 = p->m2l_tile[0];
     const int cpt = t64.width ? std::min(ncols, t64.cpt) : std::min(ncols, 64);
     const int width = t64.width ? t64.width : ncols >= 32 || ncols <= 16 ? 128 : 64;
     const int TB = std::max(1, std::min(16, width / cpt));
@@ -951,6 +958,8 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         a.row_split = tmx.width ? (tmx.flags & PBB_M2L_ROW_SPLIT) != 0 : 0;
         a.wave_rows = tmx.width ? (tmx.flags & PBB_M2L_WAVE_ROWS) != 0 : 1;
         if (a.wave_rows) a.TB = std::max(1, std::min(a.TB, dev::m2l_rows_max_width(prec, SE->R) / cptm));
+        a.split_acc = split_acc;
+        if (split_acc) a.TB = std::max(1, std::min(a.TB, 64 / cptm));
         a.side = 1 << l;
         a.kpad = ldP; a.nch = SE->nch; a.escale = SE->escale[h.ops_set(l)]; a.off = D.off; a.R = SE->R;
         a.tile_prefix[0] = 0;

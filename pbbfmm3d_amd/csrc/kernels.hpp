@@ -162,6 +162,7 @@ struct M2LMixArgs {
     int wave_rows;    // row-wave kernel: the waves split the rows, E straight to registers
     int dlo, dhi;     // only directions whose operator index (0..315) lies in [dlo, dhi)
     int accumulate;   // add to Pf (an earlier direction group's partial sums) instead of overwriting
+    int split_acc;    // row-wave kernel, two-part modes: cross products summed apart (64-column tiles)
     int64_t off;
     int tile_prefix[9];
     int64_t class_start[9];

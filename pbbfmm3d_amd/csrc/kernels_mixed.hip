@@ -415,9 +415,13 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
 // LDS, read before the step's barrier.  A step's loads and proxy stores are
 // issued between its first MFMA blocks, so that they overlap the matrix
 // cores instead of delaying the first MFMA after the barrier.
-template <typename T, int NSPLIT, int RBW, int NCB, int WCOL>
+// LO (two-part modes): the cross products of the split go to accumulators of
+// their own, so the sum of the leading products takes one FP32 rounding per
+// k chunk instead of three; the two sums are added once at the end.
+template <typename T, int NSPLIT, int RBW, int NCB, int WCOL, bool LO = false>
 __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M2LMixArgs a)
 {
+    static_assert(!LO || NSPLIT == 2, "separate cross-product sums are for the two-part modes");
     typedef typename Vec8<T>::type V8;
     constexpr int NT = 256 * WCOL, W = 16 * NCB * WCOL, BS = 36;
     constexpr int BR = W * 8 / NT;  // 16-byte proxy units per thread and step
@@ -558,10 +562,15 @@ __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M
         }
     };
     f32x4 acc[RBW][NCB];
+    constexpr int NLO = LO ? NCB : 1;
+    f32x4 accl[RBW][NLO];  // cross-product sums (LO)
 #pragma unroll
-    for (int i = 0; i < RBW; i++)
+    for (int i = 0; i < RBW; i++) {
 #pragma unroll
         for (int j = 0; j < NCB; j++) acc[i][j] = f32x4{0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < NLO; j++) accl[i][j] = f32x4{0, 0, 0, 0};
+    }
     V8 eA[RBW][NSPLIT], eB[RBW][NSPLIT];
     f32x4 pb0[BR], pb1[BR];
     Cur cm{0, 0, 0};  // next table reads: mask, E offset, proxy sources
@@ -661,13 +670,18 @@ __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M
             }
 #pragma unroll
             for (int i = 0; i < RBW; i++) {  // smallest terms first
-                if constexpr (NSPLIT == 3) {
-                    acc[i][cb] = mfma16(ea[i][2], bf[0], acc[i][cb]);
-                    acc[i][cb] = mfma16(ea[i][1], bf[1], acc[i][cb]);
-                    acc[i][cb] = mfma16(ea[i][0], bf[2], acc[i][cb]);
+                if constexpr (LO) {
+                    accl[i][cb % NLO] = mfma16(ea[i][1], bf[0], accl[i][cb % NLO]);
+                    accl[i][cb % NLO] = mfma16(ea[i][0], bf[1], accl[i][cb % NLO]);
+                } else {
+                    if constexpr (NSPLIT == 3) {
+                        acc[i][cb] = mfma16(ea[i][2], bf[0], acc[i][cb]);
+                        acc[i][cb] = mfma16(ea[i][1], bf[1], acc[i][cb]);
+                        acc[i][cb] = mfma16(ea[i][0], bf[2], acc[i][cb]);
+                    }
+                    acc[i][cb] = mfma16(ea[i][1], bf[0], acc[i][cb]);
+                    acc[i][cb] = mfma16(ea[i][0], bf[1], acc[i][cb]);
                 }
-                acc[i][cb] = mfma16(ea[i][1], bf[0], acc[i][cb]);
-                acc[i][cb] = mfma16(ea[i][0], bf[1], acc[i][cb]);
                 acc[i][cb] = mfma16(ea[i][0], bf[0], acc[i][cb]);
             }
         }
@@ -690,7 +704,8 @@ __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int row = (rb0 + i) * 16 + 4 * fq + v;
-                if (row < rf) dst[row] = acc[i][cb][v] * unscale;
+                const float x = LO ? acc[i][cb][v] + accl[i][cb % NLO][v] : acc[i][cb][v];
+                if (row < rf) dst[row] = x * unscale;
             }
     }
 }
@@ -939,6 +954,15 @@ int m2l_rows_max_width(int variant, int R)
 template <typename T, int NSPLIT, int RBW>
 static void m2l_rows_launch(const M2LMixArgs &a, dim3 grid, int w, hipStream_t st)
 {
+    if (a.split_acc) {  // 64-column tiles: the extra sums fit the two-waves-per-SIMD budget
+        if constexpr (NSPLIT == 2) {
+            if (w > 64) fail(PBB_ERR_INTERNAL, "separate cross-product sums take tiles of at most 64 columns");
+            m2l_mfma_rows<T, NSPLIT, RBW, 4, 1, true><<<grid, 256, 0, st>>>(a);
+            return;
+        } else {
+            fail(PBB_ERR_INTERNAL, "separate cross-product sums are for the two-part modes");
+        }
+    }
     if constexpr (rows_wide_ok<NSPLIT, RBW>()) {
         if (w > 128) m2l_mfma_rows<T, NSPLIT, RBW, 8, 2><<<grid, 512, 0, st>>>(a);
         else if (w > 64) m2l_mfma_rows<T, NSPLIT, RBW, 8, 1><<<grid, 256, 0, st>>>(a);

@@ -121,3 +121,39 @@ def test_m2l_direction_groups_and_wave_rows(gpu):
             assert orc.rel_err(out, ref) <= bar
         for f in (0, 1, G(3)):
             assert np.array_equal(outs[(64, f)], outs[(64, f | 4)])
+
+
+@pytest.mark.gpu
+def test_m2l_split_acc(gpu):
+    # PBB_M2L_SPLIT_ACC: the split's cross products summed apart from the
+    # leading products (one FP32 rounding of the large sums per k chunk
+    # instead of three) and the VT and U products in FP64.  On 1/r with N(0,1)
+    # weights (long range, sums that cancel) it comes closer to the FP64 path
+    # than the default two-part mode; tiles wider than 64 columns are capped
+    # and ragged column chunks work; the flag needs the row-wave kernel and
+    # PBB_PREC_MIXED ignores it (bit for bit).
+    from pbbfmm3d_amd._capi import PBBError
+    N, nc = 60000, 16
+    xyz = clustered_points(N, seed=80)
+    w = weights(N, nc, seed=81, dist="normal")
+    ops = pbb.Operators(pbb.laplacian(), 1.0, 5, 6, 1e-8)
+    plan = pbb.Plan(ops, xyz)
+    ref = _mv(plan, w, nc, pbb.capi.PREC_FP64)
+    SA = pbb.capi.M2L_SPLIT_ACC
+    with pytest.raises(PBBError):
+        plan.set_m2l_tile(64, 16, 1 | SA, pbb.capi.PREC_MIXED_F16X2)
+    for prec, bar in ((pbb.capi.PREC_MIXED_F16X2, 1e-6), (pbb.capi.PREC_MIXED_BF16X2, 1e-4)):
+        plan.set_m2l_tile(64, 16, 5, prec)
+        base = orc.rel_err(_mv(plan, w, nc, prec), ref)
+        for width, cpb in ((64, 16), (128, 16), (64, 5)):
+            print(f"case split_acc prec={prec} width={width} cpb={cpb}", flush=True)
+            plan.set_m2l_tile(width, cpb, 5 | SA, prec)
+            d = orc.rel_err(_mv(plan, w, nc, prec), ref)
+            print(f"rel diff vs FP64 GPU: {d:.3e} (without the flag {base:.3e})", flush=True)
+            assert d <= bar
+            if prec == pbb.capi.PREC_MIXED_F16X2:  # measured 6.8e-7 against 8.4e-7
+                assert d < base
+    plan.set_m2l_tile(64, 16, 5, pbb.capi.PREC_MIXED)
+    plain = _mv(plan, w, nc, pbb.capi.PREC_MIXED)
+    plan.set_m2l_tile(64, 16, 5 | SA, pbb.capi.PREC_MIXED)
+    assert np.array_equal(_mv(plan, w, nc, pbb.capi.PREC_MIXED), plain)
