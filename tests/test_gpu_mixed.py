@@ -129,8 +129,9 @@ def test_m2l_split_acc(gpu):
     # leading products (one FP32 rounding of the large sums per k chunk
     # instead of three) and the VT and U products in FP64.  On 1/r with N(0,1)
     # weights (long range, sums that cancel) it comes closer to the FP64 path
-    # than the default two-part mode; tiles wider than 64 columns are capped
-    # and ragged column chunks work; the flag needs the row-wave kernel and
+    # than the default two-part mode; tiles wider than 64 columns are capped,
+    # ragged column chunks work, and so do direction groups (later launches
+    # start from the partial sums); the flag needs the row-wave kernel and
     # PBB_PREC_MIXED ignores it (bit for bit).
     from pbbfmm3d_amd._capi import PBBError
     N, nc = 60000, 16
@@ -138,16 +139,17 @@ def test_m2l_split_acc(gpu):
     w = weights(N, nc, seed=81, dist="normal")
     ops = pbb.Operators(pbb.laplacian(), 1.0, 5, 6, 1e-8)
     plan = pbb.Plan(ops, xyz)
+    assert plan.info()["boxes"][-1] >= 8192  # leaf tiles of 4 boxes: enough workgroups for the groups
     ref = _mv(plan, w, nc, pbb.capi.PREC_FP64)
-    SA = pbb.capi.M2L_SPLIT_ACC
+    SA, G = pbb.capi.M2L_SPLIT_ACC, pbb.capi.M2L_DIR_GROUPS
     with pytest.raises(PBBError):
         plan.set_m2l_tile(64, 16, 1 | SA, pbb.capi.PREC_MIXED_F16X2)
     for prec, bar in ((pbb.capi.PREC_MIXED_F16X2, 1e-6), (pbb.capi.PREC_MIXED_BF16X2, 1e-4)):
         plan.set_m2l_tile(64, 16, 5, prec)
         base = orc.rel_err(_mv(plan, w, nc, prec), ref)
-        for width, cpb in ((64, 16), (128, 16), (64, 5)):
-            print(f"case split_acc prec={prec} width={width} cpb={cpb}", flush=True)
-            plan.set_m2l_tile(width, cpb, 5 | SA, prec)
+        for width, cpb, groups in ((64, 16, 0), (128, 16, 0), (64, 5, 0), (64, 16, G(3))):
+            print(f"case split_acc prec={prec} width={width} cpb={cpb} groups={groups >> 8}", flush=True)
+            plan.set_m2l_tile(width, cpb, 5 | SA | groups, prec)
             d = orc.rel_err(_mv(plan, w, nc, prec), ref)
             print(f"rel diff vs FP64 GPU: {d:.3e} (without the flag {base:.3e})", flush=True)
             assert d <= bar
