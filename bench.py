@@ -115,8 +115,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path (a cloud would "
+              "shard by Morton ranges of boxes with a proxy halo exchange; batches by cloud)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
