@@ -200,7 +200,10 @@ PBB_API pbb_status pbb_plan_query(const pbb_plan *plan, pbb_plan_info *info);
  * out_dev: device FP64, column-major N x ncols, accumulated (+=).
  * stream: a hipStream_t (NULL = the null stream), on which all kernels run
  * in order.  Asynchronous with respect to the host; the call returns once
- * all work is enqueued.                                                   */
+ * all work is enqueued.  Every call on one plan (matvec, transpose,
+ * gradients, tuning) uses the plan's one device workspace: issue them on
+ * one stream, or order the streams yourself; calls on different plans are
+ * independent and may run on any streams and host threads.              */
 PBB_API pbb_status pbb_plan_matvec(pbb_plan *plan, const double *w_dev,
                                    int32_t ncols, double *out_dev,
                                    int32_t precision, void *stream);
@@ -282,11 +285,14 @@ typedef struct pbb_batch pbb_batch;
 /* Captures the matvecs of nplans plans (any sizes, orders and tree levels,
  * one device) on weights w_dev[i] / outputs out_dev[i] (device FP64,
  * column-major N_i x ncols) into one HIP graph.  The buffers are bound at
- * capture: refill them in place between launches.  All workspaces are
- * allocated here, none during a launch.  The entries run as up to four
- * concurrent graph branches; entries that share a plan or an output stay
- * in one branch, in batch order.  No reference counterpart: the reference
- * runs one cloud per H2_3D_Compute.                                      */
+ * capture: refill them in place between launches.  The batch allocates
+ * its own workspaces here (one per distinct plan), none during a launch,
+ * so calls on the plans themselves never disturb it; the plans must
+ * outlive the batch.  The entries run as up to four concurrent graph
+ * branches; entries that share a plan, or whose weights or outputs overlap
+ * another entry's output (e.g. a chain A(A w)), are put in one branch and
+ * run there in batch order.  No reference counterpart: the reference runs
+ * one cloud per H2_3D_Compute.                                           */
 PBB_API pbb_status pbb_batch_create(pbb_plan *const *plans, int32_t nplans,
                                     const double *const *w_dev, double *const *out_dev,
                                     int32_t ncols, int32_t precision, int32_t flags,

@@ -223,8 +223,26 @@ class Operators:
             self._h = None
 
 
+def _f64_cuda(t, name: str, numel: Optional[int] = None, shape=None):
+    """Checks that t is a contiguous float64 CUDA tensor (of numel values or
+    of the given shape) before its pointer is handed to the kernels."""
+    import torch
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or not t.is_cuda:
+        raise TypeError(f"{name} must be a float64 CUDA tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name} must hold {numel} values, not {t.numel()}")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, not {tuple(t.shape)}")
+    return t
+
+
 class Plan:
-    """One point set's tree on the current GPU (sources are the targets)."""
+    """One point set's tree on the current GPU (sources are the targets).
+
+    Calls on one plan share its device workspace: issue them on one stream
+    (or order the streams), as the C ABI requires."""
 
     def __init__(self, ops: Operators, xyz):
         self.ops = ops
@@ -255,19 +273,28 @@ class Plan:
         """torch CUDA tensors: w (ncols, N) or (N,) float64, contiguous;
         returns out (+=) of the same shape, on torch's current stream."""
         import torch
-        if w.dtype != torch.float64 or not w.is_cuda:
-            raise TypeError("w must be a float64 CUDA tensor")
-        w = w.contiguous()
-        ncols = 1 if w.dim() == 1 else w.shape[0]
-        if w.numel() != ncols * self.npoints:
-            raise ValueError("w must hold ncols * npoints values")
-        if out is None:
-            out = torch.zeros_like(w)
-        elif not out.is_contiguous() or out.shape != w.shape or out.dtype != torch.float64:
-            raise ValueError("out must be a contiguous float64 tensor shaped like w")
+        w, out, ncols = self._io(w, out)
         stream = torch.cuda.current_stream(w.device).cuda_stream
         self.matvec_ptr(w.data_ptr(), ncols, out.data_ptr(), precision, stream)
         return out
+
+    def _io(self, w, out):
+        """(w, out, ncols) checked: w float64 CUDA (ncols, N) or (N,); out
+        of w's shape on w's device, made zero if None."""
+        import torch
+        if not isinstance(w, torch.Tensor) or w.dtype != torch.float64 or not w.is_cuda:
+            raise TypeError("w must be a float64 CUDA tensor")
+        w = w.contiguous()
+        ncols = 1 if w.dim() == 1 else w.shape[0]
+        if w.dim() > 2 or w.numel() != ncols * self.npoints or ncols < 1:
+            raise ValueError("w must be (ncols, npoints) or (npoints,)")
+        if out is None:
+            out = torch.zeros_like(w)
+        else:
+            _f64_cuda(out, "out", shape=w.shape)
+            if out.device != w.device:
+                raise ValueError("out must be on w's device")
+        return w, out, ncols
 
     def matvec_transpose_ptr(self, w_ptr: int, ncols: int, out_ptr: int,
                              precision: int = capi.PREC_FP64, stream: int = 0) -> None:
@@ -279,14 +306,7 @@ class Plan:
     def matvec_transpose(self, w, out=None, precision: int = capi.PREC_FP64):
         """Like matvec, with the transposed FMM operator."""
         import torch
-        if w.dtype != torch.float64 or not w.is_cuda:
-            raise TypeError("w must be a float64 CUDA tensor")
-        w = w.contiguous()
-        ncols = 1 if w.dim() == 1 else w.shape[0]
-        if w.numel() != ncols * self.npoints:
-            raise ValueError("w must hold ncols * npoints values")
-        if out is None:
-            out = torch.zeros_like(w)
+        w, out, ncols = self._io(w, out)
         stream = torch.cuda.current_stream(w.device).cuda_stream
         self.matvec_transpose_ptr(w.data_ptr(), ncols, out.data_ptr(), precision, stream)
         return out
@@ -295,13 +315,18 @@ class Plan:
         """grad (N x 3, float64 CUDA) += dL/dx for out = A~(x) sigma and
         psi = dL/dout (both (ncols, N) float64 CUDA tensors)."""
         import torch
-        sigma = sigma.contiguous()
-        psi = psi.contiguous()
-        ncols = 1 if sigma.dim() == 1 else sigma.shape[0]
-        if sigma.numel() != ncols * self.npoints or psi.shape != sigma.shape:
-            raise ValueError("sigma and psi must both hold ncols * npoints values")
+        sigma, _, ncols = self._io(sigma, None)
+        if not isinstance(psi, torch.Tensor):
+            raise TypeError("psi must be a float64 CUDA tensor")
+        psi = _f64_cuda(psi.contiguous(), "psi", shape=sigma.shape)
+        if psi.device != sigma.device:
+            raise ValueError("psi must be on sigma's device")
         if grad is None:
             grad = torch.zeros((self.npoints, 3), dtype=torch.float64, device=sigma.device)
+        else:
+            _f64_cuda(grad, "grad", shape=(self.npoints, 3))
+            if grad.device != sigma.device:
+                raise ValueError("grad must be on sigma's device")
         stream = torch.cuda.current_stream(sigma.device).cuda_stream
         capi.check(capi.lib().pbb_plan_grad_positions(
             self._h, C.c_void_p(sigma.data_ptr()), C.c_void_p(psi.data_ptr()), int(ncols),
@@ -325,10 +350,7 @@ class Plan:
         tensor, (ncols, N) or (N,)), keeps the fastest and returns it as
         {"width", "cols_per_box", "flags"}."""
         import torch
-        w = weights.contiguous()
-        ncols = w.numel() // self.npoints
-        if ncols * self.npoints != w.numel():
-            raise ValueError("weights must hold ncols * npoints values")
+        w, _, ncols = self._io(weights, None)
         best = (C.c_int32 * 3)()
         stream = torch.cuda.current_stream(w.device).cuda_stream
         capi.check(capi.lib().pbb_plan_tune_m2l(self._h, C.c_void_p(w.data_ptr()), int(ncols), int(precision),
@@ -356,11 +378,9 @@ class Batch:
                  transpose: bool = False, overwrite: bool = True):
         if not (len(plans) == len(weights) == len(outputs)) or not plans:
             raise ValueError("plans, weights and outputs must have the same non-zero length")
-        for p, w, o in zip(plans, weights, outputs):
-            if w.numel() != ncols * p.npoints or o.numel() != ncols * p.npoints:
-                raise ValueError("each weight/output tensor must hold ncols * npoints values")
-            if not (w.is_contiguous() and o.is_contiguous()):
-                raise ValueError("weight/output tensors must be contiguous")
+        for i, (p, w, o) in enumerate(zip(plans, weights, outputs)):
+            _f64_cuda(w, f"weights[{i}]", numel=ncols * p.npoints)
+            _f64_cuda(o, f"outputs[{i}]", numel=ncols * p.npoints)
         self.plans, self.weights, self.outputs = list(plans), list(weights), list(outputs)
         n = len(plans)
         P = (C.c_void_p * n)(*[p._h.value for p in plans])

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <functional>
 #include <mutex>
 
 #include "host.hpp"
@@ -238,6 +239,25 @@ struct pbb_ops {
     }
 };
 
+// Device buffers of one matvec.  The FP64 mode keeps the proxies P and the
+// M2L results Pf in FP64; the mixed modes keep them in FP32 (P32 k-padded to
+// kpad, a multiple of 32).  The forward and the transposed operator sets of
+// one plan fit the same layout (same n3, max(rank_f, rank_s) and kpad).
+struct Workspace {
+    double *ws = nullptr, *wsT = nullptr;  // sorted weights, column- and point-major
+    double *lp = nullptr;                  // per-level L2P results, point-major
+    double *up = nullptr;                  // partial expansions of split boxes
+    double *S = nullptr;                   // expansions S / F, [box][col][n3]
+    double *P = nullptr, *Pf = nullptr;    // FP64 mode
+    float *P32 = nullptr, *Pf32 = nullptr; // mixed modes
+    unsigned *xmax = nullptr;              // per-level max |P| (FP16 scaling)
+    int kpad = 0;
+    int ncols = 0, mixed = -1;             // what the layout was made for
+};
+
+struct pbb_plan;
+static size_t layout_workspace(const pbb_plan &p, int ncols, bool mixed, char *base, Workspace &W);
+
 struct DevLevel {
     int64_t nb = 0, off = 0;
     int32_t *coord = nullptr, *map = nullptr, *child = nullptr, *order = nullptr;
@@ -261,18 +281,14 @@ struct pbb_plan {
     int32_t *leaf_class = nullptr;              // leaves grouped by point count
     int4 *up_items = nullptr, *up_reds = nullptr;  // direct upward pass work list
     int n_up_items = 0, n_up_reds = 0, n_up_slots = 0;
-    double *up_scratch = nullptr;               // partial expansions of split boxes
     int leaf_class_off[dev::NEAR_CLASSES + 1] = {0};
     std::vector<DevLevel> lv;
     int64_t Btot = 0;
-    int ws_ncols = 0, ws_mixed = -1;
-    double *ws = nullptr, *S = nullptr, *P = nullptr, *Pf = nullptr;
-    double *wsT = nullptr, *lp = nullptr;  // near field: point-major weights, L2P results
-    float *P32 = nullptr, *Pf32 = nullptr;
-    unsigned *xmax = nullptr;
+    Workspace W;                  // the plan's own matvec buffers (one block)
+    char *ws_block = nullptr;
+    size_t ws_bytes = 0;
     double *gF = nullptr, *gU = nullptr;  // position gradients: saved leaf F, sorted weights
     int g_ncols = 0;
-    int kpad = 0;
     bool timing = false;
     struct M2LTile {
         int width = 0, cpt = 0, flags = 0;  // width 0 = built-in choice
@@ -289,63 +305,69 @@ struct pbb_plan {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
         dfree(x); dfree(y); dfree(z); dfree(perm); dfree(leaf_start); dfree(leaf_class);
-        dfree(up_items); dfree(up_reds); dfree(up_scratch);
+        dfree(up_items); dfree(up_reds);
         for (auto &L : lv) {
             dfree(L.coord); dfree(L.map); dfree(L.child); dfree(L.order); dfree(L.ctab);
             dfree(L.pstart); dfree(L.seg_box); dfree(L.seg_p0);
         }
-        dfree(ws); dfree(S); dfree(P); dfree(Pf); dfree(P32); dfree(Pf32); dfree(xmax); dfree(gF); dfree(gU);
-        dfree(wsT); dfree(lp);
+        dfree(ws_block); dfree(gF); dfree(gU);
         if (events_made)
             for (auto &e : ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
         if (prev >= 0) (void)hipSetDevice(prev);
     }
 
-    // Device buffers of one matvec: FP64 mode keeps P / Pf in FP64, the
-    // mixed modes in FP32; the other mode's buffers are released on a switch
-    // so that at most one set is resident.
+    // The plan's own matvec buffers, grown (never shrunk) to fit ncols
+    // columns in the precision class; one block, laid out by layout_workspace.
     void ensure_workspace(int ncols, bool mixed)
     {
-        const HostOps &h = ops->h;
-        if (ncols > ws_ncols) {
-            dfree(ws); dfree(S); dfree(P); dfree(Pf); dfree(P32); dfree(Pf32); dfree(wsT); dfree(lp);
-            dfree(up_scratch);
-            ws = S = P = Pf = wsT = lp = up_scratch = nullptr;
-            P32 = Pf32 = nullptr;
-            ws_ncols = 0;
-            ws_mixed = -1;
-            ws = dalloc<double>((size_t)N * ncols, "sorted weights");
-            wsT = dalloc<double>((size_t)N * ncols, "sorted weights, point-major");
-            if (Lv >= 2) lp = dalloc<double>((size_t)N * ncols, "L2P results");
-            if (n_up_slots > 0)
-                up_scratch = dalloc<double>((size_t)n_up_slots * ncols * h.n3, "partial expansions");
-            if (Lv >= 2) S = dalloc<double>((size_t)Btot * ncols * h.n3, "expansions");
-            ws_ncols = ncols;
+        const size_t need = layout_workspace(*this, ncols, mixed, nullptr, W);
+        if (need > ws_bytes) {
+            dfree(ws_block);
+            ws_block = nullptr;
+            ws_bytes = 0;
+            ws_block = dalloc<char>(need, "matvec workspace");
+            ws_bytes = need;
         }
-        if (ws_mixed == (int)mixed || Lv < 2) {
-            ws_mixed = (int)mixed;
-            return;
-        }
-        const int nc = ws_ncols;
-        if (mixed) {
-            dfree(P); dfree(Pf);
-            P = Pf = nullptr;
-            const int rmax = std::max(h.rf, h.rs);
-            kpad = 32 * ((rmax + 31) / 32);
-            P32 = dalloc<float>((size_t)Btot * nc * kpad, "FP32 proxies");
-            Pf32 = dalloc<float>((size_t)Btot * nc * rmax, "FP32 M2L results");
-            if (!xmax) xmax = dalloc<unsigned>(Lv + 1, "level maxima");
-        } else {
-            dfree(P32); dfree(Pf32);
-            P32 = Pf32 = nullptr;
-            const int rmax = std::max(h.rf, h.rs);
-            P = dalloc<double>((size_t)Btot * nc * rmax, "proxies");
-            Pf = dalloc<double>((size_t)Btot * nc * rmax, "M2L results");
-        }
-        ws_mixed = (int)mixed;
+        layout_workspace(*this, ncols, mixed, ws_block, W);
     }
 };
+
+// Lays out the buffers of one matvec of p with ncols columns from base
+// (nullptr: sizes only, pointers left null) and returns the bytes used; every
+// buffer starts on a 256-byte boundary.
+static size_t layout_workspace(const pbb_plan &p, int ncols, bool mixed, char *base, Workspace &W)
+{
+    const HostOps &h = p.ops->h;
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> void * {
+        const size_t at = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return base ? (void *)(base + at) : nullptr;
+    };
+    const size_t Nc = (size_t)p.N * ncols, Bc = (size_t)p.Btot * ncols;
+    const int rmax = std::max(h.rf, h.rs);
+    W = Workspace{};
+    W.ncols = ncols;
+    W.mixed = mixed ? 1 : 0;
+    W.kpad = 32 * ((rmax + 31) / 32);
+    W.ws = (double *)take(Nc * sizeof(double));
+    W.wsT = (double *)take(Nc * sizeof(double));
+    if (p.Lv >= 2) {
+        W.lp = (double *)take(Nc * sizeof(double));
+        W.S = (double *)take(Bc * h.n3 * sizeof(double));
+        if (mixed) {
+            W.P32 = (float *)take(Bc * W.kpad * sizeof(float));
+            W.Pf32 = (float *)take(Bc * rmax * sizeof(float));
+            W.xmax = (unsigned *)take((size_t)(p.Lv + 1) * sizeof(unsigned));
+        } else {
+            W.P = (double *)take(Bc * rmax * sizeof(double));
+            W.Pf = (double *)take(Bc * rmax * sizeof(double));
+        }
+    }
+    if (p.n_up_slots > 0) W.up = (double *)take((size_t)p.n_up_slots * ncols * h.n3 * sizeof(double));
+    return off;
+}
 
 // ---------------------------------------------------------------------------
 
@@ -777,7 +799,22 @@ PBB_API pbb_status pbb_plan_timings(pbb_plan *plan, double *ms)
     });
 }
 
-static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, double *out, hipStream_t st, int prec);
+// Ranks the M2L kernels of precision prec take (checked before any launch)
+static void check_mixed_ranks(const pbb_ops *o, int prec)
+{
+    if (prec != PBB_PREC_FP64 && o->h.rf > 224)
+        fail(PBB_ERR_UNSUPPORTED, "mixed-precision M2L supports ranks up to 224");
+}
+
+static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double *w, int ncols, double *out,
+                       hipStream_t st, int prec, bool build);
+
+// One matvec of the plan's operator set o in its own workspace.
+static void matvec_own(pbb_plan *p, pbb_ops *o, const double *w, int ncols, double *out, hipStream_t st, int prec)
+{
+    p->ensure_workspace(ncols, prec != PBB_PREC_FP64);
+    matvec_run(p, o, p->W, w, ncols, out, st, prec, true);
+}
 
 PBB_API pbb_status pbb_plan_tune_m2l(pbb_plan *plan, const double *w_dev, int32_t ncols, int32_t precision,
                                      void *stream, int32_t *best)
@@ -788,6 +825,7 @@ PBB_API pbb_status pbb_plan_tune_m2l(pbb_plan *plan, const double *w_dev, int32_
         if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
             fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
         const bool mixed = precision != PBB_PREC_FP64;
+        check_mixed_ranks(plan->ops, precision);
         DeviceGuard g(plan->device);
         hipStream_t st = (hipStream_t)stream;
         const bool was_timing = plan->timing;
@@ -803,10 +841,10 @@ PBB_API pbb_status pbb_plan_tune_m2l(pbb_plan *plan, const double *w_dev, int32_
         double best_ms = 0;
         try {
             // one untimed matvec builds the operator splits and workspaces
-            matvec_run(plan, plan->ops, w_dev, ncols, scratch, st, precision);
+            matvec_own(plan, plan->ops, w_dev, ncols, scratch, st, precision);
             for (int c = 0; c < ncand; c++) {
                 plan->m2l_tile[mixed ? 1 : 0] = {cands[c][0], std::min(cands[c][1], ncols), cands[c][2]};
-                matvec_run(plan, plan->ops, w_dev, ncols, scratch, st, precision);
+                matvec_own(plan, plan->ops, w_dev, ncols, scratch, st, precision);
                 check(hipStreamSynchronize(st), "hipStreamSynchronize");
                 float t = 0;
                 check(hipEventElapsedTime(&t, plan->ev[3], plan->ev[4]), "hipEventElapsedTime");
@@ -832,7 +870,7 @@ PBB_API pbb_status pbb_plan_tune_m2l(pbb_plan *plan, const double *w_dev, int32_
 }
 
 // One FMM application.  out == nullptr stops after the downward pass, leaving
-// the leaf local expansions in p->S and the sorted weights in p->ws.
+// the leaf local expansions in W.S and the sorted weights in W.ws.
 static void fill_levels(const pbb_plan *p, dev::LevelTab &t)
 {
     t.nlev = p->Lv - 1;
@@ -848,7 +886,42 @@ static void fill_levels(const pbb_plan *p, dev::LevelTab &t)
     t.off[t.nlev] = p->Btot;
 }
 
-static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, double *out, hipStream_t st, int prec)
+// PBB_M2L_SPLIT_ACC in force for precision prec (two-part modes only)
+static bool split_acc_on(const pbb_plan *p, int prec)
+{
+    const pbb_plan::M2LTile &tmx = p->m2l_tile[1];
+    return (prec == PBB_PREC_MIXED_F16X2 || prec == PBB_PREC_MIXED_BF16X2) && tmx.width &&
+           (tmx.flags & PBB_M2L_WAVE_ROWS) && (tmx.flags & PBB_M2L_SPLIT_ACC);
+}
+
+// The split operators a matvec of p with operator set o in precision prec
+// uses (nullptr where none): E_d in 16-bit parts for the mixed M2L, and the
+// three-part BF16 VT / U when both have at most 224 rows (PBB_M2L_SPLIT_ACC
+// keeps VT / U in FP64).  build: made on first use (allocates and
+// synchronises the stream); otherwise they must exist already.
+static void split_ops(const pbb_plan *p, pbb_ops *o, int prec, bool build, hipStream_t st, const SplitE *&SE,
+                      const pbb_ops::SplitA *&SA)
+{
+    SE = nullptr;
+    SA = nullptr;
+    const HostOps &h = o->h;
+    if (prec == PBB_PREC_FP64 || p->Lv < 2) return;
+    const bool want3 = !split_acc_on(p, prec) && h.rs <= 224 && h.n3 <= 224;
+    if (build) {
+        SE = &o->ensure_split(prec, st);
+        if (want3) SA = o->ensure_split3(st);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(o->mu);
+    if (!o->split[prec].d || (want3 && !o->vt3.d))
+        fail(PBB_ERR_INVALID_ARG, "the operators of this precision are not split yet: one pbb_plan_matvec "
+                                  "in this precision (or pbb_plan_tune_m2l) prepares them");
+    SE = &o->split[prec];
+    if (want3) SA = &o->vt3;
+}
+
+static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double *w, int ncols, double *out,
+                       hipStream_t st, int prec, bool build)
 {
     const bool mixed = prec != PBB_PREC_FP64;
     const HostOps &h = o->h;
@@ -856,26 +929,22 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
     auto mark = [&](int k) {
         if (p->timing) check(hipEventRecord(p->ev[k], st), "hipEventRecord");
     };
-    p->ensure_workspace(ncols, mixed);
-    const SplitE *SE = mixed && Lv >= 2 ? &o->ensure_split(prec, st) : nullptr;
-    // the mixed modes' VT / U products on the BF16 matrix cores (3 parts),
-    // when both operators have at most 224 rows
-    // PBB_M2L_SPLIT_ACC (two-part modes) keeps them in FP64 (P and Pf FP32)
+    if (W.ncols != ncols || W.mixed != (mixed ? 1 : 0)) fail(PBB_ERR_INTERNAL, "workspace laid out for another call");
+    const SplitE *SE = nullptr;
+    const pbb_ops::SplitA *SA = nullptr;
+    split_ops(p, o, prec, build, st, SE, SA);
+    const bool split_acc = split_acc_on(p, prec);
     const pbb_plan::M2LTile &tmx = p->m2l_tile[1];
-    const bool split_acc = (prec == PBB_PREC_MIXED_F16X2 || prec == PBB_PREC_MIXED_BF16X2) && tmx.width &&
-                           (tmx.flags & PBB_M2L_WAVE_ROWS) && (tmx.flags & PBB_M2L_SPLIT_ACC);
-    const pbb_ops::SplitA *SA =
-        mixed && !split_acc && Lv >= 2 && h.rs <= 224 && h.n3 <= 224 ? o->ensure_split3(st) : nullptr;
-    const int ldP = mixed ? p->kpad : h.rs;
-    if (mixed && h.rs > p->kpad) fail(PBB_ERR_INTERNAL, "proxy padding too small");
+    const int ldP = mixed ? W.kpad : h.rs;
+    if (mixed && h.rs > W.kpad) fail(PBB_ERR_INTERNAL, "proxy padding too small");
     mark(0);
     // orders 4..8 run the direct passes: P2M at every level instead of
     // P2M + M2M, and (with a near field to add into) L2P at every level
     // instead of L2L + L2P; both are exact rewritings (see p2m_levels)
     const bool direct = n >= 4 && n <= 8;
     const bool direct_down = direct && out != nullptr && Lv >= 2;
-    dev::launch_gather_weights(w, p->perm, p->N, ncols, p->ws, st);
-    if (direct) dev::launch_gather_weights_t(w, p->perm, p->N, ncols, p->wsT, st);
+    dev::launch_gather_weights(w, p->perm, p->N, ncols, W.ws, st);
+    if (direct) dev::launch_gather_weights_t(w, p->perm, p->N, ncols, W.wsT, st);
     const int cc_p2m = std::max(1, std::min({ncols, 64, 4096 / n3}));
     const int cc_m2m = std::max(1, std::min(ncols, 2048 / n3));
     if (Lv >= 2 && direct) {
@@ -883,18 +952,18 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         a.n = n; a.ncols = ncols;
         fill_levels(p, a.lt);
         a.items = p->up_items;
-        a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.wsT = p->wsT; a.S = p->S;
-        a.scratch = p->up_scratch;
+        a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.wsT = W.wsT; a.S = W.S;
+        a.scratch = W.up;
         dev::UpReduce r{};
         r.ncols = ncols; r.n3 = n3; r.nred = p->n_up_reds; r.reds = p->up_reds;
-        r.scratch = p->up_scratch; r.S = p->S;
+        r.scratch = W.up; r.S = W.S;
         dev::launch_p2m_levels(a, p->n_up_items, r, st);
     } else if (Lv >= 2) {
         const DevLevel &LL = p->lv[Lv];
         dev::P2MArgs a{};
         a.n = n; a.ncols = ncols; a.cc = cc_p2m; a.N = p->N; a.off = LL.off; a.len = LL.len;
         a.leaf_start = p->leaf_start; a.coord = LL.coord; a.ctab = LL.ctab; a.Tkz = o->d_Tkz;
-        a.x = p->x; a.y = p->y; a.z = p->z; a.ws = p->ws; a.S = p->S;
+        a.x = p->x; a.y = p->y; a.z = p->z; a.ws = W.ws; a.S = W.S;
         dev::launch_p2m(a, (int)LL.nb, st);
     }
     mark(1);
@@ -902,7 +971,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         dev::M2MArgs a{};
         a.n = n; a.ncols = ncols; a.cc = cc_m2m;
         a.off_parent = p->lv[l].off; a.off_child = p->lv[l + 1].off;
-        a.child = p->lv[l].child; a.Cw = o->d_Cw; a.Sparent = p->S; a.Schild = p->S;
+        a.child = p->lv[l].child; a.Cw = o->d_Cw; a.Sparent = W.S; a.Schild = W.S;
         dev::launch_m2m(a, (int)p->lv[l].nb, st);
     }
     mark(2);
@@ -911,12 +980,12 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         dev::GemmArgs g{};
         g.M = h.rs; g.K = n3; g.ldb = n3; g.ldc = ldP; g.ncol = D.nb * ncols; g.alpha = 1.0;
         g.A = o->d_VTw + (size_t)h.ops_set(l) * h.rs * n3;
-        g.B = p->S + (size_t)D.off * ncols * n3;
+        g.B = W.S + (size_t)D.off * ncols * n3;
         if (mixed) {
             g.c_f32 = 1;
-            g.C = p->P32 + (size_t)D.off * ncols * ldP;
+            g.C = W.P32 + (size_t)D.off * ncols * ldP;
         } else {
-            g.C = p->P + (size_t)D.off * ncols * h.rs;
+            g.C = W.P + (size_t)D.off * ncols * h.rs;
         }
         if (SA) {  // three-part BF16 matrix cores (P is FP32 in the mixed modes)
             dev::GemmSplitArgs q{};
@@ -928,12 +997,12 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
             dev::launch_gemm(g, st);
         }
         if (prec == PBB_PREC_MIXED_F16X2) {
-            dev::launch_zero_u32(p->xmax + l, 1, st);
-            dev::launch_absmax(p->P32 + (size_t)D.off * ncols * ldP, D.nb * ncols * ldP, p->xmax + l, st);
+            dev::launch_zero_u32(W.xmax + l, 1, st);
+            dev::launch_absmax(W.P32 + (size_t)D.off * ncols * ldP, D.nb * ncols * ldP, W.xmax + l, st);
         }
         if (prec == PBB_PREC_MIXED_F16X2 || prec == PBB_PREC_MIXED_BF16X2)
-            dev::launch_split_p2(prec, p->P32 + (size_t)D.off * ncols * ldP, D.nb * ncols * (ldP / 32),
-                                 prec == PBB_PREC_MIXED_F16X2 ? p->xmax + l : nullptr, st);
+            dev::launch_split_p2(prec, W.P32 + (size_t)D.off * ncols * ldP, D.nb * ncols * (ldP / 32),
+                                 prec == PBB_PREC_MIXED_F16X2 ? W.xmax + l : nullptr, st);
     }
     mark(3);
     // M2L tiles: W columns = TB target boxes x cpt columns.
@@ -971,8 +1040,8 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         a.order = D.order; a.coord = D.coord; a.map = D.map;
         const int nsplit = prec == PBB_PREC_MIXED ? 3 : 2;
         a.Es = (const uint16_t *)SE->d + (size_t)h.ops_set(l) * 316 * SE->nch * nsplit * SE->R * 32;
-        a.P = p->P32; a.Pf = p->Pf32;
-        a.xmax = prec == PBB_PREC_MIXED_F16X2 ? p->xmax + l : nullptr;
+        a.P = W.P32; a.Pf = W.Pf32;
+        a.xmax = prec == PBB_PREC_MIXED_F16X2 ? W.xmax + l : nullptr;
         // direction groups only where the launch fills the GPU many times over
         const int64_t wgs = (int64_t)a.tile_prefix[8] * nchunks_m * (a.row_split ? 2 : 1);
         const int ng = wgs >= 2048 ? groups_m : 1, per = (316 + ng - 1) / ng;
@@ -996,7 +1065,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         }
         a.order = D.order; a.coord = D.coord; a.map = D.map;
         a.E = o->d_E + (size_t)h.ops_set(l) * 316 * h.rf * h.rs;
-        a.P = p->P; a.Pf = p->Pf;
+        a.P = W.P; a.Pf = W.Pf;
         dev::launch_m2l(a, a.tile_prefix[8], nchunks, st);
     }
     mark(4);
@@ -1009,11 +1078,11 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         g.A = o->d_Uw + (size_t)h.ops_set(l) * n3 * h.rf;
         if (mixed) {
             g.b_f32 = 1;
-            g.B = p->Pf32 + (size_t)D.off * ncols * h.rf;
+            g.B = W.Pf32 + (size_t)D.off * ncols * h.rf;
         } else {
-            g.B = p->Pf + (size_t)D.off * ncols * h.rf;
+            g.B = W.Pf + (size_t)D.off * ncols * h.rf;
         }
-        g.C = p->S + (size_t)D.off * ncols * n3;
+        g.C = W.S + (size_t)D.off * ncols * n3;
         if (SA) {  // three-part BF16 matrix cores (Pf is FP32 in the mixed modes)
             const auto &U3 = o->u3;
             dev::GemmSplitArgs q{};
@@ -1033,14 +1102,14 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         dev::DownArgs a{};
         a.n = n; a.ncols = ncols; a.accumulate = l > 2; a.off = D.off; a.len = D.len;
         a.seg_box = D.seg_box; a.seg_p0 = D.seg_p0; a.pstart = D.pstart; a.coord = D.coord; a.ctab = D.ctab;
-        a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.F = p->S; a.lp = p->lp;
+        a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.F = W.S; a.lp = W.lp;
         dev::launch_l2p_level(a, D.nseg, st);
     }
     for (int l = 3; l <= Lv && !direct_down; l++) {
         dev::M2MArgs a{};
         a.n = n; a.ncols = ncols; a.cc = cc_m2m;
         a.off_parent = p->lv[l - 1].off; a.off_child = p->lv[l].off;
-        a.child = p->lv[l - 1].child; a.Cw = o->d_Cw; a.Sparent = p->S; a.Schild = p->S;
+        a.child = p->lv[l - 1].child; a.Cw = o->d_Cw; a.Sparent = W.S; a.Schild = W.S;
         dev::launch_l2l(a, (int)p->lv[l - 1].nb, st);
     }
     mark(6);
@@ -1050,11 +1119,11 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
         a.n = n; a.ncols = ncols; a.side = 1 << Lv; a.N = p->N; a.off = LL.off; a.len = LL.len;
         std::memcpy(a.kp.p, h.kernel.param, sizeof(a.kp.p));
         a.leaf_start = p->leaf_start; a.coord = LL.coord; a.map = LL.map; a.perm = p->perm;
-        a.ctab = LL.ctab; a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.ws = p->ws;
-        a.F = Lv >= 2 ? p->S : nullptr;
+        a.ctab = LL.ctab; a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.ws = W.ws;
+        a.F = Lv >= 2 ? W.S : nullptr;
         a.out = out;
-        a.wsT = p->wsT;
-        a.lp = direct_down ? p->lp : nullptr;
+        a.wsT = W.wsT;
+        a.lp = direct_down ? W.lp : nullptr;
         a.leaf_class = p->leaf_class;
         std::copy(std::begin(p->leaf_class_off), std::end(p->leaf_class_off), a.leaf_class_off);
         if (!(direct && dev::launch_near_split(a, h.kernel.id, (int)LL.nb, st)))
@@ -1072,10 +1141,9 @@ PBB_API pbb_status pbb_plan_matvec(pbb_plan *plan, const double *w_dev, int32_t 
         if ((int64_t)ncols * plan->N >= (1LL << 40)) fail(PBB_ERR_INVALID_ARG, "ncols * npoints too large");
         if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
             fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
-        if (precision != PBB_PREC_FP64 && plan->ops->h.rf > 224)
-            fail(PBB_ERR_UNSUPPORTED, "mixed-precision M2L supports ranks up to 224");
+        check_mixed_ranks(plan->ops, precision);
         DeviceGuard g(plan->device);
-        matvec_run(plan, plan->ops, w_dev, ncols, out_dev, (hipStream_t)stream, precision);
+        matvec_own(plan, plan->ops, w_dev, ncols, out_dev, (hipStream_t)stream, precision);
     });
 }
 
@@ -1090,11 +1158,10 @@ PBB_API pbb_status pbb_plan_matvec_transpose(pbb_plan *plan, const double *w_dev
         if (plan->ops->h.kernel.symmetry != 1)
             fail(PBB_ERR_UNSUPPORTED, "the transposed near field needs a symmetric kernel");
         pbb_ops *adj = plan->ops->adjoint();
-        if (precision != PBB_PREC_FP64 && adj->h.rf > 224)
-            fail(PBB_ERR_UNSUPPORTED, "mixed-precision M2L supports ranks up to 224");
+        check_mixed_ranks(adj, precision);
         DeviceGuard g(plan->device);
         adj->upload(plan->device);
-        matvec_run(plan, adj, w_dev, ncols, out_dev, (hipStream_t)stream, precision);
+        matvec_own(plan, adj, w_dev, ncols, out_dev, (hipStream_t)stream, precision);
     });
 }
 
@@ -1109,6 +1176,8 @@ PBB_API pbb_status pbb_plan_grad_positions(pbb_plan *plan, const double *sigma_d
         if (plan->ops->h.kernel.symmetry != 1)
             fail(PBB_ERR_UNSUPPORTED, "position gradients need a symmetric kernel");
         pbb_ops *adj = plan->ops->adjoint();
+        check_mixed_ranks(plan->ops, precision);
+        check_mixed_ranks(adj, precision);
         DeviceGuard g(plan->device);
         adj->upload(plan->device);
         hipStream_t st = (hipStream_t)stream;
@@ -1126,14 +1195,15 @@ PBB_API pbb_status pbb_plan_grad_positions(pbb_plan *plan, const double *sigma_d
         }
         const size_t leafF = (size_t)LL.nb * ncols * n3;
         // forward FMM with sigma: leaf expansions F and sorted sigma
-        matvec_run(p, p->ops, sigma_dev, ncols, nullptr, st, precision);
+        matvec_own(p, p->ops, sigma_dev, ncols, nullptr, st, precision);
+        const Workspace &W = p->W;
         if (Lv >= 2)
-            check(hipMemcpyAsync(p->gF, p->S + (size_t)LL.off * ncols * n3, leafF * sizeof(double),
+            check(hipMemcpyAsync(p->gF, W.S + (size_t)LL.off * ncols * n3, leafF * sizeof(double),
                                  hipMemcpyDeviceToDevice, st), "save leaf expansions");
-        check(hipMemcpyAsync(p->gU, p->ws, (size_t)p->N * ncols * sizeof(double), hipMemcpyDeviceToDevice, st),
+        check(hipMemcpyAsync(p->gU, W.ws, (size_t)p->N * ncols * sizeof(double), hipMemcpyDeviceToDevice, st),
               "save sorted weights");
         // transposed FMM with psi: its leaf expansions and sorted psi
-        matvec_run(p, adj, psi_dev, ncols, nullptr, st, precision);
+        matvec_own(p, adj, psi_dev, ncols, nullptr, st, precision);
         dev::NearGradArgs a{};
         a.n = h.n; a.ncols = ncols; a.side = 1 << Lv; a.N = p->N; a.len = LL.len;
         std::memcpy(a.kp.p, h.kernel.param, sizeof(a.kp.p));
@@ -1141,10 +1211,10 @@ PBB_API pbb_status pbb_plan_grad_positions(pbb_plan *plan, const double *sigma_d
         a.ctab = LL.ctab; a.Tkz = p->ops->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z;
         a.grad = grad_dev;
         // target side: d/dx_i of (A sigma)_i weighted by psi_i
-        a.v = p->ws; a.u = p->gU; a.F = Lv >= 2 ? p->gF : nullptr;
+        a.v = W.ws; a.u = p->gU; a.F = Lv >= 2 ? p->gF : nullptr;
         dev::launch_near_grad(a, h.kernel.id, (int)LL.nb, st);
         // source side: d/dx_j of (A^T psi)_j weighted by sigma_j
-        a.v = p->gU; a.u = p->ws; a.F = Lv >= 2 ? p->S + (size_t)LL.off * ncols * n3 : nullptr;
+        a.v = p->gU; a.u = W.ws; a.F = Lv >= 2 ? W.S + (size_t)LL.off * ncols * n3 : nullptr;
         dev::launch_near_grad(a, h.kernel.id, (int)LL.nb, st);
     });
 }
@@ -1157,6 +1227,7 @@ PBB_API void pbb_plan_destroy(pbb_plan *plan) { delete plan; }
 // Batches: many plans' matvecs captured once into a HIP graph and replayed.
 struct pbb_batch {
     int device = -1;
+    std::vector<char *> blocks;     // the batch's own workspaces, one per distinct plan
     hipStream_t stream = nullptr;
     std::vector<hipStream_t> side;  // capture branches
     std::vector<hipEvent_t> ev;     // fork and join points
@@ -1173,6 +1244,7 @@ struct pbb_batch {
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         for (hipStream_t s : side) (void)hipStreamDestroy(s);
         if (stream) (void)hipStreamDestroy(stream);
+        for (char *b : blocks) dfree(b);
         if (prev >= 0) (void)hipSetDevice(prev);
     }
 };
@@ -1204,47 +1276,67 @@ PBB_API pbb_status pbb_batch_create(pbb_plan *const *plans, int32_t nplans, cons
         auto b = std::make_unique<pbb_batch>();
         b->device = dev;
         check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
-        // everything that allocates or synchronises happens before the capture
+        // everything that allocates or synchronises happens before the capture:
+        // the operator splits, and one workspace per distinct plan owned by the
+        // batch (calls on the plans themselves never touch it)
         std::vector<pbb_ops *> ops(nplans);
+        std::vector<Workspace> wsp(nplans);
         for (int i = 0; i < nplans; i++) {
             pbb_plan *p = plans[i];
             ops[i] = transpose ? p->ops->adjoint() : p->ops;
             ops[i]->upload(dev);
-            if (mixed && ops[i]->h.rf > 224) fail(PBB_ERR_UNSUPPORTED, "mixed-precision M2L supports ranks up to 224");
-            p->ensure_workspace(ncols, mixed);
-            if (mixed && p->Lv >= 2) {
-                ops[i]->ensure_split(precision, b->stream);
-                if (ops[i]->h.rs <= 224 && ops[i]->h.n3 <= 224) ops[i]->ensure_split3(b->stream);
+            check_mixed_ranks(ops[i], precision);
+            const SplitE *se = nullptr;
+            const pbb_ops::SplitA *sa = nullptr;
+            split_ops(p, ops[i], precision, true, b->stream, se, sa);
+            int first = i;
+            for (int k = 0; k < i; k++)
+                if (plans[k] == p) {
+                    first = k;
+                    break;
+                }
+            if (first < i) {
+                wsp[i] = wsp[first];
+                continue;
             }
+            Workspace tmp;
+            const size_t need = layout_workspace(*p, ncols, mixed, nullptr, tmp);
+            b->blocks.push_back(dalloc<char>(need, "batch workspace"));
+            layout_workspace(*p, ncols, mixed, b->blocks.back(), wsp[i]);
         }
         check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
-        // independent entries run as concurrent graph branches: entries that
-        // share a plan (its workspace) or an output go to the same branch,
-        // in batch order; if that cannot be kept, everything runs in one
+        // independent entries run as concurrent graph branches.  Entries
+        // conflict when they share a plan (its workspace) or when one's input
+        // or output overlaps another's output; conflicting entries (closed
+        // transitively) go to one branch and run there in batch order.
         std::vector<int> branch(nplans, 0);
         int nbr = 1;
         {
-            const int maxbr = std::min(nplans, 4);
-            std::vector<std::pair<const void *, int>> seen;  // plan or output -> branch
-            auto find = [&](const void *k) {
-                for (auto &q : seen)
-                    if (q.first == k) return q.second;
-                return -1;
+            std::vector<int> root(nplans);
+            for (int i = 0; i < nplans; i++) root[i] = i;
+            std::function<int(int)> find = [&](int i) { return root[i] == i ? i : root[i] = find(root[i]); };
+            auto overlap = [&](const void *a, int64_t na, const void *b2, int64_t nb2) {
+                const char *x = (const char *)a, *y = (const char *)b2;
+                return x < y + nb2 * (int64_t)sizeof(double) && y < x + na * (int64_t)sizeof(double);
             };
-            int next = 0;
-            bool ok = true;
-            for (int i = 0; i < nplans && ok; i++) {
-                const int bp = find(plans[i]), bo = find(out_dev[i]);
-                int br = bp >= 0 ? bp : bo;
-                if (bp >= 0 && bo >= 0 && bp != bo) ok = false;
-                if (br < 0) br = next++ % maxbr;
-                branch[i] = br;
-                if (bp < 0) seen.push_back({plans[i], br});
-                if (bo < 0) seen.push_back({out_dev[i], br});
+            for (int i = 0; i < nplans; i++) {
+                const int64_t ni = plans[i]->N * (int64_t)ncols;
+                for (int j = 0; j < i; j++) {
+                    const int64_t nj = plans[j]->N * (int64_t)ncols;
+                    const bool c = plans[i] == plans[j] || overlap(out_dev[i], ni, out_dev[j], nj) ||
+                                   overlap(w_dev[i], ni, out_dev[j], nj) || overlap(w_dev[j], nj, out_dev[i], ni);
+                    if (c) root[find(i)] = find(j);
+                }
             }
-            if (!ok) std::fill(branch.begin(), branch.end(), 0);
-            else nbr = std::min(next, maxbr);
-            nbr = std::max(nbr, 1);
+            const int maxbr = std::min(nplans, 4);
+            std::vector<int> comp_branch(nplans, -1);
+            int next = 0;
+            for (int i = 0; i < nplans; i++) {
+                const int r = find(i);
+                if (comp_branch[r] < 0) comp_branch[r] = next++ % maxbr;
+                branch[i] = comp_branch[r];
+            }
+            nbr = std::max(1, std::min(next, maxbr));
         }
         b->side.resize(nbr, nullptr);
         b->ev.resize(nbr + 1, nullptr);
@@ -1262,7 +1354,7 @@ PBB_API pbb_status pbb_batch_create(pbb_plan *const *plans, int32_t nplans, cons
             for (int i = 0; i < nplans; i++) {
                 hipStream_t s = b->side[branch[i]];
                 if (overwrite) dev::launch_fill_zero(out_dev[i], plans[i]->N * (int64_t)ncols, s);
-                matvec_run(plans[i], ops[i], w_dev[i], ncols, out_dev[i], s, precision);
+                matvec_run(plans[i], ops[i], wsp[i], w_dev[i], ncols, out_dev[i], s, precision, false);
             }
             for (int k = 0; k < nbr; k++) {
                 check(hipEventRecord(b->ev[k], b->side[k]), "join");
@@ -1350,7 +1442,7 @@ PBB_API pbb_status pbb_matvec(pbb_ops *ops, const double *target, const double *
         dout = dalloc<double>((size_t)npoints * ncols, "output");
         check(hipMemcpyAsync(dw, weight, bytes, hipMemcpyHostToDevice, plan->stream), "copy weights");
         check(hipMemsetAsync(dout, 0, bytes, plan->stream), "clear output");
-        matvec_run(plan, plan->ops, dw, ncols, dout, plan->stream, PBB_PREC_FP64);
+        matvec_own(plan, plan->ops, dw, ncols, dout, plan->stream, PBB_PREC_FP64);
         std::vector<double> res((size_t)npoints * ncols);
         check(hipMemcpyAsync(res.data(), dout, bytes, hipMemcpyDeviceToHost, plan->stream), "copy output");
         check(hipStreamSynchronize(plan->stream), "hipStreamSynchronize");

@@ -78,3 +78,25 @@ def test_ops_info():
     assert not ops.homogeneous and ops.nsets == 4  # levels 2..5
     lap = pbb.Operators(pbb.laplacian(), 1.0, 5, 4, 1e-6)
     assert lap.homogeneous and lap.nsets == 1
+
+
+def test_wrappers_reject_host_tensors_before_any_call():
+    # the pointer of a host or non-FP64 tensor never reaches the C ABI
+    import torch
+    from pbbfmm3d_amd.core import Plan, _f64_cuda
+    p = Plan.__new__(Plan)
+    p.npoints = 10
+    p._h = None
+    w = torch.zeros(2, 10, dtype=torch.float64)
+    with pytest.raises(TypeError):
+        p.matvec(w)
+    with pytest.raises(TypeError):
+        p.matvec_transpose(w)
+    with pytest.raises(TypeError):
+        p.grad_positions(w, w)
+    with pytest.raises(TypeError):
+        p.tune_m2l(w)
+    with pytest.raises(TypeError):
+        _f64_cuda(w.float(), "x")
+    with pytest.raises(TypeError):
+        _f64_cuda(np.zeros(3), "x")

@@ -93,3 +93,80 @@ def test_batch_shared_plans_and_outputs(gpu):
         # sum to rounding (a race between branches would lose whole updates)
         err = float(torch.linalg.norm(o_sum - (e_a3 + e_b)) / torch.linalg.norm(e_a3 + e_b))
         assert err <= 1e-13, f"replay {rep}: the shared output is not the sum ({err:.2e})"
+
+
+def test_batch_workspace_survives_eager_calls(gpu):
+    # the batch owns its workspaces: eager calls on its plans with another
+    # precision class and more columns (which regrow the plan's own
+    # workspace) leave the captured graph valid
+    import torch
+    ncols = 4
+    kernel = pbb.exponential(0.5)
+    ops = pbb.Operators(kernel, 1.0, 4, 5, 1e-7)
+    pa = pbb.Plan(ops, clustered_points(7000, seed=90))
+    pb = pbb.Plan(ops, uniform_points(5000, seed=91))
+    g = torch.Generator(device="cuda").manual_seed(92)
+    wa = torch.randn(ncols, 7000, dtype=torch.float64, device="cuda", generator=g)
+    wb = torch.randn(ncols, 5000, dtype=torch.float64, device="cuda", generator=g)
+    oa, ob = torch.zeros_like(wa), torch.zeros_like(wb)
+    batch = pbb.Batch([pa, pb], [wa, wb], [oa, ob], ncols, precision=pbb.capi.PREC_MIXED_F16X2)
+    ea = pa.matvec(wa, precision=pbb.capi.PREC_MIXED_F16X2)
+    eb = pb.matvec(wb, precision=pbb.capi.PREC_MIXED_F16X2)
+    torch.cuda.synchronize()
+    wide = torch.randn(3 * ncols, 7000, dtype=torch.float64, device="cuda", generator=g)
+    for rep in range(2):
+        pa.matvec(wide)                     # FP64, 12 columns: regrows pa's workspace
+        pb.matvec_transpose(wb[:1].contiguous())
+        torch.cuda.synchronize()
+        batch.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(oa, ea) and torch.equal(ob, eb), f"replay {rep}"
+
+
+def test_batch_chained_entries(gpu):
+    # entry 2 reads entry 0's output (A (A w)) through a second plan of the
+    # same points: only the buffers link them, and they must run in order in
+    # one branch, beside an unrelated entry
+    import torch
+    ncols = 2
+    kernel = pbb.exponential(0.5)
+    ops = pbb.Operators(kernel, 1.0, 4, 4, 1e-7)
+    pa = pbb.Plan(ops, uniform_points(6000, seed=93))
+    pa2 = pbb.Plan(ops, uniform_points(6000, seed=93))
+    pc = pbb.Plan(ops, clustered_points(4000, seed=94))
+    g = torch.Generator(device="cuda").manual_seed(95)
+    w = torch.randn(ncols, 6000, dtype=torch.float64, device="cuda", generator=g)
+    wc = torch.randn(ncols, 4000, dtype=torch.float64, device="cuda", generator=g)
+    mid, fin, oc = torch.zeros_like(w), torch.zeros_like(w), torch.zeros_like(wc)
+    batch = pbb.Batch([pa, pc, pa2], [w, wc, mid], [mid, oc, fin], ncols)
+    e_mid = pa.matvec(w)
+    e_fin = pa2.matvec(e_mid)
+    e_c = pc.matvec(wc)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        batch.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(mid, e_mid) and torch.equal(fin, e_fin) and torch.equal(oc, e_c), f"replay {rep}"
+
+
+def test_wrappers_reject_bad_tensors(gpu):
+    import torch
+    ops = pbb.Operators(pbb.laplacian(), 1.0, 3, 4, 1e-5)
+    p = pbb.Plan(ops, uniform_points(3000, seed=96))
+    w = torch.randn(2, 3000, dtype=torch.float64, device="cuda")
+    with pytest.raises(TypeError):
+        p.matvec(w.float())
+    with pytest.raises(ValueError):
+        p.matvec(w, out=torch.zeros(3000, 2, dtype=torch.float64, device="cuda"))
+    with pytest.raises(TypeError):
+        p.matvec_transpose(w, out=torch.zeros(2, 3000, dtype=torch.float32, device="cuda"))
+    with pytest.raises(TypeError):
+        p.grad_positions(w, w.float())
+    with pytest.raises(ValueError):
+        p.grad_positions(w, w, grad=torch.zeros(3, 3000, dtype=torch.float64, device="cuda"))
+    with pytest.raises(TypeError):
+        p.grad_positions(w, w, grad=torch.zeros(3000, 3, dtype=torch.float32, device="cuda"))
+    with pytest.raises(TypeError):
+        pbb.Batch([p], [w.float()], [torch.zeros_like(w)], 2)
+    with pytest.raises(TypeError):
+        pbb.Batch([p], [w], [torch.zeros(2, 3000, dtype=torch.float64)], 2)
