@@ -24,8 +24,20 @@ sys.path.insert(0, ROOT)
 METRIC = "targets/s at N=4M nCols=32, rel-err<=1e-6 vs FP64: mixed MFMA vs FP64 GPU"
 PEAK = {"fp64": ("mfma", 78.6, "TFLOP/s"), "mixed": ("mfma", 2500.0, "TFLOP/s"),
         "f16x2": ("mfma", 2500.0, "TFLOP/s"), "bf16x2": ("mfma", 2500.0, "TFLOP/s")}
-DTYPE = {"fp64": "f64", "mixed": "bf16x3->f32 (M2L), f64 elsewhere",
-         "f16x2": "f16x2->f32 (M2L), f64 elsewhere", "bf16x2": "bf16x2->f32 (M2L), f64 elsewhere"}
+M2L_SPLIT = {"mixed": "bf16x3", "f16x2": "f16x2", "bf16x2": "bf16x2"}
+
+
+def dtype_label(precision, flags):
+    """What each phase computes in.  Mixed modes: M2L on split 16-bit MFMA
+    with FP32 sums, proxies P and M2L results Pf stored FP32, VT / U on
+    three-part BF16 MFMA (FP64 with the cross-product-sum M2L tile), the
+    upward / downward passes and the near field in FP64."""
+    if precision == "fp64":
+        return "f64"
+    split_acc = precision in ("f16x2", "bf16x2") and flags is not None and (flags & 8)
+    vtu = "f64 (P, Pf f32)" if split_acc else "bf16x3->f32 (P, Pf f32)"
+    return (f"m2l {M2L_SPLIT[precision]}->f32{' (cross products summed apart)' if split_acc else ''}; "
+            f"vt/u {vtu}; p2m, l2p, p2p f64")
 
 
 # bytes per M2L launch, averaged over the 6 launches of one f16x2 matvec of the
@@ -79,6 +91,74 @@ def cpu_baseline(args, kernel):
                       f"kernel; oracle matvec wall time {dt:.2f} s"}
 
 
+def phase_names(order):
+    """The library's eight phase slots under the names of what runs in
+    them.  Orders 4..8 run the direct passes: P2M at every level replaces
+    P2M + M2M (the M2M slot stays empty) and the per-level L2P replaces L2L
+    (the near slot then holds P2P, the sum of the levels' L2P and the
+    scatter)."""
+    if 4 <= order <= 8:
+        return {"p2m": "p2m_all_levels(=p2m+m2m)", "m2m": None, "vt_proxy": "vt_proxy", "m2l": "m2l",
+                "u_expand": "u_expand", "l2l": "l2p_all_levels(=l2l+l2p far)", "near": "p2p+scatter",
+                "total": "total"}
+    return {"p2m": "p2m", "m2m": "m2m", "vt_proxy": "vt_proxy", "m2l": "m2l", "u_expand": "u_expand",
+            "l2l": "l2l", "near": "l2p+p2p+scatter", "total": "total"}
+
+
+def named_phases(phase, order):
+    out = {}
+    for k, v in phase.items():
+        nm = phase_names(order)[k]
+        if nm is not None:
+            out[nm] = round(v, 3)
+    return out
+
+
+def phase_rooflines(phase, info, ops, nc, precision, kernel_id):
+    """Algorithmic bytes or flops of each phase over its measured time, as a
+    fraction of MI355X's peak (HBM 8 TB/s, FP64 78.6 TF/s): HBM-bound
+    expansion passes count the bytes they must move (S / P / Pf / F once,
+    points and weights once per level), P2P counts (c_K + 2 nCols) flops per
+    ordered pair (c_K = 9 for 1/r, 11 for the Gaussian)."""
+    n3, rf, rs = ops.n3, ops.rank_f, ops.rank_s
+    L = info["tree_level"]
+    N = info["npoints"]
+    B = sum(info["boxes"][2:L + 1])
+    pbytes = 8 if precision == "fp64" else 4  # P and Pf: FP64, or FP32 in the mixed modes
+    kpad = rs if precision == "fp64" else 32 * ((max(rf, rs) + 31) // 32)
+    S = B * n3 * nc * 8
+    nlev = L - 1
+    hbm = 8.0e12
+    ck = 11 if kernel_id == 3 else 9
+    res = {}
+    t = phase["p2m"] * 1e-3
+    if t > 0:
+        by = S + nlev * N * (24 + 8 * nc)
+        res["p2m_all_levels"] = {"bound": "hbm", "bytes": by, "GB/s": round(by / t / 1e9, 1),
+                                 "frac": round(by / t / hbm, 3)}
+    t = phase["vt_proxy"] * 1e-3
+    if t > 0:
+        by = S + B * nc * kpad * pbytes
+        res["vt_proxy"] = {"bound": "hbm", "bytes": by, "GB/s": round(by / t / 1e9, 1),
+                           "frac": round(by / t / hbm, 3)}
+    t = phase["u_expand"] * 1e-3
+    if t > 0:
+        by = S + B * nc * rf * pbytes
+        res["u_expand"] = {"bound": "hbm", "bytes": by, "GB/s": round(by / t / 1e9, 1),
+                           "frac": round(by / t / hbm, 3)}
+    t = phase["l2l"] * 1e-3
+    if t > 0:
+        by = S + nlev * N * 24 + (2 * (nlev - 1) + 1) * N * nc * 8
+        res["l2p_all_levels"] = {"bound": "hbm", "bytes": by, "GB/s": round(by / t / 1e9, 1),
+                                 "frac": round(by / t / hbm, 3)}
+    t = phase["near"] * 1e-3
+    if t > 0:
+        fl = info["near_pairs"] * (ck + 2 * nc)
+        res["p2p"] = {"bound": "valu_fp64", "flops": fl, "TFLOP/s": round(fl / t / 1e12, 2),
+                      "frac": round(fl / t / 78.6e12, 3)}
+    return res
+
+
 def kernel_name(args):
     return {"gaussian": "exp(-r^2/0.3^2)", "laplacian": "1/r"}[args.kernel]
 
@@ -106,6 +186,9 @@ def main():
                          "HIP-graph batch (a separate line, not the headline metric)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the built-in M2L tiles instead of running the plan's autotuner first")
+    ap.add_argument("--max-rel-diff", type=float, default=1e-6,
+                    help="accuracy bound of the autotuner in the mixed modes: keep the fastest M2L tile "
+                         "whose matvec differs from the FP64 matvec by at most this much (0: speed only)")
     ap.add_argument("--m2l-split-acc", action="store_true",
                     help="two-part modes: run the M2L with the split's cross products summed apart and "
                          "VT / U in FP64 (64-column tiles; closer to the FP64 path, slower)")
@@ -166,7 +249,8 @@ def main():
     t0 = time.perf_counter()
     widths = None
     if not args.no_tune:
-        widths = {"m2l_tile": plan.tune_m2l(w, prec)}
+        widths = {"m2l_tile": plan.tune_m2l(w, prec, args.max_rel_diff if prec != capi.PREC_FP64 else 0.0),
+                  "max_rel_diff": args.max_rel_diff if prec != capi.PREC_FP64 else None}
         if prec != capi.PREC_FP64 and args.fp64_steps > 0:
             widths["m2l_tile_fp64"] = plan.tune_m2l(w, capi.PREC_FP64)
         widths["tune_s"] = round(time.perf_counter() - t0, 3)
@@ -204,7 +288,8 @@ def main():
     if prec != capi.PREC_FP64 and args.fp64_steps > 0:
         t64, ph64 = timed(capi.PREC_FP64, 1, args.fp64_steps, "fp64")
         out64 = one(capi.PREC_FP64)
-        fp64 = {"value": N / t64, "ms_per_step": 1e3 * t64, "phase_ms": {k: round(v, 3) for k, v in ph64.items()}}
+        fp64 = {"value": N / t64, "ms_per_step": 1e3 * t64, "phase_ms": named_phases(ph64, args.order),
+                "phase_rooflines": phase_rooflines(ph64, info, ops, nc, "fp64", kernel.id)}
     t_step, phase = timed(prec, args.warmup, args.steps, args.precision)
     ms_per_step = 1e3 * t_step
     value = N / t_step
@@ -263,7 +348,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "targets/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": DTYPE[args.precision],
+        "dtype": dtype_label(args.precision, (widths or {}).get("m2l_tile", {}).get("flags")),
         "data": "synthetic",
         "config": {"workload": f"{N} clustered points (8 Gaussians sigma 0.05 + 20% uniform), "
                                f"{kernel_name(args)}, {nc} columns, level {args.level}, order "
@@ -274,7 +359,8 @@ def main():
         "cpu_baseline": cpu,
         "rel_err_vs_direct": rel_err,
         "fp64_path": fp64,
-        "phase_ms": {k: round(v, 3) for k, v in phase.items()},
+        "phase_ms": named_phases(phase, args.order),
+        "phase_rooflines": phase_rooflines(phase, info, ops, nc, args.precision, kernel.id),
         "ranks": [ops.rank_f, ops.rank_s],
         "m2l_pairs": info["m2l_pairs"], "near_pairs": info["near_pairs"],
         "precompute_s": round(t_pre, 3), "plan_s": round(t_plan, 3),
@@ -340,7 +426,8 @@ def run_c5(args, torch, pbb, capi):
     line = {
         "metric": "targets/s, 256-cloud batch (C5), one HIP graph", "value": total / dt, "unit": "targets/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": DTYPE[args.precision],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": dtype_label(args.precision, None),
         "data": "synthetic",
         "config": {"workload": f"{ncl} clouds of U[0,1)^3 points, sizes log-uniform in [2e3, 2e5] "
                                f"({total} points), levels 3..6, orders 4..6, exp(-r/0.5), {nc} columns, "

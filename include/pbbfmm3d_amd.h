@@ -195,6 +195,18 @@ PBB_API pbb_status pbb_plan_create(pbb_ops *ops, const double *xyz,
 
 PBB_API pbb_status pbb_plan_query(const pbb_plan *plan, pbb_plan_info *info);
 
+/* Moves the plan's points without rebuilding the tree: xyz (host AoS, the
+ * plan's N points in the caller's order) replaces the coordinates; box
+ * membership, the sorted order and the bbox centre subtracted at creation
+ * stay as built (the reference builds a new tree per H2_3D_Compute, so it
+ * has no counterpart; this is the tree a finite difference or an
+ * optimisation step with small moves needs).  Every point must stay in its
+ * leaf box, else PBB_ERR_INVALID_ARG and the plan is unchanged.  The
+ * copies run on `stream` after the work queued there, and the call returns
+ * once they are done.                                                     */
+PBB_API pbb_status pbb_plan_update_points(pbb_plan *plan, const double *xyz, int64_t npoints,
+                                          void *stream);
+
 /* Replaces FMMCompute (compute.hpp:192-222): upward pass, M2L, downward
  * pass, near field.  w_dev: device FP64 weights, column-major N x ncols;
  * out_dev: device FP64, column-major N x ncols, accumulated (+=).
@@ -273,6 +285,17 @@ PBB_API pbb_status pbb_plan_set_m2l_tile(pbb_plan *plan, int32_t precision, int3
 PBB_API pbb_status pbb_plan_tune_m2l(pbb_plan *plan, const double *w_dev, int32_t ncols,
                                      int32_t precision, void *stream, int32_t *best);
 
+/* The autotuner with an accuracy bound (mixed modes): runs the FP64 matvec
+ * on w_dev once, and keeps the fastest candidate whose matvec differs from
+ * it by at most max_rel_diff (||y - y64|| / ||y64||); when no candidate
+ * does, it keeps the most accurate one.  *achieved (may be NULL) receives
+ * the kept candidate's difference, so the caller can tell the two cases
+ * apart.  max_rel_diff 0 (or the FP64 mode) tunes for speed alone.  No
+ * reference counterpart.                                                  */
+PBB_API pbb_status pbb_plan_tune_m2l_within(pbb_plan *plan, const double *w_dev, int32_t ncols,
+                                            int32_t precision, double max_rel_diff, void *stream,
+                                            int32_t *best, double *achieved);
+
 PBB_API void pbb_plan_destroy(pbb_plan *plan);
 
 /* ---- batches of clouds (one HIP graph) -------------------------------- */
@@ -297,6 +320,24 @@ PBB_API pbb_status pbb_batch_create(pbb_plan *const *plans, int32_t nplans,
                                     const double *const *w_dev, double *const *out_dev,
                                     int32_t ncols, int32_t precision, int32_t flags,
                                     pbb_batch **out);
+/* One entry of a batch with flags of its own: out_dev (+)= A w_dev, or the
+ * transposed FMM with PBB_BATCH_TRANSPOSE (e.g. a forward matvec and an
+ * adjoint of the same plan in one graph launch).                         */
+typedef struct pbb_batch_entry {
+    pbb_plan *plan;
+    const double *w_dev;   /* device FP64, column-major N x ncols          */
+    double *out_dev;       /* device FP64, column-major N x ncols          */
+    int32_t flags;         /* PBB_BATCH_TRANSPOSE | PBB_BATCH_OVERWRITE    */
+    int32_t reserved;      /* 0                                            */
+} pbb_batch_entry;
+
+/* pbb_batch_create over entries that carry their own flags; same rules
+ * (workspaces, branches, conflicts: an entry and the transposed entry of
+ * the same plan share its workspace, so they run in one branch in batch
+ * order).  No reference counterpart.                                     */
+PBB_API pbb_status pbb_batch_create_entries(const pbb_batch_entry *entries, int32_t nentries,
+                                            int32_t ncols, int32_t precision, pbb_batch **out);
+
 /* Replays the graph on stream (NULL = the null stream). */
 PBB_API pbb_status pbb_batch_launch(pbb_batch *batch, void *stream);
 PBB_API void pbb_batch_destroy(pbb_batch *batch);

@@ -55,7 +55,7 @@ def lib():
         L.pbbo_direct.restype = None
         L.pbbo_matvec_flat.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                        C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp,
-                                       C.c_int, _dp, C.c_int, _dp]
+                                       C.c_int, _dp, C.c_int, _dp, C.c_void_p]
         L.pbbo_matvec_flat.restype = C.c_int
         L.pbbo_num_threads.restype = C.c_int
         _lib = L
@@ -183,21 +183,31 @@ def precompute(kid, params, symmetry, homogen, L, levels, n, eps) -> Ops:
 
 
 # ---- matvec / direct sum ------------------------------------------------------
-def matvec(ops: Ops, xyz, w, ncols, out=None):
+def matvec(ops: Ops, xyz, w, ncols, out=None, targets=None):
     """H2_3D_Compute (compute.hpp:67-149): xyz (N, 3), w column-major
-    (N*ncols,), returns out += A w (column-major, N*ncols)."""
+    (N*ncols,), returns out += A w (column-major, N*ncols).
+
+    targets (optional indices): form only these points' outputs (the other
+    entries of out are left as they are); the far field and the near field
+    then run only for the boxes that hold them, which makes the oracle
+    affordable at a million points."""
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
     N = xyz.shape[0]
     w = np.ascontiguousarray(w, dtype=np.float64).reshape(-1)
     assert w.size == N * ncols
     if out is None:
         out = np.zeros(N * ncols)
+    mask = None
+    if targets is not None:
+        mask = np.zeros(N, dtype=np.uint8)
+        mask[np.asarray(targets, dtype=np.int64)] = 1
     U = np.ascontiguousarray(ops.U); VT = np.ascontiguousarray(ops.VT)
     E = np.ascontiguousarray(ops.E)
     p = _params(ops.params)
     rc = lib().pbbo_matvec_flat(ops.n, ops.rf, ops.rs, ops.nsets, int(ops.homog),
                                 -float(ops.homogen), float(ops.L), ops.levels, ops.kid, _p(p),
-                                _p(U), _p(VT), _p(E), _p(xyz), N, _p(w), ncols, _p(out))
+                                _p(U), _p(VT), _p(E), _p(xyz), N, _p(w), ncols, _p(out),
+                                None if mask is None else mask.ctypes.data_as(C.c_void_p))
     if rc != 0:
         raise RuntimeError(f"oracle matvec failed ({rc})")
     return out

@@ -30,6 +30,9 @@
  *                                                 brute-force near field on
  *                                                 tiny inputs
  *   pbbo_direct                                   brute force (numpy)
+ *   pbbo_matvec with a target mask                equal, on the masked
+ *                                                 targets, to the full
+ *                                                 pbbo_matvec (tests)
  */
 #include <math.h>
 #include <stdint.h>
@@ -84,7 +87,7 @@ double pbbo_kernel_eval(int id, const double *p, double tx, double ty, double tz
     }
 }
 
-/* isinf -> 0 as compute.hpp:767-768 and test.hpp:306-307 */
+/* isinf -> 0 as compute.hpp:767-768 and test.hpp:29-30 */
 static inline double kernel_p2p(int id, const double *p, double tx, double ty,
                                 double tz, double sx, double sy, double sz)
 {
@@ -553,9 +556,19 @@ static inline int ops_level(const pbbo_ops *o, int l)
  * xyz: AoS N x 3; w: column-major N x nCols; out += (compute.hpp:747).
  * Levels 0 and 1 carry no far field (every level-1 box is a neighbour of
  * every other), so their expansions are not formed.
+ *
+ * tmask (optional, N bytes): when given, only the outputs of the points with
+ * tmask[i] != 0 are formed (other entries of out are left as they are).  The
+ * upward pass and the proxies stay complete; the far field and the downward
+ * pass run only for the boxes holding a masked point and their ancestors,
+ * and those leaves take their near field from all 27 neighbours directly
+ * (K(x_i, x_j) w_j for every neighbour point j) instead of sharing each
+ * pair with the partner leaf as compute.hpp:680-725 does.  Same terms, so
+ * the masked outputs equal the full ones to rounding; it only makes the
+ * oracle affordable at a million points.
  */
 int pbbo_matvec(const pbbo_ops *o, const double *xyz_in, int N, const double *w,
-                int nCols, double *out)
+                int nCols, double *out, const unsigned char *tmask)
 {
     int n = o->n, n2 = n * n, n3 = n2 * n, Lv = o->levels;
     int rf = o->rf, rs = o->rs;
@@ -588,6 +601,24 @@ int pbbo_matvec(const pbbo_ops *o, const double *xyz_in, int N, const double *w,
     otree_geometry(&T, o->L);
     otree_distribute(&T, x, N);
     otree_interactions(&T);
+
+    /* boxes whose outputs are formed: leaves holding a masked point and
+     * their ancestors (every box when there is no mask) */
+    unsigned char *mark[16] = {0};
+    if (tmask) {
+        for (int l = 0; l <= Lv; l++) mark[l] = calloc((size_t)pow8(l), 1);
+        for (int64_t p = 0; p < pow8(Lv); p++) {
+            onode_data *Ad = ND(&T, Lv, p);
+            if (!Ad) continue;
+            for (int j = 0; j < T.nodes[Lv][p].Nf; j++)
+                if (tmask[Ad->list[j]]) mark[Lv][p] = 1;
+        }
+        for (int l = Lv - 1; l >= 0; l--)
+            for (int64_t p = 0; p < pow8(l); p++)
+                for (int i = 0; i < 8; i++)
+                    if (mark[l + 1][8 * p + i]) mark[l][p] = 1;
+    }
+#define MARKED(l, p) (!tmask || mark[l][p])
 
     /* ---- upward pass: P2M at leaves (compute.hpp:287-333) ---- */
     int64_t nleafall = pow8(Lv);
@@ -681,7 +712,7 @@ int pbbo_matvec(const pbbo_ops *o, const double *xyz_in, int N, const double *w,
 #pragma omp parallel for schedule(dynamic)
         for (int64_t p = 0; p < np; p++) {
             onode *A = &T.nodes[l][p];
-            if (A->Nf == 0) continue;
+            if (A->Nf == 0 || !MARKED(l, p)) continue;
             onode_data *Ad = ND(&T, l, p);
             double iL = 1.0 / A->len;
             double scale = pow(iL, o->homogen);
@@ -722,12 +753,12 @@ int pbbo_matvec(const pbbo_ops *o, const double *xyz_in, int N, const double *w,
 #pragma omp parallel for schedule(dynamic)
         for (int64_t p = 0; p < np; p++) {
             onode *A = &T.nodes[l][p];
-            if (A->Nf == 0) continue;
+            if (A->Nf == 0 || !MARKED(l, p)) continue;
             onode_data *Ad = ND(&T, l, p);
             double *tmp = malloc(sizeof(double) * n3 * nCols * 2);
             for (int i = 0; i < 8; i++) {
                 onode_data *Cd = ND(&T, l + 1, 8 * p + i);
-                if (!Cd) continue;
+                if (!Cd || !MARKED(l + 1, 8 * p + i)) continue;
                 l2l(n, nCols, i, Cw, Ad->F, Cd->F, tmp, tmp + n3 * nCols);
             }
             free(tmp);
@@ -738,7 +769,7 @@ int pbbo_matvec(const pbbo_ops *o, const double *xyz_in, int N, const double *w,
 #pragma omp parallel for schedule(dynamic)
     for (int64_t p = 0; p < nleafall; p++) {
         onode *A = &T.nodes[Lv][p];
-        if (A->Nf == 0) continue;
+        if (A->Nf == 0 || !MARKED(Lv, p)) continue;
         onode_data *Ad = ND(&T, Lv, p);
         int Nf = A->Nf;
         Ad->phi = calloc((size_t)Nf * nCols, sizeof(double));
@@ -779,7 +810,7 @@ int pbbo_matvec(const pbbo_ops *o, const double *xyz_in, int N, const double *w,
 #pragma omp parallel for schedule(dynamic)
     for (int64_t p = 0; p < nleafall; p++) {
         onode *A = &T.nodes[Lv][p];
-        if (A->Nf == 0) continue;
+        if (A->Nf == 0 || !MARKED(Lv, p)) continue;
         onode_data *Ad = ND(&T, Lv, p);
         int Nf = A->Nf;
         double *Aout = calloc((size_t)Nf * nCols, sizeof(double));
@@ -800,6 +831,20 @@ int pbbo_matvec(const pbbo_ops *o, const double *xyz_in, int N, const double *w,
                             Aout[col * Nf + i] += kv * w[(int64_t)col * N + kj];
                             if (i != j) Aout[col * Nf + j] += kv * w[(int64_t)col * N + ki];
                         }
+                    }
+                }
+                continue;
+            }
+            if (tmask) {
+                /* masked run: this leaf takes every neighbour's share itself */
+                for (int j = 0; j < Ns; j++) {
+                    int kj = Bd->list[j];
+                    for (int i = 0; i < Nf; i++) {
+                        int ki = Ad->list[i];
+                        double kv = kernel_p2p(o->kid, o->kp, x[3 * ki], x[3 * ki + 1], x[3 * ki + 2],
+                                               x[3 * kj], x[3 * kj + 1], x[3 * kj + 2]);
+                        for (int col = 0; col < nCols; col++)
+                            Aout[col * Nf + i] += kv * w[(int64_t)col * N + kj];
                     }
                 }
                 continue;
@@ -830,7 +875,7 @@ int pbbo_matvec(const pbbo_ops *o, const double *xyz_in, int N, const double *w,
     /* ---- scatter, compute.hpp:739-749 ---- */
     for (int64_t p = 0; p < nleafall; p++) {
         onode *A = &T.nodes[Lv][p];
-        if (A->Nf == 0) continue;
+        if (A->Nf == 0 || !MARKED(Lv, p)) continue;
         onode_data *Ad = ND(&T, Lv, p);
         for (int col = 0; col < nCols; col++)
             for (int i = 0; i < A->Nf; i++)
@@ -844,7 +889,9 @@ int pbbo_matvec(const pbbo_ops *o, const double *xyz_in, int N, const double *w,
         }
         free(T.data[l]);
         free(T.nodes[l]);
+        free(mark[l]);
     }
+#undef MARKED
     free(x);
     return 0;
 }
@@ -854,14 +901,14 @@ int pbbo_matvec_flat(int n, int rf, int rs, int nops, int homog, double homogen,
                      double L, int levels, int kid, const double *kp,
                      const double *U, const double *VT, const double *E,
                      const double *xyz, int N, const double *w, int nCols,
-                     double *out)
+                     double *out, const unsigned char *tmask)
 {
     pbbo_ops o;
     o.n = n; o.rf = rf; o.rs = rs; o.nops = nops; o.homog = homog;
     o.homogen = homogen; o.L = L; o.levels = levels; o.kid = kid;
     for (int i = 0; i < 4; i++) o.kp[i] = kp ? kp[i] : 0;
     o.U = U; o.VT = VT; o.E = E;
-    return pbbo_matvec(&o, xyz, N, w, nCols, out);
+    return pbbo_matvec(&o, xyz, N, w, nCols, out, tmask);
 }
 
 int pbbo_num_threads(void)

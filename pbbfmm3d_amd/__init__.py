@@ -7,6 +7,7 @@ from . import _capi as capi
 from .core import (Batch, Kernel, Operators, Plan, constant, exponential, gaussian, inverse_quadric,
                    laplacian, logarithm, matvec_host, oneoverr2, quadric, r2_homogeneous,
                    soft_laplacian, thin_plate_spline)
+from .solve import MatrixDirect, MatrixFMM
 
 
 def fmm_matvec(*args, **kwargs):
@@ -15,6 +16,6 @@ def fmm_matvec(*args, **kwargs):
     return _f(*args, **kwargs)
 
 
-__all__ = ["capi", "fmm_matvec", "Batch", "Kernel", "Operators", "Plan", "matvec_host", "laplacian", "oneoverr2",
+__all__ = ["capi", "fmm_matvec", "MatrixFMM", "MatrixDirect", "Batch", "Kernel", "Operators", "Plan", "matvec_host", "laplacian", "oneoverr2",
            "gaussian", "exponential", "soft_laplacian", "quadric", "inverse_quadric", "logarithm",
            "thin_plate_spline", "r2_homogeneous", "constant"]

@@ -80,6 +80,11 @@ class PlanInfo(C.Structure):
                 ("outside_points", C.c_int64)]
 
 
+class BatchEntry(C.Structure):
+    _fields_ = [("plan", C.c_void_p), ("w_dev", C.c_void_p), ("out_dev", C.c_void_p),
+                ("flags", C.c_int32), ("reserved", C.c_int32)]
+
+
 HOST_KERNEL_FN = C.CFUNCTYPE(C.c_double, C.c_void_p, C.POINTER(C.c_double),
                              C.POINTER(C.c_double))
 
@@ -103,17 +108,22 @@ _SIGNATURES = {
     "pbb_ops_destroy": (None, [_vp]),
     "pbb_plan_create": (C.c_int, [_vp, _dp, C.c_int64, C.POINTER(_vp)]),
     "pbb_plan_query": (C.c_int, [_vp, C.POINTER(PlanInfo)]),
+    "pbb_plan_update_points": (C.c_int, [_vp, _dp, C.c_int64, _vp]),
     "pbb_plan_matvec": (C.c_int, [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp]),
     "pbb_plan_matvec_transpose": (C.c_int, [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp]),
     "pbb_plan_grad_positions": (C.c_int, [_vp, _vp, _vp, C.c_int32, _vp, C.c_int32, _vp]),
     "pbb_batch_create": (C.c_int, [C.POINTER(_vp), C.c_int32, C.POINTER(_vp), C.POINTER(_vp), C.c_int32,
                                    C.c_int32, C.c_int32, C.POINTER(_vp)]),
+    "pbb_batch_create_entries": (C.c_int, [C.POINTER(BatchEntry), C.c_int32, C.c_int32, C.c_int32,
+                                           C.POINTER(_vp)]),
     "pbb_batch_launch": (C.c_int, [_vp, _vp]),
     "pbb_batch_destroy": (None, [_vp]),
     "pbb_plan_timings": (C.c_int, [_vp, _dp]),
     "pbb_plan_set_timing": (C.c_int, [_vp, C.c_int32]),
     "pbb_plan_set_m2l_tile": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "pbb_plan_tune_m2l": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, _vp, C.POINTER(C.c_int32)]),
+    "pbb_plan_tune_m2l_within": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, C.c_double, _vp,
+                                           C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "pbb_plan_destroy": (None, [_vp]),
     "pbb_matvec": (C.c_int, [_vp, _dp, _dp, C.c_int64, _dp, C.c_int32, _dp]),
     "pbb_direct_sum": (C.c_int, [C.POINTER(KernelDesc), _vp, C.c_int64, _vp, C.c_int64, _vp,

@@ -6,8 +6,12 @@
   d/dxyz      through pbb_plan_grad_positions (derivatives of the Chebyshev
               interpolants and of the P2P kernel; box membership and the
               bbox centre are held fixed, as they are piecewise constant).
-The tree is built from xyz at every forward call.  Inputs of any floating
-dtype are computed in FP64 and gradients are returned in the input dtype.
+The tree is built from xyz at every forward call, unless a plan is given:
+then its points are moved to xyz (pbb_plan_update_points, which fails if a
+point leaves its leaf box) and its tree is reused.  Weights of any floating
+dtype (FP32 included) are handed to the kernels as FP64, the output has the
+weights' dtype and every gradient the dtype of its input; the arithmetic is
+the precision mode's.
 """
 from __future__ import annotations
 
@@ -19,8 +23,14 @@ from .core import Operators, Plan
 
 class FMMMatvec(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, xyz, weights, ops: Operators, precision: int):
-        plan = Plan(ops, xyz.detach().to(torch.float64).cpu().numpy())
+    def forward(ctx, xyz, weights, ops: Operators, precision: int, plan):
+        pts = xyz.detach().to(torch.float64).cpu().numpy()
+        if plan is None:
+            plan = Plan(ops, pts)
+        else:
+            if plan.ops is not ops:
+                raise ValueError("the plan was built with other operators")
+            plan.update_points(pts)
         w64 = weights.detach().to(device="cuda", dtype=torch.float64).contiguous()
         out = plan.matvec(w64, precision=precision)
         ctx.plan = plan
@@ -39,10 +49,13 @@ class FMMMatvec(torch.autograd.Function):
         if ctx.needs_input_grad[0]:
             dt, dev = ctx.xyz_meta
             gx = ctx.plan.grad_positions(w64, g64, precision=ctx.precision).to(device=dev, dtype=dt)
-        return gx, gw, None, None
+        return gx, gw, None, None, None
 
 
 def fmm_matvec(xyz: torch.Tensor, weights: torch.Tensor, ops: Operators,
-               precision: int = capi.PREC_FP64) -> torch.Tensor:
-    """A~(xyz) @ weights with gradients for both arguments."""
-    return FMMMatvec.apply(xyz, weights, ops, precision)
+               precision: int = capi.PREC_FP64, plan: Plan = None) -> torch.Tensor:
+    """A~(xyz) @ weights with gradients for both arguments.  plan: a plan of
+    ops whose tree is reused (its points are moved to xyz); it holds the
+    points of its last forward, so run a backward before the next forward
+    with the same plan."""
+    return FMMMatvec.apply(xyz, weights, ops, precision, plan)

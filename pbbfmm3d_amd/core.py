@@ -262,6 +262,18 @@ class Plan:
                 "max_leaf_points": I.max_leaf_points, "center": tuple(I.center),
                 "outside_points": I.outside_points}
 
+    def update_points(self, xyz, stream: Optional[int] = None) -> None:
+        """Replaces the coordinates (N x 3, caller's order) keeping the tree
+        as built; raises PBBError if a point leaves its leaf box."""
+        xyz = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
+        if xyz.shape[0] != self.npoints:
+            raise ValueError(f"xyz must hold {self.npoints} points")
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream if torch.cuda.is_available() else 0
+        capi.check(capi.lib().pbb_plan_update_points(self._h, xyz.ctypes.data_as(C.POINTER(C.c_double)),
+                                                     self.npoints, C.c_void_p(stream)))
+
     def matvec_ptr(self, w_ptr: int, ncols: int, out_ptr: int, precision: int = capi.PREC_FP64,
                    stream: int = 0) -> None:
         """Enqueues out += A w on raw device pointers (FP64, column-major)."""
@@ -345,17 +357,22 @@ class Plan:
         capi.check(capi.lib().pbb_plan_set_m2l_tile(self._h, int(precision), int(width), int(cols_per_box),
                                                     int(flags)))
 
-    def tune_m2l(self, weights, precision: int = capi.PREC_FP64) -> dict:
+    def tune_m2l(self, weights, precision: int = capi.PREC_FP64, max_rel_diff: float = 0.0) -> dict:
         """Times the candidate M2L tiles on these weights (float64 CUDA
         tensor, (ncols, N) or (N,)), keeps the fastest and returns it as
-        {"width", "cols_per_box", "flags"}."""
+        {"width", "cols_per_box", "flags", "rel_diff"}.  max_rel_diff > 0
+        (mixed modes) keeps only candidates whose result differs from the
+        FP64 matvec by at most that much (the most accurate one if none
+        does); rel_diff is the kept candidate's difference."""
         import torch
         w, _, ncols = self._io(weights, None)
         best = (C.c_int32 * 3)()
+        got = C.c_double(0.0)
         stream = torch.cuda.current_stream(w.device).cuda_stream
-        capi.check(capi.lib().pbb_plan_tune_m2l(self._h, C.c_void_p(w.data_ptr()), int(ncols), int(precision),
-                                                C.c_void_p(stream), best))
-        return {"width": best[0], "cols_per_box": best[1], "flags": best[2]}
+        capi.check(capi.lib().pbb_plan_tune_m2l_within(
+            self._h, C.c_void_p(w.data_ptr()), int(ncols), int(precision), float(max_rel_diff),
+            C.c_void_p(stream), best, C.byref(got)))
+        return {"width": best[0], "cols_per_box": best[1], "flags": best[2], "rel_diff": got.value}
 
     def timings(self) -> dict:
         ms = np.zeros(capi.NUM_PHASES)
@@ -372,25 +389,50 @@ class Plan:
 class Batch:
     """Many plans' matvecs (forward or transposed) captured into one HIP
     graph.  weights[i] / outputs[i] are float64 CUDA tensors (ncols, N_i)
-    bound at capture: refill them in place between launches."""
+    bound at capture: refill them in place between launches.  transpose is
+    one flag for every entry, or a sequence with one flag per entry (a
+    forward matvec and an adjoint of one plan can share a launch)."""
 
     def __init__(self, plans, weights, outputs, ncols: int, precision: int = capi.PREC_FP64,
-                 transpose: bool = False, overwrite: bool = True):
+                 transpose=False, overwrite: bool = True):
         if not (len(plans) == len(weights) == len(outputs)) or not plans:
             raise ValueError("plans, weights and outputs must have the same non-zero length")
+        n = len(plans)
+        tr = [bool(transpose)] * n if isinstance(transpose, (bool, int)) else [bool(t) for t in transpose]
+        if len(tr) != n:
+            raise ValueError("transpose must be one flag or one flag per entry")
         for i, (p, w, o) in enumerate(zip(plans, weights, outputs)):
             _f64_cuda(w, f"weights[{i}]", numel=ncols * p.npoints)
             _f64_cuda(o, f"outputs[{i}]", numel=ncols * p.npoints)
         self.plans, self.weights, self.outputs = list(plans), list(weights), list(outputs)
-        n = len(plans)
-        P = (C.c_void_p * n)(*[p._h.value for p in plans])
-        W = (C.c_void_p * n)(*[w.data_ptr() for w in weights])
-        O = (C.c_void_p * n)(*[o.data_ptr() for o in outputs])
-        flags = (capi.BATCH_TRANSPOSE if transpose else 0) | (capi.BATCH_OVERWRITE if overwrite else 0)
+        entries = (capi.BatchEntry * n)()
+        for i, (p, w, o) in enumerate(zip(plans, weights, outputs)):
+            entries[i].plan = p._h.value
+            entries[i].w_dev = w.data_ptr()
+            entries[i].out_dev = o.data_ptr()
+            entries[i].flags = (capi.BATCH_TRANSPOSE if tr[i] else 0) | \
+                (capi.BATCH_OVERWRITE if overwrite else 0)
         h = C.c_void_p()
-        capi.check(capi.lib().pbb_batch_create(P, n, W, O, int(ncols), int(precision), flags,
-                                               C.byref(h)))
+        capi.check(capi.lib().pbb_batch_create_entries(entries, n, int(ncols), int(precision),
+                                                       C.byref(h)))
         self._h = h
+
+    @staticmethod
+    def tune(plans, weights, precision: int = capi.PREC_FP64, max_rel_diff: float = 0.0) -> dict:
+        """M2L tiles per (operators, columns): the autotuner runs on the
+        largest plan of each operator set and its tile is set on every plan
+        of that set.  Returns {operators handle: tile}."""
+        groups = {}
+        for p, w in zip(plans, weights):
+            groups.setdefault(p.ops.handle.value, []).append((p, w))
+        tiles = {}
+        for key, members in groups.items():
+            p0, w0 = max(members, key=lambda pw: pw[0].npoints)
+            t = p0.tune_m2l(w0, precision, max_rel_diff)
+            for p, _ in members:
+                p.set_m2l_tile(t["width"], t["cols_per_box"], t["flags"], precision)
+            tiles[key] = t
+        return tiles
 
     def launch(self, stream: Optional[int] = None):
         import torch

@@ -288,6 +288,11 @@ struct pbb_plan {
     char *ws_block = nullptr;
     size_t ws_bytes = 0;
     double *gF = nullptr, *gU = nullptr;  // position gradients: saved leaf F, sorted weights
+    // host copies for pbb_plan_update_points: sorted position -> input
+    // index, the leaf coordinates of every sorted point, per-level centres
+    std::vector<int32_t> h_perm;
+    std::vector<uint32_t> h_leaf;
+    std::vector<Level> h_geom;
     int g_ncols = 0;
     bool timing = false;
     struct M2LTile {
@@ -710,6 +715,16 @@ PBB_API pbb_status pbb_plan_create(pbb_ops *ops, const double *xyz, int64_t npoi
             D.seg_p0 = dupload(s0, "L2P segments");
             D.nseg = (int)sb.size();
         }
+        plan->h_perm = T.perm;
+        plan->h_leaf.resize(T.N);
+        for (int64_t b = 0; b < T.lv[T.Lv].nb; b++)
+            for (int32_t q = T.leaf_start[b]; q < T.leaf_start[b + 1]; q++)
+                plan->h_leaf[q] = (uint32_t)T.lv[T.Lv].coord[b];
+        plan->h_geom.resize(T.Lv + 1);
+        for (int l = 0; l <= T.Lv; l++) {
+            plan->h_geom[l].ctab = T.lv[l].ctab;
+            plan->h_geom[l].len = T.lv[l].len;
+        }
         plan->up_items = dupload(up_items, "upward work items");
         plan->up_reds = dupload(up_reds, "upward reductions");
         plan->n_up_items = (int)up_items.size();
@@ -732,6 +747,38 @@ PBB_API pbb_status pbb_plan_create(pbb_ops *ops, const double *xyz, int64_t npoi
         for (int d = 0; d < 3; d++) I.center[d] = T.ctr[d];
         I.outside_points = T.outside;
         *out = plan.release();
+    });
+}
+
+PBB_API pbb_status pbb_plan_update_points(pbb_plan *plan, const double *xyz, int64_t npoints, void *stream)
+{
+    return guarded([&] {
+        if (!plan || !xyz) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        if (npoints != plan->N) fail(PBB_ERR_INVALID_ARG, "npoints must be the plan's point count");
+        const int64_t N = plan->N;
+        const double *c = plan->info.center;
+        std::vector<double> xs(N), ys(N), zs(N);
+        std::vector<char> moved_out(std::max<int64_t>(1, (N + 65535) / 65536), 0);
+        parallel_for((N + 65535) / 65536, [&](int64_t blk) {
+            for (int64_t k = blk * 65536, e = std::min(N, k + 65536); k < e; k++) {
+                const int64_t i = plan->h_perm[k];
+                const double p[3] = {xyz[3 * i] - c[0], xyz[3 * i + 1] - c[1], xyz[3 * i + 2] - c[2]};
+                if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2]) ||
+                    leaf_coord(plan->h_geom, plan->Lv, p) != plan->h_leaf[k])
+                    moved_out[blk] = 1;
+                xs[k] = p[0];
+                ys[k] = p[1];
+                zs[k] = p[2];
+            }
+        });
+        for (char m : moved_out)
+            if (m) fail(PBB_ERR_INVALID_ARG, "a point left its leaf box (or is not finite): build a new plan");
+        DeviceGuard g(plan->device);
+        hipStream_t st = (hipStream_t)stream;
+        check(hipMemcpyAsync(plan->x, xs.data(), N * sizeof(double), hipMemcpyHostToDevice, st), "upload x");
+        check(hipMemcpyAsync(plan->y, ys.data(), N * sizeof(double), hipMemcpyHostToDevice, st), "upload y");
+        check(hipMemcpyAsync(plan->z, zs.data(), N * sizeof(double), hipMemcpyHostToDevice, st), "upload z");
+        check(hipStreamSynchronize(st), "hipStreamSynchronize");
     });
 }
 
@@ -816,57 +863,118 @@ static void matvec_own(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
     matvec_run(p, o, p->W, w, ncols, out, st, prec, true);
 }
 
+// Relative difference ||a - b|| / ||b|| of two device FP64 arrays of n
+// values, reduced on the device (part: 2 * 1024 doubles) and summed here.
+static double device_rel_diff(const double *a, const double *b, int64_t n, double *part, hipStream_t st)
+{
+    constexpr int nblk = 1024;
+    dev::launch_diff_norms(a, b, n, part, nblk, st);
+    double h[2 * nblk];
+    check(hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, st), "copy norms");
+    check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    double d2 = 0, b2 = 0;
+    for (int i = 0; i < nblk; i++) {
+        d2 += h[2 * i];
+        b2 += h[2 * i + 1];
+    }
+    return b2 > 0 ? std::sqrt(d2 / b2) : (d2 > 0 ? INFINITY : 0.0);
+}
+
+// The M2L tile autotuner.  max_rel_diff > 0 (mixed modes): a candidate is
+// kept only if its matvec differs from the FP64 matvec on the same weights
+// by at most max_rel_diff (relative 2-norm); when none does, the most
+// accurate one is kept.  achieved: that candidate's difference (0 without
+// a bound).
+static void tune_m2l(pbb_plan *plan, const double *w_dev, int ncols, int precision, hipStream_t st,
+                     double max_rel_diff, int32_t *best, double *achieved)
+{
+    if (!plan || !w_dev) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+    if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
+    if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
+        fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
+    if (!(max_rel_diff >= 0) || !std::isfinite(max_rel_diff))
+        fail(PBB_ERR_INVALID_ARG, "max_rel_diff must be finite and >= 0");
+    const bool mixed = precision != PBB_PREC_FP64;
+    const bool bounded = mixed && max_rel_diff > 0;
+    check_mixed_ranks(plan->ops, precision);
+    DeviceGuard g(plan->device);
+    const bool was_timing = plan->timing;
+    check(pbb_plan_set_timing(plan, 1) == PBB_OK ? hipSuccess : hipErrorUnknown, "enable timing");
+    const int64_t nout = plan->N * (int64_t)ncols;
+    double *scratch = nullptr, *ref = nullptr, *part = nullptr;
+    // candidates {width, columns per box, flags}
+    static const int c64[][3] = {{32, 32, 0}, {64, 32, 0}, {128, 32, 0}, {64, 32, 1}, {128, 32, 1}};
+    static const int cmx[][3] = {{128, 16, 5}, {128, 32, 5}, {128, 8, 5}, {256, 8, 5}, {128, 16, 4},
+                                 {64, 16, 5}, {128, 32, 1}, {64, 16, 0}, {128, 16, 5 | PBB_M2L_DIR_GROUPS(4)},
+                                 {64, 16, 13}, {64, 8, 13}, {64, 32, 13}};
+    const int(*cands)[3] = mixed ? cmx : c64;
+    const int ncand = mixed ? (int)(sizeof(cmx) / sizeof(cmx[0])) : (int)(sizeof(c64) / sizeof(c64[0]));
+    pbb_plan::M2LTile bestt{}, closest{};
+    double best_ms = 0, best_err = 0, closest_err = INFINITY;
+    try {
+        scratch = dalloc<double>((size_t)nout, "tuning output");
+        if (bounded) {
+            ref = dalloc<double>((size_t)nout, "tuning reference");
+            part = dalloc<double>(2 * 1024, "tuning norms");
+            dev::launch_fill_zero(ref, nout, st);
+            matvec_own(plan, plan->ops, w_dev, ncols, ref, st, PBB_PREC_FP64);
+        }
+        // one untimed matvec builds the operator splits and workspaces
+        matvec_own(plan, plan->ops, w_dev, ncols, scratch, st, precision);
+        for (int c = 0; c < ncand; c++) {
+            plan->m2l_tile[mixed ? 1 : 0] = {cands[c][0], std::min(cands[c][1], ncols), cands[c][2]};
+            if (bounded) dev::launch_fill_zero(scratch, nout, st);
+            matvec_own(plan, plan->ops, w_dev, ncols, scratch, st, precision);
+            check(hipStreamSynchronize(st), "hipStreamSynchronize");
+            float t = 0;
+            check(hipEventElapsedTime(&t, plan->ev[3], plan->ev[4]), "hipEventElapsedTime");
+            const double err = bounded ? device_rel_diff(scratch, ref, nout, part, st) : 0.0;
+            if (err < closest_err) {
+                closest = plan->m2l_tile[mixed ? 1 : 0];
+                closest_err = err;
+            }
+            if (bounded && !(err <= max_rel_diff)) continue;
+            if (bestt.width == 0 || t < best_ms) {
+                bestt = plan->m2l_tile[mixed ? 1 : 0];
+                best_ms = t;
+                best_err = err;
+            }
+        }
+    } catch (...) {
+        dfree(scratch);
+        dfree(ref);
+        dfree(part);
+        plan->timing = was_timing;
+        throw;
+    }
+    dfree(scratch);
+    dfree(ref);
+    dfree(part);
+    plan->timing = was_timing;
+    if (bestt.width == 0) {  // nothing within the bound: the most accurate
+        bestt = closest;
+        best_err = closest_err;
+    }
+    plan->m2l_tile[mixed ? 1 : 0] = bestt;
+    if (best) {
+        best[0] = bestt.width;
+        best[1] = bestt.cpt;
+        best[2] = bestt.flags;
+    }
+    if (achieved) *achieved = best_err;
+}
+
 PBB_API pbb_status pbb_plan_tune_m2l(pbb_plan *plan, const double *w_dev, int32_t ncols, int32_t precision,
                                      void *stream, int32_t *best)
 {
-    return guarded([&] {
-        if (!plan || !w_dev) fail(PBB_ERR_INVALID_ARG, "NULL argument");
-        if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
-        if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
-            fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
-        const bool mixed = precision != PBB_PREC_FP64;
-        check_mixed_ranks(plan->ops, precision);
-        DeviceGuard g(plan->device);
-        hipStream_t st = (hipStream_t)stream;
-        const bool was_timing = plan->timing;
-        check(pbb_plan_set_timing(plan, 1) == PBB_OK ? hipSuccess : hipErrorUnknown, "enable timing");
-        double *scratch = dalloc<double>((size_t)plan->N * ncols, "tuning output");
-        // candidates {width, columns per box, flags}
-        static const int c64[][3] = {{32, 32, 0}, {64, 32, 0}, {128, 32, 0}, {64, 32, 1}, {128, 32, 1}};
-        static const int cmx[][3] = {{128, 16, 5}, {128, 32, 5}, {128, 8, 5}, {256, 8, 5}, {128, 16, 4},
-                                     {64, 16, 5}, {128, 32, 1}, {64, 16, 0}, {128, 16, 5 | PBB_M2L_DIR_GROUPS(4)}};
-        const int(*cands)[3] = mixed ? cmx : c64;
-        const int ncand = mixed ? (int)(sizeof(cmx) / sizeof(cmx[0])) : (int)(sizeof(c64) / sizeof(c64[0]));
-        pbb_plan::M2LTile bestt{};
-        double best_ms = 0;
-        try {
-            // one untimed matvec builds the operator splits and workspaces
-            matvec_own(plan, plan->ops, w_dev, ncols, scratch, st, precision);
-            for (int c = 0; c < ncand; c++) {
-                plan->m2l_tile[mixed ? 1 : 0] = {cands[c][0], std::min(cands[c][1], ncols), cands[c][2]};
-                matvec_own(plan, plan->ops, w_dev, ncols, scratch, st, precision);
-                check(hipStreamSynchronize(st), "hipStreamSynchronize");
-                float t = 0;
-                check(hipEventElapsedTime(&t, plan->ev[3], plan->ev[4]), "hipEventElapsedTime");
-                if (bestt.width == 0 || t < best_ms) {
-                    bestt = plan->m2l_tile[mixed ? 1 : 0];
-                    best_ms = t;
-                }
-            }
-        } catch (...) {
-            dfree(scratch);
-            plan->timing = was_timing;
-            throw;
-        }
-        dfree(scratch);
-        plan->timing = was_timing;
-        plan->m2l_tile[mixed ? 1 : 0] = bestt;
-        if (best) {
-            best[0] = bestt.width;
-            best[1] = bestt.cpt;
-            best[2] = bestt.flags;
-        }
-    });
+    return guarded([&] { tune_m2l(plan, w_dev, ncols, precision, (hipStream_t)stream, 0.0, best, nullptr); });
+}
+
+PBB_API pbb_status pbb_plan_tune_m2l_within(pbb_plan *plan, const double *w_dev, int32_t ncols, int32_t precision,
+                                            double max_rel_diff, void *stream, int32_t *best, double *achieved)
+{
+    return guarded(
+        [&] { tune_m2l(plan, w_dev, ncols, precision, (hipStream_t)stream, max_rel_diff, best, achieved); });
 }
 
 // One FMM application.  out == nullptr stops after the downward pass, leaving
@@ -1251,127 +1359,150 @@ struct pbb_batch {
 
 extern "C" {
 
+// Captures the entries' matvecs into one HIP graph (see pbb_batch_create).
+static void batch_create(const pbb_batch_entry *E, int nplans, int ncols, int precision, pbb_batch **out)
+{
+    if (!out) fail(PBB_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (!E || nplans < 1) fail(PBB_ERR_INVALID_ARG, "empty batch");
+    if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
+    if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
+        fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
+    const bool mixed = precision != PBB_PREC_FP64;
+    std::vector<pbb_plan *> plans(nplans);
+    std::vector<const double *> w_dev(nplans);
+    std::vector<double *> out_dev(nplans);
+    for (int i = 0; i < nplans; i++) {
+        plans[i] = E[i].plan;
+        w_dev[i] = E[i].w_dev;
+        out_dev[i] = E[i].out_dev;
+        if (!plans[i] || !w_dev[i] || !out_dev[i]) fail(PBB_ERR_INVALID_ARG, "NULL plan or buffer in batch");
+        if (E[i].flags & ~(PBB_BATCH_TRANSPOSE | PBB_BATCH_OVERWRITE))
+            fail(PBB_ERR_INVALID_ARG, "unknown batch entry flags");
+    }
+    const int dev = plans[0]->device;
+    for (int i = 0; i < nplans; i++) {
+        if (plans[i]->device != dev) fail(PBB_ERR_INVALID_ARG, "batch plans must share one device");
+        if ((E[i].flags & PBB_BATCH_TRANSPOSE) && plans[i]->ops->h.kernel.symmetry != 1)
+            fail(PBB_ERR_UNSUPPORTED, "the transposed near field needs a symmetric kernel");
+    }
+    DeviceGuard g(dev);
+    auto b = std::make_unique<pbb_batch>();
+    b->device = dev;
+    check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+    // everything that allocates or synchronises happens before the capture:
+    // the operator splits, and one workspace per distinct plan owned by the
+    // batch (calls on the plans themselves never touch it).  The forward and
+    // the transposed operator sets of a plan fit one workspace layout.
+    std::vector<pbb_ops *> ops(nplans);
+    std::vector<Workspace> wsp(nplans);
+    for (int i = 0; i < nplans; i++) {
+        pbb_plan *p = plans[i];
+        ops[i] = (E[i].flags & PBB_BATCH_TRANSPOSE) ? p->ops->adjoint() : p->ops;
+        ops[i]->upload(dev);
+        check_mixed_ranks(ops[i], precision);
+        const SplitE *se = nullptr;
+        const pbb_ops::SplitA *sa = nullptr;
+        split_ops(p, ops[i], precision, true, b->stream, se, sa);
+        int first = i;
+        for (int k = 0; k < i; k++)
+            if (plans[k] == p) {
+                first = k;
+                break;
+            }
+        if (first < i) {
+            wsp[i] = wsp[first];
+            continue;
+        }
+        Workspace tmp;
+        const size_t need = layout_workspace(*p, ncols, mixed, nullptr, tmp);
+        b->blocks.push_back(dalloc<char>(need, "batch workspace"));
+        layout_workspace(*p, ncols, mixed, b->blocks.back(), wsp[i]);
+    }
+    check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    // independent entries run as concurrent graph branches.  Entries
+    // conflict when they share a plan (its workspace) or when one's input
+    // or output overlaps another's output; conflicting entries (closed
+    // transitively) go to one branch and run there in batch order.
+    std::vector<int> branch(nplans, 0);
+    int nbr = 1;
+    {
+        std::vector<int> root(nplans);
+        for (int i = 0; i < nplans; i++) root[i] = i;
+        std::function<int(int)> find = [&](int i) { return root[i] == i ? i : root[i] = find(root[i]); };
+        auto overlap = [&](const void *a, int64_t na, const void *b2, int64_t nb2) {
+            const char *x = (const char *)a, *y = (const char *)b2;
+            return x < y + nb2 * (int64_t)sizeof(double) && y < x + na * (int64_t)sizeof(double);
+        };
+        for (int i = 0; i < nplans; i++) {
+            const int64_t ni = plans[i]->N * (int64_t)ncols;
+            for (int j = 0; j < i; j++) {
+                const int64_t nj = plans[j]->N * (int64_t)ncols;
+                const bool c = plans[i] == plans[j] || overlap(out_dev[i], ni, out_dev[j], nj) ||
+                               overlap(w_dev[i], ni, out_dev[j], nj) || overlap(w_dev[j], nj, out_dev[i], ni);
+                if (c) root[find(i)] = find(j);
+            }
+        }
+        const int maxbr = std::min(nplans, 4);
+        std::vector<int> comp_branch(nplans, -1);
+        int next = 0;
+        for (int i = 0; i < nplans; i++) {
+            const int r = find(i);
+            if (comp_branch[r] < 0) comp_branch[r] = next++ % maxbr;
+            branch[i] = comp_branch[r];
+        }
+        nbr = std::max(1, std::min(next, maxbr));
+    }
+    b->side.resize(nbr, nullptr);
+    b->ev.resize(nbr + 1, nullptr);
+    for (int k = 0; k < nbr; k++) check(hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking), "hipStreamCreate");
+    for (auto &e : b->ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    std::vector<bool> timing(nplans);
+    for (int i = 0; i < nplans; i++) timing[i] = plans[i]->timing;
+    for (int i = 0; i < nplans; i++) plans[i]->timing = false;
+    check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+        check(hipEventRecord(b->ev[nbr], b->stream), "fork");
+        for (int k = 0; k < nbr; k++) check(hipStreamWaitEvent(b->side[k], b->ev[nbr], 0), "fork");
+        for (int i = 0; i < nplans; i++) {
+            hipStream_t s = b->side[branch[i]];
+            if (E[i].flags & PBB_BATCH_OVERWRITE) dev::launch_fill_zero(out_dev[i], plans[i]->N * (int64_t)ncols, s);
+            matvec_run(plans[i], ops[i], wsp[i], w_dev[i], ncols, out_dev[i], s, precision, false);
+        }
+        for (int k = 0; k < nbr; k++) {
+            check(hipEventRecord(b->ev[k], b->side[k]), "join");
+            check(hipStreamWaitEvent(b->stream, b->ev[k], 0), "join");
+        }
+    } catch (...) {
+        hipGraph_t junk = nullptr;
+        (void)hipStreamEndCapture(b->stream, &junk);
+        if (junk) (void)hipGraphDestroy(junk);
+        for (int i = 0; i < nplans; i++) plans[i]->timing = timing[i];
+        throw;
+    }
+    check(hipStreamEndCapture(b->stream, &b->graph), "hipStreamEndCapture");
+    for (int i = 0; i < nplans; i++) plans[i]->timing = timing[i];
+    check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    *out = b.release();
+}
+
 PBB_API pbb_status pbb_batch_create(pbb_plan *const *plans, int32_t nplans, const double *const *w_dev,
                                     double *const *out_dev, int32_t ncols, int32_t precision, int32_t flags,
                                     pbb_batch **out)
 {
     return guarded([&] {
-        if (!out) fail(PBB_ERR_INVALID_ARG, "out is NULL");
-        *out = nullptr;
+        if (out) *out = nullptr;
         if (!plans || !w_dev || !out_dev || nplans < 1) fail(PBB_ERR_INVALID_ARG, "empty batch");
-        if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
-        if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
-            fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
-        const bool transpose = (flags & PBB_BATCH_TRANSPOSE) != 0;
-        const bool overwrite = (flags & PBB_BATCH_OVERWRITE) != 0;
-        const bool mixed = precision != PBB_PREC_FP64;
-        const int dev = plans[0]->device;
-        for (int i = 0; i < nplans; i++) {
-            if (!plans[i] || !w_dev[i] || !out_dev[i]) fail(PBB_ERR_INVALID_ARG, "NULL plan or buffer in batch");
-            if (plans[i]->device != dev) fail(PBB_ERR_INVALID_ARG, "batch plans must share one device");
-            if (transpose && plans[i]->ops->h.kernel.symmetry != 1)
-                fail(PBB_ERR_UNSUPPORTED, "the transposed near field needs a symmetric kernel");
-        }
-        DeviceGuard g(dev);
-        auto b = std::make_unique<pbb_batch>();
-        b->device = dev;
-        check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
-        // everything that allocates or synchronises happens before the capture:
-        // the operator splits, and one workspace per distinct plan owned by the
-        // batch (calls on the plans themselves never touch it)
-        std::vector<pbb_ops *> ops(nplans);
-        std::vector<Workspace> wsp(nplans);
-        for (int i = 0; i < nplans; i++) {
-            pbb_plan *p = plans[i];
-            ops[i] = transpose ? p->ops->adjoint() : p->ops;
-            ops[i]->upload(dev);
-            check_mixed_ranks(ops[i], precision);
-            const SplitE *se = nullptr;
-            const pbb_ops::SplitA *sa = nullptr;
-            split_ops(p, ops[i], precision, true, b->stream, se, sa);
-            int first = i;
-            for (int k = 0; k < i; k++)
-                if (plans[k] == p) {
-                    first = k;
-                    break;
-                }
-            if (first < i) {
-                wsp[i] = wsp[first];
-                continue;
-            }
-            Workspace tmp;
-            const size_t need = layout_workspace(*p, ncols, mixed, nullptr, tmp);
-            b->blocks.push_back(dalloc<char>(need, "batch workspace"));
-            layout_workspace(*p, ncols, mixed, b->blocks.back(), wsp[i]);
-        }
-        check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
-        // independent entries run as concurrent graph branches.  Entries
-        // conflict when they share a plan (its workspace) or when one's input
-        // or output overlaps another's output; conflicting entries (closed
-        // transitively) go to one branch and run there in batch order.
-        std::vector<int> branch(nplans, 0);
-        int nbr = 1;
-        {
-            std::vector<int> root(nplans);
-            for (int i = 0; i < nplans; i++) root[i] = i;
-            std::function<int(int)> find = [&](int i) { return root[i] == i ? i : root[i] = find(root[i]); };
-            auto overlap = [&](const void *a, int64_t na, const void *b2, int64_t nb2) {
-                const char *x = (const char *)a, *y = (const char *)b2;
-                return x < y + nb2 * (int64_t)sizeof(double) && y < x + na * (int64_t)sizeof(double);
-            };
-            for (int i = 0; i < nplans; i++) {
-                const int64_t ni = plans[i]->N * (int64_t)ncols;
-                for (int j = 0; j < i; j++) {
-                    const int64_t nj = plans[j]->N * (int64_t)ncols;
-                    const bool c = plans[i] == plans[j] || overlap(out_dev[i], ni, out_dev[j], nj) ||
-                                   overlap(w_dev[i], ni, out_dev[j], nj) || overlap(w_dev[j], nj, out_dev[i], ni);
-                    if (c) root[find(i)] = find(j);
-                }
-            }
-            const int maxbr = std::min(nplans, 4);
-            std::vector<int> comp_branch(nplans, -1);
-            int next = 0;
-            for (int i = 0; i < nplans; i++) {
-                const int r = find(i);
-                if (comp_branch[r] < 0) comp_branch[r] = next++ % maxbr;
-                branch[i] = comp_branch[r];
-            }
-            nbr = std::max(1, std::min(next, maxbr));
-        }
-        b->side.resize(nbr, nullptr);
-        b->ev.resize(nbr + 1, nullptr);
-        for (int k = 0; k < nbr; k++) check(hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking), "hipStreamCreate");
-        for (auto &e : b->ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-        std::vector<bool> timing(nplans);
-        for (int i = 0; i < nplans; i++) {
-            timing[i] = plans[i]->timing;
-            plans[i]->timing = false;
-        }
-        check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        try {
-            check(hipEventRecord(b->ev[nbr], b->stream), "fork");
-            for (int k = 0; k < nbr; k++) check(hipStreamWaitEvent(b->side[k], b->ev[nbr], 0), "fork");
-            for (int i = 0; i < nplans; i++) {
-                hipStream_t s = b->side[branch[i]];
-                if (overwrite) dev::launch_fill_zero(out_dev[i], plans[i]->N * (int64_t)ncols, s);
-                matvec_run(plans[i], ops[i], wsp[i], w_dev[i], ncols, out_dev[i], s, precision, false);
-            }
-            for (int k = 0; k < nbr; k++) {
-                check(hipEventRecord(b->ev[k], b->side[k]), "join");
-                check(hipStreamWaitEvent(b->stream, b->ev[k], 0), "join");
-            }
-        } catch (...) {
-            hipGraph_t junk = nullptr;
-            (void)hipStreamEndCapture(b->stream, &junk);
-            if (junk) (void)hipGraphDestroy(junk);
-            for (int i = 0; i < nplans; i++) plans[i]->timing = timing[i];
-            throw;
-        }
-        check(hipStreamEndCapture(b->stream, &b->graph), "hipStreamEndCapture");
-        for (int i = 0; i < nplans; i++) plans[i]->timing = timing[i];
-        check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-        *out = b.release();
+        std::vector<pbb_batch_entry> E(nplans);
+        for (int i = 0; i < nplans; i++) E[i] = pbb_batch_entry{plans[i], w_dev[i], out_dev[i], flags, 0};
+        batch_create(E.data(), nplans, ncols, precision, out);
     });
+}
+
+PBB_API pbb_status pbb_batch_create_entries(const pbb_batch_entry *entries, int32_t nentries, int32_t ncols,
+                                            int32_t precision, pbb_batch **out)
+{
+    return guarded([&] { batch_create(entries, nentries, ncols, precision, out); });
 }
 
 PBB_API pbb_status pbb_batch_launch(pbb_batch *batch, void *stream)

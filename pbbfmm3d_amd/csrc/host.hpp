@@ -109,6 +109,11 @@ struct HostTree {
 // BuildFMMHierarchy (H2_3D_Tree.cpp:917-977).
 void build_tree(HostTree &T, const double *xyz, int64_t N, double L, int levels);
 
+// Leaf grid coordinates, packed ix | iy << 10 | iz << 20, of a point p
+// already re-centred, by build_tree's comparison chain over the per-axis
+// node centres lv[l].ctab (strict '<', compute.hpp:1149-1173).
+uint32_t leaf_coord(const std::vector<Level> &lv, int levels, const double p[3]);
+
 // Far-field directions of an octant class (x bit 4, y bit 2, z bit 1):
 // dirs[k] = {dx, dy, dz, E index}, 189 entries.
 void class_directions(int octant, const std::vector<int> &Ktable, int dirs[189][4]);

@@ -1638,5 +1638,40 @@ void launch_fill_zero(double *out, int64_t n, hipStream_t st)
     check(hipGetLastError(), "fill_zero_f64");
 }
 
+// Per-workgroup partial sums of (a - b)^2 and b^2 over n doubles, written to
+// part[2*g] and part[2*g + 1]; the host adds the pairs (no atomics, so the
+// result does not depend on the schedule).
+__global__ __launch_bounds__(256) void diff_norms_f64(const double *__restrict__ a, const double *__restrict__ b,
+                                                      int64_t n, double *__restrict__ part)
+{
+    __shared__ double sd[256], sb[256];
+    double d2 = 0, b2 = 0;
+    for (int64_t q = blockIdx.x * (int64_t)256 + threadIdx.x; q < n; q += (int64_t)gridDim.x * 256) {
+        const double bv = b[q], dv = a[q] - bv;
+        d2 += dv * dv;
+        b2 += bv * bv;
+    }
+    sd[threadIdx.x] = d2;
+    sb[threadIdx.x] = b2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sd[threadIdx.x] += sd[threadIdx.x + s];
+            sb[threadIdx.x] += sb[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sd[0];
+        part[2 * blockIdx.x + 1] = sb[0];
+    }
+}
+
+void launch_diff_norms(const double *a, const double *b, int64_t n, double *part, int nblocks, hipStream_t st)
+{
+    diff_norms_f64<<<nblocks, 256, 0, st>>>(a, b, n, part);
+    check(hipGetLastError(), "diff_norms_f64");
+}
+
 }  // namespace dev
 }  // namespace pbb

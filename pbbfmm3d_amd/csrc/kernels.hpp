@@ -189,6 +189,8 @@ struct NearGradArgs {
 void upload_dirs(const int dirs[8][189][4]);
 void launch_near_grad(const NearGradArgs &a, int kid, int nleaf, hipStream_t st);
 void launch_fill_zero(double *out, int64_t n, hipStream_t st);
+// part[2g], part[2g+1] = sums of (a-b)^2 and b^2 over workgroup g's share of n
+void launch_diff_norms(const double *a, const double *b, int64_t n, double *part, int nblocks, hipStream_t st);
 void launch_split_e(int variant, const double *E, int rf, int rs, int R, int nch, int escale, void *Es,
                     hipStream_t st);
 void launch_absmax(const float *x, int64_t n, unsigned *out, hipStream_t st);

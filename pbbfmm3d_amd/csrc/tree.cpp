@@ -65,6 +65,17 @@ static void radix_sort_keys(std::vector<uint64_t> &v, int key_bits)
     }
 }
 
+uint32_t leaf_coord(const std::vector<Level> &lv, int levels, const double p[3])
+{
+    uint32_t c[3];
+    for (int d = 0; d < 3; d++) {
+        uint32_t ix = 0;
+        for (int l = 0; l < levels; l++) ix = 2 * ix + (p[d] < lv[l].ctab[ix] ? 0u : 1u);
+        c[d] = ix;
+    }
+    return c[0] | (c[1] << 10) | (c[2] << 20);
+}
+
 void build_tree(HostTree &T, const double *xyz, int64_t N, double L, int levels)
 {
     if (levels < 1 || levels > 9) fail(PBB_ERR_INVALID_ARG, "tree_level must be in [1, 9]");

@@ -43,3 +43,32 @@ def gpu_matvec(plan, w, ncols, device="cuda:0"):
     out = plan.matvec(wd)
     torch.cuda.synchronize()
     return out.cpu().numpy().reshape(-1)
+
+
+def gpu_direct(kernel, tgt, src, w, ncols, device="cuda:0"):
+    """FP64 direct sum on the GPU (pbb_direct_sum, the DirectCalc3D of
+    test.hpp:18-34): column-major (ncols * len(tgt),) numpy result for host
+    inputs (w column-major ncols * len(src), or a CUDA tensor)."""
+    import ctypes as C
+
+    import torch
+
+    from pbbfmm3d_amd import _capi as capi
+    t = torch.tensor(np.ascontiguousarray(tgt), dtype=torch.float64, device=device)
+    s = torch.tensor(np.ascontiguousarray(src), dtype=torch.float64, device=device)
+    if isinstance(w, torch.Tensor):
+        wd = w.to(device=device, dtype=torch.float64).contiguous()
+    else:
+        wd = torch.tensor(np.asarray(w).reshape(-1), dtype=torch.float64, device=device)
+    out = torch.zeros(ncols * len(tgt), dtype=torch.float64, device=device)
+    d = kernel.desc()
+    capi.check(capi.lib().pbb_direct_sum(C.byref(d), t.data_ptr(), len(tgt), s.data_ptr(), len(src),
+                                         wd.data_ptr(), ncols, out.data_ptr(), None))
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def rows_of(idx, N, ncols):
+    """Column-major row indices of points idx in every column."""
+    idx = np.asarray(idx, dtype=np.int64)
+    return (idx[None, :] + N * np.arange(ncols)[:, None]).ravel()

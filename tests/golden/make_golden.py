@@ -1,7 +1,7 @@
 """Writes the golden vectors under tests/golden/ (run from the repo root).
 
 reference_input_direct.npy: the direct sum K w (K = 1/r, isinf -> 0 as
-test.hpp:306-307) over the reference's example input
+test.hpp:18-34, isinf at :29-30) over the reference's example input
 (reference_input/field_test.bin, charge_test.bin: copies of PBBFMM3D's
 input/*_test.bin), computed with plain numpy, independently of the C oracle.
 """

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import pbbfmm3d_amd as pbb
-from tests._helpers import uniform_points
+from tests._helpers import gpu_direct, uniform_points
 
 pytestmark = pytest.mark.gpu
 
@@ -98,3 +98,69 @@ def test_position_gradient_vs_direct_sum(gpu):
     rel = np.linalg.norm(g - ref) / np.linalg.norm(ref)
     print(f"FMM vs direct-sum position gradient: {rel:.3e}")
     assert rel < 1e-4
+
+
+def test_softened_laplacian_500k_level6(gpu):
+    # 500k uniform points, level 6, order 5, 1/sqrt(r^2 + 1e-6), FP32 sigma
+    # and b, 1 and 4 columns: the autograd gradients against central finite
+    # differences (h = 1e-4, Richardson with h/2) of the FMM loss on 64
+    # coordinates, and against the FP64 direct-sum gradient on 64 points
+    import torch
+    N, level, n, eps, h, delta = 500_000, 6, 5, 1e-7, 1e-4, 1e-6
+    kernel = pbb.soft_laplacian(delta)
+    xyz0 = uniform_points(N, seed=80)
+    rng = np.random.default_rng(81)
+    ops = pbb.Operators(kernel, 1.0, level, n, eps)
+    plan = pbb.Plan(ops, xyz0)       # reused by the autograd calls
+    fd_plan = pbb.Plan(ops, xyz0)    # moved by the finite differences
+    picks = _safe_coords(xyz0, level, h, 64, rng, rmin=0.005)
+    pts = rng.choice(N, 64, replace=False)
+    for ncols in (1, 4):
+        print(f"case N={N} ncols={ncols} level={level} order={n}", flush=True)
+        sigma = torch.tensor(rng.standard_normal((ncols, N)), dtype=torch.float32, device="cuda")
+        b = torch.tensor(rng.standard_normal((ncols, N)), dtype=torch.float32, device="cuda")
+        xyz = torch.tensor(xyz0, dtype=torch.float64, requires_grad=True)
+        sig = sigma.clone().requires_grad_(True)
+        out = pbb.fmm_matvec(xyz, sig, ops, plan=plan)
+        assert out.dtype == torch.float32
+        L = 0.5 * ((out.double() - b.double()) ** 2).sum()
+        L.backward()
+        gx = xyz.grad.numpy()
+        assert sig.grad.dtype == torch.float32
+        s64, b64 = sigma.double(), b.double()
+
+        def central(i, d, hh):
+            o = []
+            for sgn in (1, -1):
+                xp = xyz0.copy()
+                xp[i, d] += sgn * hh
+                fd_plan.update_points(xp)
+                o.append(fd_plan.matvec(s64))
+            # L(+) - L(-) = 1/2 sum (o+ - o-)(o+ + o- - 2b), summed without
+            # forming the two large losses
+            return float(((o[0] - o[1]) * (o[0] + o[1] - 2 * b64)).sum()) / (4 * hh)
+
+        errs = []
+        for i, d in picks:
+            fd = (4 * central(i, d, h / 2) - central(i, d, h)) / 3
+            errs.append(abs(fd - gx[i, d]) / max(abs(fd), 1e-3 * np.abs(gx).max()))
+        print(f"max relative FD mismatch over {len(picks)} coordinates: {max(errs):.3e}", flush=True)
+        assert max(errs) < 1e-5
+        fd_plan.update_points(xyz0)
+        # exact gradients from FP64 direct sums: r = K sigma - b,
+        # dL/dsigma = K r, dL/dx_i = sum_j f_ij (x_j - x_i) sum_c (r_ci s_cj + s_ci r_cj)
+        # with f_ij = (|x_i - x_j|^2 + delta)^(-3/2)
+        sn = s64.cpu().numpy()
+        r = gpu_direct(kernel, xyz0, xyz0, sn.reshape(-1), ncols).reshape(ncols, N) - b64.cpu().numpy()
+        gs_ref = gpu_direct(kernel, xyz0[pts], xyz0, r.reshape(-1), ncols).reshape(ncols, len(pts))
+        gs = sig.grad.double().cpu().numpy()[:, pts]
+        e_sig = np.linalg.norm(gs - gs_ref) / np.linalg.norm(gs_ref)
+        gx_ref = np.zeros((len(pts), 3))
+        for k, i in enumerate(pts):
+            D = xyz0 - xyz0[i]
+            f = ((D ** 2).sum(1) + delta) ** -1.5
+            coef = r[:, i] @ sn + sn[:, i] @ r
+            gx_ref[k] = (f * coef) @ D
+        e_x = np.linalg.norm(gx[pts] - gx_ref) / np.linalg.norm(gx_ref)
+        print(f"vs FP64 direct-sum gradients: sigma {e_sig:.3e}, positions {e_x:.3e}", flush=True)
+        assert e_sig < 1e-4 and e_x < 1e-3

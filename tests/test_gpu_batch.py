@@ -170,3 +170,48 @@ def test_wrappers_reject_bad_tensors(gpu):
         pbb.Batch([p], [w.float()], [torch.zeros_like(w)], 2)
     with pytest.raises(TypeError):
         pbb.Batch([p], [w], [torch.zeros(2, 3000, dtype=torch.float64)], 2)
+
+
+def test_256_clouds_forward_and_adjoint_in_one_graph(gpu):
+    # 256 clouds, sizes log-uniform in [2e3, 2e5] (U[0,1)^3, seed 1000 + k),
+    # tree level round(log8(N/60)) clamped to 3..6, orders 4..6 in turn,
+    # exp(-r/0.5), 16 columns, eps 1e-7: the forward matvec AND the adjoint
+    # of every cloud in one graph launch, FP64 and f16x2 (M2L tiles tuned
+    # per operator set within 1e-6 of FP64); every replay equals the eager
+    # calls bit for bit
+    import torch
+    ncl, nc = 256, 16
+    kernel = pbb.exponential(0.5)
+    sizes = np.exp(np.random.default_rng(999).uniform(np.log(2e3), np.log(2e5), ncl)).astype(int)
+    ops, plans, sig, psi = {}, [], [], []
+    for k in range(ncl):
+        n = int(sizes[k])
+        lvl = int(min(6, max(3, round(np.log(n / 60.0) / np.log(8.0)))))
+        order = 4 + k % 3
+        if (lvl, order) not in ops:
+            ops[(lvl, order)] = pbb.Operators(kernel, 1.0, lvl, order, 1e-7)
+        plans.append(pbb.Plan(ops[(lvl, order)], np.random.default_rng(1000 + k).random((n, 3))))
+        g = torch.Generator(device="cuda").manual_seed(3000 + k)
+        sig.append(torch.randn(nc, n, dtype=torch.float64, device="cuda", generator=g))
+        psi.append(torch.randn(nc, n, dtype=torch.float64, device="cuda", generator=g))
+    print(f"{ncl} clouds, {int(sizes.sum())} points, {len(ops)} operator sets", flush=True)
+    for prec in (pbb.capi.PREC_FP64, pbb.capi.PREC_MIXED_F16X2):
+        if prec != pbb.capi.PREC_FP64:
+            tiles = pbb.Batch.tune(plans, sig, prec, max_rel_diff=1e-6)
+            assert all(t["rel_diff"] <= 1e-6 for t in tiles.values()), tiles
+        ent_plans = [p for p in plans for _ in (0, 1)]
+        ent_w = [w for pair in zip(sig, psi) for w in pair]
+        ent_out = [torch.zeros_like(w) for w in ent_w]
+        batch = pbb.Batch(ent_plans, ent_w, ent_out, nc, precision=prec, transpose=[False, True] * ncl)
+        eager = []
+        for p, s, q in zip(plans, sig, psi):
+            eager.append(p.matvec(s, precision=prec))
+            eager.append(p.matvec_transpose(q, precision=prec))
+        torch.cuda.synchronize()
+        for rep in range(2):
+            batch.launch()
+            torch.cuda.synchronize()
+            for i, (o, e) in enumerate(zip(ent_out, eager)):
+                assert torch.equal(o, e), f"precision {prec} replay {rep} entry {i} differs from eager"
+        print(f"precision {prec}: {2 * ncl} entries in one graph equal the eager calls", flush=True)
+        del batch

@@ -87,7 +87,12 @@ def test_m2l_tiles_and_tuner(gpu):
             plan.set_m2l_tile(width, cpb, flags, prec)
             assert orc.rel_err(_mv(plan, w, nc, prec), ref) <= bar
     t = plan.tune_m2l(wd, pbb.capi.PREC_MIXED_F16X2)
-    assert t["width"] in (64, 128, 256) and (t["flags"] & 0xff) in range(8)
+    assert t["width"] in (64, 128, 256) and (t["flags"] & 0xff) in range(16)
+    # with an accuracy bound the kept tile's matvec is within it
+    t = plan.tune_m2l(wd, pbb.capi.PREC_MIXED_F16X2, max_rel_diff=1e-6)
+    assert t["rel_diff"] <= 1e-6
+    plan.set_m2l_tile(t["width"], t["cols_per_box"], t["flags"], pbb.capi.PREC_MIXED_F16X2)
+    assert orc.rel_err(_mv(plan, w, nc, pbb.capi.PREC_MIXED_F16X2), ref) <= 1e-6
     t = plan.tune_m2l(wd, pbb.capi.PREC_FP64)
     assert t["width"] in (32, 64, 128)
     torch.cuda.synchronize()

@@ -175,3 +175,34 @@ def test_operators_from_reference_cache_files(gpu, tmp_path):
     a = gpu_matvec(pbb.Plan(ops, xyz), w, 2)
     b = gpu_matvec(pbb.Plan(back, xyz), w, 2)
     assert np.array_equal(a, b)
+
+
+def test_update_points_keeps_the_tree(gpu):
+    # moving points inside their leaf boxes equals a plan built on the moved
+    # points (same boxes, sorted order and centre) bit for bit, and a point
+    # that leaves its leaf box is refused with the plan unchanged
+    xyz = uniform_points(6000, seed=42)
+    w = weights(6000, 2, seed=43)
+    ops = pbb.Operators(pbb.laplacian(), 1.0, 4, 4, 1e-5)
+    plan = pbb.Plan(ops, xyz)
+    before = gpu_matvec(plan, w, 2)
+    ctr = 0.5 * (xyz.min(0) + xyz.max(0))
+    rel = (xyz - ctr + 0.5) * 16  # leaf-box units at level 4
+    inner = np.all(np.abs(rel - np.round(rel)) > 0.02, axis=1)
+    inner[xyz.argmin(0)] = False
+    inner[xyz.argmax(0)] = False
+    moved = xyz.copy()
+    moved[inner] += 1e-4  # < 0.02 leaf widths: every moved point stays in its box
+    plan.update_points(moved)
+    got = gpu_matvec(plan, w, 2)
+    fresh = gpu_matvec(pbb.Plan(ops, moved), w, 2)
+    assert np.array_equal(got, fresh)
+    assert not np.array_equal(got, before)
+    ref = orc.matvec(oracle_ops_from(ops, pbb.laplacian()), moved, w, 2)
+    assert orc.rel_err(got, ref) <= PARITY
+    out_of_box = moved.copy()
+    k = int(np.flatnonzero(inner)[0])
+    out_of_box[k, 0] += 1.0 / 16  # one leaf box over
+    with pytest.raises(pbb.capi.PBBError):
+        plan.update_points(out_of_box)
+    assert np.array_equal(gpu_matvec(plan, w, 2), got)

@@ -158,6 +158,14 @@ def test_cutoff_rule():
         assert orc.cutoff(s, eps) == expect
 
 
+def test_cutoff_rule_tie():
+    # the reference keeps a value whose tail energy equals eps^2 * total
+    # exactly (strict '<' drops, H2_3D_Tree.cpp:415-421): four unit values,
+    # eps 1/2, tail of the last one = 1 = eps^2 * 4 -> all four kept
+    assert orc.cutoff(np.ones(4), 0.5) == 4
+    assert orc.cutoff(np.array([1.0, 1.0, 1.0, 0.5]), 0.5) == 3
+
+
 def test_compress_identity():
     ops = orc.precompute(orc.LAPLACIAN, None, 1, -1.0, 1.0, 3, 3, 1e-4)
     n3 = 27
@@ -257,3 +265,25 @@ def test_reference_input_fixture():
     assert orc.rel_err(got, ref) < 1e-4
     golden = np.load(os.path.join(GOLDEN, "reference_input_direct.npy"))
     np.testing.assert_allclose(ref, golden, rtol=1e-12)
+
+
+@pytest.mark.parametrize("kid,homogen,params,levels,n,pts", [
+    (orc.LAPLACIAN, -1.0, None, 4, 4, "uniform"),
+    (orc.GAUSSIAN, 0.0, [0.3], 4, 4, "clustered"),
+    (orc.LAPLACIAN, -1.0, None, 5, 3, "clustered"),
+])
+def test_masked_matvec_equals_full(kid, homogen, params, levels, n, pts):
+    # the target-restricted oracle forms the same outputs on its targets
+    N, ncols = 6000, 2
+    xyz = uniform_points(N, seed=21) if pts == "uniform" else clustered_points(N, seed=21)
+    w = weights(N, ncols, seed=22)
+    ops = orc.precompute(kid, params, 1, homogen, 1.0, levels, n, 1e-6)
+    full = orc.matvec(ops, xyz, w, ncols)
+    tgt = np.random.default_rng(23).choice(N, 150, replace=False)
+    sentinel = np.full(N * ncols, 7.0)
+    part = orc.matvec(ops, xyz, w, ncols, out=sentinel.copy(), targets=tgt)
+    rows = (tgt[None, :] + N * np.arange(ncols)[:, None]).ravel()
+    got = part[rows] - 7.0
+    assert orc.rel_err(got, full[rows]) < 1e-13
+    # outputs of points outside the targets' leaves are left as they are
+    assert np.count_nonzero(part == 7.0) > N * ncols // 2
