@@ -220,6 +220,25 @@ PBB_API pbb_status pbb_plan_matvec(pbb_plan *plan, const double *w_dev,
                                    int32_t ncols, double *out_dev,
                                    int32_t precision, void *stream);
 
+/* pbb_plan_matvec without any device allocation: no reference counterpart.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and result as pbb_plan_matvec; the workspace comes from a
+ * fixed array of PBB_NOALLOC_SCRATCH_BYTES bytes per GPU that the library
+ * holds until the process ends, instead of the plan's own workspace.
+ * Fails with PBB_ERR_INVALID_ARG, before any launch, when the matvec needs
+ * more workspace than that (roughly 3*N*ncols*8 bytes of sorted weights
+ * plus, per non-empty box of levels >= 2, (n^3 + 2*rank)*ncols*8 bytes in
+ * FP64); a mixed precision needs its split operators made first, by one
+ * pbb_plan_matvec or pbb_plan_tune_m2l in that precision on a plan of the
+ * same operators.  Every call on one GPU shares that GPU's array, which
+ * keeps the last call's contents: calls that overlap in time on one GPU
+ * (other streams, other host threads, other plans) race; serialise them. */
+#define PBB_NOALLOC_SCRATCH_BYTES (64u << 20)
+PBB_API pbb_status pbb_plan_matvec_noalloc(pbb_plan *plan, const double *w_dev,
+                                           int32_t ncols, double *out_dev,
+                                           int32_t precision, void *stream);
+
 /* The adjoint: out += A~^T w for the same tree, where A~ is the matrix the
  * FMM applies.  Runs the transposed FMM (P2M <-> L2P and M2M <-> L2L swap
  * roles, the M2L uses V^T/U swapped and E_{-d}^T) with the same kernels;

@@ -110,6 +110,7 @@ _SIGNATURES = {
     "pbb_plan_query": (C.c_int, [_vp, C.POINTER(PlanInfo)]),
     "pbb_plan_update_points": (C.c_int, [_vp, _dp, C.c_int64, _vp]),
     "pbb_plan_matvec": (C.c_int, [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp]),
+    "pbb_plan_matvec_noalloc": (C.c_int, [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp]),
     "pbb_plan_matvec_transpose": (C.c_int, [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp]),
     "pbb_plan_grad_positions": (C.c_int, [_vp, _vp, _vp, C.c_int32, _vp, C.c_int32, _vp]),
     "pbb_batch_create": (C.c_int, [C.POINTER(_vp), C.c_int32, C.POINTER(_vp), C.POINTER(_vp), C.c_int32,

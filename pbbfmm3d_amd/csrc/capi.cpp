@@ -1255,6 +1255,66 @@ PBB_API pbb_status pbb_plan_matvec(pbb_plan *plan, const double *w_dev, int32_t 
     });
 }
 
+}  // extern "C"
+
+// ---- the matvec that allocates nothing ------------------------------------
+// pbb_plan_matvec_noalloc lays its workspace out in this file-scope array
+// instead of device memory the plan allocates.  Every GPU the library runs
+// on holds its own copy until the process ends.  Host code never passes the
+// array's name to a kernel or a copy: the name is a host-side stand-in, so
+// the copy's device address is looked up once per device (with that device
+// current) and kept in g_scratch_ptr.
+static __device__ __align__(256) char scratch[PBB_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on device dev, which must be current.
+char *scratch_ptr(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) fail(PBB_ERR_UNSUPPORTED, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            fail(PBB_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        }
+        g_scratch_ptr[dev] = (char *)p;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+PBB_API pbb_status pbb_plan_matvec_noalloc(pbb_plan *plan, const double *w_dev, int32_t ncols, double *out_dev,
+                                           int32_t precision, void *stream)
+{
+    return guarded([&] {
+        if (!plan || !w_dev || !out_dev) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
+        if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
+            fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
+        check_mixed_ranks(plan->ops, precision);
+        const bool mixed = precision != PBB_PREC_FP64;
+        Workspace W;
+        const size_t need = layout_workspace(*plan, ncols, mixed, nullptr, W);
+        if (need > (size_t)PBB_NOALLOC_SCRATCH_BYTES)
+            fail(PBB_ERR_INVALID_ARG, "this matvec needs " + std::to_string(need) +
+                                          " bytes of workspace; the allocation-free matvec has " +
+                                          std::to_string((size_t)PBB_NOALLOC_SCRATCH_BYTES));
+        DeviceGuard g(plan->device);
+        char *base = scratch_ptr(plan->device);
+        layout_workspace(*plan, ncols, mixed, base, W);
+        // build = false: the split operators of a mixed mode must exist already
+        matvec_run(plan, plan->ops, W, w_dev, ncols, out_dev, (hipStream_t)stream, precision, false);
+    });
+}
+
 PBB_API pbb_status pbb_plan_matvec_transpose(pbb_plan *plan, const double *w_dev, int32_t ncols, double *out_dev,
                                              int32_t precision, void *stream)
 {
