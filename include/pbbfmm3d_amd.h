@@ -188,10 +188,20 @@ PBB_API void pbb_ops_destroy(pbb_ops *ops);
 
 /* Replaces FMMDistribute + InteractionList (compute.hpp:157-190,
  * :1042-1203) and the bbox re-centring (compute.hpp:92-127) for one point
- * set (host AoS xyz, N points).  The tree is kept on the device; unlike
- * the reference a plan can serve any number of matvecs.                  */
+ * set (host AoS xyz, N points).  The tree is built on the device (Morton
+ * keys, radix sort, per-level box lists) and kept there; unlike the
+ * reference a plan can serve any number of matvecs.                      */
 PBB_API pbb_status pbb_plan_create(pbb_ops *ops, const double *xyz,
                                    int64_t npoints, pbb_plan **out);
+
+/* pbb_plan_create with flags: PBB_PLAN_HOST_TREE builds the tree on the
+ * host (a CPU restatement of FMMDistribute / InteractionList that the GPU
+ * build reproduces array for array, kept as its check); flags 0 builds it
+ * on the GPU, as pbb_plan_create does.                                   */
+#define PBB_PLAN_HOST_TREE 1
+PBB_API pbb_status pbb_plan_create_ex(pbb_ops *ops, const double *xyz,
+                                      int64_t npoints, int32_t flags,
+                                      pbb_plan **out);
 
 PBB_API pbb_status pbb_plan_query(const pbb_plan *plan, pbb_plan_info *info);
 

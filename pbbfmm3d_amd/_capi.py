@@ -55,6 +55,7 @@ def M2L_DIR_GROUPS(g):
 
 
 NUM_PHASES = 8
+PLAN_HOST_TREE = 1
 BATCH_TRANSPOSE = 1
 BATCH_OVERWRITE = 2
 PHASE_NAMES = ("p2m", "m2m", "vt_proxy", "m2l", "u_expand", "l2l", "near", "total")
@@ -107,6 +108,7 @@ _SIGNATURES = {
     "pbb_ops_export": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
     "pbb_ops_destroy": (None, [_vp]),
     "pbb_plan_create": (C.c_int, [_vp, _dp, C.c_int64, C.POINTER(_vp)]),
+    "pbb_plan_create_ex": (C.c_int, [_vp, _dp, C.c_int64, C.c_int32, C.POINTER(_vp)]),
     "pbb_plan_query": (C.c_int, [_vp, C.POINTER(PlanInfo)]),
     "pbb_plan_update_points": (C.c_int, [_vp, _dp, C.c_int64, _vp]),
     "pbb_plan_matvec": (C.c_int, [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp]),

@@ -244,13 +244,16 @@ class Plan:
     Calls on one plan share its device workspace: issue them on one stream
     (or order the streams), as the C ABI requires."""
 
-    def __init__(self, ops: Operators, xyz):
+    def __init__(self, ops: Operators, xyz, host_tree: bool = False):
+        """host_tree: build the tree on the host instead of the GPU (the
+        same arrays; kept as the GPU build's check)."""
         self.ops = ops
         xyz = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
         self.npoints = xyz.shape[0]
         h = C.c_void_p()
-        capi.check(capi.lib().pbb_plan_create(ops.handle, xyz.ctypes.data_as(C.POINTER(C.c_double)),
-                                              self.npoints, C.byref(h)))
+        capi.check(capi.lib().pbb_plan_create_ex(ops.handle, xyz.ctypes.data_as(C.POINTER(C.c_double)),
+                                                 self.npoints, capi.PLAN_HOST_TREE if host_tree else 0,
+                                                 C.byref(h)))
         self._h = h
 
     def info(self) -> dict:

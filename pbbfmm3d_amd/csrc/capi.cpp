@@ -12,6 +12,7 @@
 
 #include "host.hpp"
 #include "kernels.hpp"
+#include "tree_gpu.hpp"
 
 using namespace pbb;
 
@@ -288,11 +289,13 @@ struct pbb_plan {
     char *ws_block = nullptr;
     size_t ws_bytes = 0;
     double *gF = nullptr, *gU = nullptr;  // position gradients: saved leaf F, sorted weights
-    // host copies for pbb_plan_update_points: sorted position -> input
-    // index, the leaf coordinates of every sorted point, per-level centres
+    // for pbb_plan_update_points: per-level centres, and (fetched from the
+    // device on first use when the tree was built there) sorted position ->
+    // input index and the leaf coordinates of every sorted point
     std::vector<int32_t> h_perm;
     std::vector<uint32_t> h_leaf;
     std::vector<Level> h_geom;
+    int32_t *pt_leaf = nullptr;  // device: leaf coordinates of every sorted point
     int g_ncols = 0;
     bool timing = false;
     struct M2LTile {
@@ -310,7 +313,7 @@ struct pbb_plan {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
         dfree(x); dfree(y); dfree(z); dfree(perm); dfree(leaf_start); dfree(leaf_class);
-        dfree(up_items); dfree(up_reds);
+        dfree(up_items); dfree(up_reds); dfree(pt_leaf);
         for (auto &L : lv) {
             dfree(L.coord); dfree(L.map); dfree(L.child); dfree(L.order); dfree(L.ctab);
             dfree(L.pstart); dfree(L.seg_box); dfree(L.seg_p0);
@@ -614,16 +617,12 @@ PBB_API pbb_status pbb_ops_export(const pbb_ops *ops, double *U, double *VT, dou
 
 PBB_API void pbb_ops_destroy(pbb_ops *ops) { delete ops; }
 
-PBB_API pbb_status pbb_plan_create(pbb_ops *ops, const double *xyz, int64_t npoints, pbb_plan **out)
+}  // extern "C"
+
+static void plan_create_host(pbb_ops *ops, const double *xyz, int64_t npoints, pbb_plan **out)
 {
-    return guarded([&] {
-        if (!out) fail(PBB_ERR_INVALID_ARG, "out is NULL");
-        *out = nullptr;
-        if (!ops || !xyz) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+    {
         const HostOps &h = ops->h;
-        if (h.rf > 256 || h.rs > 256) fail(PBB_ERR_UNSUPPORTED, "SVD ranks above 256 are not supported");
-        if (!kernel_id_valid(h.kernel.id) || h.kernel.id == PBB_KERNEL_POLY_XY)
-            fail(PBB_ERR_UNSUPPORTED, "the kernel has no device functor for the near field");
         int dev = current_device();
         HostTree T;
         build_tree(T, xyz, npoints, h.L, h.tree_level);
@@ -747,7 +746,103 @@ PBB_API pbb_status pbb_plan_create(pbb_ops *ops, const double *xyz, int64_t npoi
         for (int d = 0; d < 3; d++) I.center[d] = T.ctr[d];
         I.outside_points = T.outside;
         *out = plan.release();
+    }
+}
+
+extern "C" {
+
+// The plan's tree built on the GPU (tree_gpu.hip): the same arrays as the
+// host build, with the plan's derived lists made there too.
+static void plan_create_gpu(pbb_ops *ops, const double *xyz, int64_t npoints, pbb_plan **out)
+{
+    const HostOps &h = ops->h;
+    const int dev = current_device();
+    const int Lv = h.tree_level;
+    ops->upload(dev);
+    auto plan = std::make_unique<pbb_plan>();
+    plan->ops = ops;
+    plan->device = dev;
+    check(hipStreamCreateWithFlags(&plan->stream, hipStreamNonBlocking), "hipStreamCreate");
+    // per-axis node centres with the arithmetic of BuildFMMHierarchy, as
+    // build_tree computes them
+    plan->h_geom.assign(Lv + 1, Level());
+    plan->h_geom[0].ctab = {0.0};
+    plan->h_geom[0].len = h.L;
+    for (int l = 0; l < Lv; l++) {
+        const Level &P = plan->h_geom[l];
+        Level &C = plan->h_geom[l + 1];
+        C.ctab.resize((size_t)1 << (l + 1));
+        const double q = 0.25 * P.len;
+        for (size_t i = 0; i < P.ctab.size(); i++) {
+            C.ctab[2 * i] = P.ctab[i] - q;
+            C.ctab[2 * i + 1] = P.ctab[i] + q;
+        }
+        C.len = 0.5 * P.len;
+    }
+    std::vector<std::vector<double>> ctab(Lv + 1);
+    for (int l = 0; l <= Lv; l++) ctab[l] = plan->h_geom[l].ctab;
+    int dirs[8][189][4];
+    for (int o = 0; o < 8; o++) class_directions(o, h.w.Ktable, dirs[o]);
+    dev::GpuTree T;
+    dev::build_tree_gpu(T, xyz, npoints, h.L, Lv, ctab, dirs, plan->stream);
+    // the plan owns the arrays from here on (its destructor frees them)
+    plan->Lv = Lv;
+    plan->N = T.N;
+    plan->x = T.x; plan->y = T.y; plan->z = T.z;
+    plan->perm = T.perm; plan->leaf_start = T.leaf_start; plan->leaf_class = T.leaf_class;
+    plan->pt_leaf = T.pt_leaf;
+    plan->up_items = T.up_items; plan->up_reds = T.up_reds;
+    plan->lv.resize(Lv + 1);
+    for (int l = 0; l <= Lv; l++) {
+        DevLevel &D = plan->lv[l];
+        const dev::GpuLevel &V = T.lv[l];
+        D.nb = V.nb; D.off = V.off; D.coord = V.coord; D.map = V.map; D.child = V.child; D.order = V.order;
+        D.pstart = V.pstart; D.seg_box = V.seg_box; D.seg_p0 = V.seg_p0; D.nseg = V.nseg;
+        std::copy(std::begin(V.class_count), std::end(V.class_count), D.class_count);
+        D.len = plan->h_geom[l].len;
+    }
+    for (int l = 0; l <= Lv; l++) plan->lv[l].ctab = dupload(ctab[l], "ctab");
+    std::copy(std::begin(T.leaf_class_off), std::end(T.leaf_class_off), plan->leaf_class_off);
+    plan->n_up_items = T.n_up_items;
+    plan->n_up_reds = T.n_up_reds;
+    plan->n_up_slots = T.n_up_slots;
+    plan->Btot = T.Btot;
+    dev::upload_dirs(dirs);
+    dev::upload_dirs_mixed(dirs);
+    pbb_plan_info &I = plan->info;
+    std::memset(&I, 0, sizeof(I));
+    I.npoints = T.N;
+    I.tree_level = Lv;
+    for (int l = 0; l <= Lv && l < 16; l++) I.boxes[l] = T.lv[l].nb;
+    I.nonempty_levels = Lv + 1;
+    I.m2l_pairs = T.m2l_pairs;
+    I.near_pairs = T.near_pairs;
+    I.max_leaf_points = T.max_leaf;
+    for (int d = 0; d < 3; d++) I.center[d] = T.ctr[d];
+    I.outside_points = T.outside;
+    *out = plan.release();
+}
+
+PBB_API pbb_status pbb_plan_create_ex(pbb_ops *ops, const double *xyz, int64_t npoints, int32_t flags,
+                                      pbb_plan **out)
+{
+    return guarded([&] {
+        if (!out) fail(PBB_ERR_INVALID_ARG, "out is NULL");
+        *out = nullptr;
+        if (!ops || !xyz) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        if (flags & ~PBB_PLAN_HOST_TREE) fail(PBB_ERR_INVALID_ARG, "unknown plan flags");
+        const HostOps &h = ops->h;
+        if (h.rf > 256 || h.rs > 256) fail(PBB_ERR_UNSUPPORTED, "SVD ranks above 256 are not supported");
+        if (!kernel_id_valid(h.kernel.id) || h.kernel.id == PBB_KERNEL_POLY_XY)
+            fail(PBB_ERR_UNSUPPORTED, "the kernel has no device functor for the near field");
+        if (flags & PBB_PLAN_HOST_TREE) plan_create_host(ops, xyz, npoints, out);
+        else plan_create_gpu(ops, xyz, npoints, out);
     });
+}
+
+PBB_API pbb_status pbb_plan_create(pbb_ops *ops, const double *xyz, int64_t npoints, pbb_plan **out)
+{
+    return pbb_plan_create_ex(ops, xyz, npoints, 0, out);
 }
 
 PBB_API pbb_status pbb_plan_update_points(pbb_plan *plan, const double *xyz, int64_t npoints, void *stream)
@@ -757,6 +852,14 @@ PBB_API pbb_status pbb_plan_update_points(pbb_plan *plan, const double *xyz, int
         if (npoints != plan->N) fail(PBB_ERR_INVALID_ARG, "npoints must be the plan's point count");
         const int64_t N = plan->N;
         const double *c = plan->info.center;
+        DeviceGuard g(plan->device);
+        if (plan->h_perm.empty()) {  // a tree built on the GPU: its ordering, once
+            std::vector<int32_t> perm(N), leaf(N);
+            check(hipMemcpy(perm.data(), plan->perm, N * sizeof(int32_t), hipMemcpyDeviceToHost), "copy perm");
+            check(hipMemcpy(leaf.data(), plan->pt_leaf, N * sizeof(int32_t), hipMemcpyDeviceToHost), "copy leaves");
+            plan->h_leaf.assign(leaf.begin(), leaf.end());
+            plan->h_perm.swap(perm);
+        }
         std::vector<double> xs(N), ys(N), zs(N);
         std::vector<char> moved_out(std::max<int64_t>(1, (N + 65535) / 65536), 0);
         parallel_for((N + 65535) / 65536, [&](int64_t blk) {
@@ -773,7 +876,6 @@ PBB_API pbb_status pbb_plan_update_points(pbb_plan *plan, const double *xyz, int
         });
         for (char m : moved_out)
             if (m) fail(PBB_ERR_INVALID_ARG, "a point left its leaf box (or is not finite): build a new plan");
-        DeviceGuard g(plan->device);
         hipStream_t st = (hipStream_t)stream;
         check(hipMemcpyAsync(plan->x, xs.data(), N * sizeof(double), hipMemcpyHostToDevice, st), "upload x");
         check(hipMemcpyAsync(plan->y, ys.data(), N * sizeof(double), hipMemcpyHostToDevice, st), "upload y");

@@ -206,3 +206,30 @@ def test_update_points_keeps_the_tree(gpu):
     with pytest.raises(pbb.capi.PBBError):
         plan.update_points(out_of_box)
     assert np.array_equal(gpu_matvec(plan, w, 2), got)
+
+
+@pytest.mark.parametrize("case", ["uniform-l4", "clustered-l7", "duplicates-l3", "one-point", "level1",
+                                  "outside-l5"])
+def test_gpu_tree_equals_host_tree(gpu, case):
+    # the tree built on the GPU (pbb_plan_create) and the host restatement
+    # (PBB_PLAN_HOST_TREE) give the same plan: same info, and matvecs equal
+    # bit for bit (same point order, boxes, lists and work items)
+    rng = np.random.default_rng(44)
+    level, xyz = {
+        "uniform-l4": (4, lambda: uniform_points(20000, seed=45)),
+        "clustered-l7": (7, lambda: clustered_points(100000, seed=46)),
+        "duplicates-l3": (3, lambda: np.repeat(uniform_points(500, seed=47), 4, axis=0)),
+        "one-point": (3, lambda: np.array([[0.3, 0.4, 0.5]])),
+        "level1": (1, lambda: uniform_points(700, seed=48)),
+        "outside-l5": (5, lambda: np.concatenate([uniform_points(5000, seed=49) * 0.6,
+                                                  [[0.0, 0.0, 0.0], [0.59, 0.59, 0.59]]])),
+    }[case]
+    xyz = xyz()
+    L = 0.5 if case == "outside-l5" else 1.0  # points outside the root cube
+    print(f"case {case} N={len(xyz)} level={level}", flush=True)
+    ops = pbb.Operators(pbb.laplacian(), L, level, 4, 1e-5)
+    g = pbb.Plan(ops, xyz)
+    h = pbb.Plan(ops, xyz, host_tree=True)
+    assert g.info() == h.info()
+    w = rng.standard_normal(len(xyz) * 3)
+    assert np.array_equal(gpu_matvec(g, w, 3), gpu_matvec(h, w, 3))
