@@ -287,7 +287,8 @@ PBB_API pbb_status pbb_plan_set_timing(pbb_plan *plan, int32_t enable);
  * modes) to split the rows of r_f over two workgroups per tile,
  * PBB_M2L_WAVE_ROWS (mixed modes, no row split) for the kernel whose
  * waves split the rows of r_f and load E_d into registers (ranks above 192,
- * or above 128 in PBB_PREC_MIXED, cap its width at 128),
+ * or above 128 in PBB_PREC_MIXED, cap its width at 128; ranks above 224
+ * always run it, as row groups of workgroups above 256 rows),
  * PBB_M2L_SPLIT_ACC (with PBB_M2L_WAVE_ROWS; PBB_PREC_MIXED_F16X2 and
  * PBB_PREC_MIXED_BF16X2, ignored in PBB_PREC_MIXED) to sum the split's
  * cross products apart from the leading products, so that the FP32 sums

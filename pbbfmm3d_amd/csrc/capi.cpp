@@ -832,7 +832,6 @@ PBB_API pbb_status pbb_plan_create_ex(pbb_ops *ops, const double *xyz, int64_t n
         if (!ops || !xyz) fail(PBB_ERR_INVALID_ARG, "NULL argument");
         if (flags & ~PBB_PLAN_HOST_TREE) fail(PBB_ERR_INVALID_ARG, "unknown plan flags");
         const HostOps &h = ops->h;
-        if (h.rf > 256 || h.rs > 256) fail(PBB_ERR_UNSUPPORTED, "SVD ranks above 256 are not supported");
         if (!kernel_id_valid(h.kernel.id) || h.kernel.id == PBB_KERNEL_POLY_XY)
             fail(PBB_ERR_UNSUPPORTED, "the kernel has no device functor for the near field");
         if (flags & PBB_PLAN_HOST_TREE) plan_create_host(ops, xyz, npoints, out);
@@ -948,12 +947,10 @@ PBB_API pbb_status pbb_plan_timings(pbb_plan *plan, double *ms)
     });
 }
 
-// Ranks the M2L kernels of precision prec take (checked before any launch)
-static void check_mixed_ranks(const pbb_ops *o, int prec)
-{
-    if (prec != PBB_PREC_FP64 && o->h.rf > 224)
-        fail(PBB_ERR_UNSUPPORTED, "mixed-precision M2L supports ranks up to 224");
-}
+// The column-wave mixed M2L tile takes at most 224 rows of r_f; larger
+// ranks always run the row-wave tile, whose workgroups split the rows into
+// row groups (as the FP64 tile does above 256 rows), so every rank runs.
+static bool wave_rows_forced(const HostOps &h) { return h.rf > 224; }
 
 static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double *w, int ncols, double *out,
                        hipStream_t st, int prec, bool build);
@@ -998,7 +995,6 @@ static void tune_m2l(pbb_plan *plan, const double *w_dev, int ncols, int precisi
         fail(PBB_ERR_INVALID_ARG, "max_rel_diff must be finite and >= 0");
     const bool mixed = precision != PBB_PREC_FP64;
     const bool bounded = mixed && max_rel_diff > 0;
-    check_mixed_ranks(plan->ops, precision);
     DeviceGuard g(plan->device);
     const bool was_timing = plan->timing;
     check(pbb_plan_set_timing(plan, 1) == PBB_OK ? hipSuccess : hipErrorUnknown, "enable timing");
@@ -1236,6 +1232,10 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         a.chunk_major = tmx.width ? (tmx.flags & PBB_M2L_CHUNK_MAJOR) : 1;
         a.row_split = tmx.width ? (tmx.flags & PBB_M2L_ROW_SPLIT) != 0 : 0;
         a.wave_rows = tmx.width ? (tmx.flags & PBB_M2L_WAVE_ROWS) != 0 : 1;
+        if (wave_rows_forced(h)) {
+            a.wave_rows = 1;
+            a.row_split = 0;
+        }
         if (a.wave_rows) a.TB = std::max(1, std::min(a.TB, dev::m2l_rows_max_width(prec, SE->R) / cptm));
         a.split_acc = split_acc;
         if (split_acc) a.TB = std::max(1, std::min(a.TB, 64 / cptm));
@@ -1351,7 +1351,6 @@ PBB_API pbb_status pbb_plan_matvec(pbb_plan *plan, const double *w_dev, int32_t 
         if ((int64_t)ncols * plan->N >= (1LL << 40)) fail(PBB_ERR_INVALID_ARG, "ncols * npoints too large");
         if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
             fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
-        check_mixed_ranks(plan->ops, precision);
         DeviceGuard g(plan->device);
         matvec_own(plan, plan->ops, w_dev, ncols, out_dev, (hipStream_t)stream, precision);
     });
@@ -1401,7 +1400,6 @@ PBB_API pbb_status pbb_plan_matvec_noalloc(pbb_plan *plan, const double *w_dev, 
         if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
         if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
             fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
-        check_mixed_ranks(plan->ops, precision);
         const bool mixed = precision != PBB_PREC_FP64;
         Workspace W;
         const size_t need = layout_workspace(*plan, ncols, mixed, nullptr, W);
@@ -1428,7 +1426,6 @@ PBB_API pbb_status pbb_plan_matvec_transpose(pbb_plan *plan, const double *w_dev
         if (plan->ops->h.kernel.symmetry != 1)
             fail(PBB_ERR_UNSUPPORTED, "the transposed near field needs a symmetric kernel");
         pbb_ops *adj = plan->ops->adjoint();
-        check_mixed_ranks(adj, precision);
         DeviceGuard g(plan->device);
         adj->upload(plan->device);
         matvec_own(plan, adj, w_dev, ncols, out_dev, (hipStream_t)stream, precision);
@@ -1446,8 +1443,6 @@ PBB_API pbb_status pbb_plan_grad_positions(pbb_plan *plan, const double *sigma_d
         if (plan->ops->h.kernel.symmetry != 1)
             fail(PBB_ERR_UNSUPPORTED, "position gradients need a symmetric kernel");
         pbb_ops *adj = plan->ops->adjoint();
-        check_mixed_ranks(plan->ops, precision);
-        check_mixed_ranks(adj, precision);
         DeviceGuard g(plan->device);
         adj->upload(plan->device);
         hipStream_t st = (hipStream_t)stream;
@@ -1562,7 +1557,6 @@ static void batch_create(const pbb_batch_entry *E, int nplans, int ncols, int pr
         pbb_plan *p = plans[i];
         ops[i] = (E[i].flags & PBB_BATCH_TRANSPOSE) ? p->ops->adjoint() : p->ops;
         ops[i]->upload(dev);
-        check_mixed_ranks(ops[i], precision);
         const SplitE *se = nullptr;
         const pbb_ops::SplitA *sa = nullptr;
         split_ops(p, ops[i], precision, true, b->stream, se, sa);

@@ -463,16 +463,20 @@ __global__ __launch_bounds__(NWC * 128) void m2l_mfma_f64(M2LArgs a)
     const int wr = wave & 1, wc = wave >> 1;
     const int rf = a.rf, rs = a.rs, TB = a.TB;
     const int nch = (rs + KC - 1) / KC;
-    const int lin = xcd_tile(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
-    // the column chunks of one tile are adjacent in launch order: they run
-    // side by side on one XCD through the same (direction, k-chunk) steps and
-    // share each E chunk through its L2
-    const int tile = lin / gridDim.y;
+    // gridDim.z row groups of R rows each (ranks above one group's rows).
+    // The row groups and column chunks of one tile are adjacent in launch
+    // order: they run side by side on one XCD through the same (direction,
+    // k-chunk) steps and share each proxy and E chunk through its L2
+    const int gyz = (int)(gridDim.y * gridDim.z);
+    const int lin = xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x * gyz);
+    const int tile = lin / gyz;
+    const int rem = lin - tile * gyz, chunk = rem / (int)gridDim.z;
+    const int row0 = (rem - chunk * (int)gridDim.z) * R;
     int o = 0;
     while (o < 7 && tile >= a.tile_prefix[o + 1]) o++;
     const int64_t first = a.class_start[o] + (int64_t)(tile - a.tile_prefix[o]) * TB;
     const int64_t cend = a.class_start[o + 1];
-    const int c0 = (lin - tile * (int)gridDim.y) * a.cpt;
+    const int c0 = chunk * a.cpt;
     const int cpt = min(a.cpt, a.ncols - c0);
     const int S = a.side;
     if (tid < TB) {
@@ -534,7 +538,7 @@ __global__ __launch_bounds__(NWC * 128) void m2l_mfma_f64(M2LArgs a)
         const double *Ed = a.E + (size_t)c_dirs[o][k][3] * rf * rs;
 #pragma unroll
         for (int r = 0; r < AR; r++) {
-            const int q = tid + r * NT, row = q % R, gk = k0 + q / R;
+            const int q = tid + r * NT, row = row0 + q % R, gk = k0 + q / R;
             const bool ok = q < KC * R && row < rf && gk < rs;
             const double v = Ed[ok ? row + (size_t)gk * rf : 0];
             ra[r] = ok ? v : 0.0;
@@ -626,7 +630,7 @@ __global__ __launch_bounds__(NWC * 128) void m2l_mfma_f64(M2LArgs a)
             const int rb_ = wr * RBW + i;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int row = rb_ * 16 + fk + 4 * v;
+                const int row = row0 + rb_ * 16 + fk + 4 * v;
                 if (rb_ < RB && row < rf) dst[row] = acc[i][cb][v];
             }
         }
@@ -1471,8 +1475,10 @@ void launch_gemm(const GemmArgs &g, hipStream_t st)
 void launch_m2l(const M2LArgs &a, int ntiles, int ncolchunks, hipStream_t st)
 {
     if (ntiles <= 0) return;
-    dim3 grid((unsigned)ntiles, (unsigned)ncolchunks);
-    const int rb = (a.rf + 15) / 16;
+    // at most 16 row blocks (256 rows) per workgroup: larger ranks run as
+    // row groups of equal size
+    const int rball = (a.rf + 15) / 16, ngroups = (rball + 15) / 16, rb = (rball + ngroups - 1) / ngroups;
+    dim3 grid((unsigned)ntiles, (unsigned)ncolchunks, (unsigned)ngroups);
     const int w = a.TB * a.cpt;  // tile width chosen by the caller: at most 128 columns
     if (w > 128) fail(PBB_ERR_INTERNAL, "FP64 M2L tile wider than 128 columns");
 #define PBB_M2L_CASE(R)                                                        \
@@ -1485,7 +1491,7 @@ void launch_m2l(const M2LArgs &a, int ntiles, int ncolchunks, hipStream_t st)
         PBB_M2L_CASE(1) PBB_M2L_CASE(2) PBB_M2L_CASE(3) PBB_M2L_CASE(4) PBB_M2L_CASE(5) PBB_M2L_CASE(6)
         PBB_M2L_CASE(7) PBB_M2L_CASE(8) PBB_M2L_CASE(9) PBB_M2L_CASE(10) PBB_M2L_CASE(11)
         PBB_M2L_CASE(12) PBB_M2L_CASE(13) PBB_M2L_CASE(14) PBB_M2L_CASE(15) PBB_M2L_CASE(16)
-    default: fail(PBB_ERR_UNSUPPORTED, "M2L rank above 256 is not supported");
+    default: fail(PBB_ERR_INTERNAL, "M2L row group above 256 rows");
     }
 #undef PBB_M2L_CASE
     check(hipGetLastError(), "m2l_mfma_f64");

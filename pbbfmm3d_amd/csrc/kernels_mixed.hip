@@ -437,13 +437,19 @@ __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M
     __shared__ int s_xyz[TB_MAX][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int rf = a.rf, TB = a.TB, nch = a.nch, RE = a.R;
-    const int lin = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    const int tile = lin / gridDim.y;  // the column chunks of one tile are adjacent in launch order
+    // gridDim.z row groups of 4 * RBW row blocks (ranks above one
+    // workgroup's rows); the row groups and column chunks of one tile are
+    // adjacent in launch order
+    const int gyz = (int)(gridDim.y * gridDim.z);
+    const int lin = xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x * gyz);
+    const int tile = lin / gyz;
+    const int rem = lin - tile * gyz, chunk = rem / (int)gridDim.z;
+    const int rgroup = rem - chunk * (int)gridDim.z;
     int o = 0;
     while (o < 7 && tile >= a.tile_prefix[o + 1]) o++;
     const int64_t first = a.class_start[o] + (int64_t)(tile - a.tile_prefix[o]) * TB;
     const int64_t cend = a.class_start[o + 1];
-    const int c0 = (lin - tile * (int)gridDim.y) * a.cpt;
+    const int c0 = chunk * a.cpt;
     const int cpt = min(a.cpt, a.ncols - c0);
     const int S = a.side;
     const int xs = p_scale_exp(a.xmax);
@@ -498,7 +504,7 @@ __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M
     const int nsteps = nd * nch;
     if (a.accumulate && nsteps == 0) return;  // nothing to add for this tile
 
-    const int fr = lane % 16, fq = lane / 16, rb0 = (wave & 3) * RBW;
+    const int fr = lane % 16, fq = lane / 16, rb0 = (rgroup * 4 + (wave & 3)) * RBW;
     const int cbase = (wave >> 2) * 16 * NCB;  // first column of this wave's group
     // boxes covered by each column block (wave-uniform)
     unsigned cbm[NCB];
@@ -944,9 +950,20 @@ constexpr bool rows_wide_ok()
     return !(RBW == 4 || (NSPLIT == 3 && RBW >= 3));
 }
 
+// row blocks per wave of the row-wave M2L: one workgroup takes up to 16
+// row blocks (256 rows); larger ranks run as row groups (gridDim.z) of at
+// most 12 row blocks each
+static int rows_rbw(int R)
+{
+    const int rball = R / 16;
+    if (rball <= 16) return (rball + 3) / 4;
+    const int ngroups = (rball + 11) / 12;
+    return ((rball + ngroups - 1) / ngroups + 3) / 4;
+}
+
 int m2l_rows_max_width(int variant, int R)
 {
-    const int rbw = (R / 16 + 3) / 4;
+    const int rbw = rows_rbw(R);
     const bool three = variant == PBB_PREC_MIXED;
     return rbw >= 4 || (three && rbw >= 3) ? 128 : 256;
 }
@@ -977,16 +994,17 @@ static void m2l_rows_launch(const M2LMixArgs &a, dim3 grid, int w, hipStream_t s
 template <typename T, int NSPLIT>
 static void m2l_rows_t(const M2LMixArgs &a, int ntiles, int ncolchunks, hipStream_t st)
 {
-    const int rbw = (a.R / 16 + 3) / 4;
+    const int rbw = rows_rbw(a.R);
+    const int ngroups = (a.R / 16 + 4 * rbw - 1) / (4 * rbw);
     const int w = a.TB * a.cpt;
     if (w > 256) fail(PBB_ERR_INTERNAL, "row-wave M2L tiles are at most 256 columns wide");
-    dim3 grid((unsigned)ntiles, (unsigned)ncolchunks);
+    dim3 grid((unsigned)ntiles, (unsigned)ncolchunks, (unsigned)ngroups);
     switch (rbw) {
     case 1: m2l_rows_launch<T, NSPLIT, 1>(a, grid, w, st); break;
     case 2: m2l_rows_launch<T, NSPLIT, 2>(a, grid, w, st); break;
     case 3: m2l_rows_launch<T, NSPLIT, 3>(a, grid, w, st); break;
     case 4: m2l_rows_launch<T, NSPLIT, 4>(a, grid, w, st); break;
-    default: fail(PBB_ERR_UNSUPPORTED, "mixed-precision M2L supports ranks up to 224");
+    default: fail(PBB_ERR_INTERNAL, "row-wave M2L row group too large");
     }
 }
 

@@ -233,3 +233,23 @@ def test_gpu_tree_equals_host_tree(gpu, case):
     assert g.info() == h.info()
     w = rng.standard_normal(len(xyz) * 3)
     assert np.array_equal(gpu_matvec(g, w, 3), gpu_matvec(h, w, 3))
+
+
+@pytest.mark.parametrize("kernel,n,eps", [(pbb.laplacian(), 7, 1e-12), (pbb.gaussian(0.3), 8, 1e-8)],
+                         ids=["laplacian-n7-r313", "gaussian-n8-r250"])
+def test_ranks_above_256_and_224(gpu, kernel, n, eps):
+    # ranks above one workgroup's rows: the FP64 M2L splits them into row
+    # groups above 256, the mixed M2L above 224 (row-wave tiles)
+    xyz = clustered_points(6000, seed=70)
+    w = weights(6000, 3, seed=71, dist="normal")
+    ops, plan, got, ref = _run(kernel, 1.0, 4, n, eps, xyz, w, 3)
+    print(f"ranks {ops.rank_f}/{ops.rank_s}", flush=True)
+    assert ops.rank_f > 224
+    assert orc.rel_err(got, ref) <= PARITY
+    import torch
+    wd = torch.tensor(w.reshape(3, -1), dtype=torch.float64, device="cuda:0")
+    for prec in (pbb.capi.PREC_MIXED, pbb.capi.PREC_MIXED_F16X2):
+        mix = plan.matvec(wd, precision=prec).cpu().numpy().reshape(-1)
+        d = orc.rel_err(mix, got)
+        print(f"precision {prec}: vs FP64 {d:.3e}", flush=True)
+        assert d <= 1e-6
