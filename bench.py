@@ -207,6 +207,11 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--err-targets", type=int, default=10000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--order-sweep", default="",
+                    help="comma-separated orders (e.g. 7,8) measured beside the headline at the same points, "
+                         "weights and sampled targets: FP64 and mixed errors against the direct sum, speed ratio")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 256-cloud forward+adjoint batch measured beside the headline")
     ap.add_argument("--workload", default="c3", choices=["c3", "c5"],
                     help="c3: the headline single cloud (default); c5: 256 independent clouds in one "
                          "HIP-graph batch (a separate line, not the headline metric)")
@@ -364,9 +369,51 @@ def main():
                 "kernel": "m2l", "launches_per_step": args.level - 1,
                 "algorithmic_flops_per_step": flops}
 
+    sweep = []
+    for order in [int(x) for x in args.order_sweep.split(",") if x.strip()]:
+        if "plan" in locals():
+            del plan, out
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        t0 = time.perf_counter()
+        ops_o = pbb.Operators(kernel, 1.0, args.level, order, args.eps)
+        t_pre_o = time.perf_counter() - t0
+        plan_o = pbb.Plan(ops_o, xyz)
+        log(f"order {order}: precompute {t_pre_o:.1f} s, ranks {ops_o.rank_f}/{ops_o.rank_s}")
+        tile_o = plan_o.tune_m2l(w, prec, args.max_rel_diff)
+        res_o = {}
+        for name, p in (("fp64", capi.PREC_FP64), (args.precision, prec)):
+            o = torch.zeros_like(w)
+            plan_o.matvec(w, o, precision=p)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            o.zero_()
+            plan_o.matvec(w, o, precision=p)
+            torch.cuda.synchronize()
+            res_o[name] = (time.perf_counter() - t0, o)
+        o64, om = res_o["fp64"][1], res_o[args.precision][1]
+        sweep.append({"order": order, "ranks": [ops_o.rank_f, ops_o.rank_s], "precompute_s": round(t_pre_o, 2),
+                      "fp64_ms": 1e3 * res_o["fp64"][0], "mixed_ms": 1e3 * res_o[args.precision][0],
+                      "speedup": res_o["fp64"][0] / res_o[args.precision][0],
+                      "fp64_rel_err_vs_direct": err(o64), "mixed_rel_err_vs_direct": err(om),
+                      "mixed_rel_diff_vs_fp64_path": float(torch.linalg.norm(om - o64) / torch.linalg.norm(o64)),
+                      "m2l_tile": tile_o})
+        log(f"order {order}: {sweep[-1]}")
+        del plan_o, o64, om, res_o
+
+    batch = None
+    if not args.no_batch:
+        if "plan" in locals():
+            del plan, w, out
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        log("256-cloud batch (forward + adjoint in one graph) ...")
+        batch = batch_clouds(args, torch, pbb, capi)
+
     cpu = None
     if not args.no_cpu_baseline:
-        del plan, w, out
+        if "plan" in locals():
+            del plan, w, out
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
         log("cpu baseline (oracle on whole problems of the reference's CPU-runnable shapes) ...")
@@ -397,23 +444,28 @@ def main():
         # compute.hpp:140-146): upload, GPU tree build and one matvec
         "with_plan_targets_per_s": N / (t_step + t_plan),
         "tuning": widths,
+        "batch_256_clouds": batch,
+        "order_sweep": sweep or None,
     }
     print(json.dumps(line), flush=True)
 
 
-def run_c5(args, torch, pbb, capi):
+def batch_clouds(args, torch, pbb, capi, log_prefix="batch"):
     """256 independent clouds (sizes log-uniform in [2e3, 2e5], U[0,1)^3
     positions, seed 1000 + k), tree level round(log8(N/60)) clamped to 3..6,
-    order 4..6 in turn, exp(-r/0.5), 16 columns, eps 1e-7: every cloud's
-    matvec captured into one HIP-graph batch (pbb_batch_*), FP64 and the
-    headline mixed mode.  Value: total targets / seconds per batch launch."""
+    order 4..6 in turn, exp(-r/0.5), 16 columns, eps 1e-7: the forward
+    matvec AND the adjoint of every cloud captured into one HIP graph
+    (pbb_batch_create_entries), M2L tiles tuned per operator set (within
+    1e-6 of FP64 in the mixed mode), FP64 and the headline mixed mode.
+    targets/s = total points / seconds per launch (each launch applies A
+    and A^T to every cloud)."""
     dev = torch.device("cuda:0")
     rng = np.random.default_rng(999)
     ncl, nc, eps = 256, 16, 1e-7
     sizes = np.exp(rng.uniform(np.log(2e3), np.log(2e5), ncl)).astype(int)
     kernel = pbb.exponential(0.5)
     t0 = time.perf_counter()
-    ops, plans, ws, outs = {}, [], [], []
+    ops, plans, ws, ps = {}, [], [], []
     for k in range(ncl):
         n = int(sizes[k])
         lvl = int(min(6, max(3, round(np.log(n / 60.0) / np.log(8.0)))))
@@ -424,17 +476,25 @@ def run_c5(args, torch, pbb, capi):
         plans.append(pbb.Plan(ops[(lvl, order)], xyz))
         ws.append(torch.tensor(np.random.default_rng(2000 + k).standard_normal((nc, n)), dtype=torch.float64,
                                device=dev))
+        ps.append(torch.tensor(np.random.default_rng(3000 + k).standard_normal((nc, n)), dtype=torch.float64,
+                               device=dev))
     t_setup = time.perf_counter() - t0
     total = int(sizes.sum())
-    log(f"c5: {ncl} clouds, {total} points, {len(ops)} operator sets, setup {t_setup:.1f} s")
+    log(f"{log_prefix}: {ncl} clouds, {total} points, {len(ops)} operator sets, setup {t_setup:.1f} s")
     precs = {"fp64": capi.PREC_FP64, "mixed": capi.PREC_MIXED, "f16x2": capi.PREC_MIXED_F16X2,
              "bf16x2": capi.PREC_MIXED_BF16X2}
     res = {}
+    t0 = time.perf_counter()
+    for name in ("fp64", args.precision):
+        pbb.Batch.tune(plans, ws, precs[name], args.max_rel_diff if name != "fp64" else 0.0)
+    t_tune = time.perf_counter() - t0
+    ent_plans = [p for p in plans for _ in (0, 1)]
+    ent_w = [w for pair in zip(ws, ps) for w in pair]
     for name in ("fp64", args.precision):
         prec = precs[name]
-        outs = [torch.zeros_like(w) for w in ws]
-        b = pbb.Batch(plans, ws, outs, nc, precision=prec)
-        for i in range(args.warmup):
+        outs = [torch.zeros_like(w) for w in ent_w]
+        b = pbb.Batch(ent_plans, ent_w, outs, nc, precision=prec, transpose=[False, True] * ncl)
+        for i in range(max(1, args.warmup)):
             b.launch()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -443,29 +503,42 @@ def run_c5(args, torch, pbb, capi):
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / args.steps
         res[name] = (dt, outs)
-        log(f"c5 {name}: {1e3 * dt:.1f} ms per batch")
-    # the batch equals one-plan matvecs (FP64, sampled clouds), and the mixed
+        log(f"{log_prefix} {name}: {1e3 * dt:.1f} ms per launch")
+        del b
+    # the graph equals the eager calls (FP64, sampled clouds); the mixed
     # batch stays within the mixed bar of the FP64 one
     _, o64 = res["fp64"]
-    same = max(float(torch.linalg.norm(plans[k].matvec(ws[k]) - o64[k]) / torch.linalg.norm(o64[k]))
-               for k in range(0, ncl, 37))
+    same = 0.0
+    for k in range(0, ncl, 37):
+        e = plans[k].matvec(ws[k])
+        same = max(same, float(torch.linalg.norm(e - o64[2 * k]) / torch.linalg.norm(o64[2 * k])))
+        e = plans[k].matvec_transpose(ps[k])
+        same = max(same, float(torch.linalg.norm(e - o64[2 * k + 1]) / torch.linalg.norm(o64[2 * k + 1])))
     dt, om = res[args.precision]
     num = sum(float(torch.linalg.norm(a - b) ** 2) for a, b in zip(om, o64))
     den = sum(float(torch.linalg.norm(b) ** 2) for b in o64)
+    return {"workload": f"{ncl} clouds of U[0,1)^3 points, sizes log-uniform in [2e3, 2e5] ({total} points), "
+                        f"levels 3..6, orders 4..6, exp(-r/0.5), {nc} columns, eps {eps}; forward and adjoint of "
+                        f"every cloud in one graph launch",
+            "clouds": ncl, "npoints": total, "ncols": nc, "precision": args.precision,
+            "targets_per_s": total / dt, "ms_per_launch": 1e3 * dt,
+            "fp64": {"targets_per_s": total / res["fp64"][0], "ms_per_launch": 1e3 * res["fp64"][0],
+                     "max_rel_diff_vs_eager": same},
+            "rel_diff_vs_fp64_batch": (num / den) ** 0.5, "setup_s": round(t_setup, 2),
+            "tune_s": round(t_tune, 2)}
+
+
+def run_c5(args, torch, pbb, capi):
+    """The 256-cloud batch as a line of its own (not the headline metric)."""
+    r = batch_clouds(args, torch, pbb, capi, "c5")
     line = {
-        "metric": "targets/s, 256-cloud batch (C5), one HIP graph", "value": total / dt, "unit": "targets/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": dtype_label(args.precision, None),
-        "data": "synthetic",
-        "config": {"workload": f"{ncl} clouds of U[0,1)^3 points, sizes log-uniform in [2e3, 2e5] "
-                               f"({total} points), levels 3..6, orders 4..6, exp(-r/0.5), {nc} columns, "
-                               f"eps {eps}", "clouds": ncl, "npoints": total, "ncols": nc,
-                   "precision": args.precision},
-        "fp64_batch": {"value": total / res["fp64"][0], "ms_per_step": 1e3 * res["fp64"][0],
-                       "max_rel_diff_vs_single_plan": same},
-        "rel_diff_vs_fp64_batch": (num / den) ** 0.5,
-        "setup_s": round(t_setup, 2),
+        "metric": "targets/s, 256-cloud batch, forward + adjoint in one HIP graph", "value": r["targets_per_s"],
+        "unit": "targets/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r["ms_per_launch"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": dtype_label(args.precision, None), "data": "synthetic",
+        "config": {k: r[k] for k in ("workload", "clouds", "npoints", "ncols", "precision")},
+        "fp64_batch": r["fp64"], "rel_diff_vs_fp64_batch": r["rel_diff_vs_fp64_batch"], "setup_s": r["setup_s"],
+        "tune_s": r["tune_s"],
     }
     print(json.dumps(line), flush=True)
 

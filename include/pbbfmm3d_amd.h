@@ -146,6 +146,19 @@ PBB_API pbb_status pbb_ops_create(const pbb_kernel_desc *kernel,
                                   double eps, int32_t use_chebyshev,
                                   pbb_ops **out);
 
+/* pbb_ops_create with flags.  By default (flags 0) the kernel blocks (the
+ * 316 n^6 evaluations of ComputeKernelCheb, H2_3D_Tree.cpp:277-374) and the
+ * compression E_d = U^T K_d V (:529-556) run on the current GPU when one is
+ * visible and the kernel has a device functor (no host callback); the two
+ * SVDs (:400, :471) always run on the host.  PBB_OPS_HOST_PRECOMPUTE keeps
+ * every step on the host.                                                 */
+#define PBB_OPS_HOST_PRECOMPUTE 1
+PBB_API pbb_status pbb_ops_create_ex(const pbb_kernel_desc *kernel,
+                                     pbb_host_kernel_fn host_eval, void *ctx,
+                                     double L, int32_t tree_level, int32_t order,
+                                     double eps, int32_t use_chebyshev, int32_t flags,
+                                     pbb_ops **out);
+
 /* Builds operators from given arrays (the layout FMMReadMatrices reads,
  * H2_3D_Tree.cpp:186-239): U nsets x (n3 x rank_f), VT nsets x (rank_s x
  * n3), E nsets x 316 x (rank_f x rank_s), all column-major FP64.          */
@@ -165,7 +178,11 @@ PBB_API pbb_status pbb_ops_create_from_arrays(
  * pbb_ops_read_cache returns PBB_ERR_NOT_FOUND when a file is missing and
  * PBB_ERR_INVALID_ARG when the files do not fit the order, or (per-level
  * operators) were written for another L or tree_level, as
- * PrecomputeAvailable (:102-131) decides.                                 */
+ * PrecomputeAvailable (:102-131) decides.  pbb_ops_write_cache also writes
+ * "<k_path>.params", the kernel id, symmetry, homogeneity, parameters,
+ * order, eps (and L, tree_level for per-level operators) the files hold;
+ * pbb_ops_read_cache refuses files whose parameter file names others, and
+ * reads files without one (the reference's) as the reference does.       */
 PBB_API pbb_status pbb_ops_write_cache(const pbb_ops *ops, const char *k_path,
                                        const char *u_path, const char *v_path);
 PBB_API pbb_status pbb_ops_read_cache(const pbb_kernel_desc *kernel, double L,
@@ -307,11 +324,24 @@ PBB_API pbb_status pbb_plan_set_timing(pbb_plan *plan, int32_t enable);
 PBB_API pbb_status pbb_plan_set_m2l_tile(pbb_plan *plan, int32_t precision, int32_t width,
                                          int32_t cols_per_box, int32_t flags);
 
+/* The M2L tile of one tree level (2..tree_level) of the precision's class,
+ * overriding the plan-wide tile there (pbb_plan_set_m2l_tile clears the
+ * overrides of its class).  PBB_M2L_SPLIT_ACC stays a plan-wide choice: it
+ * also sets the VT / U precision of the matvec.  pbb_plan_get_m2l_tile
+ * returns the tile in force at a level (level -1: the plan-wide tile) as
+ * tile[3] = {width, cols_per_box, flags}.  No reference counterpart.     */
+PBB_API pbb_status pbb_plan_set_m2l_tile_level(pbb_plan *plan, int32_t precision, int32_t level,
+                                               int32_t width, int32_t cols_per_box, int32_t flags);
+PBB_API pbb_status pbb_plan_get_m2l_tile(const pbb_plan *plan, int32_t precision, int32_t level,
+                                         int32_t *tile);
+
 /* Autotuner: times the M2L of one matvec on w_dev (device FP64, N x ncols)
  * for every candidate tile of the precision's class, keeps the fastest on
  * the plan and returns it in best[3] = {width, cols_per_box, flags} (may be
- * NULL).  Runs on `stream`, waits for it, and uses a scratch output of its
- * own.  No reference counterpart.                                         */
+ * NULL).  From the same runs it also picks the fastest tile of each tree
+ * level and keeps those per-level tiles when the combination is faster
+ * (pbb_plan_get_m2l_tile reads them).  Runs on `stream`, waits for it, and
+ * uses a scratch output of its own.  No reference counterpart.           */
 PBB_API pbb_status pbb_plan_tune_m2l(pbb_plan *plan, const double *w_dev, int32_t ncols,
                                      int32_t precision, void *stream, int32_t *best);
 

@@ -56,6 +56,7 @@ def M2L_DIR_GROUPS(g):
 
 NUM_PHASES = 8
 PLAN_HOST_TREE = 1
+OPS_HOST_PRECOMPUTE = 1
 BATCH_TRANSPOSE = 1
 BATCH_OVERWRITE = 2
 PHASE_NAMES = ("p2m", "m2m", "vt_proxy", "m2l", "u_expand", "l2l", "near", "total")
@@ -98,6 +99,8 @@ _SIGNATURES = {
     "pbb_device_count": (C.c_int, []),
     "pbb_ops_create": (C.c_int, [C.POINTER(KernelDesc), HOST_KERNEL_FN, _vp, C.c_double,
                                  C.c_int32, C.c_int32, C.c_double, C.c_int32, C.POINTER(_vp)]),
+    "pbb_ops_create_ex": (C.c_int, [C.POINTER(KernelDesc), HOST_KERNEL_FN, _vp, C.c_double,
+                                    C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_int32, C.POINTER(_vp)]),
     "pbb_ops_create_from_arrays": (C.c_int, [C.POINTER(KernelDesc), C.c_double, C.c_int32,
                                              C.c_int32, C.c_double, C.c_int32, C.c_int32,
                                              C.c_int32, _dp, _dp, _dp, C.POINTER(_vp)]),
@@ -124,6 +127,8 @@ _SIGNATURES = {
     "pbb_plan_timings": (C.c_int, [_vp, _dp]),
     "pbb_plan_set_timing": (C.c_int, [_vp, C.c_int32]),
     "pbb_plan_set_m2l_tile": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "pbb_plan_set_m2l_tile_level": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "pbb_plan_get_m2l_tile": (C.c_int, [_vp, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
     "pbb_plan_tune_m2l": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, _vp, C.POINTER(C.c_int32)]),
     "pbb_plan_tune_m2l_within": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, C.c_double, _vp,
                                            C.POINTER(C.c_int32), C.POINTER(C.c_double)]),

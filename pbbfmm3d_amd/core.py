@@ -115,7 +115,11 @@ class Operators:
 
     def __init__(self, kernel: Kernel, L: float, tree_level: int, order: int, eps: float,
                  use_chebyshev: int = 1,
-                 host_eval: Optional[Callable[[np.ndarray, np.ndarray], float]] = None):
+                 host_eval: Optional[Callable[[np.ndarray, np.ndarray], float]] = None,
+                 host_precompute: bool = False):
+        """The kernel blocks and the E compression run on the GPU when one
+        is visible (host_precompute=True keeps them on the host); the SVDs
+        run on the host."""
         self.kernel = kernel
         self._cb = None
         L_ = capi.lib()
@@ -129,8 +133,9 @@ class Operators:
         else:
             cb = capi.HOST_KERNEL_FN()
         d = kernel.desc()
-        capi.check(L_.pbb_ops_create(C.byref(d), cb, None, float(L), int(tree_level), int(order),
-                                     float(eps), int(use_chebyshev), C.byref(h)))
+        capi.check(L_.pbb_ops_create_ex(C.byref(d), cb, None, float(L), int(tree_level), int(order),
+                                        float(eps), int(use_chebyshev),
+                                        capi.OPS_HOST_PRECOMPUTE if host_precompute else 0, C.byref(h)))
         self._h = h
         self._info()
 
@@ -360,6 +365,21 @@ class Plan:
         capi.check(capi.lib().pbb_plan_set_m2l_tile(self._h, int(precision), int(width), int(cols_per_box),
                                                     int(flags)))
 
+    def set_m2l_tile_level(self, level: int, width: int, cols_per_box: int = 32, flags: int = 0,
+                           precision: int = capi.PREC_FP64):
+        """The M2L tile of one tree level (overrides the plan-wide tile)."""
+        capi.check(capi.lib().pbb_plan_set_m2l_tile_level(self._h, int(precision), int(level), int(width),
+                                                          int(cols_per_box), int(flags)))
+
+    def m2l_tiles(self, precision: int = capi.PREC_FP64) -> dict:
+        """The M2L tile in force at each level 2..L, {level: (width, cols_per_box, flags)}."""
+        out = {}
+        t = (C.c_int32 * 3)()
+        for lvl in range(2, self.info()["tree_level"] + 1):
+            capi.check(capi.lib().pbb_plan_get_m2l_tile(self._h, int(precision), lvl, t))
+            out[lvl] = (t[0], t[1], t[2])
+        return out
+
     def tune_m2l(self, weights, precision: int = capi.PREC_FP64, max_rel_diff: float = 0.0) -> dict:
         """Times the candidate M2L tiles on these weights (float64 CUDA
         tensor, (ncols, N) or (N,)), keeps the fastest and returns it as
@@ -375,7 +395,8 @@ class Plan:
         capi.check(capi.lib().pbb_plan_tune_m2l_within(
             self._h, C.c_void_p(w.data_ptr()), int(ncols), int(precision), float(max_rel_diff),
             C.c_void_p(stream), best, C.byref(got)))
-        return {"width": best[0], "cols_per_box": best[1], "flags": best[2], "rel_diff": got.value}
+        return {"width": best[0], "cols_per_box": best[1], "flags": best[2], "rel_diff": got.value,
+                "levels": self.m2l_tiles(precision)}
 
     def timings(self) -> dict:
         ms = np.zeros(capi.NUM_PHASES)
@@ -434,6 +455,10 @@ class Batch:
             t = p0.tune_m2l(w0, precision, max_rel_diff)
             for p, _ in members:
                 p.set_m2l_tile(t["width"], t["cols_per_box"], t["flags"], precision)
+                for lvl, (wd, cpb, fl) in t["levels"].items():
+                    if (wd, cpb, fl) != (t["width"], t["cols_per_box"], t["flags"]) and \
+                            lvl <= p.info()["tree_level"]:
+                        p.set_m2l_tile_level(lvl, wd, cpb, fl, precision)
             tiles[key] = t
         return tiles
 

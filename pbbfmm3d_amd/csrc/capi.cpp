@@ -301,6 +301,9 @@ struct pbb_plan {
     struct M2LTile {
         int width = 0, cpt = 0, flags = 0;  // width 0 = built-in choice
     } m2l_tile[2];                          // FP64, mixed
+    M2LTile m2l_level[2][16];               // per-level tiles (width 0: m2l_tile)
+    hipEvent_t lev_ev[17] = {};             // M2L level boundaries (autotuner)
+    bool lev_made = false, lev_timing = false;
     hipEvent_t ev[PBB_NUM_PHASES] = {};
     bool events_made = false;
     double last_ms[PBB_NUM_PHASES] = {0};
@@ -321,6 +324,8 @@ struct pbb_plan {
         dfree(ws_block); dfree(gF); dfree(gU);
         if (events_made)
             for (auto &e : ev) (void)hipEventDestroy(e);
+        if (lev_made)
+            for (auto &e : lev_ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
         if (prev >= 0) (void)hipSetDevice(prev);
     }
@@ -410,13 +415,131 @@ static void validate_params(const pbb_kernel_desc *kernel, double L, int tree_le
         fail(PBB_ERR_INVALID_ARG, "unknown kernel id");
 }
 
+}  // extern "C"
+
+namespace {
+// Device buffers of one operator precompute on the GPU (kernel blocks and
+// the compression), freed with it.
+struct DevicePrecompute {
+    int device = -1;
+    hipStream_t st = nullptr;
+    const HostOps *h = nullptr;
+    double *nodes = nullptr, *Kw = nullptr, *Kcell = nullptr, *X = nullptr, *Ut = nullptr, *V = nullptr,
+           *E = nullptr;
+    int *dirs = nullptr;
+    size_t kcell_n = 0, x_n = 0, e_n = 0;
+
+    void start(const HostOps &ops, int dev)
+    {
+        h = &ops;
+        device = dev;
+        check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+    }
+    // K blocks of one operator set on the device; the host copy feeds the
+    // SVDs.  The buffers are made at the first call, once precompute_ops has
+    // set the order's nodes, weights and n3.
+    void blocks(double box_len, std::vector<double> &Kc)
+    {
+        const HostOps &o = *h;
+        const size_t kn = (size_t)316 * o.n3 * o.n3;
+        if (o.n3 <= 0 || o.w.nodes.size() != (size_t)o.n || o.w.Kw.size() != (size_t)o.n3)
+            fail(PBB_ERR_INTERNAL, "kernel blocks requested before the Chebyshev weights");
+        if (!Kcell) {
+            nodes = dupload(o.w.nodes, "Chebyshev nodes");
+            Kw = dupload(o.w.Kw, "Kweights");
+            std::vector<int> d3;
+            for (int k1 = -3; k1 < 4; k1++)
+                for (int k2 = -3; k2 < 4; k2++)
+                    for (int k3 = -3; k3 < 4; k3++)
+                        if (std::abs(k1) > 1 || std::abs(k2) > 1 || std::abs(k3) > 1)
+                            d3.insert(d3.end(), {k1, k2, k3});
+            dirs = dupload(d3, "directions");
+            Kcell = dalloc<double>(kn, "kernel blocks");
+            kcell_n = kn;
+        }
+        if (kcell_n != kn) fail(PBB_ERR_INTERNAL, "kernel block buffer made for another order");
+        dev::KernelBlockArgs a{};
+        a.kcell_len = (int64_t)kcell_n;
+        a.n = o.n; a.n3 = o.n3; a.symm = o.kernel.symmetry; a.symm_num = 158 * (2 - std::abs(a.symm));
+        a.box_len = box_len;
+        std::memcpy(a.kp.p, o.kernel.param, sizeof(a.kp.p));
+        a.nodes = nodes; a.Kw = Kw; a.dirs = dirs; a.Kcell = Kcell;
+        dev::launch_kernel_blocks(a, o.kernel.id, st);
+        Kc.resize((size_t)316 * o.n3 * o.n3);
+        check(hipMemcpyAsync(Kc.data(), Kcell, sizeof(double) * Kc.size(), hipMemcpyDeviceToHost, st), "copy blocks");
+        check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    }
+    // E_d = U^T K_d V: X = U^T [K_0 .. K_315] in one GEMM, then E_d = X_d V
+    void compress(int rf, int rs, const double *U, const double *VT, double *Eh)
+    {
+        const int n3 = h->n3;
+        if (!Kcell || kcell_n != (size_t)316 * n3 * n3) fail(PBB_ERR_INTERNAL, "compression before the kernel blocks");
+        std::vector<double> ut((size_t)rf * n3), v((size_t)n3 * rs);
+        for (int a = 0; a < rf; a++)
+            for (int i = 0; i < n3; i++) ut[a + (size_t)i * rf] = U[i + (size_t)a * n3];
+        for (int j = 0; j < n3; j++)
+            for (int b = 0; b < rs; b++) v[j + (size_t)b * n3] = VT[b + (size_t)j * rs];
+        dfree(Ut);
+        dfree(V);
+        Ut = V = nullptr;
+        Ut = dupload(ut, "U^T");
+        V = dupload(v, "V");
+        if (x_n < (size_t)rf * 316 * n3) {
+            dfree(X);
+            X = nullptr;
+            X = dalloc<double>((size_t)rf * 316 * n3, "U^T K");
+            x_n = (size_t)rf * 316 * n3;
+        }
+        if (e_n < (size_t)316 * rf * rs) {
+            dfree(E);
+            E = nullptr;
+            E = dalloc<double>((size_t)316 * rf * rs, "E");
+            e_n = (size_t)316 * rf * rs;
+        }
+        dev::GemmArgs g{};
+        g.M = rf; g.K = n3; g.ldb = n3; g.ldc = rf; g.ncol = (int64_t)316 * n3; g.alpha = 1.0;
+        g.A = Ut; g.B = Kcell; g.C = X;
+        dev::launch_gemm(g, st);
+        for (int d = 0; d < 316; d++) {
+            dev::GemmArgs e{};
+            e.M = rf; e.K = n3; e.ldb = n3; e.ldc = rf; e.ncol = rs; e.alpha = 1.0;
+            e.A = X + (size_t)d * rf * n3; e.B = V; e.C = E + (size_t)d * rf * rs;
+            dev::launch_gemm(e, st);
+        }
+        check(hipMemcpyAsync(Eh, E, sizeof(double) * (size_t)316 * rf * rs, hipMemcpyDeviceToHost, st), "copy E");
+        check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    }
+    ~DevicePrecompute()
+    {
+        if (device < 0) return;
+        int prev = -1;
+        (void)hipGetDevice(&prev);
+        (void)hipSetDevice(device);
+        if (st) (void)hipStreamSynchronize(st);
+        dfree(nodes); dfree(Kw); dfree(dirs); dfree(Kcell); dfree(X); dfree(Ut); dfree(V); dfree(E);
+        if (st) (void)hipStreamDestroy(st);
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+}  // namespace
+
+extern "C" {
+
 PBB_API pbb_status pbb_ops_create(const pbb_kernel_desc *kernel, pbb_host_kernel_fn host_eval, void *ctx, double L,
                                   int32_t tree_level, int32_t order, double eps, int32_t use_chebyshev,
                                   pbb_ops **out)
 {
+    return pbb_ops_create_ex(kernel, host_eval, ctx, L, tree_level, order, eps, use_chebyshev, 0, out);
+}
+
+PBB_API pbb_status pbb_ops_create_ex(const pbb_kernel_desc *kernel, pbb_host_kernel_fn host_eval, void *ctx,
+                                     double L, int32_t tree_level, int32_t order, double eps,
+                                     int32_t use_chebyshev, int32_t flags, pbb_ops **out)
+{
     return guarded([&] {
         if (!out) fail(PBB_ERR_INVALID_ARG, "out is NULL");
         *out = nullptr;
+        if (flags & ~PBB_OPS_HOST_PRECOMPUTE) fail(PBB_ERR_INVALID_ARG, "unknown operator flags");
         validate_params(kernel, L, tree_level, order, eps, use_chebyshev);
         if (!host_eval && !kernel_id_valid(kernel->id))
             fail(PBB_ERR_INVALID_ARG, "no host callback and no kernel functor id");
@@ -436,7 +559,25 @@ PBB_API pbb_status pbb_ops_create(const pbb_kernel_desc *kernel, pbb_host_kernel
             std::memcpy(p, kernel->param, sizeof(p));
             eval = [id, p](const double *t, const double *s) { return kernel_eval_host(id, p, t, s); };
         }
-        precompute_ops(h, eval);
+        // the kernel blocks and the compression run on the current GPU when
+        // there is one and the kernel has a device functor; the SVDs stay on
+        // the host
+        int ndev = 0;
+        const bool on_gpu = !(flags & PBB_OPS_HOST_PRECOMPUTE) && !host_eval && kernel->id != PBB_KERNEL_POLY_XY &&
+                            kernel_id_valid(kernel->id) && hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0;
+        (void)hipGetLastError();
+        if (on_gpu) {
+            DevicePrecompute dp;
+            dp.start(h, current_device());
+            PrecomputeDevice pd;
+            pd.blocks = [&](double box_len, std::vector<double> &Kc) { dp.blocks(box_len, Kc); };
+            pd.compress = [&](int rf, int rs, const double *U, const double *VT, double *E) {
+                dp.compress(rf, rs, U, VT, E);
+            };
+            precompute_ops(h, eval, &pd);
+        } else {
+            precompute_ops(h, eval);
+        }
         *out = ops.release();
     });
 }
@@ -520,11 +661,34 @@ void read_all(FILE *f, void *p, size_t size, size_t n, const char *what)
 }
 }  // namespace
 
+namespace {
+// The parameters operator cache files were made for, kept beside the K file
+// as "<K file>.params" (the reference names its files by kernel type and
+// order only, H2_3D_Tree.cpp:81-83, so a cache made with another eps or
+// kernel parameter would be read silently).
+std::string cache_params(const pbb_kernel_desc &k, double L, int tree_level, int order, double eps, bool homog)
+{
+    char buf[512];
+    std::snprintf(buf, sizeof(buf),
+                  "pbbfmm3d_amd operator cache: kernel %d symmetry %d homogen %.17g params %.17g %.17g %.17g %.17g "
+                  "order %d eps %.17g L %.17g tree_level %d\n",
+                  k.id, k.symmetry, k.homogen, k.param[0], k.param[1], k.param[2], k.param[3], order, eps,
+                  homog ? 1.0 : L, homog ? 0 : tree_level);
+    return buf;
+}
+}  // namespace
+
 PBB_API pbb_status pbb_ops_write_cache(const pbb_ops *ops, const char *k_path, const char *u_path, const char *v_path)
 {
     return guarded([&] {
         if (!ops || !k_path || !u_path || !v_path) fail(PBB_ERR_INVALID_ARG, "NULL argument");
         const HostOps &h = ops->h;
+        {
+            File fp((std::string(k_path) + ".params").c_str(), "w");
+            if (!fp.f) fail(PBB_ERR_INVALID_ARG, "cannot open the cache parameter file for writing");
+            const std::string t = cache_params(h.kernel, h.L, h.tree_level, h.n, h.eps, h.homog);
+            write_all(fp.f, t.data(), 1, t.size(), "parameter file");
+        }
         File fk(k_path, "wb"), fu(u_path, "wb"), fv(v_path, "wb");
         if (!fk.f || !fu.f || !fv.f) fail(PBB_ERR_INVALID_ARG, "cannot open the cache files for writing");
         const double L = h.L;
@@ -549,6 +713,17 @@ PBB_API pbb_status pbb_ops_read_cache(const pbb_kernel_desc *kernel, double L, i
         validate_params(kernel, L, tree_level, order, eps, 1);
         File fk(k_path, "rb"), fu(u_path, "rb"), fv(v_path, "rb");
         if (!fk.f || !fu.f || !fv.f) fail(PBB_ERR_NOT_FOUND, "cache files not found");
+        {  // files written here carry their parameters; the reference's do not
+            File fp((std::string(k_path) + ".params").c_str(), "r");
+            if (fp.f) {
+                char buf[512] = {0};
+                const size_t got = std::fread(buf, 1, sizeof(buf) - 1, fp.f);
+                const bool homog = std::fabs(kernel->homogen) > 0.1;
+                if (std::string(buf, got) != cache_params(*kernel, L, tree_level, order, eps, homog))
+                    fail(PBB_ERR_INVALID_ARG, "the cache files were written for other kernel parameters, eps, order "
+                                              "or tree (see " + std::string(k_path) + ".params)");
+            }
+        }
         const long nk = file_size(fk.f), nu = file_size(fu.f), nv = file_size(fv.f);
         double len = 0;
         int32_t lvl = 0, rf = 0, rs = 0;
@@ -926,6 +1101,43 @@ PBB_API pbb_status pbb_plan_set_m2l_tile(pbb_plan *plan, int32_t precision, int3
             fail(PBB_ERR_INVALID_ARG, "PBB_M2L_SPLIT_ACC needs PBB_M2L_WAVE_ROWS");
         if (((flags >> 8) & 0xff) > 64) fail(PBB_ERR_INVALID_ARG, "at most 64 M2L direction groups");
         plan->m2l_tile[mixed ? 1 : 0] = {width, width ? cols_per_box : 0, flags};
+        for (auto &t : plan->m2l_level[mixed ? 1 : 0]) t = pbb_plan::M2LTile{};
+    });
+}
+
+PBB_API pbb_status pbb_plan_set_m2l_tile_level(pbb_plan *plan, int32_t precision, int32_t level, int32_t width,
+                                               int32_t cols_per_box, int32_t flags)
+{
+    return guarded([&] {
+        if (!plan) fail(PBB_ERR_INVALID_ARG, "plan is NULL");
+        if (level < 2 || level > plan->Lv) fail(PBB_ERR_INVALID_ARG, "level must be in [2, tree_level]");
+        const bool mixed = precision != PBB_PREC_FP64;
+        // validate as the plan-wide setter does, then keep it for one level
+        const pbb_plan::M2LTile keep = plan->m2l_tile[mixed ? 1 : 0];
+        pbb_plan::M2LTile levels[16];
+        std::copy(std::begin(plan->m2l_level[mixed ? 1 : 0]), std::end(plan->m2l_level[mixed ? 1 : 0]), levels);
+        const pbb_status st = pbb_plan_set_m2l_tile(plan, precision, width, cols_per_box, flags);
+        const pbb_plan::M2LTile t = plan->m2l_tile[mixed ? 1 : 0];
+        plan->m2l_tile[mixed ? 1 : 0] = keep;
+        std::copy(std::begin(levels), std::end(levels), plan->m2l_level[mixed ? 1 : 0]);
+        if (st != PBB_OK) fail(st, g_last_error);
+        plan->m2l_level[mixed ? 1 : 0][level] = t;
+    });
+}
+
+PBB_API pbb_status pbb_plan_get_m2l_tile(const pbb_plan *plan, int32_t precision, int32_t level, int32_t *tile)
+{
+    return guarded([&] {
+        if (!plan || !tile) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
+            fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
+        if (level != -1 && (level < 2 || level > plan->Lv)) fail(PBB_ERR_INVALID_ARG, "level must be -1 or in [2, tree_level]");
+        const int c = precision != PBB_PREC_FP64 ? 1 : 0;
+        const pbb_plan::M2LTile &t = level >= 0 && plan->m2l_level[c][level].width ? plan->m2l_level[c][level]
+                                                                                   : plan->m2l_tile[c];
+        tile[0] = t.width;
+        tile[1] = t.cpt;
+        tile[2] = t.flags;
     });
 }
 
@@ -956,10 +1168,33 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
                        hipStream_t st, int prec, bool build);
 
 // One matvec of the plan's operator set o in its own workspace.
+// Columns are independent: when the workspace of all ncols columns does not
+// fit beside what the device already holds, the matvec runs over chunks of
+// columns, each a complete FMM pass (the same sums, so the same results).
+// A pass that keeps its expansions for the caller (out == nullptr) is not
+// split.
 static void matvec_own(pbb_plan *p, pbb_ops *o, const double *w, int ncols, double *out, hipStream_t st, int prec)
 {
-    p->ensure_workspace(ncols, prec != PBB_PREC_FP64);
-    matvec_run(p, o, p->W, w, ncols, out, st, prec, true);
+    const bool mixed = prec != PBB_PREC_FP64;
+    int chunk = ncols;
+    if (out) {
+        Workspace tmp;
+        size_t need = layout_workspace(*p, chunk, mixed, nullptr, tmp);
+        if (need > p->ws_bytes) {
+            size_t free_b = 0, total_b = 0;
+            check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+            const size_t room = p->ws_bytes + free_b - std::min(free_b, (size_t)(1ull << 30));  // keep 1 GiB
+            while (chunk > 1 && need > room) {
+                chunk = (chunk + 1) / 2;
+                need = layout_workspace(*p, chunk, mixed, nullptr, tmp);
+            }
+        }
+    }
+    for (int c0 = 0; c0 < ncols; c0 += chunk) {
+        const int cn = std::min(chunk, ncols - c0);
+        p->ensure_workspace(cn, mixed);
+        matvec_run(p, o, p->W, w + (size_t)c0 * p->N, cn, out ? out + (size_t)c0 * p->N : nullptr, st, prec, true);
+    }
 }
 
 // Relative difference ||a - b|| / ||b|| of two device FP64 arrays of n
@@ -1009,6 +1244,15 @@ static void tune_m2l(pbb_plan *plan, const double *w_dev, int ncols, int precisi
     const int ncand = mixed ? (int)(sizeof(cmx) / sizeof(cmx[0])) : (int)(sizeof(c64) / sizeof(c64[0]));
     pbb_plan::M2LTile bestt{}, closest{};
     double best_ms = 0, best_err = 0, closest_err = INFINITY;
+    const int cls = mixed ? 1 : 0, Lv = plan->Lv;
+    std::vector<std::vector<float>> lev_ms(ncand, std::vector<float>(Lv + 1, 0.0f));
+    std::vector<bool> eligible(ncand, false);
+    for (auto &t : plan->m2l_level[cls]) t = pbb_plan::M2LTile{};
+    if (!plan->lev_made) {
+        for (auto &e : plan->lev_ev) check(hipEventCreate(&e), "hipEventCreate");
+        plan->lev_made = true;
+    }
+    plan->lev_timing = true;
     try {
         scratch = dalloc<double>((size_t)nout, "tuning output");
         if (bounded) {
@@ -1026,34 +1270,69 @@ static void tune_m2l(pbb_plan *plan, const double *w_dev, int ncols, int precisi
             check(hipStreamSynchronize(st), "hipStreamSynchronize");
             float t = 0;
             check(hipEventElapsedTime(&t, plan->ev[3], plan->ev[4]), "hipEventElapsedTime");
+            for (int l = 2; l <= Lv; l++)
+                check(hipEventElapsedTime(&lev_ms[c][l], plan->lev_ev[l], plan->lev_ev[l + 1]), "hipEventElapsedTime");
             const double err = bounded ? device_rel_diff(scratch, ref, nout, part, st) : 0.0;
             if (err < closest_err) {
                 closest = plan->m2l_tile[mixed ? 1 : 0];
                 closest_err = err;
             }
             if (bounded && !(err <= max_rel_diff)) continue;
+            eligible[c] = true;
             if (bestt.width == 0 || t < best_ms) {
                 bestt = plan->m2l_tile[mixed ? 1 : 0];
                 best_ms = t;
                 best_err = err;
             }
         }
+        if (bestt.width == 0) {  // nothing within the bound: the most accurate
+            bestt = closest;
+            best_err = closest_err;
+        }
+        plan->m2l_tile[cls] = bestt;
+        // per level, the fastest eligible candidate with the same
+        // cross-product-sum choice (which decides the VT / U precision of
+        // the whole matvec); kept if the combination is faster still, and,
+        // with a bound, within it
+        const int sacc = bestt.flags & PBB_M2L_SPLIT_ACC;
+        pbb_plan::M2LTile lev[16];
+        bool differs = false;
+        int bi = -1;
+        for (int c = 0; c < ncand; c++)
+            if (cands[c][0] == bestt.width && std::min(cands[c][1], ncols) == bestt.cpt && cands[c][2] == bestt.flags)
+                bi = c;
+        for (int l = 2; l <= Lv && bi >= 0; l++) {
+            int k = bi;
+            for (int c = 0; c < ncand; c++)
+                if (eligible[c] && (cands[c][2] & PBB_M2L_SPLIT_ACC) == sacc && lev_ms[c][l] < lev_ms[k][l]) k = c;
+            lev[l] = {cands[k][0], std::min(cands[k][1], ncols), cands[k][2]};
+            differs |= k != bi;
+        }
+        if (differs) {
+            std::copy(std::begin(lev), std::end(lev), plan->m2l_level[cls]);
+            if (bounded) dev::launch_fill_zero(scratch, nout, st);
+            matvec_own(plan, plan->ops, w_dev, ncols, scratch, st, precision);
+            check(hipStreamSynchronize(st), "hipStreamSynchronize");
+            float t = 0;
+            check(hipEventElapsedTime(&t, plan->ev[3], plan->ev[4]), "hipEventElapsedTime");
+            const double err = bounded ? device_rel_diff(scratch, ref, nout, part, st) : 0.0;
+            if (t < best_ms && (!bounded || err <= max_rel_diff)) best_err = err;
+            else
+                for (auto &x : plan->m2l_level[cls]) x = pbb_plan::M2LTile{};
+        }
     } catch (...) {
         dfree(scratch);
         dfree(ref);
         dfree(part);
         plan->timing = was_timing;
+        plan->lev_timing = false;
         throw;
     }
     dfree(scratch);
     dfree(ref);
     dfree(part);
     plan->timing = was_timing;
-    if (bestt.width == 0) {  // nothing within the bound: the most accurate
-        bestt = closest;
-        best_err = closest_err;
-    }
-    plan->m2l_tile[mixed ? 1 : 0] = bestt;
+    plan->lev_timing = false;
     if (best) {
         best[0] = bestt.width;
         best[1] = bestt.cpt;
@@ -1149,6 +1428,9 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
     // instead of L2L + L2P; both are exact rewritings (see p2m_levels)
     const bool direct = n >= 4 && n <= 8;
     const bool direct_down = direct && out != nullptr && Lv >= 2;
+    // the mixed modes keep S / F in FP32 when the direct passes read them
+    // (not with the FP64 VT / U of the cross-product-sum tile)
+    const bool s32 = mixed && direct_down && !split_acc;
     dev::launch_gather_weights(w, p->perm, p->N, ncols, W.ws, st);
     if (direct) dev::launch_gather_weights_t(w, p->perm, p->N, ncols, W.wsT, st);
     const int cc_p2m = std::max(1, std::min({ncols, 64, 4096 / n3}));
@@ -1159,10 +1441,10 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         fill_levels(p, a.lt);
         a.items = p->up_items;
         a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.wsT = W.wsT; a.S = W.S;
-        a.scratch = W.up;
+        a.scratch = W.up; a.s32 = s32;
         dev::UpReduce r{};
         r.ncols = ncols; r.n3 = n3; r.nred = p->n_up_reds; r.reds = p->up_reds;
-        r.scratch = W.up; r.S = W.S;
+        r.scratch = W.up; r.S = W.S; r.s32 = s32;
         dev::launch_p2m_levels(a, p->n_up_items, r, st);
     } else if (Lv >= 2) {
         const DevLevel &LL = p->lv[Lv];
@@ -1186,7 +1468,9 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         dev::GemmArgs g{};
         g.M = h.rs; g.K = n3; g.ldb = n3; g.ldc = ldP; g.ncol = D.nb * ncols; g.alpha = 1.0;
         g.A = o->d_VTw + (size_t)h.ops_set(l) * h.rs * n3;
-        g.B = W.S + (size_t)D.off * ncols * n3;
+        g.b_f32 = s32 ? 1 : 0;
+        g.B = s32 ? (const void *)((float *)W.S + (size_t)D.off * ncols * n3)
+                  : (const void *)(W.S + (size_t)D.off * ncols * n3);
         if (mixed) {
             g.c_f32 = 1;
             g.C = W.P32 + (size_t)D.off * ncols * ldP;
@@ -1195,7 +1479,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         }
         if (SA) {  // three-part BF16 matrix cores (P is FP32 in the mixed modes)
             dev::GemmSplitArgs q{};
-            q.M = g.M; q.K = g.K; q.ldb = g.ldb; q.ldc = g.ldc; q.b_f32 = 0; q.c_f32 = 1; q.nch = SA->nch;
+            q.M = g.M; q.K = g.K; q.ldb = g.ldb; q.ldc = g.ldc; q.b_f32 = g.b_f32; q.c_f32 = 1; q.nch = SA->nch;
             q.ncol = g.ncol; q.alpha = 1.0; q.As = (const uint16_t *)SA->d + h.ops_set(l) * SA->per_set;
             q.B = g.B; q.C = g.C;
             dev::launch_gemm_split3(q, st);
@@ -1216,22 +1500,22 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
     // per tile keep the share of empty (box, direction) slots low.
     // Mixed: the row-wave kernel, W = 128 (8 boxes x 16 columns), chunk-major
     // steps, two workgroups per CU.
-    const pbb_plan::M2LTile &t64 = p->m2l_tile[0];
-    const int cpt = t64.width ? std::min(ncols, t64.cpt) : std::min(ncols, 64);
-    const int width = t64.width ? t64.width : ncols >= 32 || ncols <= 16 ? 128 : 64;
-    const int TB = std::max(1, std::min(16, width / cpt));
-    const int nchunks = (ncols + cpt - 1) / cpt;
-    const int cptm = tmx.width ? std::min(ncols, tmx.cpt) : std::min(ncols, 16);
-    const int nchunks_m = (ncols + cptm - 1) / cptm;
-    const int width_m = tmx.width ? tmx.width : 128;
-    const int groups_m = tmx.width ? std::max(1, (tmx.flags >> 8) & 0xff) : 1;
+    auto lev_mark = [&](int l) {
+        if (p->timing && p->lev_timing) check(hipEventRecord(p->lev_ev[l], st), "hipEventRecord");
+    };
     for (int l = 2; l <= Lv && mixed; l++) {
+        lev_mark(l);
         const DevLevel &D = p->lv[l];
+        const pbb_plan::M2LTile &tl = p->m2l_level[1][l].width ? p->m2l_level[1][l] : tmx;
+        const int cptm = tl.width ? std::min(ncols, tl.cpt) : std::min(ncols, 16);
+        const int nchunks_m = (ncols + cptm - 1) / cptm;
+        const int width_m = tl.width ? tl.width : 128;
+        const int groups_m = tl.width ? std::max(1, (tl.flags >> 8) & 0xff) : 1;
         dev::M2LMixArgs a{};
         a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = std::max(1, std::min(16, width_m / cptm)); a.cpt = cptm;
-        a.chunk_major = tmx.width ? (tmx.flags & PBB_M2L_CHUNK_MAJOR) : 1;
-        a.row_split = tmx.width ? (tmx.flags & PBB_M2L_ROW_SPLIT) != 0 : 0;
-        a.wave_rows = tmx.width ? (tmx.flags & PBB_M2L_WAVE_ROWS) != 0 : 1;
+        a.chunk_major = tl.width ? (tl.flags & PBB_M2L_CHUNK_MAJOR) : 1;
+        a.row_split = tl.width ? (tl.flags & PBB_M2L_ROW_SPLIT) != 0 : 0;
+        a.wave_rows = tl.width ? (tl.flags & PBB_M2L_WAVE_ROWS) != 0 : 1;
         if (wave_rows_forced(h)) {
             a.wave_rows = 1;
             a.row_split = 0;
@@ -1263,7 +1547,13 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         }
     }
     for (int l = 2; l <= Lv && !mixed; l++) {
+        lev_mark(l);
         const DevLevel &D = p->lv[l];
+        const pbb_plan::M2LTile &t64 = p->m2l_level[0][l].width ? p->m2l_level[0][l] : p->m2l_tile[0];
+        const int cpt = t64.width ? std::min(ncols, t64.cpt) : std::min(ncols, 64);
+        const int width = t64.width ? t64.width : ncols >= 32 || ncols <= 16 ? 128 : 64;
+        const int TB = std::max(1, std::min(16, width / cpt));
+        const int nchunks = (ncols + cpt - 1) / cpt;
         dev::M2LArgs a{};
         a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = TB; a.cpt = cpt; a.side = 1 << l; a.off = D.off;
         a.chunk_major = t64.width ? (t64.flags & PBB_M2L_CHUNK_MAJOR) : 1;
@@ -1278,6 +1568,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         a.P = W.P; a.Pf = W.Pf;
         dev::launch_m2l(a, a.tile_prefix[8], nchunks, st);
     }
+    if (Lv >= 2) lev_mark(Lv + 1);
     mark(4);
     // local expansions F_l = scale_l * Uw Pf_l overwrite S_l (no longer needed)
     for (int l = 2; l <= Lv; l++) {
@@ -1292,11 +1583,12 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         } else {
             g.B = W.Pf + (size_t)D.off * ncols * h.rf;
         }
-        g.C = W.S + (size_t)D.off * ncols * n3;
+        g.c_f32 = s32 ? 1 : 0;
+        g.C = s32 ? (void *)((float *)W.S + (size_t)D.off * ncols * n3) : (void *)(W.S + (size_t)D.off * ncols * n3);
         if (SA) {  // three-part BF16 matrix cores (Pf is FP32 in the mixed modes)
             const auto &U3 = o->u3;
             dev::GemmSplitArgs q{};
-            q.M = g.M; q.K = g.K; q.ldb = g.ldb; q.ldc = g.ldc; q.b_f32 = 1; q.c_f32 = 0; q.nch = U3.nch;
+            q.M = g.M; q.K = g.K; q.ldb = g.ldb; q.ldc = g.ldc; q.b_f32 = 1; q.c_f32 = g.c_f32; q.nch = U3.nch;
             q.ncol = g.ncol; q.alpha = g.alpha; q.As = (const uint16_t *)U3.d + h.ops_set(l) * U3.per_set;
             q.B = g.B; q.C = g.C;
             dev::launch_gemm_split3(q, st);
@@ -1312,7 +1604,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         dev::DownArgs a{};
         a.n = n; a.ncols = ncols; a.accumulate = l > 2; a.off = D.off; a.len = D.len;
         a.seg_box = D.seg_box; a.seg_p0 = D.seg_p0; a.pstart = D.pstart; a.coord = D.coord; a.ctab = D.ctab;
-        a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.F = W.S; a.lp = W.lp;
+        a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.F = W.S; a.lp = W.lp; a.f32 = s32;
         dev::launch_l2p_level(a, D.nseg, st);
     }
     for (int l = 3; l <= Lv && !direct_down; l++) {

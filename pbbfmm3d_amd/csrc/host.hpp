@@ -73,8 +73,17 @@ struct HostOps {
 
 using HostEval = std::function<double(const double *, const double *)>;
 
+// Optional device help for the precompute: kernel blocks of one operator
+// set (box length, Kcell as kernel_blocks lays it out) and the compression
+// E_d = U^T K_d V of all 316 directions (Kcell of the set's last blocks
+// call).  Either may be empty; the host loops run where it is.
+struct PrecomputeDevice {
+    std::function<void(double box_len, std::vector<double> &Kcell)> blocks;
+    std::function<void(int rf, int rs, const double *U, const double *VT, double *E)> compress;
+};
+
 // Full precompute: ComputeWeights + PrecomputeM2L (H2_3D_Tree.cpp:40-66).
-void precompute_ops(HostOps &ops, const HostEval &eval);
+void precompute_ops(HostOps &ops, const HostEval &eval, const PrecomputeDevice *dev = nullptr);
 // Kernel blocks of one operator set (H2_3D_Tree.cpp:277-374):
 // Kcell[d*n6 + i + j*n3] = K(t_i, s_j^(d)) / (Kw_i Kw_j).
 void kernel_blocks(const HostOps &ops, const HostEval &eval, double boxLen,

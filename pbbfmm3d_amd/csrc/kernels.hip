@@ -809,7 +809,7 @@ constexpr int L2P_TGT = 32;  // targets per pass
 
 // one level of the direct downward pass: one workgroup per segment of at
 // most L2P_SEG points of one box; writes (or adds to) lp[k*ncols + c]
-template <int N>
+template <int N, typename TF>
 __global__ __launch_bounds__(256) void l2p_fp64(DownArgs a)
 {
     constexpr int N2 = N * N, N3 = N2 * N, FS = N3 + 1;
@@ -820,13 +820,13 @@ __global__ __launch_bounds__(256) void l2p_fp64(DownArgs a)
     const int p1 = min(p0 + L2P_SEG, a.pstart[b + 1]);
     const uint32_t cc = (uint32_t)a.coord[b];
     const double ihalfL = 2.0 / a.len;
-    const double *Fb = a.F + (size_t)(a.off + b) * a.ncols * N3;
+    const TF *Fb = (const TF *)a.F + (size_t)(a.off + b) * a.ncols * N3;
     for (int c0 = 0; c0 < a.ncols; c0 += L2P_CC) {
         const int cn = min(L2P_CC, a.ncols - c0);
         __syncthreads();  // the previous step's readers of Fs and sS are done
         for (int q = tid; q < cn * N3; q += 256) {
             const int qc = q / N3;
-            Fs[qc * FS + q - qc * N3] = Fb[(size_t)c0 * N3 + q];
+            Fs[qc * FS + q - qc * N3] = (double)Fb[(size_t)c0 * N3 + q];
         }
         for (int i0 = p0; i0 < p1; i0 += L2P_TGT) {
             const int tn = min(L2P_TGT, p1 - i0);
@@ -877,7 +877,7 @@ __global__ __launch_bounds__(256) void l2p_fp64(DownArgs a)
 // exact.  One workgroup per work item (a box of levels 2..L, or a segment of
 // P2M_SEG points of a larger box); thread pencils (column c, l1, l2) hold
 // the n values of l3 in registers.
-template <int N>
+template <int N, typename TS>
 __global__ __launch_bounds__(256) void p2m_levels(UpArgs a)
 {
     constexpr int N2 = N * N, N3 = N2 * N, PTS = 64, PP = PTS + 1;
@@ -893,8 +893,9 @@ __global__ __launch_bounds__(256) void p2m_levels(UpArgs a)
     const double *ctab = a.lt.ctab[li];
     const double cen[3] = {ctab[cc & 1023], ctab[(cc >> 10) & 1023], ctab[(cc >> 20) & 1023]};
     const double ihalfL = 2.0 / a.lt.len[li];
-    double *Sb = it.w >= 0 ? a.scratch + (size_t)it.w * a.ncols * N3
-                           : a.S + (size_t)(a.lt.off[li] + b) * a.ncols * N3;
+    // split boxes write FP64 partial sums to scratch slots, others S directly
+    double *Sp = it.w >= 0 ? a.scratch + (size_t)it.w * a.ncols * N3 : nullptr;
+    TS *Sb = it.w >= 0 ? nullptr : (TS *)a.S + (size_t)(a.lt.off[li] + b) * a.ncols * N3;
     for (int c0 = 0; c0 < a.ncols; c0 += CC) {
         const int cn = min(CC, a.ncols - c0);
         double acc[PR][N];
@@ -945,9 +946,14 @@ __global__ __launch_bounds__(256) void p2m_levels(UpArgs a)
             if (q < CC * N2) {
                 const int c = q / N2, l12 = q - c * N2;
                 if (c < cn) {
-                    double *dst = Sb + (size_t)(c0 + c) * N3 + l12 * N;
+                    const size_t at = (size_t)(c0 + c) * N3 + l12 * N;
+                    if (Sp) {
 #pragma unroll
-                    for (int m = 0; m < N; m++) dst[m] = acc[r][m];
+                        for (int m = 0; m < N; m++) Sp[at + m] = acc[r][m];
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < N; m++) Sb[at + m] = (TS)acc[r][m];
+                    }
                 }
             }
         }
@@ -956,6 +962,7 @@ __global__ __launch_bounds__(256) void p2m_levels(UpArgs a)
 
 // S of the split boxes: the sum of their segments' partial expansions, in
 // segment order
+template <typename TS>
 __global__ void p2m_reduce(UpReduce r)
 {
     const int64_t len = (int64_t)r.ncols * r.n3;
@@ -966,7 +973,7 @@ __global__ void p2m_reduce(UpReduce r)
         const int4 rd = r.reds[k];
         double s = 0.0;
         for (int i = 0; i < rd.z; i++) s += r.scratch[(int64_t)(rd.y + i) * len + e];
-        r.S[(int64_t)rd.x * len + e] = s;
+        ((TS *)r.S)[(int64_t)rd.x * len + e] = (TS)s;
     }
 }
 
@@ -1553,18 +1560,22 @@ bool launch_p2m_levels(const UpArgs &a, int nitems, const UpReduce &r, hipStream
 {
     if (a.n < 4 || a.n > 8) return false;
     if (nitems <= 0) return true;
-    switch (a.n) {
-    case 4: p2m_levels<4><<<nitems, 256, 0, st>>>(a); break;
-    case 5: p2m_levels<5><<<nitems, 256, 0, st>>>(a); break;
-    case 6: p2m_levels<6><<<nitems, 256, 0, st>>>(a); break;
-    case 7: p2m_levels<7><<<nitems, 256, 0, st>>>(a); break;
-    default: p2m_levels<8><<<nitems, 256, 0, st>>>(a); break;
+#define PBB_P2M_N(TS)                                                      \
+    switch (a.n) {                                                         \
+    case 4: p2m_levels<4, TS><<<nitems, 256, 0, st>>>(a); break;           \
+    case 5: p2m_levels<5, TS><<<nitems, 256, 0, st>>>(a); break;           \
+    case 6: p2m_levels<6, TS><<<nitems, 256, 0, st>>>(a); break;           \
+    case 7: p2m_levels<7, TS><<<nitems, 256, 0, st>>>(a); break;           \
+    default: p2m_levels<8, TS><<<nitems, 256, 0, st>>>(a); break;          \
     }
+    if (a.s32) PBB_P2M_N(float) else PBB_P2M_N(double)
+#undef PBB_P2M_N
     check(hipGetLastError(), "p2m_levels");
     if (r.nred > 0) {
         const int64_t total = (int64_t)r.nred * r.ncols * r.n3;
         const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
-        p2m_reduce<<<blocks, 256, 0, st>>>(r);
+        if (r.s32) p2m_reduce<float><<<blocks, 256, 0, st>>>(r);
+        else p2m_reduce<double><<<blocks, 256, 0, st>>>(r);
         check(hipGetLastError(), "p2m_reduce");
     }
     return true;
@@ -1574,13 +1585,16 @@ bool launch_l2p_level(const DownArgs &a, int nseg, hipStream_t st)
 {
     if (a.n < 4 || a.n > 8) return false;
     if (nseg <= 0) return true;
-    switch (a.n) {
-    case 4: l2p_fp64<4><<<nseg, 256, 0, st>>>(a); break;
-    case 5: l2p_fp64<5><<<nseg, 256, 0, st>>>(a); break;
-    case 6: l2p_fp64<6><<<nseg, 256, 0, st>>>(a); break;
-    case 7: l2p_fp64<7><<<nseg, 256, 0, st>>>(a); break;
-    default: l2p_fp64<8><<<nseg, 256, 0, st>>>(a); break;
+#define PBB_L2P_N(TF)                                                      \
+    switch (a.n) {                                                         \
+    case 4: l2p_fp64<4, TF><<<nseg, 256, 0, st>>>(a); break;               \
+    case 5: l2p_fp64<5, TF><<<nseg, 256, 0, st>>>(a); break;               \
+    case 6: l2p_fp64<6, TF><<<nseg, 256, 0, st>>>(a); break;               \
+    case 7: l2p_fp64<7, TF><<<nseg, 256, 0, st>>>(a); break;               \
+    default: l2p_fp64<8, TF><<<nseg, 256, 0, st>>>(a); break;              \
     }
+    if (a.f32) PBB_L2P_N(float) else PBB_L2P_N(double)
+#undef PBB_L2P_N
     check(hipGetLastError(), "l2p_fp64");
     return true;
 }
@@ -1634,6 +1648,64 @@ __global__ void fill_zero_f64(double *__restrict__ out, int64_t n)
 {
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
         out[q] = 0.0;
+}
+
+template <int KID>
+__global__ __launch_bounds__(256) void kernel_blocks_dev(KernelBlockArgs a)
+{
+    const int64_t n6 = (int64_t)a.n3 * a.n3, total = a.symm_num * n6;
+    const int n = a.n;
+    for (int64_t q = blockIdx.x * (int64_t)256 + threadIdx.x; q < total; q += (int64_t)gridDim.x * 256) {
+        const int d = (int)(q / n6);
+        const int64_t rem = q - d * n6;
+        const int j = (int)(rem / a.n3), i = (int)(rem - (int64_t)j * a.n3);
+        const int i1 = i / (n * n), i2 = (i / n) % n, i3 = i % n;
+        const int j1 = j / (n * n), j2 = (j / n) % n, j3 = j % n;
+        const double t0 = 0.5 * a.nodes[i1] * a.box_len, t1 = 0.5 * a.nodes[i2] * a.box_len,
+                     t2 = 0.5 * a.nodes[i3] * a.box_len;
+        const double s0 = (a.dirs[3 * d] + 0.5 * a.nodes[j1]) * a.box_len,
+                     s1 = (a.dirs[3 * d + 1] + 0.5 * a.nodes[j2]) * a.box_len,
+                     s2 = (a.dirs[3 * d + 2] + 0.5 * a.nodes[j3]) * a.box_len;
+        a.Kcell[q] = keval<KID>(s0 - t0, s1 - t1, s2 - t2, a.kp) / (a.Kw[i] * a.Kw[j]);
+    }
+}
+
+// K_d = symm * K_{315-d}^T for the directions past symm_num
+__global__ __launch_bounds__(256) void kernel_blocks_symm(KernelBlockArgs a)
+{
+    const int64_t n6 = (int64_t)a.n3 * a.n3, total = (316 - a.symm_num) * n6;
+    for (int64_t q = blockIdx.x * (int64_t)256 + threadIdx.x; q < total; q += (int64_t)gridDim.x * 256) {
+        const int d = a.symm_num + (int)(q / n6);
+        const int64_t rem = q - (int64_t)(d - a.symm_num) * n6;
+        const int j = (int)(rem / a.n3), i = (int)(rem - (int64_t)j * a.n3);
+        a.Kcell[d * n6 + i + (int64_t)j * a.n3] = a.symm * a.Kcell[(315 - d) * n6 + j + (int64_t)i * a.n3];
+    }
+}
+
+void launch_kernel_blocks(const KernelBlockArgs &a, int kid, hipStream_t st)
+{
+    const int64_t n6 = (int64_t)a.n3 * a.n3;
+    if (a.n3 <= 0 || a.n3 != a.n * a.n * a.n || a.kcell_len < 316 * n6 || a.symm_num < 0 || a.symm_num > 316)
+        fail(PBB_ERR_INTERNAL, "kernel block arguments do not fit the buffer");
+    const int blocks = (int)std::min<int64_t>((a.symm_num * n6 + 255) / 256, 65536);
+#define PBB_KB_CASE(K)                                                  \
+    case K:                                                             \
+        kernel_blocks_dev<K><<<blocks, 256, 0, st>>>(a);                \
+        break;
+    switch (kid) {
+        PBB_KB_CASE(PBB_KERNEL_LAPLACIAN) PBB_KB_CASE(PBB_KERNEL_ONEOVERR2) PBB_KB_CASE(PBB_KERNEL_GAUSSIAN)
+        PBB_KB_CASE(PBB_KERNEL_EXPONENTIAL) PBB_KB_CASE(PBB_KERNEL_SOFT_LAPLACIAN) PBB_KB_CASE(PBB_KERNEL_QUADRIC)
+        PBB_KB_CASE(PBB_KERNEL_INV_QUADRIC) PBB_KB_CASE(PBB_KERNEL_LOGARITHM) PBB_KB_CASE(PBB_KERNEL_THIN_PLATE)
+        PBB_KB_CASE(PBB_KERNEL_R2) PBB_KB_CASE(PBB_KERNEL_CONSTANT) PBB_KB_CASE(PBB_KERNEL_EXP_ANISO)
+    default: fail(PBB_ERR_UNSUPPORTED, "no device functor for this kernel id");
+    }
+#undef PBB_KB_CASE
+    check(hipGetLastError(), "kernel_blocks_dev");
+    if (a.symm_num < 316) {
+        const int b2 = (int)std::min<int64_t>(((316 - a.symm_num) * n6 + 255) / 256, 65536);
+        kernel_blocks_symm<<<b2, 256, 0, st>>>(a);
+        check(hipGetLastError(), "kernel_blocks_symm");
+    }
 }
 
 void launch_fill_zero(double *out, int64_t n, hipStream_t st)

@@ -71,14 +71,17 @@ struct UpArgs {
     // whose partial expansions go to scratch slots and are summed afterwards
     const int4 *items;
     const double *Tkz, *x, *y, *z, *wsT;
-    double *S, *scratch;
+    void *S;                   // expansions: FP64, or FP32 with s32
+    double *scratch;           // partial expansions of split boxes (FP64)
+    int s32;
 };
 
 struct UpReduce {
     int ncols, n3, nred;
     const int4 *reds;          // (global box id, first slot, slot count, 0)
     const double *scratch;
-    double *S;
+    void *S;                   // FP64, or FP32 with s32
+    int s32;
 };
 
 constexpr int L2P_SEG = 256;  // targets per workgroup of the per-level L2P
@@ -89,8 +92,9 @@ struct DownArgs {
     double len;
     const int32_t *seg_box, *seg_p0, *pstart, *coord;
     const double *ctab, *Tkz, *x, *y, *z;
-    const double *F;
+    const void *F;                             // local expansions: FP64, or FP32 with f32
     double *lp;                                // point-major L2P results
+    int f32;
 };
 
 struct GemmArgs {
@@ -229,6 +233,21 @@ bool launch_l2p_level(const DownArgs &a, int nseg, hipStream_t st);
 void launch_gather_weights_t(const double *w, const int32_t *perm, int64_t N, int ncols, double *wsT,
                              hipStream_t st);
 void launch_direct(const DirectArgs &a, int kid, hipStream_t st);
+
+// Kernel blocks of one operator set on the device (H2_3D_Tree.cpp:277-374):
+// Kcell[d*n6 + i + j*n3] = K(t_i, s_j^(d)) / (Kw_i Kw_j) for the 316
+// directions (dirs: 316 x 3 offsets in Ktable order), the first symm_num
+// evaluated, the rest filled as symm * K_{315-d}^T.
+struct KernelBlockArgs {
+    int n, n3, symm_num, symm;
+    double box_len;
+    KParams kp;
+    const double *nodes, *Kw;
+    const int *dirs;
+    double *Kcell;
+    int64_t kcell_len;          // doubles at Kcell (316 n3^2 needed)
+};
+void launch_kernel_blocks(const KernelBlockArgs &a, int kid, hipStream_t st);
 
 }  // namespace dev
 }  // namespace pbb

@@ -885,6 +885,7 @@ void launch_gemm_split3(const GemmSplitArgs &g, hipStream_t st)
     case RB:                                                                                         \
         if (!g.b_f32 && g.c_f32) gemm_split3<double, float, RB, RB <= 12 ? 2 : 1><<<grid, 512, 0, st>>>(g); \
         else if (g.b_f32 && !g.c_f32) gemm_split3<float, double, RB, RB <= 12 ? 2 : 1><<<grid, 512, 0, st>>>(g); \
+        else if (g.b_f32 && g.c_f32) gemm_split3<float, float, RB, RB <= 12 ? 2 : 1><<<grid, 512, 0, st>>>(g); \
         else fail(PBB_ERR_INTERNAL, "gemm_split3: unsupported operand types");                       \
         break;
     switch (rb) {

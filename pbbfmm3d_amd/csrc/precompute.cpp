@@ -216,7 +216,7 @@ static void compress(int n3, int rf, int rs, const double *U, const double *VT, 
     });
 }
 
-void precompute_ops(HostOps &ops, const HostEval &eval)
+void precompute_ops(HostOps &ops, const HostEval &eval, const PrecomputeDevice *dev)
 {
     const int n = ops.n;
     ops.n3 = n * n * n;
@@ -235,7 +235,8 @@ void precompute_ops(HostOps &ops, const HostEval &eval)
         double boxLen = ops.homog ? 1.0 : ops.L / 4;
         if (!ops.homog)
             for (int s = 0; s < set; s++) boxLen /= 2;
-        kernel_blocks(ops, eval, boxLen, Kcell);
+        if (dev && dev->blocks) dev->blocks(boxLen, Kcell);
+        else kernel_blocks(ops, eval, boxLen, Kcell);
         const size_t n6 = (size_t)n3 * n3;
 
         // K_fat^T: row (d, j), column i -> value K_d(i, j); left singular
@@ -276,7 +277,8 @@ void precompute_ops(HostOps &ops, const HostEval &eval)
         std::memcpy(U, Ufull.data(), sizeof(double) * (size_t)n3 * rf);
         for (int j = 0; j < n3; j++)
             for (int b = 0; b < rs; b++) VT[b + (size_t)j * rs] = V[j + (size_t)b * n3];
-        compress(n3, rf, rs, U, VT, Kcell, &ops.E[(size_t)set * 316 * rf * rs]);
+        if (dev && dev->compress) dev->compress(rf, rs, U, VT, &ops.E[(size_t)set * 316 * rf * rs]);
+        else compress(n3, rf, rs, U, VT, Kcell, &ops.E[(size_t)set * 316 * rf * rs]);
     }
 }
 

@@ -97,3 +97,22 @@ def test_cpp_dropin_build_uses_the_cache(tmp_path):
     assert "kernel_Laplacian: operator cache read" in second.stdout, second.stdout + second.stderr
     ranks = [l for l in first.stdout.splitlines() if "ranks" in l]
     assert ranks == [l for l in second.stdout.splitlines() if "ranks" in l]
+
+
+def test_parameter_file_guards_the_cache(tmp_path):
+    # files written here name the parameters they hold: a read with another
+    # eps or kernel parameter is refused; files without the parameter file
+    # (the reference's) are read as the reference reads them
+    d = str(tmp_path)
+    ops = pbb.Operators(pbb.gaussian(0.3), 1.0, 3, 4, 1e-6)
+    ops.write_cache(d, "Gaussian")
+    k = os.path.join(d, "GaussianChebK4.bin")
+    assert "eps 9.9999999999999995e-07" in open(k + ".params").read()
+    back = pbb.Operators.from_cache(pbb.gaussian(0.3), 1.0, 3, 4, 1e-6, d, "Gaussian")
+    assert np.array_equal(_export(back)[2], _export(ops)[2])
+    for kernel, eps in ((pbb.gaussian(0.3), 1e-7), (pbb.gaussian(0.5), 1e-6)):
+        with pytest.raises(PBBError) as e:
+            pbb.Operators.from_cache(kernel, 1.0, 3, 4, eps, d, "Gaussian")
+        assert e.value.status == PBB_ERR_INVALID_ARG and "other kernel parameters" in str(e.value)
+    os.remove(k + ".params")
+    pbb.Operators.from_cache(pbb.gaussian(0.5), 1.0, 3, 4, 1e-7, d, "Gaussian")

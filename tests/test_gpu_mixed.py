@@ -91,8 +91,18 @@ def test_m2l_tiles_and_tuner(gpu):
     # with an accuracy bound the kept tile's matvec is within it
     t = plan.tune_m2l(wd, pbb.capi.PREC_MIXED_F16X2, max_rel_diff=1e-6)
     assert t["rel_diff"] <= 1e-6
-    plan.set_m2l_tile(t["width"], t["cols_per_box"], t["flags"], pbb.capi.PREC_MIXED_F16X2)
+    for lvl, (width, cpb, fl) in t["levels"].items():  # the tuner's per-level tiles stay in force
+        assert width in (64, 128, 256)
     assert orc.rel_err(_mv(plan, w, nc, pbb.capi.PREC_MIXED_F16X2), ref) <= 1e-6
+    # a level's own tile, read back, and cleared by the plan-wide setter
+    f16 = pbb.capi.PREC_MIXED_F16X2
+    plan.set_m2l_tile(128, 16, 5, f16)
+    plan.set_m2l_tile_level(4, 64, 32, 5, f16)
+    tiles = plan.m2l_tiles(f16)
+    assert tiles[4] == (64, 32, 5) and tiles[3] == (128, 16, 5)
+    assert orc.rel_err(_mv(plan, w, nc, f16), ref) <= 1e-6
+    plan.set_m2l_tile(128, 16, 5, f16)
+    assert plan.m2l_tiles(f16)[4] == (128, 16, 5)
     t = plan.tune_m2l(wd, pbb.capi.PREC_FP64)
     assert t["width"] in (32, 64, 128)
     torch.cuda.synchronize()

@@ -98,3 +98,37 @@ def test_from_arrays_roundtrip():
                                   U, VT, E)
     for x, y in zip(a.export()[:3], b.export()[:3]):
         np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel,n,level", [(pbb.laplacian(), 5, 3), (pbb.gaussian(0.3), 6, 4)],
+                         ids=["laplacian-n5", "gaussian-n6"])
+def test_gpu_precompute_equals_host(gpu, kernel, n, level):
+    # kernel blocks and the E compression on the GPU, SVDs on the host: the
+    # same singular values, cutoffs and compressed operators U E V^T as the
+    # all-host precompute, and the same FMM to the SVD truncation
+    import torch
+    eps = 1e-7
+    g = pbb.Operators(kernel, 1.0, level, n, eps)
+    h = pbb.Operators(kernel, 1.0, level, n, eps, host_precompute=True)
+    Ug, VTg, Eg, sfg, ssg = g.export()
+    Uh, VTh, Eh, sfh, ssh = h.export()
+    assert (g.rank_f, g.rank_s) == (h.rank_f, h.rank_s)
+    assert np.max(np.abs(sfg - sfh)) <= 1e-12 * sfh[0] and np.max(np.abs(ssg - ssh)) <= 1e-12 * ssh[0]
+    n3, rf, rs = g.n3, g.rank_f, g.rank_s
+    for s in range(g.nsets):
+        def full(U, VT, E, d):
+            Us = U[s * n3 * rf:(s + 1) * n3 * rf].reshape(rf, n3).T
+            Vs = VT[s * rs * n3:(s + 1) * rs * n3].reshape(n3, rs).T
+            Ed = E[(s * 316 + d) * rf * rs:(s * 316 + d + 1) * rf * rs].reshape(rs, rf).T
+            return Us @ Ed @ Vs
+        # the truncation error is relative to the whole operator set, so the
+        # blocks are compared against the largest block's norm
+        pairs = [(full(Ug, VTg, Eg, d), full(Uh, VTh, Eh, d)) for d in range(0, 316, 15)]
+        top = max(np.linalg.norm(b) for _, b in pairs)
+        assert max(np.linalg.norm(a - b) for a, b in pairs) <= 10 * eps * top
+    xyz = np.random.default_rng(5).random((6000, 3))
+    w = torch.tensor(np.random.default_rng(6).standard_normal((2, 6000)), device="cuda")
+    yg = pbb.Plan(g, xyz).matvec(w)
+    yh = pbb.Plan(h, xyz).matvec(w)
+    assert float(torch.linalg.norm(yg - yh) / torch.linalg.norm(yh)) <= 10 * eps
