@@ -12,6 +12,7 @@ to device output on a prepared tree.  Prints ONE JSON line.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -40,9 +41,9 @@ def dtype_label(precision, flags):
             f"vt/u {vtu}; p2m, l2p, p2p f64")
 
 
-# bytes per M2L launch, averaged over the 6 launches of one f16x2 matvec of the
-# default workload (6758 GB read + 27 GB written per matvec)
-M2L_TRAFFIC_F16X2_C3 = 1.131e12
+# bytes per M2L launch, averaged over the launches of one f16x2 matvec of the
+# default workload (None until measured for the current kernels)
+M2L_TRAFFIC_F16X2_C3 = None
 TRAFFIC_SOURCE = "profiles/m2l_f16x2_c3_gaussian_traffic.txt"
 
 
@@ -207,9 +208,11 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--err-targets", type=int, default=10000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--order-sweep", default="",
-                    help="comma-separated orders (e.g. 7,8) measured beside the headline at the same points, "
-                         "weights and sampled targets: FP64 and mixed errors against the direct sum, speed ratio")
+    ap.add_argument("--order-sweep", default="7",
+                    help="comma-separated orders measured beside the headline at the same points, weights and "
+                         "sampled targets: FP64 and mixed errors against the direct sum, speed ratio; the "
+                         "line's accuracy_gate is the lowest of them whose FP64 path is within 1e-6 of the "
+                         "direct sum ('' to skip)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 256-cloud forward+adjoint batch measured beside the headline")
     ap.add_argument("--workload", default="c3", choices=["c3", "c5"],
@@ -351,10 +354,24 @@ def main():
             f"rel diff vs FP64 path {fp64['rel_diff_vs_fp64_path']:.3e}")
         del out64
 
-    # roofline of the dominant kernel (M2L): algorithmic flops 2*rf*rs*ncols
-    # per far-field box pair, over the M2L phase time (HIP events on the
-    # stream the M2L launches run on)
-    flops = 2.0 * ops.rank_f * ops.rank_s * nc * info["m2l_pairs"]
+    # roofline of the dominant kernel (M2L): algorithmic flops 2*kf*ks*ncols
+    # per far-field box pair of each level, with the ranks the path runs (the
+    # reference's r_f x r_s in FP64, each level's re-compressed operator set
+    # in the mixed modes), over the M2L phase time (HIP events on the stream
+    # the M2L launches run on)
+    L = info["tree_level"]
+    pairs = info["m2l_pairs_level"]
+    mranks = ops.mixed_ranks() if prec != capi.PREC_FP64 else None
+
+    def level_ranks(l):
+        if mranks is None:
+            return ops.rank_f, ops.rank_s
+        kf, ks, tail = mranks[l - 2 if ops.nsets > 1 else 0]
+        # a set that keeps its own factors runs r_f x r_s (ks is only padded)
+        return (ops.rank_f, ops.rank_s) if (kf == ops.rank_f and tail == 0.0) else (kf, ks)
+
+    flops = sum(2.0 * level_ranks(l)[0] * level_ranks(l)[1] * nc * pairs[l] for l in range(2, L + 1))
+    flops_ref = 2.0 * ops.rank_f * ops.rank_s * nc * info["m2l_pairs"]
     achieved = flops / (phase["m2l"] * 1e-3) / 1e12
     bound, peak, unit = PEAK[args.precision]
     # HBM bytes per M2L launch (FETCH_SIZE x 2 + WRITE_SIZE x 1, calibrated
@@ -367,7 +384,9 @@ def main():
                 "frac": round(achieved / peak, 4), "traffic": traffic,
                 "traffic_source": TRAFFIC_SOURCE if traffic else None,
                 "kernel": "m2l", "launches_per_step": args.level - 1,
-                "algorithmic_flops_per_step": flops}
+                "algorithmic_flops_per_step": flops,
+                "ranks_per_level": {l: list(level_ranks(l)) for l in range(2, L + 1)},
+                "flops_per_step_at_reference_ranks": flops_ref}
 
     sweep = []
     for order in [int(x) for x in args.order_sweep.split(",") if x.strip()]:
@@ -380,26 +399,46 @@ def main():
         t_pre_o = time.perf_counter() - t0
         plan_o = pbb.Plan(ops_o, xyz)
         log(f"order {order}: precompute {t_pre_o:.1f} s, ranks {ops_o.rank_f}/{ops_o.rank_s}")
-        tile_o = plan_o.tune_m2l(w, prec, args.max_rel_diff)
-        res_o = {}
-        for name, p in (("fp64", capi.PREC_FP64), (args.precision, prec)):
+
+        def run_o(p):
             o = torch.zeros_like(w)
-            plan_o.matvec(w, o, precision=p)
+            plan_o.matvec(w, o, precision=p)  # untimed: workspace and operator splits
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             o.zero_()
             plan_o.matvec(w, o, precision=p)
             torch.cuda.synchronize()
-            res_o[name] = (time.perf_counter() - t0, o)
-        o64, om = res_o["fp64"][1], res_o[args.precision][1]
+            return time.perf_counter() - t0, o
+
+        t64_o, o64 = run_o(capi.PREC_FP64)
+        e64 = err(o64)
+        log(f"order {order}: fp64 {1e3 * t64_o:.0f} ms, rel err vs direct {e64:.3e}")
+        # the mixed tile's bound: where the FP64 path is within 1e-6 of the
+        # direct sum, the room its error leaves under 1e-6 (the arithmetic
+        # difference and the truncation error are independent and add in
+        # quadrature; a tenth is kept in reserve for the sampled targets),
+        # else the headline bound against the FP64 path
+        bound = 0.9 * math.sqrt(1e-12 - e64 * e64) if e64 < 1e-6 else args.max_rel_diff
+        tile_o = plan_o.tune_m2l(w, prec, bound)
+        tm_o, om = run_o(prec)
         sweep.append({"order": order, "ranks": [ops_o.rank_f, ops_o.rank_s], "precompute_s": round(t_pre_o, 2),
-                      "fp64_ms": 1e3 * res_o["fp64"][0], "mixed_ms": 1e3 * res_o[args.precision][0],
-                      "speedup": res_o["fp64"][0] / res_o[args.precision][0],
-                      "fp64_rel_err_vs_direct": err(o64), "mixed_rel_err_vs_direct": err(om),
+                      "fp64_ms": 1e3 * t64_o, "mixed_ms": 1e3 * tm_o,
+                      "fp64_targets_per_s": N / t64_o, "mixed_targets_per_s": N / tm_o,
+                      "speedup": t64_o / tm_o,
+                      "fp64_rel_err_vs_direct": e64, "mixed_rel_err_vs_direct": err(om),
                       "mixed_rel_diff_vs_fp64_path": float(torch.linalg.norm(om - o64) / torch.linalg.norm(o64)),
-                      "m2l_tile": tile_o})
+                      "tuner_bound": bound, "m2l_tile": tile_o})
         log(f"order {order}: {sweep[-1]}")
-        del plan_o, o64, om, res_o
+        del plan_o, o64, om
+    # the lowest measured order at which the FP64 path is within 1e-6 of the
+    # direct sum, and the mixed path there
+    gate = None
+    for r in sorted(sweep, key=lambda r: r["order"]):
+        if r["fp64_rel_err_vs_direct"] <= 1e-6:
+            gate = {k: r[k] for k in ("order", "fp64_rel_err_vs_direct", "mixed_rel_err_vs_direct",
+                                      "fp64_targets_per_s", "mixed_targets_per_s", "speedup")}
+            gate["mixed_within_1e-6_of_direct"] = r["mixed_rel_err_vs_direct"] <= 1e-6
+            break
 
     batch = None
     if not args.no_batch:
@@ -446,6 +485,7 @@ def main():
         "tuning": widths,
         "batch_256_clouds": batch,
         "order_sweep": sweep or None,
+        "accuracy_gate": gate,
     }
     print(json.dumps(line), flush=True)
 

@@ -28,7 +28,7 @@ extern "C" {
 
 #define PBB_API __attribute__((visibility("default")))
 
-#define PBB_ABI_VERSION 1
+#define PBB_ABI_VERSION 2
 
 typedef enum pbb_status {
     PBB_OK = 0,
@@ -122,6 +122,10 @@ typedef struct pbb_plan_info {
     double center[3];       /* bbox centre subtracted (compute.hpp:108-111)*/
     int64_t outside_points; /* points outside the root cube (extrapolated
                                as the reference does)                      */
+    int64_t ntargets;       /* separate sets (pbb_plan_create_ts): the
+                               targets and the sources; one set: both N    */
+    int64_t nsources;
+    int64_t m2l_pairs_level[16]; /* m2l_pairs by level (0 at levels 0, 1)  */
 } pbb_plan_info;
 
 /* ---- library ---------------------------------------------------------- */
@@ -199,6 +203,27 @@ PBB_API pbb_status pbb_ops_query(const pbb_ops *ops, pbb_ops_info *info);
 PBB_API pbb_status pbb_ops_export(const pbb_ops *ops, double *U, double *VT,
                                   double *E, double *sigma_f, double *sigma_s);
 
+/* The mixed-precision modes run each operator set re-compressed: the
+ * reference keeps the level-2 SVD cutoff at every level of a
+ * non-homogeneous kernel (H2_3D_Tree.cpp:406-436), while on small boxes a
+ * smooth kernel's 316 M2L blocks E_d share far fewer singular directions.
+ * Each set's E_d are projected on the leading left / right singular
+ * vectors of all 316 blocks together (Q_L, Q_R: E'_d = Q_L^T E_d Q_R, U' =
+ * U Q_L, VT' = Q_R^T VT), keeping kf (a multiple of 16) and ks (of 32)
+ * directions so that the discarded singular values of the stacked blocks
+ * hold at most tol of their Frobenius norm on each side; where that saves
+ * nothing the set keeps its own factors.  No reference counterpart (the
+ * reference has no mixed mode).  tol in [0, 1e-6], default 1e-10; 0 keeps
+ * every set's own factors.  Set it before the first mixed-precision call
+ * on these operators (afterwards PBB_ERR_INVALID_ARG).  The FP64 mode
+ * always runs the reference's operators.                                  */
+PBB_API pbb_status pbb_ops_set_mixed_tolerance(pbb_ops *ops, double tol);
+/* The ranks the mixed modes run per operator set (arrays of nsets: sets in
+ * level order 2..L, or one set for a homogeneous kernel) and the discarded
+ * relative Frobenius norm (tail may be NULL).  Computes them on the host if
+ * needed; no device work.                                                 */
+PBB_API pbb_status pbb_ops_mixed_ranks(pbb_ops *ops, int32_t *kf, int32_t *ks, double *tail);
+
 PBB_API void pbb_ops_destroy(pbb_ops *ops);
 
 /* ---- plans (prepared tree, device resident) --------------------------- */
@@ -214,10 +239,32 @@ PBB_API pbb_status pbb_plan_create(pbb_ops *ops, const double *xyz,
 /* pbb_plan_create with flags: PBB_PLAN_HOST_TREE builds the tree on the
  * host (a CPU restatement of FMMDistribute / InteractionList that the GPU
  * build reproduces array for array, kept as its check); flags 0 builds it
- * on the GPU, as pbb_plan_create does.                                   */
+ * on the GPU, as pbb_plan_create does.  PBB_PLAN_REJECT_OUTSIDE fails with
+ * PBB_ERR_INVALID_ARG when a point lies outside the root cube (side L
+ * around the bbox centre); without it such points are extrapolated, as in
+ * the reference, and counted in pbb_plan_info.outside_points.            */
 #define PBB_PLAN_HOST_TREE 1
+#define PBB_PLAN_REJECT_OUTSIDE 2
 PBB_API pbb_status pbb_plan_create_ex(pbb_ops *ops, const double *xyz,
                                       int64_t npoints, int32_t flags,
+                                      pbb_plan **out);
+
+/* A plan of separate targets and sources: f(x_i) = sum_j K(x_i, y_j) w_j for
+ * ntargets targets x and nsources sources y (host AoS).  Replaces
+ * H2_3D_Compute's separate target and source arguments (compute.hpp:20),
+ * which the reference re-centres together on the targets' bbox
+ * (compute.hpp:92-127) but distributes as one set (targetlist,
+ * compute.hpp:1134-1139), so that it needs them equal.  Here one tree holds
+ * the targets followed by the sources, centred on the targets' bbox; the
+ * sources carry the weights and the targets receive the outputs.  Then
+ * pbb_plan_matvec takes w_dev column-major nsources x ncols and accumulates
+ * into out_dev column-major ntargets x ncols, and pbb_plan_matvec_transpose
+ * the reverse (weights on the targets, outputs at the sources).  Position
+ * gradients and pbb_plan_update_points need one point set
+ * (PBB_ERR_UNSUPPORTED).  flags: as pbb_plan_create_ex.                  */
+PBB_API pbb_status pbb_plan_create_ts(pbb_ops *ops, const double *targets,
+                                      int64_t ntargets, const double *sources,
+                                      int64_t nsources, int32_t flags,
                                       pbb_plan **out);
 
 PBB_API pbb_status pbb_plan_query(const pbb_plan *plan, pbb_plan_info *info);
@@ -236,7 +283,8 @@ PBB_API pbb_status pbb_plan_update_points(pbb_plan *plan, const double *xyz, int
 
 /* Replaces FMMCompute (compute.hpp:192-222): upward pass, M2L, downward
  * pass, near field.  w_dev: device FP64 weights, column-major N x ncols;
- * out_dev: device FP64, column-major N x ncols, accumulated (+=).
+ * out_dev: device FP64, column-major N x ncols, accumulated (+=) (with
+ * separate sets: nsources x ncols in, ntargets x ncols out).
  * stream: a hipStream_t (NULL = the null stream), on which all kernels run
  * in order.  Asynchronous with respect to the host; the call returns once
  * all work is enqueued.  Every call on one plan (matvec, transpose,
@@ -271,7 +319,9 @@ PBB_API pbb_status pbb_plan_matvec_noalloc(pbb_plan *plan, const double *w_dev,
  * roles, the M2L uses V^T/U swapped and E_{-d}^T) with the same kernels;
  * this is the gradient of a loss with respect to the weights.  No reference
  * counterpart (the reference has no adjoint); used by the autograd
- * function.  Requires a symmetric kernel for the near field.             */
+ * function.  Requires a symmetric kernel for the near field.  With
+ * separate sets (pbb_plan_create_ts) w_dev holds ntargets x ncols and
+ * out_dev nsources x ncols.                                               */
 PBB_API pbb_status pbb_plan_matvec_transpose(pbb_plan *plan, const double *w_dev,
                                              int32_t ncols, double *out_dev,
                                              int32_t precision, void *stream);
@@ -287,6 +337,12 @@ PBB_API pbb_status pbb_plan_grad_positions(pbb_plan *plan, const double *sigma_d
                                            const double *psi_dev, int32_t ncols,
                                            double *grad_dev, int32_t precision,
                                            void *stream);
+
+/* Caps the plan's matvec workspace at `bytes` (0: what the device holds).
+ * A matvec whose workspace for all its columns does not fit runs as
+ * chunks of columns, each a complete FMM pass with the same sums (phase
+ * times then cover the last chunk).  No reference counterpart.           */
+PBB_API pbb_status pbb_plan_set_workspace_limit(pbb_plan *plan, int64_t bytes);
 
 /* Phase times of the last pbb_plan_matvec on this plan, in ms
  * (PBB_NUM_PHASES values).  Synchronises the plan's events.               */
@@ -415,13 +471,21 @@ PBB_API pbb_status pbb_direct_sum(const pbb_kernel_desc *kernel, const double *t
 /* ---- one-shot drop-in ------------------------------------------------- */
 
 /* Replaces the H2_3D_Compute<T> constructor (compute.hpp:67-149): host
- * FP64 buffers, target and source must hold the same N points, weights
- * column-major N x ncols, output accumulated.  Builds a plan, runs the
- * FP64 matvec, copies back, frees the plan.                              */
+ * FP64 buffers of N targets and N sources, weights column-major N x ncols,
+ * output accumulated.  Builds a plan (one point set when target and source
+ * hold the same points, else pbb_plan_create_ts), runs the FP64 matvec,
+ * copies back, frees the plan.                                            */
 PBB_API pbb_status pbb_matvec(pbb_ops *ops, const double *target,
                               const double *source, int64_t npoints,
                               const double *weight, int32_t ncols,
                               double *output);
+
+/* pbb_matvec with ntargets targets and nsources sources: weights
+ * column-major nsources x ncols, output ntargets x ncols, accumulated.    */
+PBB_API pbb_status pbb_matvec_ts(pbb_ops *ops, const double *target,
+                                 int64_t ntargets, const double *source,
+                                 int64_t nsources, const double *weight,
+                                 int32_t ncols, double *output);
 
 #ifdef __cplusplus
 }

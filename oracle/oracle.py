@@ -55,7 +55,7 @@ def lib():
         L.pbbo_direct.restype = None
         L.pbbo_matvec_flat.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                        C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp,
-                                       C.c_int, _dp, C.c_int, _dp, C.c_void_p]
+                                       C.c_int, _dp, C.c_int, _dp, C.c_void_p, C.c_int]
         L.pbbo_matvec_flat.restype = C.c_int
         L.pbbo_num_threads.restype = C.c_int
         _lib = L
@@ -183,14 +183,18 @@ def precompute(kid, params, symmetry, homogen, L, levels, n, eps) -> Ops:
 
 
 # ---- matvec / direct sum ------------------------------------------------------
-def matvec(ops: Ops, xyz, w, ncols, out=None, targets=None):
+def matvec(ops: Ops, xyz, w, ncols, out=None, targets=None, nbox=0):
     """H2_3D_Compute (compute.hpp:67-149): xyz (N, 3), w column-major
     (N*ncols,), returns out += A w (column-major, N*ncols).
 
     targets (optional indices): form only these points' outputs (the other
     entries of out are left as they are); the far field and the near field
     then run only for the boxes that hold them, which makes the oracle
-    affordable at a million points."""
+    affordable at a million points.
+
+    nbox > 0: the tree is centred on the bbox of the first nbox points only
+    (the targets, compute.hpp:97-111), the others taking the same shift
+    (:121-126)."""
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
     N = xyz.shape[0]
     w = np.ascontiguousarray(w, dtype=np.float64).reshape(-1)
@@ -207,10 +211,27 @@ def matvec(ops: Ops, xyz, w, ncols, out=None, targets=None):
     rc = lib().pbbo_matvec_flat(ops.n, ops.rf, ops.rs, ops.nsets, int(ops.homog),
                                 -float(ops.homogen), float(ops.L), ops.levels, ops.kid, _p(p),
                                 _p(U), _p(VT), _p(E), _p(xyz), N, _p(w), ncols, _p(out),
-                                None if mask is None else mask.ctypes.data_as(C.c_void_p))
+                                None if mask is None else mask.ctypes.data_as(C.c_void_p), int(nbox))
     if rc != 0:
         raise RuntimeError(f"oracle matvec failed ({rc})")
     return out
+
+
+def matvec_ts(ops: Ops, targets_xyz, sources_xyz, w, ncols):
+    """f(x_i) = sum_j K(x_i, y_j) w_j for targets x (Nt, 3) and separate
+    sources y (Ns, 3), w column-major (Ns*ncols,): the FMM of the points
+    [targets; sources] in the tree centred on the targets' bbox (compute.hpp:
+    92-127, which centres on the targets and shifts the sources alike), with
+    weight zero on the targets, read at the targets.  Returns Nt*ncols
+    values, column-major."""
+    t = np.ascontiguousarray(targets_xyz, dtype=np.float64).reshape(-1, 3)
+    s = np.ascontiguousarray(sources_xyz, dtype=np.float64).reshape(-1, 3)
+    nt, ns = t.shape[0], s.shape[0]
+    w = np.asarray(w, dtype=np.float64).reshape(ncols, ns)
+    wu = np.zeros((ncols, nt + ns))
+    wu[:, nt:] = w
+    out = matvec(ops, np.vstack([t, s]), wu.reshape(-1), ncols, targets=np.arange(nt), nbox=nt)
+    return np.ascontiguousarray(out.reshape(ncols, nt + ns)[:, :nt]).reshape(-1)
 
 
 def direct(kid, params, tgt, src, w, ncols):

@@ -568,15 +568,20 @@ static inline int ops_level(const pbbo_ops *o, int l)
  * oracle affordable at a million points.
  */
 int pbbo_matvec(const pbbo_ops *o, const double *xyz_in, int N, const double *w,
-                int nCols, double *out, const unsigned char *tmask)
+                int nCols, double *out, const unsigned char *tmask, int nbox)
 {
     int n = o->n, n2 = n * n, n3 = n2 * n, Lv = o->levels;
     int rf = o->rf, rs = o->rs;
-    if (Lv < 1 || Lv > 10 || N <= 0 || nCols <= 0) return -1;
+    if (Lv < 1 || Lv > 10 || N <= 0 || nCols <= 0 || nbox < 0 || nbox > N) return -1;
 
-    /* bbox of the targets and re-centring, compute.hpp:92-127 */
+    /* bbox of the targets and re-centring, compute.hpp:92-127: the bbox is
+     * that of the targets only (compute.hpp:97-104) and the sources take
+     * the same shift (:121-126).  nbox > 0: the targets are the first nbox
+     * points (the rest are sources placed in the same tree); nbox = 0: all
+     * N points are the targets. */
+    if (nbox == 0) nbox = N;
     double mn[3] = {1e32, 1e32, 1e32}, mx[3] = {-1e32, -1e32, -1e32};
-    for (int i = 0; i < N; i++)
+    for (int i = 0; i < nbox; i++)
         for (int d = 0; d < 3; d++) {
             double v = xyz_in[3 * i + d];
             mn[d] = v < mn[d] ? v : mn[d];
@@ -901,14 +906,14 @@ int pbbo_matvec_flat(int n, int rf, int rs, int nops, int homog, double homogen,
                      double L, int levels, int kid, const double *kp,
                      const double *U, const double *VT, const double *E,
                      const double *xyz, int N, const double *w, int nCols,
-                     double *out, const unsigned char *tmask)
+                     double *out, const unsigned char *tmask, int nbox)
 {
     pbbo_ops o;
     o.n = n; o.rf = rf; o.rs = rs; o.nops = nops; o.homog = homog;
     o.homogen = homogen; o.L = L; o.levels = levels; o.kid = kid;
     for (int i = 0; i < 4; i++) o.kp[i] = kp ? kp[i] : 0;
     o.U = U; o.VT = VT; o.E = E;
-    return pbbo_matvec(&o, xyz, N, w, nCols, out, tmask);
+    return pbbo_matvec(&o, xyz, N, w, nCols, out, tmask, nbox);
 }
 
 int pbbo_num_threads(void)

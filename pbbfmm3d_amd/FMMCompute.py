@@ -86,12 +86,18 @@ class Compute:
                  output: vecOfdouble):
         tgt = target.data if isinstance(target, _Vec) else np.asarray(target, dtype=np.float64)
         src = source.data if isinstance(source, _Vec) else np.asarray(source, dtype=np.float64)
-        if tgt.shape != src.shape or not np.array_equal(tgt, src):
-            raise ValueError("sources must be the targets (the reference supports nothing else)")
-        N = tgt.reshape(-1, 3).shape[0]
+        tgt = tgt.reshape(-1, 3)
+        src = src.reshape(-1, 3)
+        # one point set when the sources are the targets; else separate
+        # sets, weights at the sources and outputs at the targets
+        same = tgt.shape == src.shape and np.array_equal(tgt, src)
+        N, M = tgt.shape[0], src.shape[0]
         w = weight.data if isinstance(weight, _Vec) else np.asarray(weight, dtype=np.float64)
+        w = np.asarray(w, dtype=np.float64).reshape(-1)
+        if w.size < M * nCols:
+            raise ValueError("weight must hold nCols * (number of sources) values")
         if output.data.size < N * nCols:
             output.resize(N * nCols)
         out = np.ascontiguousarray(output.data.reshape(-1)[: N * nCols])
-        matvec_host(tree.operators(), tgt, w, int(nCols), out)
+        matvec_host(tree.operators(), tgt, w[: M * nCols], int(nCols), out, sources=None if same else src)
         output.data.reshape(-1)[: N * nCols] = out

@@ -56,6 +56,7 @@ def M2L_DIR_GROUPS(g):
 
 NUM_PHASES = 8
 PLAN_HOST_TREE = 1
+PLAN_REJECT_OUTSIDE = 2
 OPS_HOST_PRECOMPUTE = 1
 BATCH_TRANSPOSE = 1
 BATCH_OVERWRITE = 2
@@ -79,7 +80,8 @@ class PlanInfo(C.Structure):
                 ("nonempty_levels", C.c_int32), ("boxes", C.c_int64 * 16),
                 ("m2l_pairs", C.c_int64), ("near_pairs", C.c_int64),
                 ("max_leaf_points", C.c_int64), ("center", C.c_double * 3),
-                ("outside_points", C.c_int64)]
+                ("outside_points", C.c_int64), ("ntargets", C.c_int64),
+                ("nsources", C.c_int64), ("m2l_pairs_level", C.c_int64 * 16)]
 
 
 class BatchEntry(C.Structure):
@@ -110,8 +112,11 @@ _SIGNATURES = {
     "pbb_ops_query": (C.c_int, [_vp, C.POINTER(OpsInfo)]),
     "pbb_ops_export": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
     "pbb_ops_destroy": (None, [_vp]),
+    "pbb_ops_set_mixed_tolerance": (C.c_int, [_vp, C.c_double]),
+    "pbb_ops_mixed_ranks": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _dp]),
     "pbb_plan_create": (C.c_int, [_vp, _dp, C.c_int64, C.POINTER(_vp)]),
     "pbb_plan_create_ex": (C.c_int, [_vp, _dp, C.c_int64, C.c_int32, C.POINTER(_vp)]),
+    "pbb_plan_create_ts": (C.c_int, [_vp, _dp, C.c_int64, _dp, C.c_int64, C.c_int32, C.POINTER(_vp)]),
     "pbb_plan_query": (C.c_int, [_vp, C.POINTER(PlanInfo)]),
     "pbb_plan_update_points": (C.c_int, [_vp, _dp, C.c_int64, _vp]),
     "pbb_plan_matvec": (C.c_int, [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp]),
@@ -125,6 +130,7 @@ _SIGNATURES = {
     "pbb_batch_launch": (C.c_int, [_vp, _vp]),
     "pbb_batch_destroy": (None, [_vp]),
     "pbb_plan_timings": (C.c_int, [_vp, _dp]),
+    "pbb_plan_set_workspace_limit": (C.c_int, [_vp, C.c_int64]),
     "pbb_plan_set_timing": (C.c_int, [_vp, C.c_int32]),
     "pbb_plan_set_m2l_tile": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "pbb_plan_set_m2l_tile_level": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
@@ -134,6 +140,7 @@ _SIGNATURES = {
                                            C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "pbb_plan_destroy": (None, [_vp]),
     "pbb_matvec": (C.c_int, [_vp, _dp, _dp, C.c_int64, _dp, C.c_int32, _dp]),
+    "pbb_matvec_ts": (C.c_int, [_vp, _dp, C.c_int64, _dp, C.c_int64, _dp, C.c_int32, _dp]),
     "pbb_direct_sum": (C.c_int, [C.POINTER(KernelDesc), _vp, C.c_int64, _vp, C.c_int64, _vp,
                                  C.c_int32, _vp, _vp]),
 }

@@ -217,6 +217,22 @@ class Operators:
                                              ss.ctypes.data_as(dp)))
         return U, VT, E, sf, ss
 
+    def set_mixed_tolerance(self, tol: float) -> None:
+        """Re-compression tolerance of the mixed modes' operator sets
+        (pbb_ops_set_mixed_tolerance; default 1e-10, 0 keeps the sets' own
+        factors); before the first mixed-precision call."""
+        capi.check(capi.lib().pbb_ops_set_mixed_tolerance(self._h, float(tol)))
+
+    def mixed_ranks(self):
+        """[(kf, ks, discarded relative norm)] per operator set, as the
+        mixed modes run them (computed on the host)."""
+        ns = self.nsets
+        kf = (C.c_int32 * ns)()
+        ks = (C.c_int32 * ns)()
+        tail = np.zeros(ns)
+        capi.check(capi.lib().pbb_ops_mixed_ranks(self._h, kf, ks, tail.ctypes.data_as(C.POINTER(C.c_double))))
+        return [(kf[i], ks[i], float(tail[i])) for i in range(ns)]
+
     @property
     def handle(self):
         return self._h
@@ -244,22 +260,43 @@ def _f64_cuda(t, name: str, numel: Optional[int] = None, shape=None):
 
 
 class Plan:
-    """One point set's tree on the current GPU (sources are the targets).
+    """A tree on the current GPU: of one point set (the sources are the
+    targets), or of separate targets and sources.
 
     Calls on one plan share its device workspace: issue them on one stream
     (or order the streams), as the C ABI requires."""
 
-    def __init__(self, ops: Operators, xyz, host_tree: bool = False):
-        """host_tree: build the tree on the host instead of the GPU (the
-        same arrays; kept as the GPU build's check)."""
+    def __init__(self, ops: Operators, xyz, host_tree: bool = False, sources=None,
+                 reject_outside: bool = False):
+        """xyz: the targets (N x 3).  sources (M x 3, optional): separate
+        sources; matvec then takes weights at the sources, (ncols, M), and
+        returns outputs at the targets, (ncols, N) (pbb_plan_create_ts).
+        host_tree: build the tree on the host instead of the GPU (the same
+        arrays; kept as the GPU build's check).  reject_outside: raise
+        PBBError if a point lies outside the root cube instead of
+        extrapolating."""
         self.ops = ops
+        dp = C.POINTER(C.c_double)
         xyz = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
-        self.npoints = xyz.shape[0]
+        flags = (capi.PLAN_HOST_TREE if host_tree else 0) | (capi.PLAN_REJECT_OUTSIDE if reject_outside else 0)
         h = C.c_void_p()
-        capi.check(capi.lib().pbb_plan_create_ex(ops.handle, xyz.ctypes.data_as(C.POINTER(C.c_double)),
-                                                 self.npoints, capi.PLAN_HOST_TREE if host_tree else 0,
-                                                 C.byref(h)))
+        if sources is None:
+            self.npoints = self.ntargets = self.nsources = xyz.shape[0]
+            capi.check(capi.lib().pbb_plan_create_ex(ops.handle, xyz.ctypes.data_as(dp), self.npoints, flags,
+                                                     C.byref(h)))
+        else:
+            src = np.ascontiguousarray(np.asarray(sources, dtype=np.float64).reshape(-1, 3))
+            self.ntargets, self.nsources = xyz.shape[0], src.shape[0]
+            self.npoints = self.ntargets + self.nsources
+            capi.check(capi.lib().pbb_plan_create_ts(ops.handle, xyz.ctypes.data_as(dp), self.ntargets,
+                                                     src.ctypes.data_as(dp), self.nsources, flags, C.byref(h)))
+        self.separate = sources is not None
         self._h = h
+
+    def sizes(self, transpose: bool = False):
+        """(points per weight column, points per output column): (sources,
+        targets) for the FMM, the reverse for the transposed FMM."""
+        return (self.ntargets, self.nsources) if transpose else (self.nsources, self.ntargets)
 
     def info(self) -> dict:
         I = capi.PlanInfo()
@@ -268,7 +305,8 @@ class Plan:
                 "boxes": [I.boxes[l] for l in range(I.tree_level + 1)],
                 "m2l_pairs": I.m2l_pairs, "near_pairs": I.near_pairs,
                 "max_leaf_points": I.max_leaf_points, "center": tuple(I.center),
-                "outside_points": I.outside_points}
+                "outside_points": I.outside_points, "ntargets": I.ntargets, "nsources": I.nsources,
+                "m2l_pairs_level": [I.m2l_pairs_level[l] for l in range(I.tree_level + 1)]}
 
     def update_points(self, xyz, stream: Optional[int] = None) -> None:
         """Replaces the coordinates (N x 3, caller's order) keeping the tree
@@ -298,20 +336,23 @@ class Plan:
         self.matvec_ptr(w.data_ptr(), ncols, out.data_ptr(), precision, stream)
         return out
 
-    def _io(self, w, out):
-        """(w, out, ncols) checked: w float64 CUDA (ncols, N) or (N,); out
-        of w's shape on w's device, made zero if None."""
+    def _io(self, w, out, transpose: bool = False):
+        """(w, out, ncols) checked: w float64 CUDA (ncols, n_in) or (n_in,);
+        out (ncols, n_out) or (n_out,) on w's device, made zero if None
+        (n_in, n_out from sizes(transpose); both N for one point set)."""
         import torch
+        n_in, n_out = self.sizes(transpose)
         if not isinstance(w, torch.Tensor) or w.dtype != torch.float64 or not w.is_cuda:
             raise TypeError("w must be a float64 CUDA tensor")
         w = w.contiguous()
         ncols = 1 if w.dim() == 1 else w.shape[0]
-        if w.dim() > 2 or w.numel() != ncols * self.npoints or ncols < 1:
-            raise ValueError("w must be (ncols, npoints) or (npoints,)")
+        if w.dim() > 2 or w.numel() != ncols * n_in or ncols < 1:
+            raise ValueError(f"w must be (ncols, {n_in}) or ({n_in},)")
+        oshape = (n_out,) if w.dim() == 1 else (ncols, n_out)
         if out is None:
-            out = torch.zeros_like(w)
+            out = torch.zeros(oshape, dtype=torch.float64, device=w.device)
         else:
-            _f64_cuda(out, "out", shape=w.shape)
+            _f64_cuda(out, "out", shape=oshape)
             if out.device != w.device:
                 raise ValueError("out must be on w's device")
         return w, out, ncols
@@ -324,9 +365,10 @@ class Plan:
             C.c_void_p(stream)))
 
     def matvec_transpose(self, w, out=None, precision: int = capi.PREC_FP64):
-        """Like matvec, with the transposed FMM operator."""
+        """Like matvec, with the transposed FMM operator (separate sets:
+        weights at the targets, outputs at the sources)."""
         import torch
-        w, out, ncols = self._io(w, out)
+        w, out, ncols = self._io(w, out, transpose=True)
         stream = torch.cuda.current_stream(w.device).cuda_stream
         self.matvec_transpose_ptr(w.data_ptr(), ncols, out.data_ptr(), precision, stream)
         return out
@@ -352,6 +394,11 @@ class Plan:
             self._h, C.c_void_p(sigma.data_ptr()), C.c_void_p(psi.data_ptr()), int(ncols),
             C.c_void_p(grad.data_ptr()), int(precision), C.c_void_p(stream)))
         return grad
+
+    def set_workspace_limit(self, nbytes: int):
+        """Caps the matvec workspace (0: device memory); larger matvecs run
+        in chunks of columns."""
+        capi.check(capi.lib().pbb_plan_set_workspace_limit(self._h, int(nbytes)))
 
     def set_timing(self, enable: bool = True):
         capi.check(capi.lib().pbb_plan_set_timing(self._h, 1 if enable else 0))
@@ -426,8 +473,9 @@ class Batch:
         if len(tr) != n:
             raise ValueError("transpose must be one flag or one flag per entry")
         for i, (p, w, o) in enumerate(zip(plans, weights, outputs)):
-            _f64_cuda(w, f"weights[{i}]", numel=ncols * p.npoints)
-            _f64_cuda(o, f"outputs[{i}]", numel=ncols * p.npoints)
+            n_in, n_out = p.sizes(tr[i])
+            _f64_cuda(w, f"weights[{i}]", numel=ncols * n_in)
+            _f64_cuda(o, f"outputs[{i}]", numel=ncols * n_out)
         self.plans, self.weights, self.outputs = list(plans), list(weights), list(outputs)
         entries = (capi.BatchEntry * n)()
         for i, (p, w, o) in enumerate(zip(plans, weights, outputs)):
@@ -475,17 +523,28 @@ class Batch:
             self._h = None
 
 
-def matvec_host(ops: Operators, xyz, weight, ncols: int, output=None) -> np.ndarray:
-    """The drop-in path (pbb_matvec): host FP64 arrays, output accumulated."""
+def matvec_host(ops: Operators, xyz, weight, ncols: int, output=None, sources=None) -> np.ndarray:
+    """The drop-in path (pbb_matvec, or pbb_matvec_ts with separate
+    sources): host FP64 arrays, weights column-major (ncols x sources),
+    output column-major (ncols x targets), accumulated."""
     xyz = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
     N = xyz.shape[0]
+    src = xyz if sources is None else np.ascontiguousarray(np.asarray(sources, dtype=np.float64).reshape(-1, 3))
+    M = src.shape[0]
     w = np.ascontiguousarray(np.asarray(weight, dtype=np.float64).reshape(-1))
-    if w.size != N * ncols:
-        raise ValueError("weight must hold ncols * N values")
+    if w.size != M * ncols:
+        raise ValueError("weight must hold ncols * nsources values")
     if output is None:
         output = np.zeros(N * ncols)
+    elif not (isinstance(output, np.ndarray) and output.dtype == np.float64 and output.size == N * ncols
+              and output.flags.c_contiguous):
+        raise ValueError("output must be a contiguous float64 array of ncols * ntargets values")
     dp = C.POINTER(C.c_double)
-    p = xyz.ctypes.data_as(dp)
-    capi.check(capi.lib().pbb_matvec(ops.handle, p, p, N, w.ctypes.data_as(dp), int(ncols),
-                                     output.ctypes.data_as(dp)))
+    if sources is None:
+        p = xyz.ctypes.data_as(dp)
+        capi.check(capi.lib().pbb_matvec(ops.handle, p, p, N, w.ctypes.data_as(dp), int(ncols),
+                                         output.ctypes.data_as(dp)))
+    else:
+        capi.check(capi.lib().pbb_matvec_ts(ops.handle, xyz.ctypes.data_as(dp), N, src.ctypes.data_as(dp), M,
+                                            w.ctypes.data_as(dp), int(ncols), output.ctypes.data_as(dp)))
     return output

@@ -98,10 +98,21 @@ struct DeviceGuard {
 
 }  // namespace
 
+// E'_d of every operator set split into 16-bit parts for one mixed
+// variant: set s at element off[s], [316][nch[s]][nsplit][R[s]][32]
 struct SplitE {
-    void *d = nullptr;   // [set][316][nch][nsplit][R][32] 16-bit parts
-    int R = 0, nch = 0;
-    std::vector<int> escale;  // per set
+    void *d = nullptr;
+    std::vector<size_t> off;
+    std::vector<int> R, nch, escale;  // per set
+};
+
+// The operator sets of the mixed modes (reduce_set): set s has kf x ks
+// blocks E'_d, VT'w (ks x n3) and U'w (n3 x kf), at the given element
+// offsets of the FP64 device arrays.
+struct MixedSet {
+    int kf = 0, ks = 0;
+    size_t e_off = 0, vt_off = 0, u_off = 0;
+    double tail = 0;  // discarded relative Frobenius norm (larger side)
 };
 
 struct pbb_ops {
@@ -110,56 +121,161 @@ struct pbb_ops {
     int device = -1;
     std::unique_ptr<pbb_ops> adj;  // transposed operator set, built on demand
     double *d_Tkz = nullptr, *d_Cw = nullptr, *d_VTw = nullptr, *d_Uw = nullptr, *d_E = nullptr;
+    // The mixed modes' operator sets, each re-compressed within level_tol
+    // (0: the sets' own factors); set before their first use
+    double level_tol = 1e-10;
+    std::vector<MixedSet> mset;
+    std::vector<double> mset_emax;  // max |E'_d| per set
+    std::vector<double> hm_E, hm_VTw, hm_Uw;  // host copies until uploaded
+    double *dm_E = nullptr, *dm_VTw = nullptr, *dm_Uw = nullptr;
     SplitE split[4];
-    // VTw and Uw of every operator set split into three BF16 parts for the
-    // mixed modes' proxy / expansion products ([set][3][R][32*nch])
+    // VT'w and U'w of every set split into three BF16 parts for the mixed
+    // modes' proxy / expansion products: set s at off[s], [3][R][32*nch[s]]
     struct SplitA {
         void *d = nullptr;
-        int R = 0, nch = 0;
-        size_t per_set = 0;
+        std::vector<size_t> off;
+        std::vector<int> nch;
     } vt3, u3;
+
+    // The mixed operator sets on the host (caller holds mu)
+    void compute_mixed_sets()
+    {
+        if (!mset.empty()) return;
+        const int n3 = h.n3, rf = h.rf, rs = h.rs;
+        std::vector<std::vector<double>> VTw(h.nsets), Uw(h.nsets);
+        for (int s = 0; s < h.nsets; s++) {
+            VTw[s].resize((size_t)rs * n3);
+            Uw[s].resize((size_t)n3 * rf);
+            for (int q = 0; q < n3; q++) {
+                for (int a = 0; a < rs; a++)
+                    VTw[s][a + (size_t)q * rs] = h.VT[(size_t)s * rs * n3 + a + (size_t)q * rs] * h.w.Kw[q];
+                for (int a = 0; a < rf; a++)
+                    Uw[s][q + (size_t)a * n3] = h.U[(size_t)s * n3 * rf + q + (size_t)a * n3] * h.w.Kw[q];
+            }
+        }
+        std::vector<ReducedSet> red(h.nsets);
+        for (int s = 0; s < h.nsets; s++) {
+            if (level_tol > 0) {
+                reduce_set(h, s, VTw[s].data(), Uw[s].data(), level_tol, red[s]);
+            } else {
+                red[s].kf = rf;
+                red[s].ks = rs;
+                red[s].E.assign(h.E.begin() + (size_t)s * 316 * rf * rs, h.E.begin() + (size_t)(s + 1) * 316 * rf * rs);
+                red[s].VTw = VTw[s];
+                red[s].Uw = Uw[s];
+            }
+        }
+        // ks padded to whole k chunks with zero columns of E' and zero rows of
+        // VT'w, so that the proxy GEMM writes every row the M2L reads
+        for (auto &r : red) {
+            const int ks32 = 32 * ((r.ks + 31) / 32);
+            if (ks32 == r.ks) continue;
+            std::vector<double> E2((size_t)316 * r.kf * ks32, 0.0), V2((size_t)ks32 * n3, 0.0);
+            for (int d = 0; d < 316; d++)
+                std::copy(r.E.begin() + (size_t)d * r.kf * r.ks, r.E.begin() + (size_t)(d + 1) * r.kf * r.ks,
+                          E2.begin() + (size_t)d * r.kf * ks32);
+            for (int q = 0; q < n3; q++)
+                for (int a = 0; a < r.ks; a++) V2[a + (size_t)q * ks32] = r.VTw[a + (size_t)q * r.ks];
+            r.E.swap(E2);
+            r.VTw.swap(V2);
+            r.ks = ks32;
+        }
+        std::vector<double> E, V, U;
+        mset.assign(h.nsets, MixedSet{});
+        mset_emax.assign(h.nsets, 0.0);
+        for (int s = 0; s < h.nsets; s++) {
+            MixedSet &m = mset[s];
+            m.kf = red[s].kf;
+            m.ks = red[s].ks;
+            m.tail = std::max(red[s].tail_f, red[s].tail_s);
+            m.e_off = E.size();
+            m.vt_off = V.size();
+            m.u_off = U.size();
+            E.insert(E.end(), red[s].E.begin(), red[s].E.end());
+            V.insert(V.end(), red[s].VTw.begin(), red[s].VTw.end());
+            U.insert(U.end(), red[s].Uw.begin(), red[s].Uw.end());
+            for (double x : red[s].E) mset_emax[s] = std::max(mset_emax[s], std::fabs(x));
+        }
+        hm_E.swap(E);
+        hm_VTw.swap(V);
+        hm_Uw.swap(U);
+    }
+
+    // ... and on the device (caller holds mu; the operators are uploaded)
+    void make_mixed_sets()
+    {
+        if (dm_E) return;
+        compute_mixed_sets();
+        dm_E = dupload(hm_E, "mixed E");
+        dm_VTw = dupload(hm_VTw, "mixed VT");
+        dm_Uw = dupload(hm_Uw, "mixed U");
+        std::vector<double>().swap(hm_E);
+        std::vector<double>().swap(hm_VTw);
+        std::vector<double>().swap(hm_Uw);
+    }
 
     const SplitA *ensure_split3(hipStream_t st)
     {
         std::lock_guard<std::mutex> lk(mu);
         if (vt3.d) return &vt3;
-        auto make = [&](SplitA &S, const double *A, int M, int K, const char *what) {
-            S.R = 16 * ((M + 15) / 16);
-            S.nch = (K + 31) / 32;
-            S.per_set = (size_t)3 * S.R * S.nch * 32;
-            S.d = dalloc<uint16_t>(S.per_set * h.nsets, what);
-            for (int s = 0; s < h.nsets; s++)
-                dev::launch_split_a3(A + (size_t)s * M * K, M, K, S.R, S.nch * 32, (uint16_t *)S.d + s * S.per_set, st);
-        };
-        make(vt3, d_VTw, h.rs, h.n3, "split VT");
-        make(u3, d_Uw, h.n3, h.rf, "split U");
+        make_mixed_sets();
+        const int n3 = h.n3, ns = h.nsets;
+        // VT'w: M = ks rows, K = n3; U'w: M = n3 rows, K = kf
+        size_t nv = 0, nu = 0;
+        vt3.off.assign(ns, 0);
+        vt3.nch.assign(ns, 0);
+        u3.off.assign(ns, 0);
+        u3.nch.assign(ns, 0);
+        for (int s = 0; s < ns; s++) {
+            vt3.off[s] = nv;
+            vt3.nch[s] = (n3 + 31) / 32;
+            nv += (size_t)3 * (16 * ((mset[s].ks + 15) / 16)) * vt3.nch[s] * 32;
+            u3.off[s] = nu;
+            u3.nch[s] = (mset[s].kf + 31) / 32;
+            nu += (size_t)3 * (16 * ((n3 + 15) / 16)) * u3.nch[s] * 32;
+        }
+        vt3.d = dalloc<uint16_t>(nv, "split VT");
+        u3.d = dalloc<uint16_t>(nu, "split U");
+        for (int s = 0; s < ns; s++) {
+            const MixedSet &m = mset[s];
+            dev::launch_split_a3(dm_VTw + m.vt_off, m.ks, n3, 16 * ((m.ks + 15) / 16), vt3.nch[s] * 32,
+                                 (uint16_t *)vt3.d + vt3.off[s], st);
+            dev::launch_split_a3(dm_Uw + m.u_off, n3, m.kf, 16 * ((n3 + 15) / 16), u3.nch[s] * 32,
+                                 (uint16_t *)u3.d + u3.off[s], st);
+        }
         check(hipStreamSynchronize(st), "split VT / U");
         return &vt3;
     }
 
-    // E_d split into 16-bit parts for a mixed-precision variant (built once)
+    // E'_d split into 16-bit parts for a mixed-precision variant (built once)
     const SplitE &ensure_split(int variant, hipStream_t st)
     {
         std::lock_guard<std::mutex> lk(mu);
         SplitE &S = split[variant];
         if (S.d) return S;
-        const int nsplit = variant == PBB_PREC_MIXED ? 3 : 2;
-        S.R = 16 * ((h.rf + 15) / 16);
-        S.nch = (h.rs + 31) / 32;
-        const size_t per_set = (size_t)316 * S.nch * nsplit * S.R * 32;
-        S.d = dalloc<uint16_t>(per_set * h.nsets, "split E");
-        S.escale.assign(h.nsets, 0);
-        for (int s = 0; s < h.nsets; s++) {
+        make_mixed_sets();
+        const int nsplit = variant == PBB_PREC_MIXED ? 3 : 2, ns = h.nsets;
+        S.off.assign(ns, 0);
+        S.R.assign(ns, 0);
+        S.nch.assign(ns, 0);
+        S.escale.assign(ns, 0);
+        size_t total = 0;
+        for (int s = 0; s < ns; s++) {
+            S.R[s] = 16 * ((mset[s].kf + 15) / 16);
+            S.nch[s] = (mset[s].ks + 31) / 32;
+            S.off[s] = total;
+            total += (size_t)316 * S.nch[s] * nsplit * S.R[s] * 32;
+        }
+        S.d = dalloc<uint16_t>(total, "split E");
+        for (int s = 0; s < ns; s++) {
+            const MixedSet &m = mset[s];
             if (variant == PBB_PREC_MIXED_F16X2) {
-                double mx = 0;
-                const size_t n = (size_t)316 * h.rf * h.rs;
-                for (size_t i = 0; i < n; i++) mx = std::max(mx, std::fabs(h.E[(size_t)s * n + i]));
                 int e = 0;
-                if (mx > 0) std::frexp(mx, &e);
-                S.escale[s] = mx > 0 ? 12 - e : 0;
+                if (mset_emax[s] > 0) std::frexp(mset_emax[s], &e);
+                S.escale[s] = mset_emax[s] > 0 ? 12 - e : 0;
             }
-            dev::launch_split_e(variant, d_E + (size_t)s * 316 * h.rf * h.rs, h.rf, h.rs, S.R, S.nch, S.escale[s],
-                                (uint16_t *)S.d + (size_t)s * per_set, st);
+            dev::launch_split_e(variant, dm_E + m.e_off, m.kf, m.ks, S.R[s], S.nch[s], S.escale[s],
+                                (uint16_t *)S.d + S.off[s], st);
         }
         check(hipStreamSynchronize(st), "split E");
         return S;
@@ -198,6 +314,7 @@ struct pbb_ops {
         std::lock_guard<std::mutex> lk(mu);
         if (adj) return adj.get();
         auto a = std::make_unique<pbb_ops>();
+        a->level_tol = level_tol;
         HostOps &g = a->h;
         g = h;
         g.rf = h.rs;
@@ -233,6 +350,7 @@ struct pbb_ops {
             (void)hipGetDevice(&prev);
             (void)hipSetDevice(device);
             dfree(d_Tkz); dfree(d_Cw); dfree(d_VTw); dfree(d_Uw); dfree(d_E);
+            dfree(dm_E); dfree(dm_VTw); dfree(dm_Uw);
             for (auto &x : split) dfree(x.d);
             dfree(vt3.d); dfree(u3.d);
             if (prev >= 0) (void)hipSetDevice(prev);
@@ -276,6 +394,10 @@ struct pbb_plan {
     hipStream_t stream = nullptr;
     int Lv = 0;
     int64_t N = 0;
+    // A plan of separate targets and sources holds the nt targets followed
+    // by the ns sources (N = nt + ns; sep); otherwise nt = ns = N, one set.
+    bool sep = false;
+    int64_t nt = 0, ns = 0;
     pbb_plan_info info{};
     double *x = nullptr, *y = nullptr, *z = nullptr;
     int32_t *perm = nullptr, *leaf_start = nullptr;
@@ -304,6 +426,7 @@ struct pbb_plan {
     M2LTile m2l_level[2][16];               // per-level tiles (width 0: m2l_tile)
     hipEvent_t lev_ev[17] = {};             // M2L level boundaries (autotuner)
     bool lev_made = false, lev_timing = false;
+    size_t ws_limit = 0;                    // workspace cap in bytes (0: device memory)
     hipEvent_t ev[PBB_NUM_PHASES] = {};
     bool events_made = false;
     double last_ms[PBB_NUM_PHASES] = {0};
@@ -770,6 +893,34 @@ PBB_API pbb_status pbb_ops_query(const pbb_ops *ops, pbb_ops_info *info)
     });
 }
 
+PBB_API pbb_status pbb_ops_set_mixed_tolerance(pbb_ops *ops, double tol)
+{
+    return guarded([&] {
+        if (!ops) fail(PBB_ERR_INVALID_ARG, "ops is NULL");
+        if (!(tol >= 0) || tol > 1e-6) fail(PBB_ERR_INVALID_ARG, "tol must be in [0, 1e-6]");
+        std::lock_guard<std::mutex> lk(ops->mu);
+        if (!ops->mset.empty() || (ops->adj && !ops->adj->mset.empty()))
+            fail(PBB_ERR_INVALID_ARG, "the mixed-precision operators are prepared already: set the tolerance "
+                                      "before the first mixed-precision call");
+        ops->level_tol = tol;
+        if (ops->adj) ops->adj->level_tol = tol;
+    });
+}
+
+PBB_API pbb_status pbb_ops_mixed_ranks(pbb_ops *ops, int32_t *kf, int32_t *ks, double *tail)
+{
+    return guarded([&] {
+        if (!ops || !kf || !ks) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        std::lock_guard<std::mutex> lk(ops->mu);
+        ops->compute_mixed_sets();
+        for (int s = 0; s < ops->h.nsets; s++) {
+            kf[s] = ops->mset[s].kf;
+            ks[s] = ops->mset[s].ks;
+            if (tail) tail[s] = ops->mset[s].tail;
+        }
+    });
+}
+
 PBB_API pbb_status pbb_ops_export(const pbb_ops *ops, double *U, double *VT, double *E, double *sigma_f,
                                   double *sigma_s)
 {
@@ -794,20 +945,20 @@ PBB_API void pbb_ops_destroy(pbb_ops *ops) { delete ops; }
 
 }  // extern "C"
 
-static void plan_create_host(pbb_ops *ops, const double *xyz, int64_t npoints, pbb_plan **out)
+static void plan_create_host(pbb_ops *ops, const double *xyz, int64_t npoints, int64_t nbox, pbb_plan **out)
 {
     {
         const HostOps &h = ops->h;
         int dev = current_device();
         HostTree T;
-        build_tree(T, xyz, npoints, h.L, h.tree_level);
+        build_tree(T, xyz, npoints, h.L, h.tree_level, nbox);
         ops->upload(dev);
         auto plan = std::make_unique<pbb_plan>();
         plan->ops = ops;
         plan->device = dev;
         check(hipStreamCreateWithFlags(&plan->stream, hipStreamNonBlocking), "hipStreamCreate");
         plan->Lv = T.Lv;
-        plan->N = T.N;
+        plan->N = plan->nt = plan->ns = T.N;
         plan->x = dupload(T.xs, "x");
         plan->y = dupload(T.ys, "y");
         plan->z = dupload(T.zs, "z");
@@ -916,10 +1067,12 @@ static void plan_create_host(pbb_ops *ops, const double *xyz, int64_t npoints, p
         for (int l = 0; l <= T.Lv && l < 16; l++) I.boxes[l] = T.lv[l].nb;
         I.nonempty_levels = T.Lv + 1;
         I.m2l_pairs = T.m2l_pairs;
+        std::copy(std::begin(T.m2l_pairs_level), std::end(T.m2l_pairs_level), I.m2l_pairs_level);
         I.near_pairs = T.near_pairs;
         I.max_leaf_points = T.max_leaf;
         for (int d = 0; d < 3; d++) I.center[d] = T.ctr[d];
         I.outside_points = T.outside;
+        I.ntargets = I.nsources = T.N;
         *out = plan.release();
     }
 }
@@ -928,7 +1081,7 @@ extern "C" {
 
 // The plan's tree built on the GPU (tree_gpu.hip): the same arrays as the
 // host build, with the plan's derived lists made there too.
-static void plan_create_gpu(pbb_ops *ops, const double *xyz, int64_t npoints, pbb_plan **out)
+static void plan_create_gpu(pbb_ops *ops, const double *xyz, int64_t npoints, int64_t nbox, pbb_plan **out)
 {
     const HostOps &h = ops->h;
     const int dev = current_device();
@@ -959,10 +1112,10 @@ static void plan_create_gpu(pbb_ops *ops, const double *xyz, int64_t npoints, pb
     int dirs[8][189][4];
     for (int o = 0; o < 8; o++) class_directions(o, h.w.Ktable, dirs[o]);
     dev::GpuTree T;
-    dev::build_tree_gpu(T, xyz, npoints, h.L, Lv, ctab, dirs, plan->stream);
+    dev::build_tree_gpu(T, xyz, npoints, nbox, h.L, Lv, ctab, dirs, plan->stream);
     // the plan owns the arrays from here on (its destructor frees them)
     plan->Lv = Lv;
-    plan->N = T.N;
+    plan->N = plan->nt = plan->ns = T.N;
     plan->x = T.x; plan->y = T.y; plan->z = T.z;
     plan->perm = T.perm; plan->leaf_start = T.leaf_start; plan->leaf_class = T.leaf_class;
     plan->pt_leaf = T.pt_leaf;
@@ -991,10 +1144,33 @@ static void plan_create_gpu(pbb_ops *ops, const double *xyz, int64_t npoints, pb
     for (int l = 0; l <= Lv && l < 16; l++) I.boxes[l] = T.lv[l].nb;
     I.nonempty_levels = Lv + 1;
     I.m2l_pairs = T.m2l_pairs;
+    std::copy(std::begin(T.m2l_pairs_level), std::end(T.m2l_pairs_level), I.m2l_pairs_level);
     I.near_pairs = T.near_pairs;
     I.max_leaf_points = T.max_leaf;
     for (int d = 0; d < 3; d++) I.center[d] = T.ctr[d];
     I.outside_points = T.outside;
+    I.ntargets = I.nsources = T.N;
+    *out = plan.release();
+}
+
+// A plan of npoints points whose first nbox define the bbox centre, built
+// as flags say; with PBB_PLAN_REJECT_OUTSIDE, fails when a point lies
+// outside the root cube.
+static void plan_create(pbb_ops *ops, const double *xyz, int64_t npoints, int64_t nbox, int32_t flags,
+                        pbb_plan **out)
+{
+    if (flags & ~(PBB_PLAN_HOST_TREE | PBB_PLAN_REJECT_OUTSIDE)) fail(PBB_ERR_INVALID_ARG, "unknown plan flags");
+    const HostOps &h = ops->h;
+    if (!kernel_id_valid(h.kernel.id) || h.kernel.id == PBB_KERNEL_POLY_XY)
+        fail(PBB_ERR_UNSUPPORTED, "the kernel has no device functor for the near field");
+    pbb_plan *p = nullptr;
+    if (flags & PBB_PLAN_HOST_TREE) plan_create_host(ops, xyz, npoints, nbox, &p);
+    else plan_create_gpu(ops, xyz, npoints, nbox, &p);
+    std::unique_ptr<pbb_plan> plan(p);
+    if ((flags & PBB_PLAN_REJECT_OUTSIDE) && plan->info.outside_points > 0)
+        fail(PBB_ERR_INVALID_ARG, std::to_string(plan->info.outside_points) +
+                                      " points lie outside the root cube of side L around the targets' bbox "
+                                      "centre: build the operators with a larger L");
     *out = plan.release();
 }
 
@@ -1005,12 +1181,33 @@ PBB_API pbb_status pbb_plan_create_ex(pbb_ops *ops, const double *xyz, int64_t n
         if (!out) fail(PBB_ERR_INVALID_ARG, "out is NULL");
         *out = nullptr;
         if (!ops || !xyz) fail(PBB_ERR_INVALID_ARG, "NULL argument");
-        if (flags & ~PBB_PLAN_HOST_TREE) fail(PBB_ERR_INVALID_ARG, "unknown plan flags");
-        const HostOps &h = ops->h;
-        if (!kernel_id_valid(h.kernel.id) || h.kernel.id == PBB_KERNEL_POLY_XY)
-            fail(PBB_ERR_UNSUPPORTED, "the kernel has no device functor for the near field");
-        if (flags & PBB_PLAN_HOST_TREE) plan_create_host(ops, xyz, npoints, out);
-        else plan_create_gpu(ops, xyz, npoints, out);
+        plan_create(ops, xyz, npoints, npoints, flags, out);
+    });
+}
+
+PBB_API pbb_status pbb_plan_create_ts(pbb_ops *ops, const double *targets, int64_t ntargets, const double *sources,
+                                      int64_t nsources, int32_t flags, pbb_plan **out)
+{
+    return guarded([&] {
+        if (!out) fail(PBB_ERR_INVALID_ARG, "out is NULL");
+        *out = nullptr;
+        if (!ops || !targets || !sources) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        if (ntargets <= 0 || nsources <= 0) fail(PBB_ERR_INVALID_ARG, "ntargets and nsources must be positive");
+        if (ntargets + nsources >= (1LL << 31)) fail(PBB_ERR_INVALID_ARG, "ntargets + nsources must be < 2^31");
+        // one tree over [targets; sources], centred on the targets' bbox
+        // (compute.hpp:92-127 centres on the targets and shifts the sources
+        // by the same vector)
+        std::vector<double> both((size_t)3 * (ntargets + nsources));
+        std::memcpy(both.data(), targets, sizeof(double) * 3 * (size_t)ntargets);
+        std::memcpy(both.data() + 3 * (size_t)ntargets, sources, sizeof(double) * 3 * (size_t)nsources);
+        pbb_plan *p = nullptr;
+        plan_create(ops, both.data(), ntargets + nsources, ntargets, flags, &p);
+        p->sep = true;
+        p->nt = ntargets;
+        p->ns = nsources;
+        p->info.ntargets = ntargets;
+        p->info.nsources = nsources;
+        *out = p;
     });
 }
 
@@ -1023,6 +1220,7 @@ PBB_API pbb_status pbb_plan_update_points(pbb_plan *plan, const double *xyz, int
 {
     return guarded([&] {
         if (!plan || !xyz) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        if (plan->sep) fail(PBB_ERR_UNSUPPORTED, "moving points needs a plan whose sources are its targets");
         if (npoints != plan->N) fail(PBB_ERR_INVALID_ARG, "npoints must be the plan's point count");
         const int64_t N = plan->N;
         const double *c = plan->info.center;
@@ -1141,6 +1339,15 @@ PBB_API pbb_status pbb_plan_get_m2l_tile(const pbb_plan *plan, int32_t precision
     });
 }
 
+PBB_API pbb_status pbb_plan_set_workspace_limit(pbb_plan *plan, int64_t bytes)
+{
+    return guarded([&] {
+        if (!plan) fail(PBB_ERR_INVALID_ARG, "plan is NULL");
+        if (bytes < 0) fail(PBB_ERR_INVALID_ARG, "bytes must be >= 0");
+        plan->ws_limit = (size_t)bytes;
+    });
+}
+
 PBB_API pbb_status pbb_plan_timings(pbb_plan *plan, double *ms)
 {
     return guarded([&] {
@@ -1159,10 +1366,21 @@ PBB_API pbb_status pbb_plan_timings(pbb_plan *plan, double *ms)
     });
 }
 
-// The column-wave mixed M2L tile takes at most 224 rows of r_f; larger
-// ranks always run the row-wave tile, whose workgroups split the rows into
-// row groups (as the FP64 tile does above 256 rows), so every rank runs.
-static bool wave_rows_forced(const HostOps &h) { return h.rf > 224; }
+// Which of the plan's sorted points carry the weights and which receive the
+// outputs: weights w[c*in_n + j] belong to the points of input index
+// in_lo + j, outputs out[c*out_n + i] to those of index out_lo + i.  One
+// point set: both ranges are all N points.  Separate sets: the FMM (o =
+// p->ops) takes weights on the sources and writes the targets; the
+// transposed FMM (the adjoint operators) the other way round.
+struct PointIo {
+    int64_t in_lo, in_n, out_lo, out_n;
+};
+static PointIo point_io(const pbb_plan *p, const pbb_ops *o)
+{
+    if (!p->sep) return {0, p->N, 0, p->N};
+    if (o == p->ops) return {p->nt, p->ns, 0, p->nt};
+    return {0, p->nt, p->nt, p->ns};
+}
 
 static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double *w, int ncols, double *out,
                        hipStream_t st, int prec, bool build);
@@ -1176,14 +1394,20 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
 static void matvec_own(pbb_plan *p, pbb_ops *o, const double *w, int ncols, double *out, hipStream_t st, int prec)
 {
     const bool mixed = prec != PBB_PREC_FP64;
+    const PointIo io = point_io(p, o);
     int chunk = ncols;
     if (out) {
         Workspace tmp;
         size_t need = layout_workspace(*p, chunk, mixed, nullptr, tmp);
-        if (need > p->ws_bytes) {
+        if (need > p->ws_bytes || (p->ws_limit && need > p->ws_limit)) {
             size_t free_b = 0, total_b = 0;
             check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-            const size_t room = p->ws_bytes + free_b - std::min(free_b, (size_t)(1ull << 30));  // keep 1 GiB
+            // The workspace stays with the plan after the call, so leave the
+            // caller a sixteenth of the device (at least 1 GiB) for its own
+            // allocations.
+            const size_t margin = std::max((size_t)(1ull << 30), total_b / 16);
+            size_t room = p->ws_bytes + free_b - std::min(free_b, margin);
+            if (p->ws_limit) room = std::min(room, p->ws_limit);
             while (chunk > 1 && need > room) {
                 chunk = (chunk + 1) / 2;
                 need = layout_workspace(*p, chunk, mixed, nullptr, tmp);
@@ -1193,7 +1417,8 @@ static void matvec_own(pbb_plan *p, pbb_ops *o, const double *w, int ncols, doub
     for (int c0 = 0; c0 < ncols; c0 += chunk) {
         const int cn = std::min(chunk, ncols - c0);
         p->ensure_workspace(cn, mixed);
-        matvec_run(p, o, p->W, w + (size_t)c0 * p->N, cn, out ? out + (size_t)c0 * p->N : nullptr, st, prec, true);
+        matvec_run(p, o, p->W, w + (size_t)c0 * io.in_n, cn, out ? out + (size_t)c0 * io.out_n : nullptr, st, prec,
+                   true);
     }
 }
 
@@ -1233,13 +1458,16 @@ static void tune_m2l(pbb_plan *plan, const double *w_dev, int ncols, int precisi
     DeviceGuard g(plan->device);
     const bool was_timing = plan->timing;
     check(pbb_plan_set_timing(plan, 1) == PBB_OK ? hipSuccess : hipErrorUnknown, "enable timing");
-    const int64_t nout = plan->N * (int64_t)ncols;
+    const int64_t nout = point_io(plan, plan->ops).out_n * (int64_t)ncols;
     double *scratch = nullptr, *ref = nullptr, *part = nullptr;
     // candidates {width, columns per box, flags}
     static const int c64[][3] = {{32, 32, 0}, {64, 32, 0}, {128, 32, 0}, {64, 32, 1}, {128, 32, 1}};
+    // (the column-wave tiles of 128 and 256 columns suit the small ranks of
+    // re-compressed deep levels, whose rows would leave row-wave waves idle)
     static const int cmx[][3] = {{128, 16, 5}, {128, 32, 5}, {128, 8, 5}, {256, 8, 5}, {128, 16, 4},
                                  {64, 16, 5}, {128, 32, 1}, {64, 16, 0}, {128, 16, 5 | PBB_M2L_DIR_GROUPS(4)},
-                                 {64, 16, 13}, {64, 8, 13}, {64, 32, 13}};
+                                 {64, 16, 13}, {64, 8, 13}, {64, 32, 13}, {128, 16, 1}, {256, 16, 1},
+                                 {256, 32, 1}};
     const int(*cands)[3] = mixed ? cmx : c64;
     const int ncand = mixed ? (int)(sizeof(cmx) / sizeof(cmx[0])) : (int)(sizeof(c64) / sizeof(c64[0]));
     pbb_plan::M2LTile bestt{}, closest{};
@@ -1431,8 +1659,9 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
     // the mixed modes keep S / F in FP32 when the direct passes read them
     // (not with the FP64 VT / U of the cross-product-sum tile)
     const bool s32 = mixed && direct_down && !split_acc;
-    dev::launch_gather_weights(w, p->perm, p->N, ncols, W.ws, st);
-    if (direct) dev::launch_gather_weights_t(w, p->perm, p->N, ncols, W.wsT, st);
+    const PointIo io = point_io(p, o);
+    dev::launch_gather_weights(w, p->perm, p->N, ncols, W.ws, io.in_lo, io.in_n, st);
+    if (direct) dev::launch_gather_weights_t(w, p->perm, p->N, ncols, W.wsT, io.in_lo, io.in_n, st);
     const int cc_p2m = std::max(1, std::min({ncols, 64, 4096 / n3}));
     const int cc_m2m = std::max(1, std::min(ncols, 2048 / n3));
     if (Lv >= 2 && direct) {
@@ -1463,11 +1692,19 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         dev::launch_m2m(a, (int)p->lv[l].nb, st);
     }
     mark(2);
+    // the mixed modes run each level with its operator set's own ranks
+    // (kf x ks, pbb_ops::mset); Pf keeps the stride r_f
+    auto mset = [&](int l) -> const MixedSet & { return o->mset[h.ops_set(l)]; };
     for (int l = 2; l <= Lv; l++) {
         const DevLevel &D = p->lv[l];
+        const int sl = h.ops_set(l);
         dev::GemmArgs g{};
         g.M = h.rs; g.K = n3; g.ldb = n3; g.ldc = ldP; g.ncol = D.nb * ncols; g.alpha = 1.0;
-        g.A = o->d_VTw + (size_t)h.ops_set(l) * h.rs * n3;
+        g.A = o->d_VTw + (size_t)sl * h.rs * n3;
+        if (mixed) {
+            g.M = mset(l).ks;
+            g.A = o->dm_VTw + mset(l).vt_off;
+        }
         g.b_f32 = s32 ? 1 : 0;
         g.B = s32 ? (const void *)((float *)W.S + (size_t)D.off * ncols * n3)
                   : (const void *)(W.S + (size_t)D.off * ncols * n3);
@@ -1479,20 +1716,26 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         }
         if (SA) {  // three-part BF16 matrix cores (P is FP32 in the mixed modes)
             dev::GemmSplitArgs q{};
-            q.M = g.M; q.K = g.K; q.ldb = g.ldb; q.ldc = g.ldc; q.b_f32 = g.b_f32; q.c_f32 = 1; q.nch = SA->nch;
-            q.ncol = g.ncol; q.alpha = 1.0; q.As = (const uint16_t *)SA->d + h.ops_set(l) * SA->per_set;
+            q.M = g.M; q.K = g.K; q.ldb = g.ldb; q.ldc = g.ldc; q.b_f32 = g.b_f32; q.c_f32 = 1; q.nch = SA->nch[sl];
+            q.ncol = g.ncol; q.alpha = 1.0; q.As = (const uint16_t *)SA->d + SA->off[sl];
             q.B = g.B; q.C = g.C;
             dev::launch_gemm_split3(q, st);
         } else {
             dev::launch_gemm(g, st);
         }
-        if (prec == PBB_PREC_MIXED_F16X2) {
-            dev::launch_zero_u32(W.xmax + l, 1, st);
-            dev::launch_absmax(W.P32 + (size_t)D.off * ncols * ldP, D.nb * ncols * ldP, W.xmax + l, st);
+        if (mixed) {
+            // the k chunks the M2L reads: the GEMM writes every row of them
+            // (rows past ks come from zero rows of the split operator)
+            const int used = 32 * SE->nch[sl];
+            if (used > ldP) fail(PBB_ERR_INTERNAL, "proxy chunks past the padding");
+            if (prec == PBB_PREC_MIXED_F16X2) {
+                dev::launch_zero_u32(W.xmax + l, 1, st);
+                dev::launch_absmax_rows(W.P32 + (size_t)D.off * ncols * ldP, D.nb * ncols, ldP, used, W.xmax + l, st);
+            }
+            if (prec == PBB_PREC_MIXED_F16X2 || prec == PBB_PREC_MIXED_BF16X2)
+                dev::launch_split_p2(prec, W.P32 + (size_t)D.off * ncols * ldP, D.nb * ncols, ldP, used / 32,
+                                     prec == PBB_PREC_MIXED_F16X2 ? W.xmax + l : nullptr, st);
         }
-        if (prec == PBB_PREC_MIXED_F16X2 || prec == PBB_PREC_MIXED_BF16X2)
-            dev::launch_split_p2(prec, W.P32 + (size_t)D.off * ncols * ldP, D.nb * ncols * (ldP / 32),
-                                 prec == PBB_PREC_MIXED_F16X2 ? W.xmax + l : nullptr, st);
     }
     mark(3);
     // M2L tiles: W columns = TB target boxes x cpt columns.
@@ -1511,20 +1754,22 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         const int nchunks_m = (ncols + cptm - 1) / cptm;
         const int width_m = tl.width ? tl.width : 128;
         const int groups_m = tl.width ? std::max(1, (tl.flags >> 8) & 0xff) : 1;
+        const int sl = h.ops_set(l);
         dev::M2LMixArgs a{};
-        a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = std::max(1, std::min(16, width_m / cptm)); a.cpt = cptm;
+        a.rf = mset(l).kf; a.rs = mset(l).ks; a.ldf = h.rf;
+        a.ncols = ncols; a.TB = std::max(1, std::min(16, width_m / cptm)); a.cpt = cptm;
         a.chunk_major = tl.width ? (tl.flags & PBB_M2L_CHUNK_MAJOR) : 1;
         a.row_split = tl.width ? (tl.flags & PBB_M2L_ROW_SPLIT) != 0 : 0;
         a.wave_rows = tl.width ? (tl.flags & PBB_M2L_WAVE_ROWS) != 0 : 1;
-        if (wave_rows_forced(h)) {
+        if (a.rf > 224) {  // the column-wave tile takes at most 224 rows
             a.wave_rows = 1;
             a.row_split = 0;
         }
-        if (a.wave_rows) a.TB = std::max(1, std::min(a.TB, dev::m2l_rows_max_width(prec, SE->R) / cptm));
+        if (a.wave_rows) a.TB = std::max(1, std::min(a.TB, dev::m2l_rows_max_width(prec, SE->R[sl]) / cptm));
         a.split_acc = split_acc;
         if (split_acc) a.TB = std::max(1, std::min(a.TB, 64 / cptm));
         a.side = 1 << l;
-        a.kpad = ldP; a.nch = SE->nch; a.escale = SE->escale[h.ops_set(l)]; a.off = D.off; a.R = SE->R;
+        a.kpad = ldP; a.nch = SE->nch[sl]; a.escale = SE->escale[sl]; a.off = D.off; a.R = SE->R[sl];
         a.tile_prefix[0] = 0;
         a.class_start[0] = 0;
         for (int c = 0; c < 8; c++) {
@@ -1532,8 +1777,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
             a.tile_prefix[c + 1] = a.tile_prefix[c] + (int)((D.class_count[c] + a.TB - 1) / a.TB);
         }
         a.order = D.order; a.coord = D.coord; a.map = D.map;
-        const int nsplit = prec == PBB_PREC_MIXED ? 3 : 2;
-        a.Es = (const uint16_t *)SE->d + (size_t)h.ops_set(l) * 316 * SE->nch * nsplit * SE->R * 32;
+        a.Es = (const uint16_t *)SE->d + SE->off[sl];
         a.P = W.P32; a.Pf = W.Pf32;
         a.xmax = prec == PBB_PREC_MIXED_F16X2 ? W.xmax + l : nullptr;
         // direction groups only where the launch fills the GPU many times over
@@ -1573,11 +1817,14 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
     // local expansions F_l = scale_l * Uw Pf_l overwrite S_l (no longer needed)
     for (int l = 2; l <= Lv; l++) {
         const DevLevel &D = p->lv[l];
+        const int sl = h.ops_set(l);
         dev::GemmArgs g{};
         g.M = n3; g.K = h.rf; g.ldb = h.rf; g.ldc = n3; g.ncol = D.nb * ncols;
         g.alpha = h.homog ? std::pow(1.0 / D.len, h.homogen_flipped) : 1.0;  // compute.hpp:433-435
-        g.A = o->d_Uw + (size_t)h.ops_set(l) * n3 * h.rf;
+        g.A = o->d_Uw + (size_t)sl * n3 * h.rf;
         if (mixed) {
+            g.K = mset(l).kf;
+            g.A = o->dm_Uw + mset(l).u_off;
             g.b_f32 = 1;
             g.B = W.Pf32 + (size_t)D.off * ncols * h.rf;
         } else {
@@ -1588,8 +1835,8 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         if (SA) {  // three-part BF16 matrix cores (Pf is FP32 in the mixed modes)
             const auto &U3 = o->u3;
             dev::GemmSplitArgs q{};
-            q.M = g.M; q.K = g.K; q.ldb = g.ldb; q.ldc = g.ldc; q.b_f32 = 1; q.c_f32 = g.c_f32; q.nch = U3.nch;
-            q.ncol = g.ncol; q.alpha = g.alpha; q.As = (const uint16_t *)U3.d + h.ops_set(l) * U3.per_set;
+            q.M = g.M; q.K = g.K; q.ldb = g.ldb; q.ldc = g.ldc; q.b_f32 = 1; q.c_f32 = g.c_f32; q.nch = U3.nch[sl];
+            q.ncol = g.ncol; q.alpha = g.alpha; q.As = (const uint16_t *)U3.d + U3.off[sl];
             q.B = g.B; q.C = g.C;
             dev::launch_gemm_split3(q, st);
         } else {
@@ -1624,6 +1871,8 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         a.ctab = LL.ctab; a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.ws = W.ws;
         a.F = Lv >= 2 ? W.S : nullptr;
         a.out = out;
+        a.out_lo = io.out_lo;
+        a.out_n = io.out_n;
         a.wsT = W.wsT;
         a.lp = direct_down ? W.lp : nullptr;
         a.leaf_class = p->leaf_class;
@@ -1734,6 +1983,7 @@ PBB_API pbb_status pbb_plan_grad_positions(pbb_plan *plan, const double *sigma_d
             fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
         if (plan->ops->h.kernel.symmetry != 1)
             fail(PBB_ERR_UNSUPPORTED, "position gradients need a symmetric kernel");
+        if (plan->sep) fail(PBB_ERR_UNSUPPORTED, "position gradients need a plan whose sources are its targets");
         pbb_ops *adj = plan->ops->adjoint();
         DeviceGuard g(plan->device);
         adj->upload(plan->device);
@@ -1882,12 +2132,14 @@ static void batch_create(const pbb_batch_entry *E, int nplans, int ncols, int pr
             const char *x = (const char *)a, *y = (const char *)b2;
             return x < y + nb2 * (int64_t)sizeof(double) && y < x + na * (int64_t)sizeof(double);
         };
+        std::vector<PointIo> io(nplans);
+        for (int i = 0; i < nplans; i++) io[i] = point_io(plans[i], ops[i]);
         for (int i = 0; i < nplans; i++) {
-            const int64_t ni = plans[i]->N * (int64_t)ncols;
+            const int64_t wi = io[i].in_n * (int64_t)ncols, oi = io[i].out_n * (int64_t)ncols;
             for (int j = 0; j < i; j++) {
-                const int64_t nj = plans[j]->N * (int64_t)ncols;
-                const bool c = plans[i] == plans[j] || overlap(out_dev[i], ni, out_dev[j], nj) ||
-                               overlap(w_dev[i], ni, out_dev[j], nj) || overlap(w_dev[j], nj, out_dev[i], ni);
+                const int64_t wj = io[j].in_n * (int64_t)ncols, oj = io[j].out_n * (int64_t)ncols;
+                const bool c = plans[i] == plans[j] || overlap(out_dev[i], oi, out_dev[j], oj) ||
+                               overlap(w_dev[i], wi, out_dev[j], oj) || overlap(w_dev[j], wj, out_dev[i], oi);
                 if (c) root[find(i)] = find(j);
             }
         }
@@ -1914,7 +2166,8 @@ static void batch_create(const pbb_batch_entry *E, int nplans, int ncols, int pr
         for (int k = 0; k < nbr; k++) check(hipStreamWaitEvent(b->side[k], b->ev[nbr], 0), "fork");
         for (int i = 0; i < nplans; i++) {
             hipStream_t s = b->side[branch[i]];
-            if (E[i].flags & PBB_BATCH_OVERWRITE) dev::launch_fill_zero(out_dev[i], plans[i]->N * (int64_t)ncols, s);
+            if (E[i].flags & PBB_BATCH_OVERWRITE)
+                dev::launch_fill_zero(out_dev[i], point_io(plans[i], ops[i]).out_n * (int64_t)ncols, s);
             matvec_run(plans[i], ops[i], wsp[i], w_dev[i], ncols, out_dev[i], s, precision, false);
         }
         for (int k = 0; k < nbr; k++) {
@@ -1999,33 +2252,29 @@ PBB_API pbb_status pbb_direct_sum(const pbb_kernel_desc *kernel, const double *t
     });
 }
 
-PBB_API pbb_status pbb_matvec(pbb_ops *ops, const double *target, const double *source, int64_t npoints,
-                              const double *weight, int32_t ncols, double *output)
+// One matvec from host buffers: a plan (one point set when same, else the
+// targets followed by the sources), the weights uploaded, out += A w.
+static pbb_status matvec_host(pbb_ops *ops, const double *target, int64_t nt, const double *source, int64_t ns,
+                              bool same, const double *weight, int32_t ncols, double *output)
 {
     pbb_plan *plan = nullptr;
-    pbb_status st = guarded([&] {
-        if (!ops || !target || !source || !weight || !output) fail(PBB_ERR_INVALID_ARG, "NULL argument");
-        if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
-        if (npoints <= 0) fail(PBB_ERR_INVALID_ARG, "npoints must be positive");
-        if (target != source && std::memcmp(target, source, sizeof(double) * 3 * (size_t)npoints) != 0)
-            fail(PBB_ERR_UNSUPPORTED, "sources must be the targets (the reference supports nothing else)");
-    });
-    if (st != PBB_OK) return st;
-    st = pbb_plan_create(ops, target, npoints, &plan);
+    pbb_status st = same ? pbb_plan_create(ops, target, nt, &plan)
+                         : pbb_plan_create_ts(ops, target, nt, source, ns, 0, &plan);
     if (st != PBB_OK) return st;
     double *dw = nullptr, *dout = nullptr;
     st = guarded([&] {
         DeviceGuard g(plan->device);
-        size_t bytes = sizeof(double) * (size_t)npoints * ncols;
-        dw = dalloc<double>((size_t)npoints * ncols, "weights");
-        dout = dalloc<double>((size_t)npoints * ncols, "output");
-        check(hipMemcpyAsync(dw, weight, bytes, hipMemcpyHostToDevice, plan->stream), "copy weights");
-        check(hipMemsetAsync(dout, 0, bytes, plan->stream), "clear output");
+        const size_t nw = (size_t)ns * ncols, no = (size_t)nt * ncols;
+        dw = dalloc<double>(nw, "weights");
+        dout = dalloc<double>(no, "output");
+        check(hipMemcpyAsync(dw, weight, nw * sizeof(double), hipMemcpyHostToDevice, plan->stream), "copy weights");
+        check(hipMemsetAsync(dout, 0, no * sizeof(double), plan->stream), "clear output");
         matvec_own(plan, plan->ops, dw, ncols, dout, plan->stream, PBB_PREC_FP64);
-        std::vector<double> res((size_t)npoints * ncols);
-        check(hipMemcpyAsync(res.data(), dout, bytes, hipMemcpyDeviceToHost, plan->stream), "copy output");
+        std::vector<double> res(no);
+        check(hipMemcpyAsync(res.data(), dout, no * sizeof(double), hipMemcpyDeviceToHost, plan->stream),
+              "copy output");
         check(hipStreamSynchronize(plan->stream), "hipStreamSynchronize");
-        for (size_t i = 0; i < res.size(); i++) output[i] += res[i];
+        for (size_t i = 0; i < no; i++) output[i] += res[i];
     });
     {
         int prev = -1;
@@ -2037,6 +2286,32 @@ PBB_API pbb_status pbb_matvec(pbb_ops *ops, const double *target, const double *
     }
     pbb_plan_destroy(plan);
     return st;
+}
+
+PBB_API pbb_status pbb_matvec(pbb_ops *ops, const double *target, const double *source, int64_t npoints,
+                              const double *weight, int32_t ncols, double *output)
+{
+    bool same = true;
+    pbb_status st = guarded([&] {
+        if (!ops || !target || !source || !weight || !output) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
+        if (npoints <= 0) fail(PBB_ERR_INVALID_ARG, "npoints must be positive");
+        same = target == source || std::memcmp(target, source, sizeof(double) * 3 * (size_t)npoints) == 0;
+    });
+    if (st != PBB_OK) return st;
+    return matvec_host(ops, target, npoints, source, npoints, same, weight, ncols, output);
+}
+
+PBB_API pbb_status pbb_matvec_ts(pbb_ops *ops, const double *target, int64_t ntargets, const double *source,
+                                 int64_t nsources, const double *weight, int32_t ncols, double *output)
+{
+    pbb_status st = guarded([&] {
+        if (!ops || !target || !source || !weight || !output) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
+        if (ntargets <= 0 || nsources <= 0) fail(PBB_ERR_INVALID_ARG, "ntargets and nsources must be positive");
+    });
+    if (st != PBB_OK) return st;
+    return matvec_host(ops, target, ntargets, source, nsources, false, weight, ncols, output);
 }
 
 }  // extern "C"

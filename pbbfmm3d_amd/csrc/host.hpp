@@ -84,6 +84,25 @@ struct PrecomputeDevice {
 
 // Full precompute: ComputeWeights + PrecomputeM2L (H2_3D_Tree.cpp:40-66).
 void precompute_ops(HostOps &ops, const HostEval &eval, const PrecomputeDevice *dev = nullptr);
+// One operator set re-compressed for the mixed modes.  The reference keeps
+// the level-2 cutoff at every level of a non-homogeneous kernel
+// (H2_3D_Tree.cpp:406-436), but a smooth kernel's far field is of much lower
+// numerical rank on small boxes.  With Q_L (r_f x kf) and Q_R (r_s x ks) the
+// leading left and right singular vectors of all 316 E_d together
+// ([E_0 .. E_315] and [E_0; ..; E_315]), E'_d = Q_L^T E_d Q_R, U' = Uw Q_L
+// and VT' = Q_R^T VTw apply the same far field within tau (the relative
+// Frobenius norm of the discarded singular values of the stacked E, each
+// side), with kf a multiple of 16 and ks of 32 (chunk sizes of the M2L).
+// When that saves nothing the set keeps its own factors (kf = r_f, ks = r_s).
+struct ReducedSet {
+    int kf = 0, ks = 0;
+    std::vector<double> E;    // 316 x kf x ks (col-major per direction)
+    std::vector<double> VTw;  // ks x n3 (col-major), Kweights applied
+    std::vector<double> Uw;   // n3 x kf (col-major), Kweights applied
+    double tail_f = 0, tail_s = 0;  // discarded relative Frobenius norms
+};
+void reduce_set(const HostOps &ops, int set, const double *VTw, const double *Uw, double tau, ReducedSet &out);
+
 // Kernel blocks of one operator set (H2_3D_Tree.cpp:277-374):
 // Kcell[d*n6 + i + j*n3] = K(t_i, s_j^(d)) / (Kw_i Kw_j).
 void kernel_blocks(const HostOps &ops, const HostEval &eval, double boxLen,
@@ -111,12 +130,15 @@ struct HostTree {
     std::vector<int32_t> leaf_start;     // nb[Lv]+1
     std::vector<Level> lv;               // 0..Lv
     int64_t m2l_pairs = 0, near_pairs = 0, max_leaf = 0, outside = 0;
+    int64_t m2l_pairs_level[16] = {0};
 };
 
 // Replicates compute.hpp:92-127 (re-centring on the bbox centre) and
 // DistributeFieldPoints (compute.hpp:1126-1203) with the node centres of
-// BuildFMMHierarchy (H2_3D_Tree.cpp:917-977).
-void build_tree(HostTree &T, const double *xyz, int64_t N, double L, int levels);
+// BuildFMMHierarchy (H2_3D_Tree.cpp:917-977).  The bbox (and so the centre)
+// is that of the first nbox points: the targets, when the point set is the
+// targets followed by separate sources; nbox = N when they are the same set.
+void build_tree(HostTree &T, const double *xyz, int64_t N, double L, int levels, int64_t nbox);
 
 // Leaf grid coordinates, packed ix | iy << 10 | iz << 20, of a point p
 // already re-centred, by build_tree's comparison chain over the per-axis

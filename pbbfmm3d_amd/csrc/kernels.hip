@@ -81,27 +81,31 @@ __device__ __forceinline__ void sn_eval_to(int n, const double *__restrict__ Tkz
 }
 
 // ---------------------------------------------------------------------------
-// ws[c*N + k] = w[c*N + perm[k]]
+// ws[c*N + k] = w[c*w_n + j] with j = perm[k] - w_lo when 0 <= j < w_n,
+// else 0: the points that carry no weight (the targets of a plan whose
+// sources are not its targets) enter the sums with weight zero
 __global__ void gather_weights(const double *__restrict__ w, const int32_t *__restrict__ perm, int64_t N,
-                               int ncols, double *__restrict__ ws)
+                               int ncols, double *__restrict__ ws, int64_t w_lo, int64_t w_n)
 {
     int64_t total = N * ncols;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total;
          q += (int64_t)gridDim.x * blockDim.x) {
         int64_t c = q / N, k = q - c * N;
-        ws[q] = w[c * N + perm[k]];
+        const int64_t j = perm[k] - w_lo;
+        ws[q] = (uint64_t)j < (uint64_t)w_n ? w[c * w_n + j] : 0.0;
     }
 }
 
-// wsT[k*ncols + c] = w[c*N + perm[k]] (point-major copy for the near field)
+// wsT[k*ncols + c], the same values point-major (for the near field)
 __global__ void gather_weights_t(const double *__restrict__ w, const int32_t *__restrict__ perm, int64_t N,
-                                 int ncols, double *__restrict__ wsT)
+                                 int ncols, double *__restrict__ wsT, int64_t w_lo, int64_t w_n)
 {
     int64_t total = N * ncols;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total;
          q += (int64_t)gridDim.x * blockDim.x) {
         int64_t k = q / ncols, c = q - k * ncols;
-        wsT[q] = w[c * N + perm[k]];
+        const int64_t j = perm[k] - w_lo;
+        wsT[q] = (uint64_t)j < (uint64_t)w_n ? w[c * w_n + j] : 0.0;
     }
 }
 
@@ -760,12 +764,12 @@ __global__ __launch_bounds__(64) void near_fp64(NearArgs a)
                     }
                 }
             }
-            if (active) {
-                const int64_t oi = a.perm[t0 + i0 + i];
+            const int64_t oi = active ? a.perm[t0 + i0 + i] - a.out_lo : -1;
+            if ((uint64_t)oi < (uint64_t)a.out_n) {
 #pragma unroll
                 for (int k = 0; k < NV_COL / 4; k++) {
                     const int cl = g + G * k;
-                    if (cl < cn) a.out[(int64_t)(c0 + cl) * a.N + oi] += acc[k];
+                    if (cl < cn) a.out[(int64_t)(c0 + cl) * a.out_n + oi] += acc[k];
                 }
             }
         }
@@ -1076,12 +1080,12 @@ __device__ __forceinline__ void p2p_pass(const NearArgs &a, P2PShared &sh, int t
                 }
         }
     }
-    if (active) {
-        const int64_t oi = a.perm[ti];
+    const int64_t oi = active ? a.perm[ti] - a.out_lo : -1;
+    if ((uint64_t)oi < (uint64_t)a.out_n) {
 #pragma unroll
         for (int k = 0; k < CPL; k++) {
             const int cl = g + G * k;
-            if (cl < cn) a.out[(int64_t)(c0 + cl) * a.N + oi] += acc[k];
+            if (cl < cn) a.out[(int64_t)(c0 + cl) * a.out_n + oi] += acc[k];
         }
     }
 }
@@ -1408,21 +1412,22 @@ void upload_dirs(const int dirs[8][189][4])
     check(hipMemcpyToSymbol(HIP_SYMBOL(c_dirs), dirs, sizeof(int) * 8 * 189 * 4), "hipMemcpyToSymbol(c_dirs)");
 }
 
-void launch_gather_weights(const double *w, const int32_t *perm, int64_t N, int ncols, double *ws, hipStream_t st)
+void launch_gather_weights(const double *w, const int32_t *perm, int64_t N, int ncols, double *ws, int64_t w_lo,
+                           int64_t w_n, hipStream_t st)
 {
     int64_t total = N * ncols;
     int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
-    gather_weights<<<blocks, 256, 0, st>>>(w, perm, N, ncols, ws);
+    gather_weights<<<blocks, 256, 0, st>>>(w, perm, N, ncols, ws, w_lo, w_n);
     check(hipGetLastError(), "gather_weights");
 }
 
 void launch_gather_weights_t(const double *w, const int32_t *perm, int64_t N, int ncols, double *wsT,
-                             hipStream_t st)
+                             int64_t w_lo, int64_t w_n, hipStream_t st)
 {
     int64_t total = N * ncols;
     if (total <= 0) return;
     int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
-    gather_weights_t<<<blocks, 256, 0, st>>>(w, perm, N, ncols, wsT);
+    gather_weights_t<<<blocks, 256, 0, st>>>(w, perm, N, ncols, wsT, w_lo, w_n);
     check(hipGetLastError(), "gather_weights_t");
 }
 

@@ -142,7 +142,11 @@ struct NearArgs {
     const double *ctab, *Tkz;
     const double *x, *y, *z, *ws;
     const double *F;            // leaf-level local expansions (nullptr: no far field)
+    // out[c*out_n + perm[k] - out_lo] += ... for the sorted points k whose
+    // input index perm[k] lies in [out_lo, out_lo + out_n); the others
+    // (sources of a plan whose sources are not its targets) write nothing
     double *out;
+    int64_t out_lo, out_n;
     const double *wsT;          // sorted weights, point-major wsT[k*ncols + c]
     double *lp;                 // L2P results, point-major (nullptr: no far field)
     // leaves grouped by size: class c holds leaves of at most 2 << c points
@@ -161,6 +165,7 @@ struct DirectArgs {
 
 struct M2LMixArgs {
     int rf, rs, ncols, TB, cpt, side, kpad, nch, escale, R;  // R: rows of the split E (16-multiple)
+    int ldf;                                                 // Pf stride per (box, column), >= rf
     int chunk_major;                                         // step order: see m2l_step
     int row_split;                                           // rows of r_f split over two workgroups
     int wave_rows;    // row-wave kernel: the waves split the rows, E straight to registers
@@ -197,15 +202,20 @@ void launch_fill_zero(double *out, int64_t n, hipStream_t st);
 void launch_diff_norms(const double *a, const double *b, int64_t n, double *part, int nblocks, hipStream_t st);
 void launch_split_e(int variant, const double *E, int rf, int rs, int R, int nch, int escale, void *Es,
                     hipStream_t st);
-void launch_absmax(const float *x, int64_t n, unsigned *out, hipStream_t st);
-// two-part modes: rewrite nchunks 32-float k-chunks of P as scaled 16-bit parts
-void launch_split_p2(int variant, float *P, int64_t nchunks, const unsigned *xmax, hipStream_t st);
+// max |x| over the first `used` floats of rows rows of stride ld
+void launch_absmax_rows(const float *x, int64_t rows, int ld, int used, unsigned *out, hipStream_t st);
+// two-part modes: rewrite the first cpr 32-float k-chunks of each of rows
+// rows of P (stride ld floats) as scaled 16-bit parts
+void launch_split_p2(int variant, float *P, int64_t rows, int ld, int cpr, const unsigned *xmax, hipStream_t st);
 void upload_dirs_mixed(const int dirs[8][189][4]);
 void launch_zero_u32(unsigned *p, int n, hipStream_t st);
 // widest row-wave M2L tile (PBB_M2L_WAVE_ROWS) for a variant and split-E row count
 int m2l_rows_max_width(int variant, int R);
 void launch_m2l_mixed(int variant, const M2LMixArgs &a, int ntiles, int ncolchunks, hipStream_t st);
-void launch_gather_weights(const double *w, const int32_t *perm, int64_t N, int ncols, double *ws, hipStream_t st);
+// ws[c*N + k] = w[c*w_n + perm[k] - w_lo] for perm[k] in [w_lo, w_lo + w_n),
+// else 0 (w_n = N, w_lo = 0 for a plan whose sources are its targets)
+void launch_gather_weights(const double *w, const int32_t *perm, int64_t N, int ncols, double *ws, int64_t w_lo,
+                           int64_t w_n, hipStream_t st);
 void launch_p2m(const P2MArgs &a, int nleaf, hipStream_t st);
 void launch_m2m(const M2MArgs &a, int nparent, hipStream_t st);
 void launch_l2l(const M2MArgs &a, int nparent, hipStream_t st);
@@ -230,8 +240,9 @@ bool launch_near_split(const NearArgs &a, int kid, int nleaf, hipStream_t st);
 bool launch_p2m_levels(const UpArgs &a, int nitems, const UpReduce &r, hipStream_t st);
 // direct downward pass, one level: lp (+)= L2P of the level's F at its points
 bool launch_l2p_level(const DownArgs &a, int nseg, hipStream_t st);
+// the point-major copy: wsT[k*ncols + c], the same rule
 void launch_gather_weights_t(const double *w, const int32_t *perm, int64_t N, int ncols, double *wsT,
-                             hipStream_t st);
+                             int64_t w_lo, int64_t w_n, hipStream_t st);
 void launch_direct(const DirectArgs &a, int kid, hipStream_t st);
 
 // Kernel blocks of one operator set on the device (H2_3D_Tree.cpp:277-374):

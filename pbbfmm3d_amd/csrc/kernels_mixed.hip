@@ -80,12 +80,16 @@ __global__ void split_e(const double *__restrict__ E, int rf, int rs, int R, int
     }
 }
 
-// max |x| over n floats into *out (as ordered bits; out cleared by caller)
-__global__ void absmax_f32(const float *__restrict__ x, int64_t n, unsigned *__restrict__ out)
+// max |x| over the first `used` floats of rows rows of stride ld into *out
+// (as ordered bits; out cleared by caller)
+__global__ void absmax_f32(const float *__restrict__ x, int64_t rows, int ld, int used, unsigned *__restrict__ out)
 {
     float m = 0.0f;
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
-        m = fmaxf(m, fabsf(x[q]));
+    const int64_t n = rows * used;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = q / used;
+        m = fmaxf(m, fabsf(x[r * ld + (q - r * used)]));
+    }
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
     if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
 }
@@ -106,16 +110,19 @@ __device__ __forceinline__ int p_scale_exp(const unsigned *xmax)
     return xs;
 }
 
-// Two-part proxies, in place: every 32-float k-chunk of P (128 bytes) is
-// rewritten as 32 high then 32 low 16-bit parts of x * 2^xs, so that the
-// M2L loads its B fragments ready-made.  One thread per chunk.
+// Two-part proxies, in place: each of the first cpr 32-float k-chunks of a
+// row of P (128 bytes) is rewritten as 32 high then 32 low 16-bit parts of
+// x * 2^xs, so that the M2L loads its B fragments ready-made.  One thread
+// per chunk.
 template <typename T>
-__global__ void split_p2(float *__restrict__ P, int64_t nchunks, const unsigned *__restrict__ xmax)
+__global__ void split_p2(float *__restrict__ P, int64_t rows, int ld, int cpr, const unsigned *__restrict__ xmax)
 {
     const float xscale = ldexpf(1.0f, p_scale_exp(xmax));
+    const int64_t nchunks = rows * cpr;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nchunks;
          q += (int64_t)gridDim.x * blockDim.x) {
-        float4 *c = (float4 *)(P + q * 32);
+        const int64_t r = q / cpr;
+        float4 *c = (float4 *)(P + r * ld + (q - r * cpr) * 32);
         float x[32];
 #pragma unroll
         for (int i = 0; i < 8; i++) {
@@ -140,12 +147,14 @@ __global__ void split_p2(float *__restrict__ P, int64_t nchunks, const unsigned 
     }
 }
 
-void launch_split_p2(int variant, float *P, int64_t nchunks, const unsigned *xmax, hipStream_t st)
+void launch_split_p2(int variant, float *P, int64_t rows, int ld, int cpr, const unsigned *xmax, hipStream_t st)
 {
+    const int64_t nchunks = rows * cpr;
     if (nchunks <= 0) return;
+    if (ld % 4 || cpr * 32 > ld) fail(PBB_ERR_INTERNAL, "split_p2: chunks past the row stride");
     const int blocks = (int)std::min<int64_t>((nchunks + 255) / 256, 65536);
-    if (variant == PBB_PREC_MIXED_F16X2) split_p2<_Float16><<<blocks, 256, 0, st>>>(P, nchunks, xmax);
-    else if (variant == PBB_PREC_MIXED_BF16X2) split_p2<__bf16><<<blocks, 256, 0, st>>>(P, nchunks, xmax);
+    if (variant == PBB_PREC_MIXED_F16X2) split_p2<_Float16><<<blocks, 256, 0, st>>>(P, rows, ld, cpr, xmax);
+    else if (variant == PBB_PREC_MIXED_BF16X2) split_p2<__bf16><<<blocks, 256, 0, st>>>(P, rows, ld, cpr, xmax);
     else fail(PBB_ERR_INVALID_ARG, "proxies are pre-split for the two-part modes only");
     check(hipGetLastError(), "split_p2");
 }
@@ -302,7 +311,7 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
             const int col = (wave * NCB + cb) * 16 + fr;
             const int tb = col / cpt, cl = min(col - tb * cpt, cpt - 1);
             const int t = tb < TB ? s_tgt[tb] : -1;
-            const float *src = a.Pf + ((size_t)(a.off + (t >= 0 ? t : s_tgt[0])) * a.ncols + c0 + cl) * rf;
+            const float *src = a.Pf + ((size_t)(a.off + (t >= 0 ? t : s_tgt[0])) * a.ncols + c0 + cl) * a.ldf;
 #pragma unroll
             for (int rb_ = 0; rb_ < RB; rb_++)
 #pragma unroll
@@ -385,7 +394,7 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
         if (tb >= TB) continue;
         const int t = s_tgt[tb];
         if (t < 0) continue;
-        float *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + cl) * rf;
+        float *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + cl) * a.ldf;
 #pragma unroll
         for (int rb_ = 0; rb_ < RB; rb_++)
 #pragma unroll
@@ -607,7 +616,7 @@ __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M
             const int col = cbase + cb * 16 + fr;
             const int tb = col / cpt, cl = min(col - tb * cpt, cpt - 1);
             const int t = tb < TB ? s_tgt[tb] : -1;
-            const float *src = a.Pf + ((size_t)(a.off + (t >= 0 ? t : s_tgt[0])) * a.ncols + c0 + cl) * rf;
+            const float *src = a.Pf + ((size_t)(a.off + (t >= 0 ? t : s_tgt[0])) * a.ncols + c0 + cl) * a.ldf;
 #pragma unroll
             for (int i = 0; i < RBW; i++)
 #pragma unroll
@@ -704,7 +713,7 @@ __global__ __launch_bounds__(256 * WCOL, WCOL == 1 ? 2 : 1) void m2l_mfma_rows(M
         if (tb >= TB) continue;
         const int t = s_tgt[tb];
         if (t < 0) continue;
-        float *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + cl) * rf;
+        float *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + cl) * a.ldf;
 #pragma unroll
         for (int i = 0; i < RBW; i++)
 #pragma unroll
@@ -936,11 +945,13 @@ void launch_zero_u32(unsigned *p, int n, hipStream_t st)
     check(hipGetLastError(), "zero_u32");
 }
 
-void launch_absmax(const float *x, int64_t n, unsigned *out, hipStream_t st)
+void launch_absmax_rows(const float *x, int64_t rows, int ld, int used, unsigned *out, hipStream_t st)
 {
+    const int64_t n = rows * used;
     if (n <= 0) return;
+    if (used > ld) fail(PBB_ERR_INTERNAL, "absmax: rows wider than their stride");
     int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
-    absmax_f32<<<blocks, 256, 0, st>>>(x, n, out);
+    absmax_f32<<<blocks, 256, 0, st>>>(x, rows, ld, used, out);
     check(hipGetLastError(), "absmax_f32");
 }
 

@@ -312,4 +312,97 @@ void class_directions(int octant, const std::vector<int> &Ktable, int dirs[189][
     if (k != 189) fail(PBB_ERR_INTERNAL, "far-direction count is not 189");
 }
 
+// Right singular vectors and values of a tall m x n matrix A (col-major,
+// destroyed): TSQR, then the one-sided Jacobi SVD of R.
+static void right_svd(std::vector<double> &A, int64_t m, int n, std::vector<double> &sigma,
+                      std::vector<double> &V)
+{
+    std::vector<double> R;
+    tsqr_r(A.data(), m, n, R);
+    jacobi_svd_right(R.data(), n, sigma, V);
+}
+
+// Smallest k (a multiple of `mult`, at most r) whose discarded singular
+// values hold at most tau of the Frobenius norm; tail: what they hold.
+static int reduced_rank(const std::vector<double> &sigma, int r, int mult, double tau, double &tail)
+{
+    double tot = 0;
+    for (double v : sigma) tot += v * v;
+    std::vector<double> rest(sigma.size() + 1, 0.0);
+    for (int i = (int)sigma.size() - 1; i >= 0; i--) rest[i] = rest[i + 1] + sigma[i] * sigma[i];
+    for (int k = mult; k < r; k += mult)
+        if (rest[k] <= tau * tau * tot) {
+            tail = tot > 0 ? std::sqrt(rest[k] / tot) : 0.0;
+            return k;
+        }
+    tail = 0;
+    return r;
+}
+
+void reduce_set(const HostOps &h, int set, const double *VTw, const double *Uw, double tau, ReducedSet &out)
+{
+    const int rf = h.rf, rs = h.rs, n3 = h.n3;
+    const double *E = &h.E[(size_t)set * 316 * rf * rs];
+    const int64_t mR = 316LL * rf, mL = 316LL * rs;
+    // [E_0; ..; E_315] (316 rf x rs) for Q_R, [E_0^T; ..; E_315^T] (316 rs x rf) for Q_L
+    std::vector<double> BR((size_t)mR * rs), BL((size_t)mL * rf);
+    parallel_for(316, [&](int64_t d) {
+        const double *Ed = E + (size_t)d * rf * rs;
+        for (int j = 0; j < rs; j++)
+            for (int i = 0; i < rf; i++) {
+                const double v = Ed[i + (size_t)j * rf];
+                BR[(size_t)(d * rf + i) + (size_t)j * mR] = v;
+                BL[(size_t)(d * rs + j) + (size_t)i * mL] = v;
+            }
+    });
+    std::vector<double> sR, VR, sL, VL;
+    right_svd(BR, mR, rs, sR, VR);
+    right_svd(BL, mL, rf, sL, VL);
+    const int ks = reduced_rank(sR, rs, 32, tau, out.tail_s);
+    const int kf = reduced_rank(sL, rf, 16, tau, out.tail_f);
+    out.kf = kf;
+    out.ks = ks;
+    if (kf >= rf && ks >= rs) {  // nothing to gain: the set's own factors
+        out.kf = rf;
+        out.ks = rs;
+        out.tail_f = out.tail_s = 0;
+        out.E.assign(E, E + (size_t)316 * rf * rs);
+        out.VTw.assign(VTw, VTw + (size_t)rs * n3);
+        out.Uw.assign(Uw, Uw + (size_t)n3 * rf);
+        return;
+    }
+    // Q_L = VL[:, :kf], Q_R = VR[:, :ks] (column-major, ld rf / rs)
+    out.E.assign((size_t)316 * kf * ks, 0.0);
+    parallel_for(316, [&](int64_t d) {
+        const double *Ed = E + (size_t)d * rf * rs;
+        std::vector<double> T((size_t)rf * ks, 0.0);  // E_d Q_R
+        for (int j = 0; j < ks; j++)
+            for (int q = 0; q < rs; q++) {
+                const double v = VR[q + (size_t)j * rs];
+                for (int i = 0; i < rf; i++) T[i + (size_t)j * rf] += Ed[i + (size_t)q * rf] * v;
+            }
+        double *Eo = &out.E[(size_t)d * kf * ks];
+        for (int j = 0; j < ks; j++)
+            for (int a = 0; a < kf; a++) {
+                double acc = 0;
+                for (int i = 0; i < rf; i++) acc += VL[i + (size_t)a * rf] * T[i + (size_t)j * rf];
+                Eo[a + (size_t)j * kf] = acc;
+            }
+    });
+    out.VTw.assign((size_t)ks * n3, 0.0);  // Q_R^T VTw
+    out.Uw.assign((size_t)n3 * kf, 0.0);   // Uw Q_L
+    parallel_for(n3, [&](int64_t q) {
+        for (int a = 0; a < ks; a++) {
+            double acc = 0;
+            for (int i = 0; i < rs; i++) acc += VR[i + (size_t)a * rs] * VTw[i + (size_t)q * rs];
+            out.VTw[a + (size_t)q * ks] = acc;
+        }
+        for (int a = 0; a < kf; a++) {
+            double acc = 0;
+            for (int i = 0; i < rf; i++) acc += Uw[q + (size_t)i * n3] * VL[i + (size_t)a * rf];
+            out.Uw[q + (size_t)a * n3] = acc;
+        }
+    });
+}
+
 }  // namespace pbb

@@ -76,14 +76,16 @@ uint32_t leaf_coord(const std::vector<Level> &lv, int levels, const double p[3])
     return c[0] | (c[1] << 10) | (c[2] << 20);
 }
 
-void build_tree(HostTree &T, const double *xyz, int64_t N, double L, int levels)
+void build_tree(HostTree &T, const double *xyz, int64_t N, double L, int levels, int64_t nbox)
 {
     if (levels < 1 || levels > 9) fail(PBB_ERR_INVALID_ARG, "tree_level must be in [1, 9]");
     if (N <= 0) fail(PBB_ERR_INVALID_ARG, "npoints must be positive");
     if (N >= (1LL << 31)) fail(PBB_ERR_INVALID_ARG, "npoints must be < 2^31");
+    if (nbox < 1 || nbox > N) fail(PBB_ERR_INTERNAL, "bbox point count out of range");
     T.Lv = levels;
     T.N = N;
-    // bbox of the targets, centre, shift (compute.hpp:92-127)
+    // bbox of the targets, centre, shift (compute.hpp:92-127); every point
+    // is checked to be finite
     const int64_t BLK = 65536, nblk = (N + BLK - 1) / BLK;
     std::vector<double> bmn((size_t)nblk * 3, 1e32), bmx((size_t)nblk * 3, -1e32);
     std::vector<char> bad(nblk, 0);
@@ -92,6 +94,7 @@ void build_tree(HostTree &T, const double *xyz, int64_t N, double L, int levels)
             for (int d = 0; d < 3; d++) {
                 double v = xyz[3 * i + d];
                 if (!std::isfinite(v)) bad[b] = 1;
+                if (i >= nbox) continue;
                 bmn[3 * b + d] = std::min(bmn[3 * b + d], v);
                 bmx[3 * b + d] = std::max(bmx[3 * b + d], v);
             }
@@ -258,7 +261,8 @@ void build_tree(HostTree &T, const double *xyz, int64_t N, double L, int levels)
             }
             part[blk] = q;
         });
-        for (int64_t q : part) T.m2l_pairs += q;
+        for (int64_t q : part) T.m2l_pairs_level[l] += q;
+        T.m2l_pairs += T.m2l_pairs_level[l];
     }
     // ordered near-field point pairs (27 neighbours incl. self)
     {

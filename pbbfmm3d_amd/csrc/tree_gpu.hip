@@ -54,9 +54,11 @@ __device__ __forceinline__ int32_t key_coord(uint32_t k, int l)
     return (int32_t)(ix | (iy << 10) | (iz << 20));
 }
 
-// per-workgroup min and max of each axis (part[6 g + d] = min, [6 g + 3 + d] = max)
-__global__ __launch_bounds__(TPB) void bbox_part(const double *__restrict__ xyz, int64_t N, double *__restrict__ part,
-                                                 int *__restrict__ bad)
+// per-workgroup min and max of each axis over the first nbox points
+// (part[6 g + d] = min, [6 g + 3 + d] = max); all N points are checked to be
+// finite
+__global__ __launch_bounds__(TPB) void bbox_part(const double *__restrict__ xyz, int64_t N, int64_t nbox,
+                                                 double *__restrict__ part, int *__restrict__ bad)
 {
     __shared__ double s[6][TPB];
     double mn[3] = {1e32, 1e32, 1e32}, mx[3] = {-1e32, -1e32, -1e32};
@@ -65,8 +67,10 @@ __global__ __launch_bounds__(TPB) void bbox_part(const double *__restrict__ xyz,
         for (int d = 0; d < 3; d++) {
             const double v = xyz[3 * i + d];
             nf |= !isfinite(v);
-            mn[d] = fmin(mn[d], v);
-            mx[d] = fmax(mx[d], v);
+            if (i < nbox) {
+                mn[d] = fmin(mn[d], v);
+                mx[d] = fmax(mx[d], v);
+            }
         }
     for (int d = 0; d < 3; d++) {
         s[d][threadIdx.x] = mn[d];
@@ -410,12 +414,13 @@ void sort_by_class(Temp &tmp, Pool &pool, const uint32_t *cls, const int32_t *id
 
 }  // namespace
 
-void build_tree_gpu(GpuTree &T, const double *xyz_host, int64_t N, double L, int levels,
+void build_tree_gpu(GpuTree &T, const double *xyz_host, int64_t N, int64_t nbox, double L, int levels,
                     const std::vector<std::vector<double>> &ctab, const int dirs[8][189][4], hipStream_t st)
 {
     if (levels < 1 || levels > 9) fail(PBB_ERR_INVALID_ARG, "tree_level must be in [1, 9]");
     if (N <= 0) fail(PBB_ERR_INVALID_ARG, "npoints must be positive");
     if (N >= (1LL << 31)) fail(PBB_ERR_INVALID_ARG, "npoints must be < 2^31");
+    if (nbox < 1 || nbox > N) fail(PBB_ERR_INTERNAL, "bbox point count out of range");
     Pool pool{st, {}};
     Temp tmp{pool};
     T = GpuTree{};
@@ -430,7 +435,7 @@ void build_tree_gpu(GpuTree &T, const double *xyz_host, int64_t N, double L, int
     unsigned long long *cnt = pool.get<unsigned long long>(16, "counters");
     check(hipMemsetAsync(bad, 0, sizeof(int), st), "clear flag");
     check(hipMemsetAsync(cnt, 0, 16 * sizeof(unsigned long long), st), "clear counters");
-    bbox_part<<<nbb, TPB, 0, st>>>(xyz, N, part, bad);
+    bbox_part<<<nbb, TPB, 0, st>>>(xyz, N, nbox, part, bad);
     check(hipGetLastError(), "bbox_part");
     std::vector<double> hp((size_t)6 * nbb);
     check(hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, st), "copy bbox");
@@ -622,17 +627,23 @@ void build_tree_gpu(GpuTree &T, const double *xyz_host, int64_t N, double L, int
     }
     int *ddirs = pool.get<int>(8 * 189 * 4, "directions");
     check(hipMemcpyAsync(ddirs, dirs, sizeof(int) * 8 * 189 * 4, hipMemcpyHostToDevice, st), "upload directions");
+    // counters: 0 outside points, 1 near pairs, 2 + l far pairs of level l
+    static_assert(2 + 9 < 16, "counter slots");
     for (int l = 2; l <= levels; l++) {
         m2l_pair_count<<<nblocks(T.lv[l].nb, 4096), TPB, 0, st>>>(T.lv[l].coord, T.lv[l].map, T.lv[l].nb, l, ddirs,
-                                                                  cnt + 2);
+                                                                  cnt + 2 + l);
         check(hipGetLastError(), "m2l_pair_count");
     }
-    unsigned long long hc[3];
+    unsigned long long hc[16];
     check(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, st), "copy counters");
     check(hipStreamSynchronize(st), "hipStreamSynchronize");
     T.outside = (int64_t)hc[0];
     T.near_pairs = (int64_t)hc[1];
-    T.m2l_pairs = (int64_t)hc[2];
+    T.m2l_pairs = 0;
+    for (int l = 2; l <= levels; l++) {
+        T.m2l_pairs_level[l] = (int64_t)hc[2 + l];
+        T.m2l_pairs += T.m2l_pairs_level[l];
+    }
 
     // hand the tree's arrays over; everything else is freed with the pool
     for (void *p : {(void *)T.x, (void *)T.y, (void *)T.z, (void *)T.perm, (void *)T.pt_leaf, (void *)T.leaf_start,

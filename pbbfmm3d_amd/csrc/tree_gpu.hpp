@@ -36,12 +36,14 @@ struct GpuTree {
     std::vector<GpuLevel> lv;                         // 0..Lv
     int64_t Btot = 0;
     int64_t m2l_pairs = 0, near_pairs = 0, max_leaf = 0, outside = 0;
+    int64_t m2l_pairs_level[16] = {0};
 };
 
 // Builds the tree of N points (host AoS xyz) on stream st.  ctab: per-axis
 // node centres of levels 0..levels-1 (H2_3D_Tree.cpp:939-961), dirs: the far
-// directions of each octant class (class_directions).  Synchronises st.
-void build_tree_gpu(GpuTree &T, const double *xyz_host, int64_t N, double L, int levels,
+// directions of each octant class (class_directions).  The bbox centre is
+// that of the first nbox points (the targets).  Synchronises st.
+void build_tree_gpu(GpuTree &T, const double *xyz_host, int64_t N, int64_t nbox, double L, int levels,
                     const std::vector<std::vector<double>> &ctab, const int dirs[8][189][4], hipStream_t st);
 
 }  // namespace dev

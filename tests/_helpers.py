@@ -38,7 +38,7 @@ def oracle_ops_from(ops, kernel):
 
 def gpu_matvec(plan, w, ncols, device="cuda:0"):
     import torch
-    N = plan.npoints
+    N = plan.nsources  # the plan's points, or its separate sources
     wd = torch.tensor(np.asarray(w).reshape(ncols, N), dtype=torch.float64, device=device)
     out = plan.matvec(wd)
     torch.cuda.synchronize()

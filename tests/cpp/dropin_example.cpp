@@ -56,6 +56,38 @@ static int run(const char *name, T &tree, std::vector<vector3> &pts, std::vector
     DirectCalc3D(&tree, pts, pts, w, nCols, exact, rows);
     double err = ComputeError(out, exact, N, nCols, rows);
     std::printf("%s: relative error for first %d rows: %.3e\n", name, rows, err);
+    if (err >= tol) return 1;
+
+    // separate targets: Nf = N / 3 other points of the same cube, the N
+    // points as sources (weights column-major Ns x nCols, outputs Nf x nCols)
+    const int Nf = N / 3;
+    std::vector<vector3> tgt(Nf);
+    const double L = tree.L;
+    for (auto &p : tgt) p = vector3(frand(-0.5, 0.5) * L, frand(-0.5, 0.5) * L, frand(-0.5, 0.5) * L);
+    std::vector<double> out_t((size_t)Nf * nCols, 0.0);
+    try {
+        H2_3D_Compute<T> separate(tree, tgt, pts, w, nCols, out_t);
+    } catch (const std::runtime_error &e) {
+        std::printf("%s: %s\n", name, e.what());
+        return 2;
+    }
+    rows = std::min(Nf, 400);
+    double diff = 0, sum = 0;
+    for (int k = 0; k < nCols; k++)
+        for (int i = 0; i < rows; i++) {
+            double acc = 0;
+            for (int j = 0; j < N; j++) {
+                double v = tree.EvaluateKernel(tgt[i], pts[j]);
+                if (std::isinf(v)) v = 0;
+                acc += v * w[(size_t)k * N + j];
+            }
+            const double d = out_t[(size_t)k * Nf + i] - acc;
+            diff += d * d;
+            sum += acc * acc;
+        }
+    err = std::sqrt(diff / sum);
+    std::printf("%s: separate targets (%d) and sources (%d): relative error for first %d rows: %.3e\n", name, Nf,
+                N, rows, err);
     return err < tol ? 0 : 1;
 }
 

@@ -28,7 +28,7 @@ def test_exports_every_declared_symbol():
 
 
 def test_abi_version_and_no_device_here():
-    assert capi.lib().pbb_abi_version() == 1
+    assert capi.lib().pbb_abi_version() == 2
     assert capi.lib().pbb_device_count() >= 0
 
 
@@ -85,7 +85,7 @@ def test_wrappers_reject_host_tensors_before_any_call():
     import torch
     from pbbfmm3d_amd.core import Plan, _f64_cuda
     p = Plan.__new__(Plan)
-    p.npoints = 10
+    p.npoints = p.ntargets = p.nsources = 10
     p._h = None
     w = torch.zeros(2, 10, dtype=torch.float64)
     with pytest.raises(TypeError):

@@ -46,6 +46,32 @@ def test_mixed_vs_fp64(gpu, prec, bar, kernel, n, eps, ncols):
     assert orc.rel_err(f64, ref) <= 1e-10
 
 
+def test_recompressed_operator_sets(gpu):
+    # the mixed modes run each level's operator set re-compressed
+    # (pbb_ops_set_mixed_tolerance, default 1e-10): the deepest sets keep far
+    # fewer directions, the FP64 path keeps the reference's operators, and
+    # the mixed result stays as close to it as with every set's own factors
+    N, nc = 60000, 16
+    xyz = clustered_points(N, seed=62)
+    w = weights(N, nc, seed=63)
+    kernel = pbb.gaussian(0.3)
+    red = pbb.Operators(kernel, 1.0, 6, 6, 1e-8)
+    own = pbb.Operators(kernel, 1.0, 6, 6, 1e-8)
+    own.set_mixed_tolerance(0.0)
+    ranks = red.mixed_ranks()
+    print(f"ranks per set {ranks}", flush=True)
+    assert ranks[-1][0] <= 48 and ranks[-1][1] <= 64  # level 6
+    assert all(k == (own.rank_f, 32 * ((own.rank_s + 31) // 32), 0.0) for k in own.mixed_ranks())
+    pr, po = pbb.Plan(red, xyz), pbb.Plan(own, xyz)
+    f64 = _mv(pr, w, nc, pbb.capi.PREC_FP64)
+    assert np.array_equal(f64, _mv(po, w, nc, pbb.capi.PREC_FP64))
+    for prec in (pbb.capi.PREC_MIXED, pbb.capi.PREC_MIXED_F16X2):
+        a = orc.rel_err(_mv(pr, w, nc, prec), f64)
+        b = orc.rel_err(_mv(po, w, nc, prec), f64)
+        print(f"case prec={prec}: re-compressed {a:.3e}, own factors {b:.3e}", flush=True)
+        assert a <= 1e-6 and a <= 1.5 * b + 1e-8
+
+
 def test_mixed_switching_and_accumulate(gpu):
     import torch
     N = 8000

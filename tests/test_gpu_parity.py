@@ -253,3 +253,23 @@ def test_ranks_above_256_and_224(gpu, kernel, n, eps):
         d = orc.rel_err(mix, got)
         print(f"precision {prec}: vs FP64 {d:.3e}", flush=True)
         assert d <= 1e-6
+
+
+def test_column_chunks_when_the_workspace_is_capped(gpu):
+    # a workspace cap below what 24 columns need runs the matvec in chunks of
+    # columns: FP64 equals the one-pass matvec bit for bit, the mixed modes
+    # stay within their bar of it
+    import torch
+    xyz = clustered_points(8000, seed=72)
+    ops = pbb.Operators(pbb.laplacian(), 1.0, 4, 4, 1e-5)
+    plan = pbb.Plan(ops, xyz)
+    w = torch.tensor(np.random.default_rng(73).standard_normal((24, 8000)), device="cuda")
+    ref = plan.matvec(w)
+    ref_m = plan.matvec(w, precision=pbb.capi.PREC_MIXED_F16X2)
+    capped = pbb.Plan(ops, xyz)
+    capped.set_workspace_limit(6 << 20)  # a few columns' worth
+    got = capped.matvec(w)
+    assert torch.equal(got, ref)
+    got_m = capped.matvec(w, precision=pbb.capi.PREC_MIXED_F16X2)
+    assert float(torch.linalg.norm(got_m - ref) / torch.linalg.norm(ref)) <= 1e-6
+    assert float(torch.linalg.norm(got_m - ref_m) / torch.linalg.norm(ref)) <= 1e-6

@@ -287,3 +287,39 @@ def test_masked_matvec_equals_full(kid, homogen, params, levels, n, pts):
     assert orc.rel_err(got, full[rows]) < 1e-13
     # outputs of points outside the targets' leaves are left as they are
     assert np.count_nonzero(part == 7.0) > N * ncols // 2
+
+
+@pytest.mark.parametrize("kid,homogen,n,levels", [(orc.R2, 2.0, 3, 3), (orc.QUADRIC, 0.0, 4, 4)])
+def test_matvec_ts_exact_for_polynomial_kernels(kid, homogen, n, levels):
+    # separate targets and sources (different counts and bboxes; a few
+    # sources outside the targets' root cube): for a polynomial kernel of
+    # degree < n the FMM is exact, so it must equal the direct sum
+    tgt = clustered_points(1800, seed=31)
+    src = 0.9 * uniform_points(2600, seed=32) + 0.04
+    w = weights(2600, 2, seed=33)
+    ops = orc.precompute(kid, None, 1, homogen, 1.0, levels, n, 1e-14)
+    got = orc.matvec_ts(ops, tgt, src, w, 2)
+    ref = orc.direct(kid, None, tgt, src, w, 2)
+    assert got.shape == (1800 * 2,)
+    assert orc.rel_err(got, ref) < 1e-12
+
+
+def test_matvec_ts_converges_to_direct_sum():
+    tgt = uniform_points(1500, seed=34)
+    src = clustered_points(2500, seed=35)
+    w = weights(2500, 1, seed=36)
+    ref = orc.direct(orc.GAUSSIAN, [0.3], tgt, src, w, 1)
+    errs = []
+    for n in (4, 6):
+        ops = orc.precompute(orc.GAUSSIAN, [0.3], 1, 0.0, 1.0, 3, n, 1e-10)
+        errs.append(orc.rel_err(orc.matvec_ts(ops, tgt, src, w, 1), ref))
+    assert errs[1] < errs[0] and errs[1] < 1e-5
+
+
+def test_matvec_ts_of_the_same_points_is_matvec():
+    # sources equal to the targets: every point appears twice (once without
+    # weight), the sums are the same terms
+    xyz = clustered_points(3000, seed=37)
+    w = weights(3000, 2, seed=38)
+    ops = orc.precompute(orc.LAPLACIAN, None, 1, -1.0, 1.0, 4, 4, 1e-6)
+    assert orc.rel_err(orc.matvec_ts(ops, xyz, xyz, w, 2), orc.matvec(ops, xyz, w, 2)) < 1e-13
