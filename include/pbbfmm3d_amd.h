@@ -218,6 +218,12 @@ PBB_API pbb_status pbb_ops_export(const pbb_ops *ops, double *U, double *VT,
  * on these operators (afterwards PBB_ERR_INVALID_ARG).  The FP64 mode
  * always runs the reference's operators.                                  */
 PBB_API pbb_status pbb_ops_set_mixed_tolerance(pbb_ops *ops, double tol);
+/* enable != 0: the FP64 mode runs the re-compressed operator sets too
+ * (faster on smooth non-homogeneous kernels; its result then differs from
+ * the reference's by about the tolerance); 0 (default): the reference's
+ * operators.  Takes effect from the next call on these operators; a batch
+ * captured before keeps the operators it was captured with.               */
+PBB_API pbb_status pbb_ops_set_fp64_recompressed(pbb_ops *ops, int32_t enable);
 /* The ranks the mixed modes run per operator set (arrays of nsets: sets in
  * level order 2..L, or one set for a homogeneous kernel) and the discarded
  * relative Frobenius norm (tail may be NULL).  Computes them on the host if

@@ -113,6 +113,7 @@ _SIGNATURES = {
     "pbb_ops_export": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
     "pbb_ops_destroy": (None, [_vp]),
     "pbb_ops_set_mixed_tolerance": (C.c_int, [_vp, C.c_double]),
+    "pbb_ops_set_fp64_recompressed": (C.c_int, [_vp, C.c_int32]),
     "pbb_ops_mixed_ranks": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _dp]),
     "pbb_plan_create": (C.c_int, [_vp, _dp, C.c_int64, C.POINTER(_vp)]),
     "pbb_plan_create_ex": (C.c_int, [_vp, _dp, C.c_int64, C.c_int32, C.POINTER(_vp)]),

@@ -223,6 +223,11 @@ class Operators:
         factors); before the first mixed-precision call."""
         capi.check(capi.lib().pbb_ops_set_mixed_tolerance(self._h, float(tol)))
 
+    def set_fp64_recompressed(self, enable: bool = True) -> None:
+        """Run the FP64 mode with the re-compressed operator sets too (its
+        result then differs from the reference's by about the tolerance)."""
+        capi.check(capi.lib().pbb_ops_set_fp64_recompressed(self._h, 1 if enable else 0))
+
     def mixed_ranks(self):
         """[(kf, ks, discarded relative norm)] per operator set, as the
         mixed modes run them (computed on the host)."""

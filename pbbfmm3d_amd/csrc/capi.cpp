@@ -122,8 +122,10 @@ struct pbb_ops {
     std::unique_ptr<pbb_ops> adj;  // transposed operator set, built on demand
     double *d_Tkz = nullptr, *d_Cw = nullptr, *d_VTw = nullptr, *d_Uw = nullptr, *d_E = nullptr;
     // The mixed modes' operator sets, each re-compressed within level_tol
-    // (0: the sets' own factors); set before their first use
+    // (0: the sets' own factors); set before their first use.  fp64_reduced:
+    // the FP64 mode runs them too (off: the reference's operators)
     double level_tol = 1e-10;
+    bool fp64_reduced = false;
     std::vector<MixedSet> mset;
     std::vector<double> mset_emax;  // max |E'_d| per set
     std::vector<double> hm_E, hm_VTw, hm_Uw;  // host copies until uploaded
@@ -315,6 +317,7 @@ struct pbb_ops {
         if (adj) return adj.get();
         auto a = std::make_unique<pbb_ops>();
         a->level_tol = level_tol;
+        a->fp64_reduced = fp64_reduced;
         HostOps &g = a->h;
         g = h;
         g.rf = h.rs;
@@ -904,6 +907,16 @@ PBB_API pbb_status pbb_ops_set_mixed_tolerance(pbb_ops *ops, double tol)
                                       "before the first mixed-precision call");
         ops->level_tol = tol;
         if (ops->adj) ops->adj->level_tol = tol;
+    });
+}
+
+PBB_API pbb_status pbb_ops_set_fp64_recompressed(pbb_ops *ops, int32_t enable)
+{
+    return guarded([&] {
+        if (!ops) fail(PBB_ERR_INVALID_ARG, "ops is NULL");
+        std::lock_guard<std::mutex> lk(ops->mu);
+        ops->fp64_reduced = enable != 0;
+        if (ops->adj) ops->adj->fp64_reduced = enable != 0;
     });
 }
 
@@ -1618,6 +1631,16 @@ static void split_ops(const pbb_plan *p, pbb_ops *o, int prec, bool build, hipSt
     SE = nullptr;
     SA = nullptr;
     const HostOps &h = o->h;
+    if (prec == PBB_PREC_FP64 && o->fp64_reduced && p->Lv >= 2) {
+        std::lock_guard<std::mutex> lk(o->mu);
+        if (!o->dm_E) {
+            if (!build)
+                fail(PBB_ERR_INVALID_ARG, "the re-compressed FP64 operators are not prepared yet: one "
+                                          "pbb_plan_matvec prepares them");
+            o->make_mixed_sets();
+        }
+        return;
+    }
     if (prec == PBB_PREC_FP64 || p->Lv < 2) return;
     const bool want3 = !split_acc_on(p, prec) && h.rs <= 224 && h.n3 <= 224;
     if (build) {
@@ -1692,16 +1715,21 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         dev::launch_m2m(a, (int)p->lv[l].nb, st);
     }
     mark(2);
-    // the mixed modes run each level with its operator set's own ranks
-    // (kf x ks, pbb_ops::mset); Pf keeps the stride r_f
+    // the mixed modes (and FP64 with fp64_reduced) run each level with its
+    // re-compressed operator set (kf x ks, pbb_ops::mset); P and Pf keep
+    // their strides
+    const bool reduced = mixed || o->fp64_reduced;
     auto mset = [&](int l) -> const MixedSet & { return o->mset[h.ops_set(l)]; };
+    // FP64 keeps a set's own factors where its padded ks would pass the
+    // FP64 proxy stride r_s (a set that was not re-compressed)
+    auto red = [&](int l) { return mixed || (reduced && mset(l).ks <= h.rs); };
     for (int l = 2; l <= Lv; l++) {
         const DevLevel &D = p->lv[l];
         const int sl = h.ops_set(l);
         dev::GemmArgs g{};
         g.M = h.rs; g.K = n3; g.ldb = n3; g.ldc = ldP; g.ncol = D.nb * ncols; g.alpha = 1.0;
         g.A = o->d_VTw + (size_t)sl * h.rs * n3;
-        if (mixed) {
+        if (red(l)) {
             g.M = mset(l).ks;
             g.A = o->dm_VTw + mset(l).vt_off;
         }
@@ -1799,7 +1827,8 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         const int TB = std::max(1, std::min(16, width / cpt));
         const int nchunks = (ncols + cpt - 1) / cpt;
         dev::M2LArgs a{};
-        a.rf = h.rf; a.rs = h.rs; a.ncols = ncols; a.TB = TB; a.cpt = cpt; a.side = 1 << l; a.off = D.off;
+        a.rf = h.rf; a.rs = h.rs; a.ldp = h.rs; a.ldf = h.rf;
+        a.ncols = ncols; a.TB = TB; a.cpt = cpt; a.side = 1 << l; a.off = D.off;
         a.chunk_major = t64.width ? (t64.flags & PBB_M2L_CHUNK_MAJOR) : 1;
         a.tile_prefix[0] = 0;
         a.class_start[0] = 0;
@@ -1809,6 +1838,11 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         }
         a.order = D.order; a.coord = D.coord; a.map = D.map;
         a.E = o->d_E + (size_t)h.ops_set(l) * 316 * h.rf * h.rs;
+        if (red(l)) {
+            a.rf = mset(l).kf;
+            a.rs = mset(l).ks;
+            a.E = o->dm_E + mset(l).e_off;
+        }
         a.P = W.P; a.Pf = W.Pf;
         dev::launch_m2l(a, a.tile_prefix[8], nchunks, st);
     }
@@ -1822,9 +1856,11 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         g.M = n3; g.K = h.rf; g.ldb = h.rf; g.ldc = n3; g.ncol = D.nb * ncols;
         g.alpha = h.homog ? std::pow(1.0 / D.len, h.homogen_flipped) : 1.0;  // compute.hpp:433-435
         g.A = o->d_Uw + (size_t)sl * n3 * h.rf;
-        if (mixed) {
+        if (red(l)) {
             g.K = mset(l).kf;
             g.A = o->dm_Uw + mset(l).u_off;
+        }
+        if (mixed) {
             g.b_f32 = 1;
             g.B = W.Pf32 + (size_t)D.off * ncols * h.rf;
         } else {

@@ -566,7 +566,7 @@ __global__ __launch_bounds__(NWC * 128) void m2l_mfma_f64(M2LArgs a)
         for (int cb = 0; cb < 2; cb++) {
             const int sid0 = btb[cb] >= 0 ? s_sid[k * TB_MAX + btb[cb]] : -1;
             okm |= (sid0 >= 0) << cb;
-            const double *col = a.P + ((size_t)(a.off + max(sid0, 0)) * a.ncols + c0 + bcl[cb]) * rs;
+            const double *col = a.P + ((size_t)(a.off + max(sid0, 0)) * a.ncols + c0 + bcl[cb]) * a.ldp;
 #pragma unroll
             for (int e = 0; e < 4; e++) {
                 const double v = col[min(k0 + e, rs - 1)];
@@ -628,7 +628,7 @@ __global__ __launch_bounds__(NWC * 128) void m2l_mfma_f64(M2LArgs a)
         if (btb[cb] < 0) continue;
         const int t = s_tgt[btb[cb]];
         if (t < 0) continue;
-        double *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + bcl[cb]) * rf;
+        double *dst = a.Pf + ((size_t)(a.off + t) * a.ncols + c0 + bcl[cb]) * a.ldf;
 #pragma unroll
         for (int i = 0; i < RBW; i++) {
             const int rb_ = wr * RBW + i;

@@ -109,6 +109,7 @@ struct GemmArgs {
 
 struct M2LArgs {
     int rf, rs, ncols, TB, cpt, side, chunk_major;  // step order: see m2l_step
+    int ldp, ldf;                                   // P and Pf strides per (box, column), >= rs, rf
     int64_t off;
     int tile_prefix[9];
     int64_t class_start[9];

@@ -70,6 +70,19 @@ def test_recompressed_operator_sets(gpu):
         b = orc.rel_err(_mv(po, w, nc, prec), f64)
         print(f"case prec={prec}: re-compressed {a:.3e}, own factors {b:.3e}", flush=True)
         assert a <= 1e-6 and a <= 1.5 * b + 1e-8
+    # FP64 with the re-compressed sets: the same far field within about the
+    # tolerance; the transposed FMM too
+    red.set_fp64_recompressed(True)
+    f64r = _mv(pr, w, nc, pbb.capi.PREC_FP64)
+    d = orc.rel_err(f64r, f64)
+    print(f"FP64 re-compressed vs reference operators: {d:.3e}", flush=True)
+    assert 0 < d <= 1e-8
+    import torch
+    wd = torch.tensor(w.reshape(nc, N), device="cuda:0")
+    t_red = pr.matvec_transpose(wd)
+    red.set_fp64_recompressed(False)
+    t_ref = pr.matvec_transpose(wd)
+    assert float(torch.linalg.norm(t_red - t_ref) / torch.linalg.norm(t_ref)) <= 1e-8
 
 
 def test_mixed_switching_and_accumulate(gpu):
