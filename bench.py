@@ -141,43 +141,39 @@ def named_phases(phase, order):
     return out
 
 
-def phase_rooflines(phase, info, ops, nc, precision, kernel_id):
+def phase_rooflines(phase, info, ops, nc, precision, kernel_id, ranks=None):
     """Algorithmic bytes or flops of each phase over its measured time, as a
     fraction of MI355X's peak (HBM 8 TB/s, FP64 78.6 TF/s): HBM-bound
     expansion passes count the bytes they must move (S / P / Pf / F once,
     points and weights once per level), P2P counts (c_K + 2 nCols) flops per
-    ordered pair (c_K = 9 for 1/r, 11 for the Gaussian)."""
+    ordered pair (c_K = 9 for 1/r, 11 for the Gaussian).  ranks: {level:
+    (kf, ks)} the path runs (default r_f x r_s); S and F are FP64 in the FP64
+    mode and FP32 in the mixed modes, P and Pf likewise."""
     n3, rf, rs = ops.n3, ops.rank_f, ops.rank_s
     L = info["tree_level"]
     N = info["npoints"]
-    B = sum(info["boxes"][2:L + 1])
-    pbytes = 8 if precision == "fp64" else 4  # P and Pf: FP64, or FP32 in the mixed modes
-    kpad = rs if precision == "fp64" else 32 * ((max(rf, rs) + 31) // 32)
-    S = B * n3 * nc * 8
+    boxes = info["boxes"]
+    B = sum(boxes[2:L + 1])
+    eb = 8 if precision == "fp64" else 4  # S / F and P / Pf element bytes
+    ranks = ranks or {l: (rf, rs) for l in range(2, L + 1)}
+    S = B * n3 * nc * eb
+    P = sum(boxes[l] * nc * ranks[l][1] for l in range(2, L + 1)) * eb
+    Pf = sum(boxes[l] * nc * ranks[l][0] for l in range(2, L + 1)) * eb
     nlev = L - 1
     hbm = 8.0e12
     ck = 11 if kernel_id == 3 else 9
     res = {}
-    t = phase["p2m"] * 1e-3
-    if t > 0:
-        by = S + nlev * N * (24 + 8 * nc)
-        res["p2m_all_levels"] = {"bound": "hbm", "bytes": by, "GB/s": round(by / t / 1e9, 1),
-                                 "frac": round(by / t / hbm, 3)}
-    t = phase["vt_proxy"] * 1e-3
-    if t > 0:
-        by = S + B * nc * kpad * pbytes
-        res["vt_proxy"] = {"bound": "hbm", "bytes": by, "GB/s": round(by / t / 1e9, 1),
-                           "frac": round(by / t / hbm, 3)}
-    t = phase["u_expand"] * 1e-3
-    if t > 0:
-        by = S + B * nc * rf * pbytes
-        res["u_expand"] = {"bound": "hbm", "bytes": by, "GB/s": round(by / t / 1e9, 1),
-                           "frac": round(by / t / hbm, 3)}
-    t = phase["l2l"] * 1e-3
-    if t > 0:
-        by = S + nlev * N * 24 + (2 * (nlev - 1) + 1) * N * nc * 8
-        res["l2p_all_levels"] = {"bound": "hbm", "bytes": by, "GB/s": round(by / t / 1e9, 1),
-                                 "frac": round(by / t / hbm, 3)}
+
+    def hbm_entry(name, t_ms, by):
+        t = t_ms * 1e-3
+        if t > 0:
+            res[name] = {"bound": "hbm", "bytes": by, "GB/s": round(by / t / 1e9, 1),
+                         "frac": round(by / t / hbm, 3)}
+
+    hbm_entry("p2m_all_levels", phase["p2m"], S + nlev * N * (24 + 8 * nc))
+    hbm_entry("vt_proxy", phase["vt_proxy"], S + P)
+    hbm_entry("u_expand", phase["u_expand"], Pf + S)
+    hbm_entry("l2p_all_levels", phase["l2l"], S + nlev * N * 24 + (2 * (nlev - 1) + 1) * N * nc * 8)
     t = phase["near"] * 1e-3
     if t > 0:
         fl = info["near_pairs"] * (ck + 2 * nc)
@@ -218,6 +214,9 @@ def main():
     ap.add_argument("--workload", default="c3", choices=["c3", "c5"],
                     help="c3: the headline single cloud (default); c5: 256 independent clouds in one "
                          "HIP-graph batch (a separate line, not the headline metric)")
+    ap.add_argument("--m2l-tiles", default="",
+                    help="JSON {level: [width, cols_per_box, flags]} of the headline precision's M2L tiles "
+                         "(as the tuner reports them) set instead of running the tuner (profiled runs)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the built-in M2L tiles instead of running the plan's autotuner first")
     ap.add_argument("--max-rel-diff", type=float, default=1e-6,
@@ -282,7 +281,12 @@ def main():
     # M2L tile widths picked by the plan's autotuner (untimed, like the tree)
     t0 = time.perf_counter()
     widths = None
-    if not args.no_tune:
+    if args.m2l_tiles:
+        tiles = {int(k): v for k, v in json.loads(args.m2l_tiles).items()}
+        for lvl, (wd, cpb, fl) in sorted(tiles.items()):
+            plan.set_m2l_tile_level(lvl, wd, cpb, fl, prec)
+        widths = {"m2l_tile": {"levels": tiles, "set_from": "--m2l-tiles"}}
+    elif not args.no_tune:
         widths = {"m2l_tile": plan.tune_m2l(w, prec, args.max_rel_diff if prec != capi.PREC_FP64 else 0.0),
                   "max_rel_diff": args.max_rel_diff if prec != capi.PREC_FP64 else None}
         if prec != capi.PREC_FP64 and args.fp64_steps > 0:
@@ -491,7 +495,8 @@ def main():
         "rel_err_vs_direct": rel_err,
         "fp64_path": fp64,
         "phase_ms": named_phases(phase, args.order),
-        "phase_rooflines": phase_rooflines(phase, info, ops, nc, args.precision, kernel.id),
+        "phase_rooflines": phase_rooflines(phase, info, ops, nc, args.precision, kernel.id,
+                                           {l: level_ranks(l) for l in range(2, info["tree_level"] + 1)}),
         "ranks": [ops.rank_f, ops.rank_s],
         "m2l_pairs": info["m2l_pairs"], "near_pairs": info["near_pairs"],
         "precompute_s": round(t_pre, 3), "plan_s": round(t_plan, 3),

@@ -627,21 +627,24 @@ void build_tree_gpu(GpuTree &T, const double *xyz_host, int64_t N, int64_t nbox,
     }
     int *ddirs = pool.get<int>(8 * 189 * 4, "directions");
     check(hipMemcpyAsync(ddirs, dirs, sizeof(int) * 8 * 189 * 4, hipMemcpyHostToDevice, st), "upload directions");
-    // counters: 0 outside points, 1 near pairs, 2 + l far pairs of level l
-    static_assert(2 + 9 < 16, "counter slots");
+    // far-field box pairs per level (the shared counters' slots 8.. hold
+    // the leaf-class histogram)
+    unsigned long long *lpairs = pool.get<unsigned long long>(16, "pair counters");
+    check(hipMemsetAsync(lpairs, 0, 16 * sizeof(unsigned long long), st), "clear pair counters");
     for (int l = 2; l <= levels; l++) {
         m2l_pair_count<<<nblocks(T.lv[l].nb, 4096), TPB, 0, st>>>(T.lv[l].coord, T.lv[l].map, T.lv[l].nb, l, ddirs,
-                                                                  cnt + 2 + l);
+                                                                  lpairs + l);
         check(hipGetLastError(), "m2l_pair_count");
     }
-    unsigned long long hc[16];
+    unsigned long long hc[2], hl[16];
     check(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, st), "copy counters");
+    check(hipMemcpyAsync(hl, lpairs, sizeof(hl), hipMemcpyDeviceToHost, st), "copy pair counters");
     check(hipStreamSynchronize(st), "hipStreamSynchronize");
     T.outside = (int64_t)hc[0];
     T.near_pairs = (int64_t)hc[1];
     T.m2l_pairs = 0;
     for (int l = 2; l <= levels; l++) {
-        T.m2l_pairs_level[l] = (int64_t)hc[2 + l];
+        T.m2l_pairs_level[l] = (int64_t)hl[l];
         T.m2l_pairs += T.m2l_pairs_level[l];
     }
 
