@@ -884,10 +884,137 @@ __global__ __launch_bounds__(512) void gemm_split3(GemmSplitArgs g)
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same product for thin shapes (few rows, or a short k): one wave per
+// 32 columns, no LDS and no barriers.  Each wave loads its columns' B
+// fragments straight from memory one k chunk at a time, splits them into
+// three BF16 parts in registers and runs the RB row blocks against A
+// fragments read from the pre-split operator (L2-resident).  The proxy
+// product of the re-compressed deep levels (M = 32) and their expansion
+// product (K = 32) move far more B or C than A, which the workgroup tile
+// above would stage through LDS for nothing.
+template <typename TBt, typename TC, int RB>
+__global__ __launch_bounds__(256) void gemm_thin3(GemmSplitArgs g)
+{
+    constexpr int NCB = 2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, fr = lane % 16, fq = lane / 16;
+    const int64_t n0 = ((int64_t)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave) * 16 * NCB;
+    if (n0 >= g.ncol) return;
+    const TBt *B = (const TBt *)g.B;
+    TC *C = (TC *)g.C;
+    const bf16x8 *As = (const bf16x8 *)g.As;  // [3][16*RB][Kp] bf16, 8 at a time
+    const int Kp = 32 * g.nch, R = 16 * RB;
+    f32x4 acc[RB][NCB];
+#pragma unroll
+    for (int i = 0; i < RB; i++)
+#pragma unroll
+        for (int j = 0; j < NCB; j++) acc[i][j] = f32x4{0, 0, 0, 0};
+    for (int ch = 0; ch < g.nch; ch++) {
+        bf16x8 b[NCB][3];
+#pragma unroll
+        for (int j = 0; j < NCB; j++) {
+            const int64_t col = min(n0 + j * 16 + fr, g.ncol - 1);
+            const TBt *src = B + col * g.ldb;
+            const int k0 = ch * 32 + 8 * fq;
+            float x[8];
+            if (k0 + 8 <= g.K && (g.ldb % 4) == 0) {  // whole 16-byte groups (the columns start 16-byte aligned)
+                if constexpr (sizeof(TBt) == 4) {
+                    const float4 v0 = *(const float4 *)(src + k0), v1 = *(const float4 *)(src + k0 + 4);
+                    x[0] = v0.x; x[1] = v0.y; x[2] = v0.z; x[3] = v0.w;
+                    x[4] = v1.x; x[5] = v1.y; x[6] = v1.z; x[7] = v1.w;
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; e += 2) {
+                        const double2 v = *(const double2 *)(src + k0 + e);
+                        x[e] = (float)v.x;
+                        x[e + 1] = (float)v.y;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; e++) x[e] = k0 + e < g.K ? (float)src[k0 + e] : 0.0f;
+            }
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const __bf16 h0 = (__bf16)x[e];
+                const float r1 = x[e] - (float)h0;
+                const __bf16 h1 = (__bf16)r1;
+                b[j][0][e] = h0;
+                b[j][1][e] = h1;
+                b[j][2][e] = (__bf16)(r1 - (float)h1);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < RB; i++) {
+            const size_t at = ((size_t)(i * 16 + fr) * Kp + ch * 32 + 8 * fq) / 8;
+            const bf16x8 a0 = As[at], a1 = As[at + (size_t)R * Kp / 8], a2 = As[at + (size_t)2 * R * Kp / 8];
+#pragma unroll
+            for (int j = 0; j < NCB; j++) {  // smallest terms first
+                acc[i][j] = mfma16(a2, b[j][0], acc[i][j]);
+                acc[i][j] = mfma16(a1, b[j][1], acc[i][j]);
+                acc[i][j] = mfma16(a0, b[j][2], acc[i][j]);
+                acc[i][j] = mfma16(a1, b[j][0], acc[i][j]);
+                acc[i][j] = mfma16(a0, b[j][1], acc[i][j]);
+                acc[i][j] = mfma16(a0, b[j][0], acc[i][j]);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NCB; j++) {
+        const int64_t col = n0 + j * 16 + fr;
+        if (col >= g.ncol) continue;
+        TC *dst = C + col * g.ldc;
+        const bool vec = (g.ldc % 4) == 0;  // FP32: 16-byte stores of 4 rows
+#pragma unroll
+        for (int i = 0; i < RB; i++) {
+            const int row = i * 16 + 4 * fq;
+            if constexpr (sizeof(TC) == 4) {
+                if (vec && row + 4 <= g.M) {
+                    float4 o;
+                    o.x = (float)(g.alpha * (double)acc[i][j][0]);
+                    o.y = (float)(g.alpha * (double)acc[i][j][1]);
+                    o.z = (float)(g.alpha * (double)acc[i][j][2]);
+                    o.w = (float)(g.alpha * (double)acc[i][j][3]);
+                    *(float4 *)(dst + row) = o;
+                    continue;
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                if (row + v < g.M) dst[row + v] = (TC)(g.alpha * (double)acc[i][j][v]);
+        }
+    }
+}
+
+// thin shapes (at most 64 rows, or one or two k chunks): the wave-per-32-columns
+// kernel; RB row blocks of 16
+template <typename TBt, typename TC>
+static void gemm_thin_t(const GemmSplitArgs &g, int rb, hipStream_t st)
+{
+    dim3 grid((unsigned)((g.ncol + 127) / 128));
+#define PBB_T3(RB)                                                        \
+    case RB: gemm_thin3<TBt, TC, RB><<<grid, 256, 0, st>>>(g); break;
+    switch (rb) {
+        PBB_T3(1) PBB_T3(2) PBB_T3(3) PBB_T3(4) PBB_T3(5) PBB_T3(6) PBB_T3(7) PBB_T3(8) PBB_T3(9) PBB_T3(10)
+        PBB_T3(11) PBB_T3(12) PBB_T3(13) PBB_T3(14)
+    default: fail(PBB_ERR_UNSUPPORTED, "split GEMM supports at most 224 rows");
+    }
+#undef PBB_T3
+}
+
 void launch_gemm_split3(const GemmSplitArgs &g, hipStream_t st)
 {
     if (g.ncol <= 0) return;
     if (g.ldb < g.K || g.ldc < g.M) fail(PBB_ERR_INTERNAL, "gemm: leading dimension too small");
+    if (g.M <= 64 || g.nch <= 2) {
+        const int rb = (g.M + 15) / 16;
+        if (!g.b_f32 && g.c_f32) gemm_thin_t<double, float>(g, rb, st);
+        else if (g.b_f32 && !g.c_f32) gemm_thin_t<float, double>(g, rb, st);
+        else if (g.b_f32 && g.c_f32) gemm_thin_t<float, float>(g, rb, st);
+        else fail(PBB_ERR_INTERNAL, "gemm_thin3: unsupported operand types");
+        check(hipGetLastError(), "gemm_thin3");
+        return;
+    }
     const int rb = (g.M + 15) / 16, w = rb > 12 ? 128 : 256;
     dim3 grid((unsigned)((g.ncol + w - 1) / w));
 #define PBB_G3(RB)                                                                                   \
