@@ -874,6 +874,72 @@ __global__ __launch_bounds__(256) void l2p_fp64(DownArgs a)
     }
 }
 
+// The same evaluation with FP32 local expansions (the mixed modes): all
+// columns of a box's F staged at once (32 per step, 16 from order 7), as
+// FP32 in LDS, so that a box with few points costs one load of F and two
+// barriers; lanes take (target, column) pairs of 8 targets at a time and
+// each sums the whole n^3 expansion in FP64.
+template <int N>
+__global__ __launch_bounds__(256) void l2p_f32(DownArgs a)
+{
+    constexpr int N2 = N * N, N3 = N2 * N, FS = N3 + 1;
+    constexpr int CC = N <= 6 ? 32 : 16, TG = 256 / CC;
+    __shared__ float Fs[CC * FS];
+    __shared__ double sS[3][N][TG];
+    const int tid = threadIdx.x;
+    const int b = a.seg_box[blockIdx.x], p0 = a.seg_p0[blockIdx.x];
+    const int p1 = min(p0 + L2P_SEG, a.pstart[b + 1]);
+    const uint32_t cc = (uint32_t)a.coord[b];
+    const double ihalfL = 2.0 / a.len;
+    const float *Fb = (const float *)a.F + (size_t)(a.off + b) * a.ncols * N3;
+    const int i = tid / CC, c = tid % CC;
+    for (int c0 = 0; c0 < a.ncols; c0 += CC) {
+        const int cn = min(CC, a.ncols - c0);
+        __syncthreads();  // the previous step's readers of Fs and sS are done
+        for (int q = tid; q < cn * N3; q += 256) {
+            const int qc = q / N3;
+            Fs[qc * FS + q - qc * N3] = Fb[(size_t)c0 * N3 + q];
+        }
+        for (int i0 = p0; i0 < p1; i0 += TG) {
+            const int tn = min(TG, p1 - i0);
+            if (i0 != p0) __syncthreads();  // the previous chunk's readers of sS are done
+            if (tid < 3 * TG) {
+                const int ti = tid % TG, ax = tid / TG;
+                if (ti < tn) {
+                    const double *xa = ax == 0 ? a.x : ax == 1 ? a.y : a.z;
+                    const double cen = a.ctab[(cc >> (10 * ax)) & 1023];
+                    double S[N];
+                    sn_eval_n<N>(a.Tkz, ihalfL * (xa[i0 + ti] - cen), S);
+#pragma unroll
+                    for (int m = 0; m < N; m++) sS[ax][m][ti] = S[m];
+                }
+            }
+            __syncthreads();
+            if (i < tn && c < cn) {
+                double Sz[N];
+#pragma unroll
+                for (int m = 0; m < N; m++) Sz[m] = sS[2][m][i];
+                const float *f = &Fs[c * FS];
+                double s = 0.0;
+                for (int l1 = 0; l1 < N; l1++) {
+                    const double s1 = sS[0][l1][i];
+                    double s2 = 0.0;
+#pragma unroll
+                    for (int l2 = 0; l2 < N; l2++) {
+                        double s3 = 0.0;
+#pragma unroll
+                        for (int l3 = 0; l3 < N; l3++) s3 += (double)f[(l1 * N + l2) * N + l3] * Sz[l3];
+                        s2 += sS[1][l2][i] * s3;
+                    }
+                    s += s1 * s2;
+                }
+                double *dst = a.lp + (int64_t)(i0 + i) * a.ncols + c0 + c;
+                *dst = a.accumulate ? *dst + s : s;
+            }
+        }
+    }
+}
+
 // Direct upward pass.  P2M at the leaves followed by M2M (compute.hpp:
 // 234-333, :956-1034) equals P2M at every level straight from the points:
 // the child-to-parent transfer interpolates the parent's Chebyshev
@@ -887,8 +953,14 @@ __global__ __launch_bounds__(256) void p2m_levels(UpArgs a)
     constexpr int N2 = N * N, N3 = N2 * N, PTS = 64, PP = PTS + 1;
     constexpr int CC = N <= 6 ? 32 : 16;        // columns per pass
     constexpr int PR = (CC * N2 + 255) / 256;   // pencils per thread
-    __shared__ double sS[3][N][PP];
-    __shared__ double sw[PTS][CC + 1];
+    // LDS: the point tables of a pass, then (FP32 expansions) the pass's
+    // expansion, staged so that it is stored with whole 16-byte writes
+    constexpr int U_SS = 3 * N * PP, U_SW = PTS * (CC + 1), U_SF = (CC * N3 + 1) / 2;
+    constexpr int U_ALL = U_SS + U_SW > U_SF ? U_SS + U_SW : U_SF;
+    __shared__ double smem[U_ALL];
+    double(*sS)[N][PP] = reinterpret_cast<double(*)[N][PP]>(smem);
+    double(*sw)[CC + 1] = reinterpret_cast<double(*)[CC + 1]>(smem + U_SS);
+    float *sF = reinterpret_cast<float *>(smem);
     const int tid = threadIdx.x;
     const int4 it = a.items[blockIdx.x];
     const int li = it.x, b = it.y, p0 = it.z;
@@ -942,6 +1014,29 @@ __global__ __launch_bounds__(256) void p2m_levels(UpArgs a)
                         for (int m = 0; m < N; m++) acc[r][m] += t * sz[m];
                     }
                 }
+            }
+        }
+        if constexpr (sizeof(TS) == 4) {
+            if (!Sp) {  // FP32 S: through LDS, then contiguous (16-byte where aligned) stores
+                __syncthreads();  // the pass's point tables are consumed
+#pragma unroll
+                for (int r = 0; r < PR; r++) {
+                    const int q = tid + 256 * r;
+                    if (q < CC * N2) {
+                        const int c = q / N2, l12 = q - c * N2;
+#pragma unroll
+                        for (int m = 0; m < N; m++) sF[c * N3 + l12 * N + m] = (float)acc[r][m];
+                    }
+                }
+                __syncthreads();
+                float *dst = (float *)Sb + (size_t)c0 * N3;
+                const int n = cn * N3;
+                if ((n & 3) == 0 && ((uintptr_t)dst & 15) == 0) {
+                    for (int q = tid; q < n / 4; q += 256) ((float4 *)dst)[q] = ((const float4 *)sF)[q];
+                } else {
+                    for (int q = tid; q < n; q += 256) dst[q] = sF[q];
+                }
+                continue;
             }
         }
 #pragma unroll
@@ -1590,16 +1685,24 @@ bool launch_l2p_level(const DownArgs &a, int nseg, hipStream_t st)
 {
     if (a.n < 4 || a.n > 8) return false;
     if (nseg <= 0) return true;
-#define PBB_L2P_N(TF)                                                      \
-    switch (a.n) {                                                         \
-    case 4: l2p_fp64<4, TF><<<nseg, 256, 0, st>>>(a); break;               \
-    case 5: l2p_fp64<5, TF><<<nseg, 256, 0, st>>>(a); break;               \
-    case 6: l2p_fp64<6, TF><<<nseg, 256, 0, st>>>(a); break;               \
-    case 7: l2p_fp64<7, TF><<<nseg, 256, 0, st>>>(a); break;               \
-    default: l2p_fp64<8, TF><<<nseg, 256, 0, st>>>(a); break;              \
+    if (a.f32) {
+        switch (a.n) {
+        case 4: l2p_f32<4><<<nseg, 256, 0, st>>>(a); break;
+        case 5: l2p_f32<5><<<nseg, 256, 0, st>>>(a); break;
+        case 6: l2p_f32<6><<<nseg, 256, 0, st>>>(a); break;
+        case 7: l2p_f32<7><<<nseg, 256, 0, st>>>(a); break;
+        default: l2p_f32<8><<<nseg, 256, 0, st>>>(a); break;
+        }
+        check(hipGetLastError(), "l2p_f32");
+        return true;
     }
-    if (a.f32) PBB_L2P_N(float) else PBB_L2P_N(double)
-#undef PBB_L2P_N
+    switch (a.n) {
+    case 4: l2p_fp64<4, double><<<nseg, 256, 0, st>>>(a); break;
+    case 5: l2p_fp64<5, double><<<nseg, 256, 0, st>>>(a); break;
+    case 6: l2p_fp64<6, double><<<nseg, 256, 0, st>>>(a); break;
+    case 7: l2p_fp64<7, double><<<nseg, 256, 0, st>>>(a); break;
+    default: l2p_fp64<8, double><<<nseg, 256, 0, st>>>(a); break;
+    }
     check(hipGetLastError(), "l2p_fp64");
     return true;
 }
