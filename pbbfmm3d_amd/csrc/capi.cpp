@@ -1913,6 +1913,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         a.lp = direct_down ? W.lp : nullptr;
         a.leaf_class = p->leaf_class;
         std::copy(std::begin(p->leaf_class_off), std::end(p->leaf_class_off), a.leaf_class_off);
+        a.kf32 = mixed ? 1 : 0;
         if (!(direct && dev::launch_near_split(a, h.kernel.id, (int)LL.nb, st)))
             dev::launch_near(a, h.kernel.id, (int)LL.nb, st);
     }

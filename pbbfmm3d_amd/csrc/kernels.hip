@@ -48,6 +48,26 @@ __device__ __forceinline__ double keval(double dx, double dy, double dz, const K
     else return 0.0;
 }
 
+// The same functors with the transcendental part in FP32 (the mixed modes'
+// near field): r^2 from the FP64 differences, then the FP32 library
+// function; relative error a few 1e-7 per value
+template <int KID>
+__device__ __forceinline__ double keval_f32(double dx, double dy, double dz, const KParams &p)
+{
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const float f = (float)r2;
+    if constexpr (KID == PBB_KERNEL_LAPLACIAN) return (double)(1.0f / sqrtf(f));
+    else if constexpr (KID == PBB_KERNEL_ONEOVERR2) return (double)(1.0f / f);
+    else if constexpr (KID == PBB_KERNEL_GAUSSIAN) return (double)expf(-f / (float)(p.p[0] * p.p[0]));
+    else if constexpr (KID == PBB_KERNEL_EXPONENTIAL) return p.p[1] * (double)expf(-sqrtf(f) / (float)p.p[0]);
+    else if constexpr (KID == PBB_KERNEL_SOFT_LAPLACIAN) return (double)(1.0f / sqrtf(f + (float)p.p[0]));
+    else if constexpr (KID == PBB_KERNEL_LOGARITHM) return r2 == 0 ? 0.0 : (double)(0.5f * logf(f));
+    else if constexpr (KID == PBB_KERNEL_THIN_PLATE) return r2 == 0 ? 0.0 : r2 * (double)(0.5f * logf(f));
+    else if constexpr (KID == PBB_KERNEL_EXP_ANISO)
+        return p.p[0] * (double)expf(-sqrtf((float)(p.p[1] * dx * dx + p.p[2] * dy * dy + p.p[3] * dz * dz)));
+    else return keval<KID>(dx, dy, dz, p);  // polynomial and rational ones: FP64 is as cheap
+}
+
 // S_m(u), m < n: Chebyshev interpolant by Clenshaw (H2_3D_Tree.cpp:830-879)
 __device__ __forceinline__ void sn_eval(int n, const double *__restrict__ Tkz, double u, double *S)
 {
@@ -1215,6 +1235,121 @@ __global__ __launch_bounds__(64) void p2p_fp64(NearArgs a, const int32_t *__rest
     }
 }
 
+// Near field of the dense leaves on the FP64 matrix cores (leaves of 16 or
+// more targets).  K (targets x sources) times W (sources x columns) is a
+// product of 16 x 4 blocks of K with 4 x 16 blocks of W: the A fragment of
+// v_mfma_f64_16x16x4f64 holds one value per lane, so each lane evaluates
+// the kernel for its (target, source) entry of every block and the matrix
+// cores accumulate 16*TB targets x 32 columns in FP64; the VALU only
+// evaluates kernels.  Sources are staged 32 at a time in LDS as in
+// p2p_fp64; the result (with the leaf's L2P sums) is added to the output
+// through perm, in FP64 as there.  KF32 (mixed modes): the kernel values in
+// FP32 (keval_f32), the sums still FP64.
+constexpr int PM_SC = 32;
+
+template <int KID, int TB, bool KF32>
+__global__ __launch_bounds__(64) void p2p_mfma(NearArgs a, const int32_t *__restrict__ leaves)
+{
+    __shared__ double sw[PM_SC][33];
+    __shared__ double sx[PM_SC], sy[PM_SC], sz[PM_SC];
+    __shared__ int s_start[27], s_pre[28];
+    const int lane = threadIdx.x, fr = lane & 15, fk = lane >> 4;
+    const int b = leaves[blockIdx.x];
+    const int t0 = a.leaf_start[b], nt = a.leaf_start[b + 1] - t0;
+    const uint32_t cc = (uint32_t)a.coord[b];
+    const int ix = cc & 1023, iy = (cc >> 10) & 1023, iz = (cc >> 20) & 1023;
+    const int S = a.side;
+    if (lane < 27) {
+        const int x = ix + lane / 9 - 1, y = iy + (lane / 3) % 3 - 1, z = iz + lane % 3 - 1;
+        const int sb = (x >= 0 && y >= 0 && z >= 0 && x < S && y < S && z < S) ? a.map[((int64_t)x * S + y) * S + z] : -1;
+        s_start[lane] = sb >= 0 ? a.leaf_start[sb] : 0;
+        s_pre[lane + 1] = sb >= 0 ? a.leaf_start[sb + 1] - a.leaf_start[sb] : 0;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        s_pre[0] = 0;
+        for (int m = 0; m < 27; m++) s_pre[m + 1] += s_pre[m];
+    }
+    __syncthreads();
+    const int ns = s_pre[27];
+    for (int c0 = 0; c0 < a.ncols; c0 += 32) {
+        const int cn = min(32, a.ncols - c0);
+        for (int i0 = 0; i0 < nt; i0 += 16 * TB) {
+            // this lane's target row fr of each 16-target block (rows past
+            // the leaf repeat its last target; their sums are not stored)
+            double tx[TB], ty[TB], tz[TB];
+#pragma unroll
+            for (int tb = 0; tb < TB; tb++) {
+                const int64_t ti = t0 + min(i0 + tb * 16 + fr, nt - 1);
+                tx[tb] = a.x[ti];
+                ty[tb] = a.y[ti];
+                tz[tb] = a.z[ti];
+            }
+            // accumulators: D[fk + 4v][fr] of block (tb, cb), with the L2P sums
+            dbl4 acc[TB][2];
+#pragma unroll
+            for (int tb = 0; tb < TB; tb++)
+#pragma unroll
+                for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int ti = i0 + tb * 16 + fk + 4 * v, col = cb * 16 + fr;
+                        acc[tb][cb][v] = (a.lp && ti < nt && col < cn) ? a.lp[(int64_t)(t0 + ti) * a.ncols + c0 + col] : 0.0;
+                    }
+            for (int j0 = 0; j0 < ns; j0 += PM_SC) {
+                const int jn = min(PM_SC, ns - j0);
+                __syncthreads();  // the previous chunk is consumed
+                int gi = 0;
+                if (lane < jn) {
+                    const int q = j0 + lane;
+                    int m = 0;
+                    while (s_pre[m + 1] <= q) m++;
+                    gi = s_start[m] + q - s_pre[m];
+                    sx[lane] = a.x[gi];
+                    sy[lane] = a.y[gi];
+                    sz[lane] = a.z[gi];
+                }
+#pragma unroll
+                for (int r = 0; r < PM_SC * 32 / 64; r++) {
+                    const int u = lane + 64 * r, j = u / 32, c = u % 32;
+                    const int gj = __shfl(gi, j);
+                    sw[j][c] = (j < jn && c < cn) ? a.wsT[(int64_t)gj * a.ncols + c0 + c] : 0.0;
+                }
+                __syncthreads();
+                for (int k0 = 0; k0 < jn; k0 += 4) {
+                    const int jj = k0 + fk;  // this lane's source of the 4-block (rows past jn carry zero weights)
+                    const int js = min(jj, jn - 1);
+                    const double px = sx[js], py = sy[js], pz = sz[js];
+                    const double w0 = sw[jj & (PM_SC - 1)][fr], w1 = sw[jj & (PM_SC - 1)][16 + fr];
+                    const double b0 = jj < jn ? w0 : 0.0, b1 = jj < jn ? w1 : 0.0;
+#pragma unroll
+                    for (int tb = 0; tb < TB; tb++) {
+                        double kv = KF32 ? keval_f32<KID>(px - tx[tb], py - ty[tb], pz - tz[tb], a.kp)
+                                         : keval<KID>(px - tx[tb], py - ty[tb], pz - tz[tb], a.kp);
+                        if (isinf(kv) || jj >= jn) kv = 0.0;  // (padding sources add nothing)
+                        acc[tb][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv, b0, acc[tb][0], 0, 0, 0);
+                        acc[tb][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv, b1, acc[tb][1], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int tb = 0; tb < TB; tb++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int ti = i0 + tb * 16 + fk + 4 * v;
+                    if (ti >= nt) continue;
+                    const int64_t oi = a.perm[t0 + ti] - a.out_lo;
+                    if ((uint64_t)oi >= (uint64_t)a.out_n) continue;
+#pragma unroll
+                    for (int cb = 0; cb < 2; cb++) {
+                        const int col = cb * 16 + fr;
+                        if (col < cn) a.out[(int64_t)(c0 + col) * a.out_n + oi] += acc[tb][cb][v];
+                    }
+                }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Position gradients of the FMM matvec (no reference counterpart: the
 // reference has no derivatives).  For each target i of a leaf:
@@ -1634,7 +1769,12 @@ bool launch_near_split(const NearArgs &a, int kid, int nleaf, hipStream_t st)
         const int32_t *lst = a.leaf_class + a.leaf_class_off[c];
 #define PBB_P2P_TT(K, T)                                                    \
     case T:                                                                 \
-        p2p_fp64<K, T><<<cnt, 64, 0, st>>>(b, lst);                         \
+        if constexpr (T >= 32) {                                            \
+            if (a.kf32) p2p_mfma<K, T / 16, true><<<cnt, 64, 0, st>>>(b, lst);  \
+            else p2p_mfma<K, T / 16, false><<<cnt, 64, 0, st>>>(b, lst);        \
+        } else {                                                            \
+            p2p_fp64<K, T><<<cnt, 64, 0, st>>>(b, lst);                     \
+        }                                                                   \
         break;
 #define PBB_P2P_CASE(K)                                                     \
     case K:                                                                 \

@@ -154,6 +154,7 @@ struct NearArgs {
     // (the last class also the larger ones)
     const int32_t *leaf_class;
     int leaf_class_off[NEAR_CLASSES + 1];
+    int kf32;                   // dense leaves: kernel values in FP32 (mixed modes)
 };
 
 struct DirectArgs {
