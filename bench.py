@@ -30,20 +30,23 @@ M2L_SPLIT = {"mixed": "bf16x3", "f16x2": "f16x2", "bf16x2": "bf16x2"}
 
 def dtype_label(precision, flags):
     """What each phase computes in.  Mixed modes: M2L on split 16-bit MFMA
-    with FP32 sums, proxies P and M2L results Pf stored FP32, VT / U on
-    three-part BF16 MFMA (FP64 with the cross-product-sum M2L tile), the
-    upward / downward passes and the near field in FP64."""
+    with FP32 sums (each level's re-compressed operator set), proxies P and
+    M2L results Pf stored FP32, VT / U on three-part BF16 MFMA (FP64 with the
+    cross-product-sum M2L tile), expansions S / F stored FP32 with FP64
+    arithmetic, the near field of dense leaves with FP32 kernel values and
+    FP64 MFMA sums, the rest of the near field in FP64."""
     if precision == "fp64":
         return "f64"
     split_acc = precision in ("f16x2", "bf16x2") and flags is not None and (flags & 8)
     vtu = "f64 (P, Pf f32)" if split_acc else "bf16x3->f32 (P, Pf f32)"
     return (f"m2l {M2L_SPLIT[precision]}->f32{' (cross products summed apart)' if split_acc else ''}; "
-            f"vt/u {vtu}; p2m, l2p, p2p f64")
+            f"vt/u {vtu}; p2m, l2p f64 (S, F f32); p2p f64 (dense leaves: f32 kernel values, f64 mfma sums)")
 
 
-# bytes per M2L launch, averaged over the launches of one f16x2 matvec of the
-# default workload (None until measured for the current kernels)
-M2L_TRAFFIC_F16X2_C3 = None
+# bytes per M2L launch (FETCH_SIZE x 2 + WRITE_SIZE x 1, calibrated on a plain
+# copy), averaged over the 6 launches of one f16x2 matvec of the default
+# workload (174.85 GB per matvec)
+M2L_TRAFFIC_F16X2_C3 = 2.914e10
 TRAFFIC_SOURCE = "profiles/m2l_f16x2_c3_gaussian_traffic.txt"
 
 

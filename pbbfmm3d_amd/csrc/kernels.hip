@@ -900,7 +900,7 @@ __global__ __launch_bounds__(256) void l2p_fp64(DownArgs a)
 // barriers; lanes take (target, column) pairs of 8 targets at a time and
 // each sums the whole n^3 expansion in FP64.
 template <int N>
-__global__ __launch_bounds__(256) void l2p_f32(DownArgs a)
+__global__ __launch_bounds__(256, 5) void l2p_f32(DownArgs a)
 {
     constexpr int N2 = N * N, N3 = N2 * N, FS = N3 + 1;
     constexpr int CC = N <= 6 ? 32 : 16, TG = 256 / CC;
