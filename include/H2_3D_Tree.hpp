@@ -126,6 +126,9 @@ class H2_3D_Tree {
     bool computed = false;
     int device_kernel = PBB_KERNEL_NONE;
     double device_param[4];
+    /* precision mode of H2_3D_Compute on this tree (pbb_precision); FP64,
+       the reference's arithmetic, unless the caller chooses a mixed mode */
+    int precision = PBB_PREC_FP64;
 
   private:
     static double eval_trampoline(void *ctx, const double *t, const double *s)

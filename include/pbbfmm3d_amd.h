@@ -493,6 +493,20 @@ PBB_API pbb_status pbb_matvec_ts(pbb_ops *ops, const double *target,
                                  int64_t nsources, const double *weight,
                                  int32_t ncols, double *output);
 
+/* The H2_3D_Compute<T> constructor (compute.hpp:67-149) in a chosen
+ * precision mode (pbb_precision): the reference computes in FP64 only, so
+ * pbb_matvec and pbb_matvec_ts do too; this entry lets callers of the
+ * reference-shaped surfaces (H2_3D_Tree::precision, FMMCompute.Compute's
+ * precision argument) run the mixed modes.  Targets and sources holding
+ * the same points run as one point set.  Each call builds a fresh plan, so
+ * the mixed M2L runs its default tiles (no tuning); a plan kept across
+ * calls (pbb_plan_create + pbb_plan_tune_m2l) is the way to tuned tiles.  */
+PBB_API pbb_status pbb_matvec_prec(pbb_ops *ops, const double *target,
+                                   int64_t ntargets, const double *source,
+                                   int64_t nsources, const double *weight,
+                                   int32_t ncols, double *output,
+                                   int32_t precision);
+
 #ifdef __cplusplus
 }
 #endif

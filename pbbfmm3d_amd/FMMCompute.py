@@ -5,11 +5,14 @@
 ``Compute(tree, target, source, weight, nCols, output)`` runs the FP64
 matvec on the GPU and adds the result into ``output`` (column-major
 ``output[c*N + i]``), as the reference's H2_3D_Compute constructor does.
+A ``precision`` argument (or the tree's ``precision`` attribute) selects a
+mixed mode instead; the reference has FP64 only.
 """
 from __future__ import annotations
 
 import numpy as np
 
+from ._capi import PREC_FP64
 from .core import matvec_host
 
 __all__ = ["Compute", "vecOfvec3", "vecOfdouble", "convert_to_numpy", "convert_to_vecOfdouble",
@@ -83,7 +86,11 @@ class Compute:
     (src/FMMCompute.cpp:68-70, include/compute.hpp:67-149)."""
 
     def __init__(self, tree, target: vecOfvec3, source: vecOfvec3, weight: vecOfdouble, nCols: int,
-                 output: vecOfdouble):
+                 output: vecOfdouble, precision: int | None = None):
+        # precision: a pbb_precision mode; None takes the tree's
+        # (tree.precision, FP64 unless the caller set a mixed mode)
+        if precision is None:
+            precision = getattr(tree, "precision", PREC_FP64)
         tgt = target.data if isinstance(target, _Vec) else np.asarray(target, dtype=np.float64)
         src = source.data if isinstance(source, _Vec) else np.asarray(source, dtype=np.float64)
         tgt = tgt.reshape(-1, 3)
@@ -99,5 +106,6 @@ class Compute:
         if output.data.size < N * nCols:
             output.resize(N * nCols)
         out = np.ascontiguousarray(output.data.reshape(-1)[: N * nCols])
-        matvec_host(tree.operators(), tgt, w[: M * nCols], int(nCols), out, sources=None if same else src)
+        matvec_host(tree.operators(), tgt, w[: M * nCols], int(nCols), out, sources=None if same else src,
+                    precision=int(precision))
         output.data.reshape(-1)[: N * nCols] = out

@@ -52,6 +52,9 @@ class Atree:
         self.homogen = 0.0
         self.symmetry = 1
         self.kernelType = ""
+        # precision mode of FMMCompute.Compute on this tree: FP64 (the
+        # reference's arithmetic) unless the caller picks a mixed mode
+        self.precision = capi.PREC_FP64
         self._ops = None
 
     def SetKernelProperty(self):

@@ -142,6 +142,7 @@ _SIGNATURES = {
     "pbb_plan_destroy": (None, [_vp]),
     "pbb_matvec": (C.c_int, [_vp, _dp, _dp, C.c_int64, _dp, C.c_int32, _dp]),
     "pbb_matvec_ts": (C.c_int, [_vp, _dp, C.c_int64, _dp, C.c_int64, _dp, C.c_int32, _dp]),
+    "pbb_matvec_prec": (C.c_int, [_vp, _dp, C.c_int64, _dp, C.c_int64, _dp, C.c_int32, _dp, C.c_int32]),
     "pbb_direct_sum": (C.c_int, [C.POINTER(KernelDesc), _vp, C.c_int64, _vp, C.c_int64, _vp,
                                  C.c_int32, _vp, _vp]),
 }

@@ -528,10 +528,14 @@ class Batch:
             self._h = None
 
 
-def matvec_host(ops: Operators, xyz, weight, ncols: int, output=None, sources=None) -> np.ndarray:
+def matvec_host(ops: Operators, xyz, weight, ncols: int, output=None, sources=None,
+                precision: int = capi.PREC_FP64) -> np.ndarray:
     """The drop-in path (pbb_matvec, or pbb_matvec_ts with separate
-    sources): host FP64 arrays, weights column-major (ncols x sources),
-    output column-major (ncols x targets), accumulated."""
+    sources; pbb_matvec_prec in a mixed mode): host FP64 arrays, weights
+    column-major (ncols x sources), output column-major (ncols x targets),
+    accumulated."""
+    if precision not in (capi.PREC_FP64, capi.PREC_MIXED, capi.PREC_MIXED_F16X2, capi.PREC_MIXED_BF16X2):
+        raise ValueError(f"unknown precision mode {precision}")
     xyz = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
     N = xyz.shape[0]
     src = xyz if sources is None else np.ascontiguousarray(np.asarray(sources, dtype=np.float64).reshape(-1, 3))
@@ -545,7 +549,11 @@ def matvec_host(ops: Operators, xyz, weight, ncols: int, output=None, sources=No
               and output.flags.c_contiguous):
         raise ValueError("output must be a contiguous float64 array of ncols * ntargets values")
     dp = C.POINTER(C.c_double)
-    if sources is None:
+    if precision != capi.PREC_FP64:
+        capi.check(capi.lib().pbb_matvec_prec(ops.handle, xyz.ctypes.data_as(dp), N, src.ctypes.data_as(dp), M,
+                                              w.ctypes.data_as(dp), int(ncols), output.ctypes.data_as(dp),
+                                              int(precision)))
+    elif sources is None:
         p = xyz.ctypes.data_as(dp)
         capi.check(capi.lib().pbb_matvec(ops.handle, p, p, N, w.ctypes.data_as(dp), int(ncols),
                                          output.ctypes.data_as(dp)))

@@ -2292,7 +2292,7 @@ PBB_API pbb_status pbb_direct_sum(const pbb_kernel_desc *kernel, const double *t
 // One matvec from host buffers: a plan (one point set when same, else the
 // targets followed by the sources), the weights uploaded, out += A w.
 static pbb_status matvec_host(pbb_ops *ops, const double *target, int64_t nt, const double *source, int64_t ns,
-                              bool same, const double *weight, int32_t ncols, double *output)
+                              bool same, const double *weight, int32_t ncols, double *output, int32_t precision)
 {
     pbb_plan *plan = nullptr;
     pbb_status st = same ? pbb_plan_create(ops, target, nt, &plan)
@@ -2306,7 +2306,7 @@ static pbb_status matvec_host(pbb_ops *ops, const double *target, int64_t nt, co
         dout = dalloc<double>(no, "output");
         check(hipMemcpyAsync(dw, weight, nw * sizeof(double), hipMemcpyHostToDevice, plan->stream), "copy weights");
         check(hipMemsetAsync(dout, 0, no * sizeof(double), plan->stream), "clear output");
-        matvec_own(plan, plan->ops, dw, ncols, dout, plan->stream, PBB_PREC_FP64);
+        matvec_own(plan, plan->ops, dw, ncols, dout, plan->stream, precision);
         std::vector<double> res(no);
         check(hipMemcpyAsync(res.data(), dout, no * sizeof(double), hipMemcpyDeviceToHost, plan->stream),
               "copy output");
@@ -2336,7 +2336,7 @@ PBB_API pbb_status pbb_matvec(pbb_ops *ops, const double *target, const double *
         same = target == source || std::memcmp(target, source, sizeof(double) * 3 * (size_t)npoints) == 0;
     });
     if (st != PBB_OK) return st;
-    return matvec_host(ops, target, npoints, source, npoints, same, weight, ncols, output);
+    return matvec_host(ops, target, npoints, source, npoints, same, weight, ncols, output, PBB_PREC_FP64);
 }
 
 PBB_API pbb_status pbb_matvec_ts(pbb_ops *ops, const double *target, int64_t ntargets, const double *source,
@@ -2348,7 +2348,25 @@ PBB_API pbb_status pbb_matvec_ts(pbb_ops *ops, const double *target, int64_t nta
         if (ntargets <= 0 || nsources <= 0) fail(PBB_ERR_INVALID_ARG, "ntargets and nsources must be positive");
     });
     if (st != PBB_OK) return st;
-    return matvec_host(ops, target, ntargets, source, nsources, false, weight, ncols, output);
+    return matvec_host(ops, target, ntargets, source, nsources, false, weight, ncols, output, PBB_PREC_FP64);
+}
+
+PBB_API pbb_status pbb_matvec_prec(pbb_ops *ops, const double *target, int64_t ntargets, const double *source,
+                                   int64_t nsources, const double *weight, int32_t ncols, double *output,
+                                   int32_t precision)
+{
+    bool same = false;
+    pbb_status st = guarded([&] {
+        if (!ops || !target || !source || !weight || !output) fail(PBB_ERR_INVALID_ARG, "NULL argument");
+        if (ncols < 1) fail(PBB_ERR_INVALID_ARG, "ncols must be >= 1");
+        if (ntargets <= 0 || nsources <= 0) fail(PBB_ERR_INVALID_ARG, "ntargets and nsources must be positive");
+        if (precision < PBB_PREC_FP64 || precision > PBB_PREC_MIXED_BF16X2)
+            fail(PBB_ERR_INVALID_ARG, "unknown precision mode");
+        same = ntargets == nsources &&
+               (target == source || std::memcmp(target, source, sizeof(double) * 3 * (size_t)ntargets) == 0);
+    });
+    if (st != PBB_OK) return st;
+    return matvec_host(ops, target, ntargets, source, nsources, same, weight, ncols, output, precision);
 }
 
 }  // extern "C"

@@ -35,7 +35,7 @@ class myKernel : public H2_3D_Tree {
 
 template <typename T>
 static int run(const char *name, T &tree, std::vector<vector3> &pts, std::vector<double> &w, int nCols,
-               double tol, const char *cache)
+               double tol, double mixed_tol, const char *cache)
 {
     if (cache) tree.cache_dir = cache;
     tree.buildFMMTree();
@@ -57,6 +57,27 @@ static int run(const char *name, T &tree, std::vector<vector3> &pts, std::vector
     double err = ComputeError(out, exact, N, nCols, rows);
     std::printf("%s: relative error for first %d rows: %.3e\n", name, rows, err);
     if (err >= tol) return 1;
+
+    // the same matvec in the f16x2 mixed mode, against the FP64 result
+    {
+        std::vector<double> out_m((size_t)N * nCols, 0.0);
+        tree.precision = PBB_PREC_MIXED_F16X2;
+        try {
+            H2_3D_Compute<T> mixed(tree, pts, pts, w, nCols, out_m);
+        } catch (const std::runtime_error &e) {
+            std::printf("%s: %s\n", name, e.what());
+            return 2;
+        }
+        tree.precision = PBB_PREC_FP64;
+        double d2 = 0, s2 = 0;
+        for (size_t i = 0; i < out.size(); i++) {
+            d2 += (out_m[i] - out[i]) * (out_m[i] - out[i]);
+            s2 += out[i] * out[i];
+        }
+        const double rel = std::sqrt(d2 / s2);
+        std::printf("%s: f16x2 mode against FP64: relative difference %.3e\n", name, rel);
+        if (!(rel < mixed_tol)) return 1;
+    }
 
     // separate targets: Nf = N / 3 other points of the same cube, the N
     // points as sources (weights column-major Ns x nCols, outputs Nf x nCols)
@@ -112,8 +133,8 @@ int main(int argc, char **argv)
     for (auto &p : pts) p = vector3(frand(-0.5, 0.5) * L, frand(-0.5, 0.5) * L, frand(-0.5, 0.5) * L);
 
     kernel_Laplacian lap(L, level, order, eps, cheb);
-    int rc = run("kernel_Laplacian", lap, pts, w, nCols, 1e-5, cache);
+    int rc = run("kernel_Laplacian", lap, pts, w, nCols, 1e-5, 1e-6, cache);
     if (rc) return rc;
     myKernel mk(L, level, order, eps, cheb);
-    return run("myKernel", mk, pts, w, nCols, 1e-6, cache);
+    return run("myKernel", mk, pts, w, nCols, 1e-6, 1e-6, cache);
 }

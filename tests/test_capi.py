@@ -100,3 +100,28 @@ def test_wrappers_reject_host_tensors_before_any_call():
         _f64_cuda(w.float(), "x")
     with pytest.raises(TypeError):
         _f64_cuda(np.zeros(3), "x")
+
+
+def test_matvec_prec_rejects_bad_arguments():
+    # argument checks of the precision-mode drop-in run before any device work
+    from pbbfmm3d_amd.core import matvec_host
+    L = capi.lib()
+    ops = pbb.Operators(pbb.gaussian(0.3), 1.0, 3, 4, 1e-6)
+    dp = C.POINTER(C.c_double)
+    x = np.random.default_rng(1).random((50, 3)) - 0.5
+    w = np.ones(50)
+    out = np.zeros(50)
+    args = (x.ctypes.data_as(dp), 50, x.ctypes.data_as(dp), 50, w.ctypes.data_as(dp), 1, out.ctypes.data_as(dp))
+    assert L.pbb_matvec_prec(ops.handle, *args, 9) == capi.PBB_ERR_INVALID_ARG
+    assert b"precision" in L.pbb_last_error()
+    assert L.pbb_matvec_prec(ops.handle, *args, -1) == capi.PBB_ERR_INVALID_ARG
+    bad = list(args)
+    bad[1] = 0
+    assert L.pbb_matvec_prec(ops.handle, *bad, capi.PREC_MIXED) == capi.PBB_ERR_INVALID_ARG
+    bad = list(args)
+    bad[5] = 0
+    assert L.pbb_matvec_prec(ops.handle, *bad, capi.PREC_MIXED) == capi.PBB_ERR_INVALID_ARG
+    assert L.pbb_matvec_prec(None, *args, capi.PREC_MIXED) == capi.PBB_ERR_INVALID_ARG
+    with pytest.raises(ValueError):
+        matvec_host(ops, x, w, 1, precision=5)
+    assert not out.any()

@@ -109,3 +109,40 @@ def test_exp_cov_grid_workflow(gpu):
     # the kernel's z scale (1.5) is about the leaf size (2.8) at level 5: order-5
     # interpolation error, identical in the oracle (measured 7.2e-3)
     assert err <= 1e-2
+
+
+@pytest.mark.gpu
+def test_compute_precision_modes(gpu):
+    """Compute in the mixed modes (the tree's precision attribute, or the
+    precision argument), against the FP64 Compute on the same inputs, for
+    one point set and for separate targets and sources."""
+    from pbbfmm3d_amd import _capi as capi
+    rng = np.random.default_rng(91)
+    pts = rng.random((20000, 3))
+    tgt = rng.random((7000, 3))
+    ncols = 8
+    W = rng.random((pts.shape[0], ncols))
+    tree = myKernel(1, 4, 5, 1e-7, 1)
+    tree.buildFMMTree()
+
+    def run(targets, precision=None):
+        t_c, s_c, w_c, o_c = vecOfvec3(), vecOfvec3(), vecOfdouble(), vecOfdouble()
+        convert_to_vecOfvec3(targets, t_c)
+        convert_to_vecOfvec3(pts, s_c)
+        convert_to_vecOfdouble(W.T.reshape(-1), w_c)
+        o_c[:] = np.zeros(targets.shape[0] * ncols)
+        Compute(tree, t_c, s_c, w_c, ncols, o_c, precision=precision)
+        return o_c.data.reshape(ncols, -1).T.copy()
+
+    ref_same, ref_sep = run(pts), run(tgt)
+    tree.precision = capi.PREC_MIXED_F16X2
+    f16_same, f16_sep = run(pts), run(tgt)
+    tree.precision = capi.PREC_FP64
+    bf16_same = run(pts, precision=capi.PREC_MIXED)
+    assert orc.rel_err(run(pts), ref_same) <= 1e-13  # back to FP64
+    d = [orc.rel_err(f16_same, ref_same), orc.rel_err(f16_sep, ref_sep), orc.rel_err(bf16_same, ref_same)]
+    print("case compute precision modes: f16x2 same/separate, bf16x3 same: " +
+          " ".join(f"{x:.2e}" for x in d), flush=True)
+    assert max(d) <= 1e-6
+    with pytest.raises(ValueError):
+        run(pts, precision=7)
