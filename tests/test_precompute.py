@@ -132,3 +132,34 @@ def test_gpu_precompute_equals_host(gpu, kernel, n, level):
     yg = pbb.Plan(g, xyz).matvec(w)
     yh = pbb.Plan(h, xyz).matvec(w)
     assert float(torch.linalg.norm(yg - yh) / torch.linalg.norm(yh)) <= 10 * eps
+
+
+def test_deeper_levels_keep_the_level2_cutoff():
+    # A non-homogeneous kernel gets one operator set per level, and every set
+    # keeps the cutoff of the level-2 blocks (H2_3D_Tree.cpp:406-436), although
+    # the blocks of smaller boxes have a lower numerical rank.  Pinned against
+    # numpy SVDs of each level's stacked kernel blocks: the ranks are the
+    # level-2 cutoffs, and the deepest level's own cutoff is smaller, so a
+    # per-level cutoff would fail here.
+    # (cutoffs of levels 2..5 here: 54, 32, 18, 10 of n^3 = 64)
+    kernel, L, levels, n, eps = pbb.gaussian(0.5), 1.0, 5, 4, 1e-6
+    ops = pbb.Operators(kernel, L, levels, n, eps)
+    U, VT, E, sf, ss = ops.export()
+    own = []
+    for s in range(ops.nsets):
+        B = orc.kernel_blocks(kernel.id, kernel.params, kernel.symmetry, n, L / 4 / 2 ** s)
+        fat = np.hstack([B[d].T for d in range(316)])
+        thin = np.vstack([B[d].T for d in range(316)])
+        sv_f = np.linalg.svd(fat, compute_uv=False)
+        sv_s = np.linalg.svd(thin, compute_uv=False)
+        if s == 0:
+            k = min(len(sf), len(sv_f))
+            np.testing.assert_allclose(sf[:k], sv_f[:k], rtol=0, atol=1e-12 * sv_f[0])
+            np.testing.assert_allclose(ss[:k], sv_s[:k], rtol=0, atol=1e-12 * sv_s[0])
+        own.append((orc.cutoff(sv_f, eps), orc.cutoff(sv_s, eps)))
+    assert ops.nsets == levels - 1
+    assert (ops.rank_f, ops.rank_s) == own[0] and ops.rank_f < ops.n3
+    assert own[-1][0] < own[0][0] and own[-1][1] < own[0][1]
+    rf, rs, n3 = ops.rank_f, ops.rank_s, ops.n3
+    assert U.size == ops.nsets * n3 * rf and VT.size == ops.nsets * rs * n3
+    assert E.size == ops.nsets * 316 * rf * rs
