@@ -248,9 +248,15 @@ PBB_API pbb_status pbb_plan_create(pbb_ops *ops, const double *xyz,
  * on the GPU, as pbb_plan_create does.  PBB_PLAN_REJECT_OUTSIDE fails with
  * PBB_ERR_INVALID_ARG when a point lies outside the root cube (side L
  * around the bbox centre); without it such points are extrapolated, as in
- * the reference, and counted in pbb_plan_info.outside_points.            */
+ * the reference, and counted in pbb_plan_info.outside_points.
+ * PBB_PLAN_EXPANSION_ROUTE keeps the n^3-coefficient expansions S and F of
+ * every level in the mixed modes: without it, the levels whose re-compressed
+ * ranks are at most 32 form their proxies straight from the points and
+ * evaluate the M2L results straight at the points (the same sums in
+ * another order, without the expansions in device memory).               */
 #define PBB_PLAN_HOST_TREE 1
 #define PBB_PLAN_REJECT_OUTSIDE 2
+#define PBB_PLAN_EXPANSION_ROUTE 4
 PBB_API pbb_status pbb_plan_create_ex(pbb_ops *ops, const double *xyz,
                                       int64_t npoints, int32_t flags,
                                       pbb_plan **out);

@@ -57,6 +57,7 @@ def M2L_DIR_GROUPS(g):
 NUM_PHASES = 8
 PLAN_HOST_TREE = 1
 PLAN_REJECT_OUTSIDE = 2
+PLAN_EXPANSION_ROUTE = 4
 OPS_HOST_PRECOMPUTE = 1
 BATCH_TRANSPOSE = 1
 BATCH_OVERWRITE = 2

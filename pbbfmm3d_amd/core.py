@@ -272,18 +272,21 @@ class Plan:
     (or order the streams), as the C ABI requires."""
 
     def __init__(self, ops: Operators, xyz, host_tree: bool = False, sources=None,
-                 reject_outside: bool = False):
+                 reject_outside: bool = False, expansion_route: bool = False):
         """xyz: the targets (N x 3).  sources (M x 3, optional): separate
         sources; matvec then takes weights at the sources, (ncols, M), and
         returns outputs at the targets, (ncols, N) (pbb_plan_create_ts).
         host_tree: build the tree on the host instead of the GPU (the same
         arrays; kept as the GPU build's check).  reject_outside: raise
         PBBError if a point lies outside the root cube instead of
-        extrapolating."""
+        extrapolating.  expansion_route: keep the n^3-coefficient expansions
+        of every level in the mixed modes (no proxy passes at the levels of
+        rank <= 32; the same sums in another order)."""
         self.ops = ops
         dp = C.POINTER(C.c_double)
         xyz = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
-        flags = (capi.PLAN_HOST_TREE if host_tree else 0) | (capi.PLAN_REJECT_OUTSIDE if reject_outside else 0)
+        flags = ((capi.PLAN_HOST_TREE if host_tree else 0) | (capi.PLAN_REJECT_OUTSIDE if reject_outside else 0)
+                 | (capi.PLAN_EXPANSION_ROUTE if expansion_route else 0))
         h = C.c_void_p()
         if sources is None:
             self.npoints = self.ntargets = self.nsources = xyz.shape[0]

@@ -386,6 +386,7 @@ struct DevLevel {
     int32_t *pstart = nullptr;                       // box -> first sorted point (levels >= 2)
     int32_t *seg_box = nullptr, *seg_p0 = nullptr;   // L2P segments of at most L2P_SEG points
     int nseg = 0;
+    int up_item0 = 0, up_red0 = 0;                  // the level's first direct upward work item and reduction
     double *ctab = nullptr;
     double len = 0;
     int64_t class_count[8] = {0};
@@ -407,6 +408,7 @@ struct pbb_plan {
     int32_t *leaf_class = nullptr;              // leaves grouped by point count
     int4 *up_items = nullptr, *up_reds = nullptr;  // direct upward pass work list
     int n_up_items = 0, n_up_reds = 0, n_up_slots = 0;
+    bool expansion_route = false;               // PBB_PLAN_EXPANSION_ROUTE: no proxy passes
     int leaf_class_off[dev::NEAR_CLASSES + 1] = {0};
     std::vector<DevLevel> lv;
     int64_t Btot = 0;
@@ -1037,6 +1039,8 @@ static void plan_create_host(pbb_ops *ops, const double *xyz, int64_t npoints, i
                     s0.push_back(q);
                 }
             }
+            plan->lv[l].up_item0 = (int)up_items.size();
+            plan->lv[l].up_red0 = (int)up_reds.size();
             for (int64_t b = 0; b < V.nb; b++) {  // direct upward pass work items
                 const int32_t np = ps[b + 1] - ps[b];
                 if (np <= dev::P2M_SEG) {
@@ -1139,6 +1143,7 @@ static void plan_create_gpu(pbb_ops *ops, const double *xyz, int64_t npoints, in
         const dev::GpuLevel &V = T.lv[l];
         D.nb = V.nb; D.off = V.off; D.coord = V.coord; D.map = V.map; D.child = V.child; D.order = V.order;
         D.pstart = V.pstart; D.seg_box = V.seg_box; D.seg_p0 = V.seg_p0; D.nseg = V.nseg;
+        D.up_item0 = V.up_item0; D.up_red0 = V.up_red0;
         std::copy(std::begin(V.class_count), std::end(V.class_count), D.class_count);
         D.len = plan->h_geom[l].len;
     }
@@ -1172,7 +1177,8 @@ static void plan_create_gpu(pbb_ops *ops, const double *xyz, int64_t npoints, in
 static void plan_create(pbb_ops *ops, const double *xyz, int64_t npoints, int64_t nbox, int32_t flags,
                         pbb_plan **out)
 {
-    if (flags & ~(PBB_PLAN_HOST_TREE | PBB_PLAN_REJECT_OUTSIDE)) fail(PBB_ERR_INVALID_ARG, "unknown plan flags");
+    if (flags & ~(PBB_PLAN_HOST_TREE | PBB_PLAN_REJECT_OUTSIDE | PBB_PLAN_EXPANSION_ROUTE))
+        fail(PBB_ERR_INVALID_ARG, "unknown plan flags");
     const HostOps &h = ops->h;
     if (!kernel_id_valid(h.kernel.id) || h.kernel.id == PBB_KERNEL_POLY_XY)
         fail(PBB_ERR_UNSUPPORTED, "the kernel has no device functor for the near field");
@@ -1180,6 +1186,7 @@ static void plan_create(pbb_ops *ops, const double *xyz, int64_t npoints, int64_
     if (flags & PBB_PLAN_HOST_TREE) plan_create_host(ops, xyz, npoints, nbox, &p);
     else plan_create_gpu(ops, xyz, npoints, nbox, &p);
     std::unique_ptr<pbb_plan> plan(p);
+    plan->expansion_route = (flags & PBB_PLAN_EXPANSION_ROUTE) != 0;
     if ((flags & PBB_PLAN_REJECT_OUTSIDE) && plan->info.outside_points > 0)
         fail(PBB_ERR_INVALID_ARG, std::to_string(plan->info.outside_points) +
                                       " points lie outside the root cube of side L around the targets' bbox "
@@ -1682,6 +1689,27 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
     // the mixed modes keep S / F in FP32 when the direct passes read them
     // (not with the FP64 VT / U of the cross-product-sum tile)
     const bool s32 = mixed && direct_down && !split_acc;
+    // levels Lp..Lv run the proxy passes (P2M straight to P, L2P straight
+    // from Pf, no S / F of the level): the deepest levels whose re-compressed
+    // ranks are at most 32.  Not with points outside the root cube: their
+    // extrapolated S_n terms are large and cancel, which the proxy passes'
+    // FP32 rank vectors do not resolve (the expansion route sums S in FP64).
+    int Lp = Lv + 1;
+    if (s32 && !p->expansion_route && p->info.outside_points == 0 && n <= 8 && ldP % 4 == 0 &&
+        ldP >= dev::PX_K && !o->mset.empty())
+        while (Lp > 2 && o->mset[h.ops_set(Lp - 1)].kf <= dev::PX_K && o->mset[h.ops_set(Lp - 1)].ks <= dev::PX_K &&
+               h.rf >= dev::PX_K)
+            Lp--;
+    // the M2L results of a proxy level keep rows of PX_K floats (128 bytes)
+    // inside the level's own region of Pf (rows of r_f), for whole-row loads
+    auto pf_level = [&](int l, int &ldf) -> float * {
+        if (l < Lp) {
+            ldf = h.rf;
+            return W.Pf32;
+        }
+        ldf = dev::PX_K;
+        return W.Pf32 + (size_t)p->lv[l].off * ncols * (h.rf - dev::PX_K);
+    };
     const PointIo io = point_io(p, o);
     dev::launch_gather_weights(w, p->perm, p->N, ncols, W.ws, io.in_lo, io.in_n, st);
     if (direct) dev::launch_gather_weights_t(w, p->perm, p->N, ncols, W.wsT, io.in_lo, io.in_n, st);
@@ -1695,9 +1723,10 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.wsT = W.wsT; a.S = W.S;
         a.scratch = W.up; a.s32 = s32;
         dev::UpReduce r{};
-        r.ncols = ncols; r.n3 = n3; r.nred = p->n_up_reds; r.reds = p->up_reds;
+        r.ncols = ncols; r.n3 = n3; r.reds = p->up_reds;
+        r.nred = Lp <= Lv ? p->lv[Lp].up_red0 : p->n_up_reds;  // the work items are in level order
         r.scratch = W.up; r.S = W.S; r.s32 = s32;
-        dev::launch_p2m_levels(a, p->n_up_items, r, st);
+        dev::launch_p2m_levels(a, Lp <= Lv ? p->lv[Lp].up_item0 : p->n_up_items, r, st);
     } else if (Lv >= 2) {
         const DevLevel &LL = p->lv[Lv];
         dev::P2MArgs a{};
@@ -1742,7 +1771,14 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         } else {
             g.C = W.P + (size_t)D.off * ncols * h.rs;
         }
-        if (SA) {  // three-part BF16 matrix cores (P is FP32 in the mixed modes)
+        if (l >= Lp) {  // proxies straight from the points (rows past ks are zero)
+            dev::ProxyArgs x{};
+            x.ncols = ncols; x.nb = (int)D.nb; x.rank = mset(l).ks; x.ldp = ldP; x.off = D.off; x.len = D.len;
+            x.pstart = D.pstart; x.coord = D.coord; x.ctab = D.ctab; x.Tkz = o->d_Tkz;
+            x.x = p->x; x.y = p->y; x.z = p->z; x.wsT = W.wsT;
+            x.op = o->dm_VTw + mset(l).vt_off; x.P = W.P32; x.npts = p->N;
+            dev::launch_p2m_proxy(x, n, p->N, st);
+        } else if (SA) {  // three-part BF16 matrix cores (P is FP32 in the mixed modes)
             dev::GemmSplitArgs q{};
             q.M = g.M; q.K = g.K; q.ldb = g.ldb; q.ldc = g.ldc; q.b_f32 = g.b_f32; q.c_f32 = 1; q.nch = SA->nch[sl];
             q.ncol = g.ncol; q.alpha = 1.0; q.As = (const uint16_t *)SA->d + SA->off[sl];
@@ -1784,7 +1820,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         const int groups_m = tl.width ? std::max(1, (tl.flags >> 8) & 0xff) : 1;
         const int sl = h.ops_set(l);
         dev::M2LMixArgs a{};
-        a.rf = mset(l).kf; a.rs = mset(l).ks; a.ldf = h.rf;
+        a.rf = mset(l).kf; a.rs = mset(l).ks;
         a.ncols = ncols; a.TB = std::max(1, std::min(16, width_m / cptm)); a.cpt = cptm;
         a.chunk_major = tl.width ? (tl.flags & PBB_M2L_CHUNK_MAJOR) : 1;
         a.row_split = tl.width ? (tl.flags & PBB_M2L_ROW_SPLIT) != 0 : 0;
@@ -1806,7 +1842,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         }
         a.order = D.order; a.coord = D.coord; a.map = D.map;
         a.Es = (const uint16_t *)SE->d + SE->off[sl];
-        a.P = W.P32; a.Pf = W.Pf32;
+        a.P = W.P32; a.Pf = pf_level(l, a.ldf);
         a.xmax = prec == PBB_PREC_MIXED_F16X2 ? W.xmax + l : nullptr;
         // direction groups only where the launch fills the GPU many times over
         const int64_t wgs = (int64_t)a.tile_prefix[8] * nchunks_m * (a.row_split ? 2 : 1);
@@ -1848,8 +1884,9 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
     }
     if (Lv >= 2) lev_mark(Lv + 1);
     mark(4);
-    // local expansions F_l = scale_l * Uw Pf_l overwrite S_l (no longer needed)
-    for (int l = 2; l <= Lv; l++) {
+    // local expansions F_l = scale_l * Uw Pf_l overwrite S_l (no longer needed);
+    // the proxy-pass levels evaluate Pf_l at their points instead
+    for (int l = 2; l <= Lv && l < Lp; l++) {
         const DevLevel &D = p->lv[l];
         const int sl = h.ops_set(l);
         dev::GemmArgs g{};
@@ -1884,6 +1921,19 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
     // the direct pass evaluates every level's F at the points instead
     for (int l = 2; l <= Lv && direct_down; l++) {
         const DevLevel &D = p->lv[l];
+        if (l >= Lp) {
+            dev::ProxyArgs x{};
+            x.ncols = ncols; x.nb = (int)D.nb; x.accumulate = l > 2; x.rank = mset(l).kf;
+            x.off = D.off; x.len = D.len;
+            x.alpha = h.homog ? std::pow(1.0 / D.len, h.homogen_flipped) : 1.0;  // as the U product's scale
+            x.pstart = D.pstart; x.coord = D.coord; x.ctab = D.ctab; x.Tkz = o->d_Tkz;
+            x.x = p->x; x.y = p->y; x.z = p->z;
+            x.op = o->dm_Uw + mset(l).u_off; x.lp = W.lp; x.npts = p->N;
+            x.Pf = pf_level(l, x.ldf);
+            x.pf_vec = ((uintptr_t)(x.Pf + (size_t)D.off * ncols * x.ldf) & 15) == 0;
+            dev::launch_l2p_proxy(x, n, p->N, st);
+            continue;
+        }
         dev::DownArgs a{};
         a.n = n; a.ncols = ncols; a.accumulate = l > 2; a.off = D.off; a.len = D.len;
         a.seg_box = D.seg_box; a.seg_p0 = D.seg_p0; a.pstart = D.pstart; a.coord = D.coord; a.ctab = D.ctab;

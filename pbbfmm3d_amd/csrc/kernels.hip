@@ -811,22 +811,6 @@ __global__ __launch_bounds__(64) void near_fp64(NearArgs a)
 //             in LDS while the next chunk is loaded.  Passes of up to 16
 //             targets evaluate each K(t, s) once into LDS; wider passes
 //             evaluate it per lane (at most 2 lanes per target).
-template <int N>
-__device__ __forceinline__ void sn_eval_n(const double *__restrict__ Tkz, double u, double *S)
-{
-    const double pfac = 2.0 / N;
-#pragma unroll
-    for (int m = 0; m < N; m++) {
-        double d0 = 0, d1 = 0;
-#pragma unroll
-        for (int j = N - 1; j > 0; j--) {
-            const double d2 = d0;
-            d0 = 2.0 * u * d0 - d1 + Tkz[m * N + j];
-            d1 = d2;
-        }
-        S[m] = pfac * (u * d0 - d1 + 0.5 * Tkz[m * N]);
-    }
-}
 
 constexpr int L2P_CC = 8;    // columns of F staged per step
 constexpr int L2P_TGT = 32;  // targets per pass

@@ -97,6 +97,30 @@ struct DownArgs {
     int f32;
 };
 
+// Proxy passes of the levels whose operator ranks are at most 32 (mixed
+// modes): P2M straight to the proxies P = VT' S, and L2P straight from the
+// M2L results Pf, without the n^3 expansions S and F of the level.  Range k
+// holds the boxes whose first point lies in [k*PX_PT, (k+1)*PX_PT);
+// persistent workgroups walk the ranges in passes of PX_PT points.
+constexpr int PX_PT = 128;
+constexpr int PX_K = 32;       // ranks held per point (zero-padded)
+struct ProxyArgs {
+    int ncols, nb, accumulate;
+    int rank;                  // ks (P2M) or kf (L2P), <= PX_K
+    int64_t npts;              // points of the tree (pstart[nb])
+    int ldp, ldf;              // P and Pf strides per (box, column)
+    int pf_vec;                // L2P: Pf rows 16-byte aligned (ldf = PX_K)
+    int64_t off;               // global box offset of the level
+    double len, alpha;         // box length; L2P scale (homogeneous kernels)
+    const int32_t *pstart, *coord;
+    const double *ctab, *Tkz, *x, *y, *z;
+    const double *wsT;         // P2M: point-major sorted weights
+    const double *op;          // P2M: VT' (rank x n^3), L2P: U' (n^3 x rank), column-major
+    float *P;                  // P2M: proxies, rows 0..PX_K-1 written
+    const float *Pf;           // L2P: M2L results
+    double *lp;                // L2P: point-major results
+};
+
 struct GemmArgs {
     int M, K, ldb, ldc;
     int b_f32, c_f32;          // B / C element type: 0 FP64, 1 FP32
@@ -131,6 +155,26 @@ __device__ __forceinline__ void m2l_step(int s, int nd, int nch, int chunk_major
     } else {
         di = s / nch;
         ch = s - di * nch;
+    }
+}
+
+// S_m(u), m = 0..N-1: the Chebyshev interpolation weights of a point at
+// u in [-1, 1] of its box (ComputeSn + ClenshawSum, H2_3D_Tree.cpp:830-879),
+// from Tkz[m*N + j] = T_j(node_m).
+template <int N>
+__device__ __forceinline__ void sn_eval_n(const double *__restrict__ Tkz, double u, double *S)
+{
+    const double pfac = 2.0 / N;
+#pragma unroll
+    for (int m = 0; m < N; m++) {
+        double d0 = 0, d1 = 0;
+#pragma unroll
+        for (int j = N - 1; j > 0; j--) {
+            const double d2 = d0;
+            d0 = 2.0 * u * d0 - d1 + Tkz[m * N + j];
+            d1 = d2;
+        }
+        S[m] = pfac * (u * d0 - d1 + 0.5 * Tkz[m * N]);
     }
 }
 
@@ -242,6 +286,10 @@ bool launch_near_split(const NearArgs &a, int kid, int nleaf, hipStream_t st);
 bool launch_p2m_levels(const UpArgs &a, int nitems, const UpReduce &r, hipStream_t st);
 // direct downward pass, one level: lp (+)= L2P of the level's F at its points
 bool launch_l2p_level(const DownArgs &a, int nseg, hipStream_t st);
+// the proxy passes of one level (orders 4..8; false for other orders);
+// npts: points of the tree
+bool launch_p2m_proxy(const ProxyArgs &a, int n, int64_t npts, hipStream_t st);
+bool launch_l2p_proxy(const ProxyArgs &a, int n, int64_t npts, hipStream_t st);
 // the point-major copy: wsT[k*ncols + c], the same rule
 void launch_gather_weights_t(const double *w, const int32_t *perm, int64_t N, int ncols, double *wsT,
                              int64_t w_lo, int64_t w_n, hipStream_t st);

@@ -1184,5 +1184,361 @@ void launch_m2l_mixed(int variant, const M2LMixArgs &a, int ntiles, int ncolchun
     check(hipGetLastError(), "m2l_mfma_split");
 }
 
+// ---------------------------------------------------------------------------
+// Proxy passes of the levels with operator ranks of at most 32 (mixed modes):
+//   p2m_proxy  P[b][c][r] = sum_{p in b} w_p[c] y_p[r],   y_p = VT' S_n(x_p)
+//   l2p_proxy  lp[p][c] (+)= alpha sum_r z_p[r] Pf[b][c][r],   z_p = U'^T S_n(x_p)
+// By associativity these equal the GEMM route (S_b = sum_p S_n(x_p) w_p^T,
+// P_b = VT' S_b; F_b = U' Pf_b, value = S_n(x_p)^T F_b), without the n^3 x
+// ncols expansions S and F of the level's boxes, which at a few points per
+// box are several times the points' own data.
+// Range k holds the boxes whose first point lies in [k*PX_PT, (k+1)*PX_PT);
+// a workgroup stages the operator once and walks ranges k = blockIdx.x,
+// + gridDim.x, ..., each in passes of PX_PT points.
+// The rank vectors of a pass (PX_PT points x 32 ranks, K = n^3) run on the
+// BF16 matrix cores in three parts per operand (6 products, FP32
+// accumulation), as the expansion route's proxy and expansion products do:
+// A = S_n of 16 points (products of the per-axis weights, split in
+// registers), B = the operator, split once into LDS as [part][rank][k].
+// The box sums (P2M) and the Pf products (L2P) run in FP64 on the VALU.
+constexpr int PX_VS = PX_K + 4;  // LDS stride of a point's rank vector (16-byte aligned rows)
+constexpr int PX_HALF = 64;      // points whose S_n tables are staged at once (4 waves x 16)
+
+template <int N>
+struct PxDims {
+    static constexpr int N3 = N * N * N, KC = (N3 + 31) / 32, KP = 32 * KC;
+    static constexpr int KS = KP + 8;  // row stride of the split operator (bf16): conflict-free 16-byte reads
+    static constexpr int TS = N * N + 1, ZS = N + 1;
+};
+
+// The boxes [bA, bB) of range k: bA, the first box with pstart >= k*PX_PT,
+// by a 256-way search (every thread probes one position per round, a
+// handful of rounds), then bB - bA, the boxes that start before
+// (k+1)*PX_PT (at most PX_PT), counted in one more round.  Block-uniform.
+__device__ __forceinline__ void px_range(const ProxyArgs &a, int64_t k, int &bA, int &bB)
+{
+    const int64_t t0 = k * PX_PT, t1 = t0 + PX_PT;
+    int lo = 0, hi = a.nb;  // bA lies in [lo, hi]; pstart[nb] = N
+    while (lo < hi) {
+        const int step = (hi - lo + 255) / 256;
+        const int64_t q = (int64_t)lo + (int64_t)threadIdx.x * step;
+        const int c = __syncthreads_count(q < hi && (int64_t)a.pstart[q] < t0);
+        const int nlo = c > 0 ? lo + (c - 1) * step + 1 : lo;
+        const int64_t qc = (int64_t)lo + (int64_t)c * step;  // the first probe at or past t0, if any
+        hi = c > 0 ? (qc < hi ? (int)qc : hi) : lo;
+        lo = nlo;
+    }
+    bA = lo;
+    const int b = bA + (int)threadIdx.x;
+    bB = bA + __syncthreads_count(threadIdx.x < PX_PT && b < a.nb && (int64_t)a.pstart[b] < t1);
+}
+
+// the operator split into three BF16 parts, Ob[p][r][k] (ranks past a.rank
+// and k past n^3 zero); n3_major: the operator is n^3 x rank (U'), else
+// rank x n^3 (VT'), both column-major
+template <int N>
+__device__ __forceinline__ void px_stage_op(const ProxyArgs &a, bool n3_major, __bf16 *Ob)
+{
+    typedef PxDims<N> D;
+    for (int q = threadIdx.x; q < PX_K * D::KP; q += blockDim.x) {
+        const int r = q / D::KP, k = q - r * D::KP;
+        double x = 0.0;
+        if (r < a.rank && k < D::N3) x = n3_major ? a.op[k + (int64_t)r * D::N3] : a.op[r + (int64_t)k * a.rank];
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            const __bf16 h = (__bf16)(float)x;
+            Ob[(p * PX_K + r) * D::KS + k] = h;
+            x -= (double)(float)h;
+        }
+    }
+}
+
+// The boxes of a pass: s_ps[j] = first point of box jb + j (j <= PX_PT, the
+// entry past the range's last box is its end); returns how many start
+// before sub_end.  Includes the barrier after which s_ps may be read.
+__device__ __forceinline__ int px_pass_boxes(const ProxyArgs &a, int *s_ps, int jb, int bB, int sub_end)
+{
+    const int tid = threadIdx.x;
+    int pred = 0;
+    if (tid <= PX_PT) {
+        const int v = a.pstart[min(jb + tid, bB)];
+        s_ps[tid] = v;
+        pred = tid < PX_PT && jb + tid < bB && v < sub_end;
+    }
+    return __syncthreads_count(pred);
+}
+
+// Points [base, base + np) of a pass, whose boxes are jb + 0..nbx-1 with
+// first points s_ps[0..nbx-1]: vs[pp][r] = sum_i Ob[i][r] S_n,i(x_{base+pp}).
+// Per half of PX_HALF points: one thread per point evaluates S_n along the
+// three axes (FP64 Clenshaw) into Sxy[pp][l1*N + l2] and Sz[pp][l3]; then
+// wave w forms the 16 x 32 block of points 16w..16w+15 on the matrix cores.
+// Begins and ends with a barrier-separated use of tab (which the caller may
+// reuse after the final barrier).
+template <int N>
+__device__ __forceinline__ void px_point_vec(const ProxyArgs &a, const __bf16 *Ob, const int *s_ps, int nbx, int jb,
+                                             int base, int np, float *vs, float *tab)
+{
+    typedef PxDims<N> D;
+    typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+    float *Sxy = tab, *Sz = tab + PX_HALF * D::TS;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int pl = lane & 15, g = lane >> 4;
+    for (int h0 = 0; h0 < PX_PT; h0 += PX_HALF) {
+        if (h0 >= np) break;  // block-uniform
+        if (tid < PX_HALF) {
+            const int pp = h0 + tid;
+            if (pp < np) {
+                const int q = base + pp;
+                int lo = 0, hi = nbx - 1;  // the last box of the pass that starts at or before q
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_ps[mid] <= q) lo = mid;
+                    else hi = mid - 1;
+                }
+                const uint32_t cc = (uint32_t)a.coord[jb + lo];
+                const double ihalfL = 2.0 / a.len;
+                double Sx[N], Sy[N], Szd[N];
+                sn_eval_n<N>(a.Tkz, ihalfL * (a.x[q] - a.ctab[cc & 1023]), Sx);
+                sn_eval_n<N>(a.Tkz, ihalfL * (a.y[q] - a.ctab[(cc >> 10) & 1023]), Sy);
+                sn_eval_n<N>(a.Tkz, ihalfL * (a.z[q] - a.ctab[(cc >> 20) & 1023]), Szd);
+#pragma unroll
+                for (int l1 = 0; l1 < N; l1++)
+#pragma unroll
+                    for (int l2 = 0; l2 < N; l2++) Sxy[tid * D::TS + l1 * N + l2] = (float)(Sx[l1] * Sy[l2]);
+#pragma unroll
+                for (int m = 0; m < N; m++) Sz[tid * D::ZS + m] = (float)Szd[m];
+            } else {  // rows of points past the pass: zeros (their results are not read)
+                for (int m = 0; m < N * N; m++) Sxy[tid * D::TS + m] = 0.0f;
+                for (int m = 0; m < N; m++) Sz[tid * D::ZS + m] = 0.0f;
+            }
+        }
+        __syncthreads();
+        const int pt = 16 * wave + pl;  // this lane's point (row of A) within the half
+        f32x4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};
+#pragma unroll 1
+        for (int kc = 0; kc < D::KC; kc++) {
+            bf16x8_t a0, a1, a2;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int k = 32 * kc + 8 * g + e;
+                float v = 0.0f;
+                if (k < D::N3) {
+                    const int l12 = k / N, l3 = k - l12 * N;
+                    v = Sxy[pt * D::TS + l12] * Sz[pt * D::ZS + l3];
+                }
+                const __bf16 p0 = (__bf16)v;
+                v -= (float)p0;
+                const __bf16 p1 = (__bf16)v;
+                v -= (float)p1;
+                a0[e] = p0;
+                a1[e] = p1;
+                a2[e] = (__bf16)v;
+            }
+            const __bf16 *ob = Ob + (size_t)pl * D::KS + 32 * kc + 8 * g;
+#pragma unroll
+            for (int rb = 0; rb < 2; rb++) {
+                const bf16x8_t b0 = *(const bf16x8_t *)(ob + (0 * PX_K + 16 * rb) * D::KS);
+                const bf16x8_t b1 = *(const bf16x8_t *)(ob + (1 * PX_K + 16 * rb) * D::KS);
+                const bf16x8_t b2 = *(const bf16x8_t *)(ob + (2 * PX_K + 16 * rb) * D::KS);
+                f32x4 d = rb ? d1 : d0;
+                d = mfma16(a2, b0, d);  // smallest terms first
+                d = mfma16(a1, b1, d);
+                d = mfma16(a0, b2, d);
+                d = mfma16(a1, b0, d);
+                d = mfma16(a0, b1, d);
+                d = mfma16(a0, b0, d);
+                if (rb) d1 = d;
+                else d0 = d;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            float *row = vs + (size_t)(h0 + 16 * wave + 4 * g + v) * PX_VS;
+            row[pl] = d0[v];
+            row[16 + pl] = d1[v];
+        }
+        __syncthreads();  // tab is free again
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void p2m_proxy(ProxyArgs a)
+{
+    typedef PxDims<N> D;
+    constexpr int SWS = PX_K + 1;
+    constexpr int TAB = PX_HALF * (D::TS + D::ZS), UNI = TAB > PX_PT * SWS ? TAB : PX_PT * SWS;
+    __shared__ __attribute__((aligned(16))) __bf16 Ob[3 * PX_K * D::KS];
+    __shared__ __attribute__((aligned(16))) float vs[PX_PT * PX_VS];
+    __shared__ float uni[UNI];  // S_n tables of the rank-vector phase, then the pass's weights
+    __shared__ int s_ps[PX_PT + 1];
+    float *sw = uni;
+    const int tid = threadIdx.x;
+    px_stage_op<N>(a, false, Ob);
+    const int64_t nranges = (a.npts + PX_PT - 1) / PX_PT;
+    const int c = tid >> 3, rq = tid & 7;  // output phase: column c, ranks 4rq..4rq+3
+    for (int64_t k = blockIdx.x; k < nranges; k += gridDim.x) {
+        int bA, bB;
+        px_range(a, k, bA, bB);
+        if (bA >= bB) continue;  // no box starts in this range
+        const int p_begin = a.pstart[bA], p_end = a.pstart[bB];
+        for (int c0 = 0; c0 < a.ncols; c0 += PX_K) {
+            const int cn = min(PX_K, a.ncols - c0);
+            int jb = bA;
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};  // the current box's sums (FP64), carried across passes
+            for (int base = p_begin; base < p_end; base += PX_PT) {
+                const int sub_end = min(base + PX_PT, p_end), np = sub_end - base;
+                __syncthreads();  // the previous pass's readers are done; Ob is staged
+                const int nbx = px_pass_boxes(a, s_ps, jb, bB, sub_end);
+                px_point_vec<N>(a, Ob, s_ps, nbx, jb, base, np, vs, uni);
+                for (int q = tid; q < PX_PT * PX_K; q += 256) {
+                    const int pp = q / PX_K, cl = q - pp * PX_K;
+                    sw[pp * SWS + cl] =
+                        pp < np && cl < cn ? (float)a.wsT[(int64_t)(base + pp) * a.ncols + c0 + cl] : 0.0f;
+                }
+                __syncthreads();
+                int done = 0;
+                for (int j = 0; j < nbx; j++) {
+                    const int lo = max(s_ps[j], base) - base, hi = min(s_ps[j + 1], sub_end) - base;
+#pragma unroll 4
+                    for (int qq = lo; qq < hi; qq++) {
+                        const double w = sw[qq * SWS + c];
+                        const float4 y = *(const float4 *)&vs[qq * PX_VS + 4 * rq];
+                        acc[0] += w * y.x;
+                        acc[1] += w * y.y;
+                        acc[2] += w * y.z;
+                        acc[3] += w * y.w;
+                    }
+                    if (s_ps[j + 1] <= sub_end) {  // the box ends in this pass
+                        if (c < cn)
+                            *(float4 *)&a.P[((a.off + jb + j) * a.ncols + c0 + c) * a.ldp + 4 * rq] =
+                                float4{(float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]};
+                        acc[0] = acc[1] = acc[2] = acc[3] = 0.0;
+                        done++;
+                    }
+                }
+                jb += done;
+            }
+        }
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void l2p_proxy(ProxyArgs a)
+{
+    typedef PxDims<N> D;
+    __shared__ __attribute__((aligned(16))) __bf16 Ob[3 * PX_K * D::KS];
+    __shared__ __attribute__((aligned(16))) float vs[PX_PT * PX_VS];
+    __shared__ float tab[PX_HALF * (D::TS + D::ZS)];
+    __shared__ int s_ps[PX_PT + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    px_stage_op<N>(a, true, Ob);
+    const int64_t nranges = (a.npts + PX_PT - 1) / PX_PT;
+    const int cl = lane >> 1, hf = lane & 1;  // output phase: column cl, ranks 16hf..16hf+15
+    for (int64_t k = blockIdx.x; k < nranges; k += gridDim.x) {
+        int bA, bB;
+        px_range(a, k, bA, bB);
+        if (bA >= bB) continue;
+        const int p_begin = a.pstart[bA], p_end = a.pstart[bB];
+        for (int c0 = 0; c0 < a.ncols; c0 += PX_K) {
+            const int cn = min(PX_K, a.ncols - c0);
+            int jb = bA;
+            for (int base = p_begin; base < p_end; base += PX_PT) {
+                const int sub_end = min(base + PX_PT, p_end), np = sub_end - base;
+                __syncthreads();
+                const int nbx = px_pass_boxes(a, s_ps, jb, bB, sub_end);
+                px_point_vec<N>(a, Ob, s_ps, nbx, jb, base, np, vs, tab);
+                // one wave per box: lane (cl, hf) holds half of the box's Pf row cl;
+                // the next box's row is loaded while this box's points run.
+                // Each output element gets one addition per launch, so the
+                // no-return atomic adds leave the sums deterministic.
+                auto load_row = [&](int j, float (&pr)[16]) __attribute__((always_inline)) {
+                    const int jj = min(j, nbx - 1);
+                    const float *row = a.Pf + ((a.off + jb + jj) * a.ncols + c0 + min(cl, cn - 1)) * a.ldf + 16 * hf;
+                    if (a.pf_vec) {
+#pragma unroll
+                        for (int k4 = 0; k4 < 4; k4++) {
+                            const float4 v = ((const float4 *)row)[k4];
+                            pr[4 * k4] = v.x;
+                            pr[4 * k4 + 1] = v.y;
+                            pr[4 * k4 + 2] = v.z;
+                            pr[4 * k4 + 3] = v.w;
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 16; k++) pr[k] = row[k];
+                    }
+                };
+                float nxt[16];
+                if (wave < nbx) load_row(wave, nxt);
+                for (int j = wave; j < nbx; j += 4) {
+                    const int lo = max(s_ps[j], base) - base, hi = min(s_ps[j + 1], sub_end) - base;
+                    double pf[16];  // rows past the rank are not written by the M2L
+#pragma unroll
+                    for (int k = 0; k < 16; k++) pf[k] = 16 * hf + k < a.rank ? (double)nxt[k] : 0.0;
+                    if (j + 4 < nbx) load_row(j + 4, nxt);
+                    for (int qq = lo; qq < hi; qq++) {
+                        const float4 *zv = (const float4 *)&vs[qq * PX_VS + 16 * hf];
+                        double s = 0.0;
+#pragma unroll
+                        for (int k4 = 0; k4 < 4; k4++) {
+                            const float4 z = zv[k4];
+                            s += (double)z.x * pf[4 * k4] + (double)z.y * pf[4 * k4 + 1] + (double)z.z * pf[4 * k4 + 2] +
+                                 (double)z.w * pf[4 * k4 + 3];
+                        }
+                        s += __shfl_xor(s, 1);
+                        if (hf == 0 && cl < cn) {
+                            double *dst = a.lp + (int64_t)(base + qq) * a.ncols + c0 + cl;
+                            if (a.accumulate) unsafeAtomicAdd(dst, a.alpha * s);
+                            else *dst = a.alpha * s;
+                        }
+                    }
+                }
+                jb += nbx - (s_ps[nbx] > sub_end ? 1 : 0);  // boxes that end in this pass
+            }
+        }
+    }
+}
+
+static unsigned proxy_grid(const ProxyArgs &a, int64_t npts)
+{
+    if (a.rank < 1 || a.rank > PX_K) fail(PBB_ERR_INTERNAL, "proxy pass rank out of range");
+    if (npts <= 0 || npts != a.npts) fail(PBB_ERR_INTERNAL, "proxy pass size");
+    return (unsigned)std::min<int64_t>((npts + PX_PT - 1) / PX_PT, 1024);  // persistent workgroups
+}
+
+bool launch_p2m_proxy(const ProxyArgs &a, int n, int64_t npts, hipStream_t st)
+{
+    if (n < 4 || n > 8) return false;
+    const unsigned g = proxy_grid(a, npts);
+    if (a.ldp < PX_K || a.ldp % 4) fail(PBB_ERR_INTERNAL, "proxy stride must hold 32 rows in 16-byte units");
+    switch (n) {
+    case 4: p2m_proxy<4><<<g, 256, 0, st>>>(a); break;
+    case 5: p2m_proxy<5><<<g, 256, 0, st>>>(a); break;
+    case 6: p2m_proxy<6><<<g, 256, 0, st>>>(a); break;
+    case 7: p2m_proxy<7><<<g, 256, 0, st>>>(a); break;
+    default: p2m_proxy<8><<<g, 256, 0, st>>>(a); break;
+    }
+    check(hipGetLastError(), "p2m_proxy");
+    return true;
+}
+
+bool launch_l2p_proxy(const ProxyArgs &a, int n, int64_t npts, hipStream_t st)
+{
+    if (n < 4 || n > 8) return false;
+    const unsigned g = proxy_grid(a, npts);
+    if (a.ldf < a.rank) fail(PBB_ERR_INTERNAL, "M2L result stride below the rank");
+    if (a.pf_vec && a.ldf % 4) fail(PBB_ERR_INTERNAL, "whole-row loads need a stride of 4 floats");
+    switch (n) {
+    case 4: l2p_proxy<4><<<g, 256, 0, st>>>(a); break;
+    case 5: l2p_proxy<5><<<g, 256, 0, st>>>(a); break;
+    case 6: l2p_proxy<6><<<g, 256, 0, st>>>(a); break;
+    case 7: l2p_proxy<7><<<g, 256, 0, st>>>(a); break;
+    default: l2p_proxy<8><<<g, 256, 0, st>>>(a); break;
+    }
+    check(hipGetLastError(), "l2p_proxy");
+    return true;
+}
+
 }  // namespace dev
 }  // namespace pbb

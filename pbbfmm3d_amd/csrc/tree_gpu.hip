@@ -578,6 +578,8 @@ void build_tree_gpu(GpuTree &T, const double *xyz_host, int64_t N, int64_t nbox,
         C.nslot = exclusive_count(tmp, C.slot, C.sslot, V.nb, st);
         C.nred = exclusive_count(tmp, C.red, C.sred, V.nb, st);
         base[l] = LevelBase{l - 2, V.off, items_total, slots_total, reds_total};
+        V.up_item0 = items_total;
+        V.up_red0 = reds_total;
         items_total += C.nitem;
         slots_total += C.nslot;
         reds_total += C.nred;

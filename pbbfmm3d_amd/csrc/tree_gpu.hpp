@@ -16,6 +16,7 @@ struct GpuLevel {
     int32_t *coord = nullptr, *map = nullptr, *child = nullptr, *order = nullptr;
     int32_t *pstart = nullptr, *seg_box = nullptr, *seg_p0 = nullptr;
     int nseg = 0;
+    int up_item0 = 0, up_red0 = 0;  // the level's first direct upward work item and reduction
     int64_t class_count[8] = {0};
 };
 
