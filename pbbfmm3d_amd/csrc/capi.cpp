@@ -1711,8 +1711,13 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         return W.Pf32 + (size_t)p->lv[l].off * ncols * (h.rf - dev::PX_K);
     };
     const PointIo io = point_io(p, o);
-    dev::launch_gather_weights(w, p->perm, p->N, ncols, W.ws, io.in_lo, io.in_n, st);
-    if (direct) dev::launch_gather_weights_t(w, p->perm, p->N, ncols, W.wsT, io.in_lo, io.in_n, st);
+    // the direct passes and the split near field read only the point-major
+    // sorted weights wsT; W.ws (column-major sorted) serves the other paths
+    // and, on the direct path, holds the transposed input on the way to wsT
+    const bool ws_needed = !direct_down;  // (the split near field covers the direct orders 4..8)
+    if (ws_needed) dev::launch_gather_weights(w, p->perm, p->N, ncols, W.ws, io.in_lo, io.in_n, st);
+    if (direct && ws_needed) dev::launch_gather_weights_t(w, p->perm, p->N, ncols, W.wsT, io.in_lo, io.in_n, st);
+    else if (direct) dev::launch_gather_rows_t(w, p->perm, p->N, ncols, W.ws, W.wsT, io.in_lo, io.in_n, st);
     const int cc_p2m = std::max(1, std::min({ncols, 64, 4096 / n3}));
     const int cc_m2m = std::max(1, std::min(ncols, 2048 / n3));
     if (Lv >= 2 && direct) {
@@ -1777,6 +1782,10 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
             x.pstart = D.pstart; x.coord = D.coord; x.ctab = D.ctab; x.Tkz = o->d_Tkz;
             x.x = p->x; x.y = p->y; x.z = p->z; x.wsT = W.wsT;
             x.op = o->dm_VTw + mset(l).vt_off; x.P = W.P32; x.npts = p->N;
+            if (prec == PBB_PREC_MIXED_F16X2) {  // the proxy pass also finds the level's max |P|
+                dev::launch_zero_u32(W.xmax + l, 1, st);
+                x.xmax = W.xmax + l;
+            }
             dev::launch_p2m_proxy(x, n, p->N, st);
         } else if (SA) {  // three-part BF16 matrix cores (P is FP32 in the mixed modes)
             dev::GemmSplitArgs q{};
@@ -1792,7 +1801,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
             // (rows past ks come from zero rows of the split operator)
             const int used = 32 * SE->nch[sl];
             if (used > ldP) fail(PBB_ERR_INTERNAL, "proxy chunks past the padding");
-            if (prec == PBB_PREC_MIXED_F16X2) {
+            if (prec == PBB_PREC_MIXED_F16X2 && l < Lp) {
                 dev::launch_zero_u32(W.xmax + l, 1, st);
                 dev::launch_absmax_rows(W.P32 + (size_t)D.off * ncols * ldP, D.nb * ncols, ldP, used, W.xmax + l, st);
             }

@@ -129,6 +129,37 @@ __global__ void gather_weights_t(const double *__restrict__ w, const int32_t *__
     }
 }
 
+// The same wsT in two coalesced steps: wT[j*ncols + c] = w[c*w_n + j] (a
+// transpose through LDS tiles of 32 points x 32 columns), then whole rows
+// wsT[k] = wT[perm[k] - w_lo] (zero rows for points outside the range).
+__global__ __launch_bounds__(256) void transpose_weights(const double *__restrict__ w, int64_t w_n, int ncols,
+                                                         double *__restrict__ wT)
+{
+    __shared__ double tile[32][33];
+    const int64_t j0 = (int64_t)blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows of 32 lanes
+    for (int c0 = 0; c0 < ncols; c0 += 32) {
+        __syncthreads();
+        for (int c = ty; c < 32; c += 8)
+            tile[c][tx] = c0 + c < ncols && j0 + tx < w_n ? w[(int64_t)(c0 + c) * w_n + j0 + tx] : 0.0;
+        __syncthreads();
+        for (int j = ty; j < 32; j += 8)
+            if (j0 + j < w_n && c0 + tx < ncols) wT[(j0 + j) * ncols + c0 + tx] = tile[tx][j];
+    }
+}
+
+__global__ void gather_rows(const double *__restrict__ wT, const int32_t *__restrict__ perm, int64_t N, int ncols,
+                            double *__restrict__ wsT, int64_t w_lo, int64_t w_n)
+{
+    const int64_t total = N * ncols;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = q / ncols, c = q - k * ncols;
+        const int64_t j = perm[k] - w_lo;
+        wsT[q] = (uint64_t)j < (uint64_t)w_n ? wT[j * ncols + c] : 0.0;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // P2M, compute.hpp:287-323: one workgroup per non-empty leaf.
 // S[(g*ncols + c)*n3 + (l1 n + l2) n + l3] = sum_j w_jc Sx_l1(j) Sy_l2(j) Sz_l3(j)
@@ -1643,6 +1674,19 @@ void launch_gather_weights_t(const double *w, const int32_t *perm, int64_t N, in
     int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
     gather_weights_t<<<blocks, 256, 0, st>>>(w, perm, N, ncols, wsT, w_lo, w_n);
     check(hipGetLastError(), "gather_weights_t");
+}
+
+void launch_gather_rows_t(const double *w, const int32_t *perm, int64_t N, int ncols, double *wT, double *wsT,
+                          int64_t w_lo, int64_t w_n, hipStream_t st)
+{
+    if (N * ncols <= 0 || w_n <= 0) return;
+    const int64_t tb = (w_n + 31) / 32;
+    if (tb > INT32_MAX) fail(PBB_ERR_UNSUPPORTED, "too many points for the weight transpose");
+    transpose_weights<<<(unsigned)tb, 256, 0, st>>>(w, w_n, ncols, wT);
+    check(hipGetLastError(), "transpose_weights");
+    const int blocks = (int)std::min<int64_t>((N * ncols + 255) / 256, 65536);
+    gather_rows<<<blocks, 256, 0, st>>>(wT, perm, N, ncols, wsT, w_lo, w_n);
+    check(hipGetLastError(), "gather_rows");
 }
 
 void launch_p2m(const P2MArgs &a, int nleaf, hipStream_t st)

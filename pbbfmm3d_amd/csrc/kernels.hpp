@@ -117,6 +117,7 @@ struct ProxyArgs {
     const double *wsT;         // P2M: point-major sorted weights
     const double *op;          // P2M: VT' (rank x n^3), L2P: U' (n^3 x rank), column-major
     float *P;                  // P2M: proxies, rows 0..PX_K-1 written
+    unsigned *xmax;            // P2M: atomic max of |P| as FP32 bits (cleared by the caller), or nullptr
     const float *Pf;           // L2P: M2L results
     double *lp;                // L2P: point-major results
 };
@@ -291,6 +292,10 @@ bool launch_l2p_level(const DownArgs &a, int nseg, hipStream_t st);
 bool launch_p2m_proxy(const ProxyArgs &a, int n, int64_t npts, hipStream_t st);
 bool launch_l2p_proxy(const ProxyArgs &a, int n, int64_t npts, hipStream_t st);
 // the point-major copy: wsT[k*ncols + c], the same rule
+// wsT through a point-major copy wT of the caller's weights (w_n x ncols
+// doubles of scratch): a coalesced transpose, then whole-row gathers
+void launch_gather_rows_t(const double *w, const int32_t *perm, int64_t N, int ncols, double *wT, double *wsT,
+                          int64_t w_lo, int64_t w_n, hipStream_t st);
 void launch_gather_weights_t(const double *w, const int32_t *perm, int64_t N, int ncols, double *wsT,
                              int64_t w_lo, int64_t w_n, hipStream_t st);
 void launch_direct(const DirectArgs &a, int kid, hipStream_t st);

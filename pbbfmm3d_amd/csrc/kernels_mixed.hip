@@ -1377,6 +1377,7 @@ __global__ __launch_bounds__(256) void p2m_proxy(ProxyArgs a)
     px_stage_op<N>(a, false, Ob);
     const int64_t nranges = (a.npts + PX_PT - 1) / PX_PT;
     const int c = tid >> 3, rq = tid & 7;  // output phase: column c, ranks 4rq..4rq+3
+    float pmax = 0.0f;                     // max |P| stored by this thread
     for (int64_t k = blockIdx.x; k < nranges; k += gridDim.x) {
         int bA, bB;
         px_range(a, k, bA, bB);
@@ -1410,9 +1411,11 @@ __global__ __launch_bounds__(256) void p2m_proxy(ProxyArgs a)
                         acc[3] += w * y.w;
                     }
                     if (s_ps[j + 1] <= sub_end) {  // the box ends in this pass
-                        if (c < cn)
-                            *(float4 *)&a.P[((a.off + jb + j) * a.ncols + c0 + c) * a.ldp + 4 * rq] =
-                                float4{(float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]};
+                        if (c < cn) {
+                            const float4 v = float4{(float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]};
+                            *(float4 *)&a.P[((a.off + jb + j) * a.ncols + c0 + c) * a.ldp + 4 * rq] = v;
+                            pmax = fmaxf(pmax, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+                        }
                         acc[0] = acc[1] = acc[2] = acc[3] = 0.0;
                         done++;
                     }
@@ -1420,6 +1423,10 @@ __global__ __launch_bounds__(256) void p2m_proxy(ProxyArgs a)
                 jb += done;
             }
         }
+    }
+    if (a.xmax) {  // the level's scale for the two-part FP16 proxies
+        for (int o = 32; o > 0; o >>= 1) pmax = fmaxf(pmax, __shfl_xor(pmax, o));
+        if ((tid & 63) == 0) atomicMax(a.xmax, __float_as_uint(pmax));
     }
 }
 
