@@ -1710,6 +1710,13 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         ldf = dev::PX_K;
         return W.Pf32 + (size_t)p->lv[l].off * ncols * (h.rf - dev::PX_K);
     };
+    // With proxy levels, the expansion levels 2..Le (Le = Lp - 1) take the
+    // reference's structure: P2M straight from the points at Le only, M2M up
+    // to level 2, L2L back down to Le, L2P at Le only.  The transfers are
+    // exact (the direct passes' rewriting in reverse) and cost a few boxes'
+    // worth of work instead of every point at every level.
+    const bool hyb = direct_down && Lp <= Lv;
+    const int Le = hyb ? Lp - 1 : Lv;
     const PointIo io = point_io(p, o);
     // the direct passes and the split near field read only the point-major
     // sorted weights wsT; W.ws (column-major sorted) serves the other paths
@@ -1727,11 +1734,17 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         a.items = p->up_items;
         a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.wsT = W.wsT; a.S = W.S;
         a.scratch = W.up; a.s32 = s32;
+        // the work items and reductions are in level order: levels 2..Lp-1,
+        // or only Le with the M2M above it
+        const int i1 = Lp <= Lv ? p->lv[Lp].up_item0 : p->n_up_items;
+        const int r1 = Lp <= Lv ? p->lv[Lp].up_red0 : p->n_up_reds;
+        const int i0 = hyb ? (Le >= 2 ? p->lv[Le].up_item0 : i1) : 0;
+        const int r0 = hyb ? (Le >= 2 ? p->lv[Le].up_red0 : r1) : 0;
+        a.items = p->up_items + i0;
         dev::UpReduce r{};
-        r.ncols = ncols; r.n3 = n3; r.reds = p->up_reds;
-        r.nred = Lp <= Lv ? p->lv[Lp].up_red0 : p->n_up_reds;  // the work items are in level order
+        r.ncols = ncols; r.n3 = n3; r.reds = p->up_reds + r0; r.nred = r1 - r0;
         r.scratch = W.up; r.S = W.S; r.s32 = s32;
-        dev::launch_p2m_levels(a, Lp <= Lv ? p->lv[Lp].up_item0 : p->n_up_items, r, st);
+        dev::launch_p2m_levels(a, i1 - i0, r, st);
     } else if (Lv >= 2) {
         const DevLevel &LL = p->lv[Lv];
         dev::P2MArgs a{};
@@ -1741,11 +1754,11 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         dev::launch_p2m(a, (int)LL.nb, st);
     }
     mark(1);
-    for (int l = Lv - 1; l >= 2 && !direct; l--) {
+    for (int l = (direct ? Le : Lv) - 1; l >= 2 && (!direct || hyb); l--) {
         dev::M2MArgs a{};
         a.n = n; a.ncols = ncols; a.cc = cc_m2m;
         a.off_parent = p->lv[l].off; a.off_child = p->lv[l + 1].off;
-        a.child = p->lv[l].child; a.Cw = o->d_Cw; a.Sparent = W.S; a.Schild = W.S;
+        a.child = p->lv[l].child; a.Cw = o->d_Cw; a.Sparent = W.S; a.Schild = W.S; a.s32 = s32;
         dev::launch_m2m(a, (int)p->lv[l].nb, st);
     }
     mark(2);
@@ -1928,8 +1941,16 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
     mark(5);
     // downward pass, top-down: F_l += L2L(F_{l-1}), F_{l-1} already complete;
     // the direct pass evaluates every level's F at the points instead
+    for (int l = 3; l <= Le && hyb; l++) {  // F_l += L2L(F_{l-1}) down to Le
+        dev::M2MArgs a{};
+        a.n = n; a.ncols = ncols; a.cc = cc_m2m;
+        a.off_parent = p->lv[l - 1].off; a.off_child = p->lv[l].off;
+        a.child = p->lv[l - 1].child; a.Cw = o->d_Cw; a.Sparent = W.S; a.Schild = W.S; a.s32 = s32;
+        dev::launch_l2l(a, (int)p->lv[l - 1].nb, st);
+    }
     for (int l = 2; l <= Lv && direct_down; l++) {
         const DevLevel &D = p->lv[l];
+        if (hyb && l < Le) continue;  // reaches the points through L2L into level Le
         if (l >= Lp) {
             dev::ProxyArgs x{};
             x.ncols = ncols; x.nb = (int)D.nb; x.accumulate = l > 2; x.rank = mset(l).kf;
@@ -1944,7 +1965,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
             continue;
         }
         dev::DownArgs a{};
-        a.n = n; a.ncols = ncols; a.accumulate = l > 2; a.off = D.off; a.len = D.len;
+        a.n = n; a.ncols = ncols; a.accumulate = hyb ? 0 : l > 2; a.off = D.off; a.len = D.len;
         a.seg_box = D.seg_box; a.seg_p0 = D.seg_p0; a.pstart = D.pstart; a.coord = D.coord; a.ctab = D.ctab;
         a.Tkz = o->d_Tkz; a.x = p->x; a.y = p->y; a.z = p->z; a.F = W.S; a.lp = W.lp; a.f32 = s32;
         dev::launch_l2p_level(a, D.nseg, st);

@@ -222,6 +222,9 @@ __global__ __launch_bounds__(256) void p2m_fp64(P2MArgs a)
 // ---------------------------------------------------------------------------
 // M2M, compute.hpp:234-283 + :956-1034: one workgroup per parent box; the
 // children's expansions are contracted mode by mode (z, y, x) in LDS.
+// TS: the storage type of the expansions (FP64, or FP32 in the mixed modes);
+// the transfers run in FP64 either way.
+template <typename TS>
 __global__ __launch_bounds__(256) void m2m_fp64(M2MArgs a)
 {
     extern __shared__ double lds[];
@@ -230,7 +233,7 @@ __global__ __launch_bounds__(256) void m2m_fp64(M2MArgs a)
     double *A = lds, *B = lds + CC * n3, *Cw = lds + 2 * CC * n3;
     for (int q = tid; q < 2 * n2; q += 256) Cw[q] = a.Cw[q];
     const int b = blockIdx.x;
-    double *Sp = a.Sparent + (size_t)(a.off_parent + b) * a.ncols * n3;
+    TS *Sp = (TS *)a.Sparent + (size_t)(a.off_parent + b) * a.ncols * n3;
     for (int c0 = 0; c0 < a.ncols; c0 += CC) {
         const int cn = min(CC, a.ncols - c0);
         const int tot = cn * n3;
@@ -240,7 +243,7 @@ __global__ __launch_bounds__(256) void m2m_fp64(M2MArgs a)
         for (int o = 0; o < 8; o++) {
             const int ch = a.child[(int64_t)b * 8 + o];
             if (ch < 0) continue;
-            const double *Sc = a.Schild + ((size_t)(a.off_child + ch) * a.ncols + c0) * n3;
+            const TS *Sc = (const TS *)a.Schild + ((size_t)(a.off_child + ch) * a.ncols + c0) * n3;
             const double *Cx = Cw + ((o >> 2) & 1) * n2, *Cy = Cw + ((o >> 1) & 1) * n2, *Cz = Cw + (o & 1) * n2;
             __syncthreads();
             for (int q = tid; q < tot; q += 256) A[q] = Sc[q];
@@ -276,7 +279,7 @@ __global__ __launch_bounds__(256) void m2m_fp64(M2MArgs a)
 #pragma unroll
         for (int r = 0; r < P2M_R; r++) {
             int q = tid + 256 * r;
-            if (q < tot) Sp[(size_t)c0 * n3 + q] = acc[r];
+            if (q < tot) Sp[(size_t)c0 * n3 + q] = (TS)acc[r];
         }
         __syncthreads();
     }
@@ -285,6 +288,7 @@ __global__ __launch_bounds__(256) void m2m_fp64(M2MArgs a)
 // ---------------------------------------------------------------------------
 // L2L, compute.hpp:551-587 + :798-878: one workgroup per parent box, the
 // transpose of M2M added into each non-empty child.
+template <typename TS>
 __global__ __launch_bounds__(256) void l2l_fp64(M2MArgs a)
 {
     extern __shared__ double lds[];
@@ -293,7 +297,7 @@ __global__ __launch_bounds__(256) void l2l_fp64(M2MArgs a)
     double *Pp = lds, *X = lds + CC * n3, *Y = lds + 2 * CC * n3, *Cw = lds + 3 * CC * n3;
     for (int q = tid; q < 2 * n2; q += 256) Cw[q] = a.Cw[q];
     const int b = blockIdx.x;
-    const double *Fp = a.Sparent + (size_t)(a.off_parent + b) * a.ncols * n3;
+    const TS *Fp = (const TS *)a.Sparent + (size_t)(a.off_parent + b) * a.ncols * n3;
     for (int c0 = 0; c0 < a.ncols; c0 += CC) {
         const int cn = min(CC, a.ncols - c0);
         const int tot = cn * n3;
@@ -302,7 +306,7 @@ __global__ __launch_bounds__(256) void l2l_fp64(M2MArgs a)
         for (int o = 0; o < 8; o++) {
             const int ch = a.child[(int64_t)b * 8 + o];
             if (ch < 0) continue;
-            double *Fc = a.Schild + ((size_t)(a.off_child + ch) * a.ncols + c0) * n3;
+            TS *Fc = (TS *)a.Schild + ((size_t)(a.off_child + ch) * a.ncols + c0) * n3;
             const double *Cx = Cw + ((o >> 2) & 1) * n2, *Cy = Cw + ((o >> 1) & 1) * n2, *Cz = Cw + (o & 1) * n2;
             __syncthreads();
             for (int q = tid; q < tot; q += 256) {  // x: X[l1,l2,l3] = sum_k P[k,l2,l3] Cx[k n + l1]
@@ -326,7 +330,7 @@ __global__ __launch_bounds__(256) void l2l_fp64(M2MArgs a)
                 const double *src = Y + c * n3 + ab * n;
                 double s = 0;
                 for (int k = 0; k < n; k++) s += src[k] * Cz[k * n + l3];
-                Fc[q] += s;
+                Fc[q] = (TS)((double)Fc[q] + s);
             }
         }
     }
@@ -1700,7 +1704,8 @@ void launch_m2m(const M2MArgs &a, int nparent, hipStream_t st)
 {
     if (nparent <= 0) return;
     size_t lds = sizeof(double) * (2 * (size_t)a.cc * a.n * a.n * a.n + 2 * a.n * a.n);
-    m2m_fp64<<<nparent, 256, lds, st>>>(a);
+    if (a.s32) m2m_fp64<float><<<nparent, 256, lds, st>>>(a);
+    else m2m_fp64<double><<<nparent, 256, lds, st>>>(a);
     check(hipGetLastError(), "m2m_fp64");
 }
 
@@ -1708,7 +1713,8 @@ void launch_l2l(const M2MArgs &a, int nparent, hipStream_t st)
 {
     if (nparent <= 0) return;
     size_t lds = sizeof(double) * (3 * (size_t)a.cc * a.n * a.n * a.n + 2 * a.n * a.n);
-    l2l_fp64<<<nparent, 256, lds, st>>>(a);
+    if (a.s32) l2l_fp64<float><<<nparent, 256, lds, st>>>(a);
+    else l2l_fp64<double><<<nparent, 256, lds, st>>>(a);
     check(hipGetLastError(), "l2l_fp64");
 }
 

@@ -48,6 +48,7 @@ struct M2MArgs {
     const double *Cw;
     double *Sparent;            // M2M: written; L2L: read (parent F)
     double *Schild;             // M2M: read;    L2L: accumulated
+    int s32;                    // the expansions are stored in FP32 (same offsets in floats)
 };
 
 // per-level tables of the direct upward / downward passes (levels 2..L)
