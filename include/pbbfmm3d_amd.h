@@ -209,11 +209,12 @@ PBB_API pbb_status pbb_ops_export(const pbb_ops *ops, double *U, double *VT,
  * smooth kernel's 316 M2L blocks E_d share far fewer singular directions.
  * Each set's E_d are projected on the leading left / right singular
  * vectors of all 316 blocks together (Q_L, Q_R: E'_d = Q_L^T E_d Q_R, U' =
- * U Q_L, VT' = Q_R^T VT), keeping kf (a multiple of 16) and ks (of 32)
- * directions so that the discarded singular values of the stacked blocks
- * hold at most tol of their Frobenius norm on each side; where that saves
- * nothing the set keeps its own factors.  No reference counterpart (the
- * reference has no mixed mode).  tol in [0, 1e-6], default 1e-10; 0 keeps
+ * U Q_L, VT' = Q_R^T VT), keeping kf (a multiple of 16) and ks (16 or a
+ * multiple of 32) directions so that the discarded singular values of the
+ * stacked blocks hold at most tol of their Frobenius norm on each side;
+ * where that saves nothing the set keeps its own factors.  No reference
+ * counterpart (the reference has no mixed mode).  tol in [0, 1e-6],
+ * default 2e-8 (far below the mixed modes' 1e-6 bar); 0 keeps
  * every set's own factors.  Set it before the first mixed-precision call
  * on these operators (afterwards PBB_ERR_INVALID_ARG).  The FP64 mode
  * always runs the reference's operators.                                  */

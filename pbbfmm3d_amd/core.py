@@ -219,7 +219,7 @@ class Operators:
 
     def set_mixed_tolerance(self, tol: float) -> None:
         """Re-compression tolerance of the mixed modes' operator sets
-        (pbb_ops_set_mixed_tolerance; default 1e-10, 0 keeps the sets' own
+        (pbb_ops_set_mixed_tolerance; default 2e-8, 0 keeps the sets' own
         factors); before the first mixed-precision call."""
         capi.check(capi.lib().pbb_ops_set_mixed_tolerance(self._h, float(tol)))
 

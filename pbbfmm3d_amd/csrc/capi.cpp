@@ -111,6 +111,7 @@ struct SplitE {
 // offsets of the FP64 device arrays.
 struct MixedSet {
     int kf = 0, ks = 0;
+    int ks_used = 0;  // ks before the padding to 32-deep chunks (rows / columns past it are zero)
     size_t e_off = 0, vt_off = 0, u_off = 0;
     double tail = 0;  // discarded relative Frobenius norm (larger side)
 };
@@ -124,7 +125,7 @@ struct pbb_ops {
     // The mixed modes' operator sets, each re-compressed within level_tol
     // (0: the sets' own factors); set before their first use.  fp64_reduced:
     // the FP64 mode runs them too (off: the reference's operators)
-    double level_tol = 1e-10;
+    double level_tol = 2e-8;
     bool fp64_reduced = false;
     std::vector<MixedSet> mset;
     std::vector<double> mset_emax;  // max |E'_d| per set
@@ -169,6 +170,8 @@ struct pbb_ops {
         }
         // ks padded to whole k chunks with zero columns of E' and zero rows of
         // VT'w, so that the proxy GEMM writes every row the M2L reads
+        std::vector<int> ks_used(h.nsets);
+        for (int s = 0; s < h.nsets; s++) ks_used[s] = red[s].ks;
         for (auto &r : red) {
             const int ks32 = 32 * ((r.ks + 31) / 32);
             if (ks32 == r.ks) continue;
@@ -189,6 +192,7 @@ struct pbb_ops {
             MixedSet &m = mset[s];
             m.kf = red[s].kf;
             m.ks = red[s].ks;
+            m.ks_used = ks_used[s];
             m.tail = std::max(red[s].tail_f, red[s].tail_s);
             m.e_off = E.size();
             m.vt_off = V.size();
@@ -1770,6 +1774,12 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
     // FP64 keeps a set's own factors where its padded ks would pass the
     // FP64 proxy stride r_s (a set that was not re-compressed)
     auto red = [&](int l) { return mixed || (reduced && mset(l).ks <= h.rs); };
+    // f16 x2 sets of ks <= 16 (stored padded to one 32-deep chunk): the M2L
+    // runs 16-deep steps on proxies pre-split to 16 + 16 parts
+    auto k16 = [&](int l) {
+        return prec == PBB_PREC_MIXED_F16X2 && SE && mset(l).ks_used <= 16 && mset(l).kf <= 32 &&
+               SE->nch[h.ops_set(l)] == 1;
+    };
     for (int l = 2; l <= Lv; l++) {
         const DevLevel &D = p->lv[l];
         const int sl = h.ops_set(l);
@@ -1819,8 +1829,9 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
                 dev::launch_absmax_rows(W.P32 + (size_t)D.off * ncols * ldP, D.nb * ncols, ldP, used, W.xmax + l, st);
             }
             if (prec == PBB_PREC_MIXED_F16X2 || prec == PBB_PREC_MIXED_BF16X2)
-                dev::launch_split_p2(prec, W.P32 + (size_t)D.off * ncols * ldP, D.nb * ncols, ldP, used / 32,
-                                     prec == PBB_PREC_MIXED_F16X2 ? W.xmax + l : nullptr, st);
+                dev::launch_split_p2(prec, W.P32 + (size_t)D.off * ncols * ldP, D.nb * ncols, ldP,
+                                     k16(l) ? 1 : used / 32, prec == PBB_PREC_MIXED_F16X2 ? W.xmax + l : nullptr,
+                                     st, k16(l) ? 16 : 32);
         }
     }
     mark(3);
@@ -1854,6 +1865,13 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         if (a.wave_rows) a.TB = std::max(1, std::min(a.TB, dev::m2l_rows_max_width(prec, SE->R[sl]) / cptm));
         a.split_acc = split_acc;
         if (split_acc) a.TB = std::max(1, std::min(a.TB, 64 / cptm));
+        a.k16 = k16(l) ? 1 : 0;
+        if (a.k16) {  // the column-wave kernel with all rows
+            a.wave_rows = 0;
+            a.row_split = 0;
+            a.split_acc = 0;
+            a.TB = std::max(1, std::min(16, width_m / cptm));
+        }
         a.side = 1 << l;
         a.kpad = ldP; a.nch = SE->nch[sl]; a.escale = SE->escale[sl]; a.off = D.off; a.R = SE->R[sl];
         a.tile_prefix[0] = 0;

@@ -92,7 +92,8 @@ void precompute_ops(HostOps &ops, const HostEval &eval, const PrecomputeDevice *
 // ([E_0 .. E_315] and [E_0; ..; E_315]), E'_d = Q_L^T E_d Q_R, U' = Uw Q_L
 // and VT' = Q_R^T VTw apply the same far field within tau (the relative
 // Frobenius norm of the discarded singular values of the stacked E, each
-// side), with kf a multiple of 16 and ks of 32 (chunk sizes of the M2L).
+// side), with kf a multiple of 16 and ks 16 or a multiple of 32 (chunk
+// sizes of the M2L's matrix-core steps).
 // When that saves nothing the set keeps its own factors (kf = r_f, ks = r_s).
 struct ReducedSet {
     int kf = 0, ks = 0;

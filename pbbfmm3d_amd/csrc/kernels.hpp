@@ -228,6 +228,7 @@ struct M2LMixArgs {
     const float *P;             // FP32 proxies, kpad per column
     const unsigned *xmax;       // max |P| of the level (FP16 scaling) or nullptr
     float *Pf;
+    int k16;          // f16 x2, ks at most 16: 16-deep steps (v_mfma_f32_16x16x16_f16) on 16-deep pre-split proxies
 };
 
 struct NearGradArgs {
@@ -254,7 +255,9 @@ void launch_split_e(int variant, const double *E, int rf, int rs, int R, int nch
 void launch_absmax_rows(const float *x, int64_t rows, int ld, int used, unsigned *out, hipStream_t st);
 // two-part modes: rewrite the first cpr 32-float k-chunks of each of rows
 // rows of P (stride ld floats) as scaled 16-bit parts
-void launch_split_p2(int variant, float *P, int64_t rows, int ld, int cpr, const unsigned *xmax, hipStream_t st);
+// kd: k-chunk depth (32; 16 for the one chunk of a set whose ks is at most 16, f16 x2)
+void launch_split_p2(int variant, float *P, int64_t rows, int ld, int cpr, const unsigned *xmax, hipStream_t st,
+                     int kd = 32);
 void upload_dirs_mixed(const int dirs[8][189][4]);
 void launch_zero_u32(unsigned *p, int n, hipStream_t st);
 // widest row-wave M2L tile (PBB_M2L_WAVE_ROWS) for a variant and split-E row count

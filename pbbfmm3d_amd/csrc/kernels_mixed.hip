@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -51,6 +52,12 @@ __device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c)
 __device__ __forceinline__ f32x4 mfma16(f16x8 a, f16x8 b, f32x4 c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+// 16-deep steps: A[i][k] i = l % 16, k = 4*(l/16) + e; B[k][j] j = l % 16, k = 4*(l/16) + e (e < 4)
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f32x4 mfma16(f16x4 a, f16x4 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -113,8 +120,9 @@ __device__ __forceinline__ int p_scale_exp(const unsigned *xmax)
 // Two-part proxies, in place: each of the first cpr 32-float k-chunks of a
 // row of P (128 bytes) is rewritten as 32 high then 32 low 16-bit parts of
 // x * 2^xs, so that the M2L loads its B fragments ready-made.  One thread
-// per chunk.
-template <typename T>
+// per chunk.  KD = 16: one 16-deep chunk per row (sets whose ks is at most
+// 16), rewritten as 16 high then 16 low parts in its first 64 bytes.
+template <typename T, int KD>
 __global__ void split_p2(float *__restrict__ P, int64_t rows, int ld, int cpr, const unsigned *__restrict__ xmax)
 {
     const float xscale = ldexpf(1.0f, p_scale_exp(xmax));
@@ -122,39 +130,44 @@ __global__ void split_p2(float *__restrict__ P, int64_t rows, int ld, int cpr, c
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nchunks;
          q += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = q / cpr;
-        float4 *c = (float4 *)(P + r * ld + (q - r * cpr) * 32);
-        float x[32];
+        float4 *c = (float4 *)(P + r * ld + (q - r * cpr) * KD);
+        float x[KD];
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
+        for (int i = 0; i < KD / 4; i++) {
             const float4 v = c[i];
             x[4 * i] = v.x * xscale;
             x[4 * i + 1] = v.y * xscale;
             x[4 * i + 2] = v.z * xscale;
             x[4 * i + 3] = v.w * xscale;
         }
-        T h[32], l[32];
+        T h[KD], l[KD];
 #pragma unroll
-        for (int e = 0; e < 32; e++) {
+        for (int e = 0; e < KD; e++) {
             h[e] = (T)x[e];
             l[e] = (T)(x[e] - (float)h[e]);
         }
         uint4 *o = (uint4 *)c;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < KD / 8; i++) {
             o[i] = *(const uint4 *)&h[8 * i];
-            o[4 + i] = *(const uint4 *)&l[8 * i];
+            o[KD / 8 + i] = *(const uint4 *)&l[8 * i];
         }
     }
 }
 
-void launch_split_p2(int variant, float *P, int64_t rows, int ld, int cpr, const unsigned *xmax, hipStream_t st)
+void launch_split_p2(int variant, float *P, int64_t rows, int ld, int cpr, const unsigned *xmax, hipStream_t st,
+                     int kd)
 {
     const int64_t nchunks = rows * cpr;
     if (nchunks <= 0) return;
-    if (ld % 4 || cpr * 32 > ld) fail(PBB_ERR_INTERNAL, "split_p2: chunks past the row stride");
+    if (kd != 32 && !(kd == 16 && cpr == 1 && variant == PBB_PREC_MIXED_F16X2))
+        fail(PBB_ERR_INTERNAL, "split_p2: 16-deep chunks are for one f16 chunk per row");
+    if (ld % 4 || cpr * kd > ld) fail(PBB_ERR_INTERNAL, "split_p2: chunks past the row stride");
     const int blocks = (int)std::min<int64_t>((nchunks + 255) / 256, 65536);
-    if (variant == PBB_PREC_MIXED_F16X2) split_p2<_Float16><<<blocks, 256, 0, st>>>(P, rows, ld, cpr, xmax);
-    else if (variant == PBB_PREC_MIXED_BF16X2) split_p2<__bf16><<<blocks, 256, 0, st>>>(P, rows, ld, cpr, xmax);
+    if (variant == PBB_PREC_MIXED_F16X2 && kd == 16)
+        split_p2<_Float16, 16><<<blocks, 256, 0, st>>>(P, rows, ld, cpr, xmax);
+    else if (variant == PBB_PREC_MIXED_F16X2) split_p2<_Float16, 32><<<blocks, 256, 0, st>>>(P, rows, ld, cpr, xmax);
+    else if (variant == PBB_PREC_MIXED_BF16X2) split_p2<__bf16, 32><<<blocks, 256, 0, st>>>(P, rows, ld, cpr, xmax);
     else fail(PBB_ERR_INVALID_ARG, "proxies are pre-split for the two-part modes only");
     check(hipGetLastError(), "split_p2");
 }
@@ -183,14 +196,24 @@ void launch_split_p2(int variant, float *P, int64_t rows, int ld, int cpr, const
 constexpr int TB_MAX = 16;
 __device__ __attribute__((aligned(16))) float g_zero32[32];  // a zero k chunk (empty slots), never written
 
-template <typename T, int NSPLIT, int RB, int NCB>
+// KD = 16 (f16 x2, sets whose ks is at most 16): 16-deep steps on proxies
+// pre-split as 16 high + 16 low parts in the first 64 bytes of each row
+// (half the proxy bytes per (box, direction) pair, which bound these levels),
+// E_d read as the first 16 k of its 32-deep chunk.
+template <typename T, int NSPLIT, int RB, int NCB, int KD = 32>
 __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2LMixArgs a)
 {
+    static_assert(KD == 32 || (KD == 16 && NSPLIT == 2), "16-deep steps are for pre-split two-part proxies");
     typedef typename Vec8<T>::type V8;
-    constexpr int NT = 256, W = 64 * NCB, R = 16 * RB, KS = 40, BS = 36;  // R: rows of this workgroup
-    constexpr int NA = NSPLIT * R * 4;  // 16-byte units of one E chunk
+    typedef T VK __attribute__((ext_vector_type(KD / 4)));  // one lane's k run of a fragment
+    constexpr int NT = 256, W = 64 * NCB, R = 16 * RB;      // R: rows of this workgroup
+    constexpr int KU = KD / 8;          // 16-byte units of one E row's k chunk
+    constexpr int KS = KD + 8;          // LDS row stride of As (16-bit elements)
+    constexpr int PU = KD / 4;          // 16-byte units of one column's proxy chunk
+    constexpr int BS = KD + 4;          // LDS row stride of Bs (floats)
+    constexpr int NA = NSPLIT * R * KU;  // 16-byte units of one E chunk
     constexpr int AR = (NA + NT - 1) / NT;
-    constexpr int BR = W * 8 / NT;      // 16-byte units of one proxy chunk per thread
+    constexpr int BR = W * PU / NT;     // 16-byte units of one proxy chunk per thread
     constexpr bool PRESPLIT = NSPLIT == 2;
     __shared__ __attribute__((aligned(16))) T As[NSPLIT * R * KS];
     __shared__ __attribute__((aligned(16))) float Bs[W * BS];
@@ -273,8 +296,9 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
         const u32x4 *src = (const u32x4 *)((const T *)a.Es + ((size_t)c_dirs[o][k][3] * nch + ch) * NSPLIT * RE * 32);
 #pragma unroll
         for (int r = 0; r < AR; r++) {
-            const int u = min(tid + r * NT, NA - 1), p = u / (R * 4), row = min(r0 + (u / 4) % R, RE - 1), q = u % 4;
-            ra[r] = src[(p * RE + row) * 4 + q];
+            const int u = min(tid + r * NT, NA - 1), p = u / (R * KU), row = min(r0 + (u / KU) % R, RE - 1),
+                      q = u % KU;
+            ra[r] = src[(p * RE + row) * 4 + q];  // rows of 32-deep chunks: 4 units each
         }
     };
     auto loadP = [&](int step, f32x4 (&pb)[BR]) __attribute__((always_inline)) {
@@ -283,7 +307,7 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
         const int k = s_dir[di];
 #pragma unroll
         for (int r = 0; r < BR; r++) {
-            const int u = tid + r * NT, col = u / 8, k4 = u % 8;
+            const int u = tid + r * NT, col = u / PU, k4 = u % PU;
             const int tb = col / cpt, cl = col - tb * cpt;
             const int sid = tb < TB && step <= last ? s_sid[k * TB_MAX + tb] : -1;  // zeros past the last step
             const float *src = sid >= 0 ? a.P + ((size_t)(a.off + sid) * a.ncols + c0 + cl) * a.kpad + ch * 32 : g_zero32;
@@ -329,25 +353,25 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
         for (int r = 0; r < AR; r++) {
             const int u = tid + r * NT;
             if (AR * NT == NA || u < NA) {
-                const int p = u / (R * 4), row = (u / 4) % R, q = u % 4;
+                const int p = u / (R * KU), row = (u / KU) % R, q = u % KU;
                 *(u32x4 *)&As[(p * R + row) * KS + 8 * q] = ra[r];
             }
         }
 #pragma unroll
         for (int r = 0; r < BR; r++) {
             const int u = tid + r * NT;
-            *(f32x4 *)&Bs[(u / 8) * BS + 4 * (u % 8)] = pb[r];
+            *(f32x4 *)&Bs[(u / PU) * BS + 4 * (u % PU)] = pb[r];
         }
         __syncthreads();
         loadE(step + 1);  // in flight during the MFMAs
         loadP(step + 2, pb);
-        V8 bfr[NCB][NSPLIT];
+        VK bfr[NCB][NSPLIT];
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++) {
             const float *bp = &Bs[((wave * NCB + cb) * 16 + fr) * BS];
-            if constexpr (PRESPLIT) {  // plane p of k 8fq..8fq+7 at floats 16p + 4fq
-                bfr[cb][0] = *(const V8 *)(bp + 4 * fq);
-                bfr[cb][1] = *(const V8 *)(bp + 16 + 4 * fq);
+            if constexpr (PRESPLIT) {  // plane p of k (KD/4)fq.. at floats (KD/2)p + (KD/8)fq
+                bfr[cb][0] = *(const VK *)(bp + (KD / 8) * fq);
+                bfr[cb][1] = *(const VK *)(bp + KD / 2 + (KD / 8) * fq);
             } else {
                 const float4 v0 = *(const float4 *)(bp + 8 * fq), v1 = *(const float4 *)(bp + 8 * fq + 4);
                 float x[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
@@ -365,9 +389,9 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
         }
 #pragma unroll
         for (int rb_ = 0; rb_ < RB; rb_++) {
-            V8 afr[NSPLIT];
+            VK afr[NSPLIT];
 #pragma unroll
-            for (int p = 0; p < NSPLIT; p++) afr[p] = *(const V8 *)&As[(p * R + rb_ * 16 + fr) * KS + 8 * fq];
+            for (int p = 0; p < NSPLIT; p++) afr[p] = *(const VK *)&As[(p * R + rb_ * 16 + fr) * KS + (KD / 4) * fq];
 #pragma unroll
             for (int cb = 0; cb < NCB; cb++) {
                 // smallest terms first
@@ -1150,6 +1174,27 @@ static void m2l_rows_t(const M2LMixArgs &a, int ntiles, int ncolchunks, hipStrea
 template <typename T, int NSPLIT>
 static void m2l_split_t(const M2LMixArgs &a, int ntiles, int ncolchunks, hipStream_t st)
 {
+    if (a.k16) {  // 16-deep steps, column-wave tiles, all rows (at most 32) in one workgroup
+        if constexpr (NSPLIT == 2 && sizeof(T) == 2 && !std::is_same<T, __bf16>::value) {
+            if (a.nch != 1 || a.rf > 32) fail(PBB_ERR_INTERNAL, "16-deep M2L steps need ks <= 16 and kf <= 32");
+            const int w = a.TB * a.cpt;
+            if (w > 256) fail(PBB_ERR_INTERNAL, "mixed-precision M2L tile wider than 256 columns");
+            dim3 grid((unsigned)ntiles, (unsigned)ncolchunks, 1);
+#define PBB_K16(RB)                                                                        \
+    if (w > 128) m2l_mfma_split<T, 2, RB, 4, 16><<<grid, 256, 0, st>>>(a);                 \
+    else if (w > 64) m2l_mfma_split<T, 2, RB, 2, 16><<<grid, 256, 0, st>>>(a);             \
+    else m2l_mfma_split<T, 2, RB, 1, 16><<<grid, 256, 0, st>>>(a);
+            if (a.rf <= 16) {
+                PBB_K16(1)
+            } else {
+                PBB_K16(2)
+            }
+#undef PBB_K16
+            return;
+        } else {
+            fail(PBB_ERR_INTERNAL, "16-deep M2L steps are for the f16 x2 mode");
+        }
+    }
     if (a.wave_rows) {
         m2l_rows_t<T, NSPLIT>(a, ntiles, ncolchunks, st);
         return;

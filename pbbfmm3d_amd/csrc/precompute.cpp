@@ -324,13 +324,15 @@ static void right_svd(std::vector<double> &A, int64_t m, int n, std::vector<doub
 
 // Smallest k (a multiple of `mult`, at most r) whose discarded singular
 // values hold at most tau of the Frobenius norm; tail: what they hold.
-static int reduced_rank(const std::vector<double> &sigma, int r, int mult, double tau, double &tail)
+// the smallest candidate rank (first, then the multiples of mult) whose
+// discarded singular values hold at most tau of the Frobenius norm
+static int reduced_rank(const std::vector<double> &sigma, int r, int first, int mult, double tau, double &tail)
 {
     double tot = 0;
     for (double v : sigma) tot += v * v;
     std::vector<double> rest(sigma.size() + 1, 0.0);
     for (int i = (int)sigma.size() - 1; i >= 0; i--) rest[i] = rest[i + 1] + sigma[i] * sigma[i];
-    for (int k = mult; k < r; k += mult)
+    for (int k = first; k < r; k = k < mult ? mult : k + mult)
         if (rest[k] <= tau * tau * tot) {
             tail = tot > 0 ? std::sqrt(rest[k] / tot) : 0.0;
             return k;
@@ -358,8 +360,9 @@ void reduce_set(const HostOps &h, int set, const double *VTw, const double *Uw, 
     std::vector<double> sR, VR, sL, VL;
     right_svd(BR, mR, rs, sR, VR);
     right_svd(BL, mL, rf, sL, VL);
-    const int ks = reduced_rank(sR, rs, 32, tau, out.tail_s);
-    const int kf = reduced_rank(sL, rf, 16, tau, out.tail_f);
+    // ks: 16 (one 16-deep k chunk of the matrix cores) or whole 32-deep chunks
+    const int ks = reduced_rank(sR, rs, 16, 32, tau, out.tail_s);
+    const int kf = reduced_rank(sL, rf, 16, 16, tau, out.tail_f);
     out.kf = kf;
     out.ks = ks;
     if (kf >= rf && ks >= rs) {  // nothing to gain: the set's own factors

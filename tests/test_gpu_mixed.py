@@ -48,9 +48,10 @@ def test_mixed_vs_fp64(gpu, prec, bar, kernel, n, eps, ncols):
 
 def test_recompressed_operator_sets(gpu):
     # the mixed modes run each level's operator set re-compressed
-    # (pbb_ops_set_mixed_tolerance, default 1e-10): the deepest sets keep far
+    # (pbb_ops_set_mixed_tolerance, default 2e-8): the deepest sets keep far
     # fewer directions, the FP64 path keeps the reference's operators, and
     # the mixed result stays as close to it as with every set's own factors
+    tol = 2e-8
     N, nc = 60000, 16
     xyz = clustered_points(N, seed=62)
     w = weights(N, nc, seed=63)
@@ -76,13 +77,13 @@ def test_recompressed_operator_sets(gpu):
     f64r = _mv(pr, w, nc, pbb.capi.PREC_FP64)
     d = orc.rel_err(f64r, f64)
     print(f"FP64 re-compressed vs reference operators: {d:.3e}", flush=True)
-    assert 0 < d <= 1e-8
+    assert 0 < d <= 2 * tol
     import torch
     wd = torch.tensor(w.reshape(nc, N), device="cuda:0")
     t_red = pr.matvec_transpose(wd)
     red.set_fp64_recompressed(False)
     t_ref = pr.matvec_transpose(wd)
-    assert float(torch.linalg.norm(t_red - t_ref) / torch.linalg.norm(t_ref)) <= 1e-8
+    assert float(torch.linalg.norm(t_red - t_ref) / torch.linalg.norm(t_ref)) <= 2 * tol
 
 
 def test_mixed_switching_and_accumulate(gpu):
