@@ -30,17 +30,22 @@ M2L_SPLIT = {"mixed": "bf16x3", "f16x2": "f16x2", "bf16x2": "bf16x2"}
 
 def dtype_label(precision, flags):
     """What each phase computes in.  Mixed modes: M2L on split 16-bit MFMA
-    with FP32 sums (each level's re-compressed operator set), proxies P and
-    M2L results Pf stored FP32, VT / U on three-part BF16 MFMA (FP64 with the
-    cross-product-sum M2L tile), expansions S / F stored FP32 with FP64
-    arithmetic, the near field of dense leaves with FP32 kernel values and
-    FP64 MFMA sums, the rest of the near field in FP64."""
+    with FP32 sums (each level's re-compressed operator set; 16-deep steps
+    where the kept source rank is at most 16), proxies P and M2L results Pf
+    stored FP32; on the levels of rank <= 32 the proxy passes (rank vectors
+    on three-part BF16 MFMA, FP64 box sums and Pf products); above them VT /
+    U on three-part BF16 MFMA (FP64 with the cross-product-sum M2L tile),
+    expansions S / F stored FP32 with FP64 arithmetic (P2M, M2M, L2L, L2P);
+    the near field of dense leaves with FP32 kernel values and FP64 MFMA
+    sums, the rest of the near field in FP64."""
     if precision == "fp64":
         return "f64"
     split_acc = precision in ("f16x2", "bf16x2") and flags is not None and (flags & 8)
     vtu = "f64 (P, Pf f32)" if split_acc else "bf16x3->f32 (P, Pf f32)"
     return (f"m2l {M2L_SPLIT[precision]}->f32{' (cross products summed apart)' if split_acc else ''}; "
-            f"vt/u {vtu}; p2m, l2p f64 (S, F f32); p2p f64 (dense leaves: f32 kernel values, f64 mfma sums)")
+            f"levels of rank<=32: proxy passes bf16x3->f32 rank vectors, f64 box sums; "
+            f"other levels: vt/u {vtu}, p2m/m2m/l2l/l2p f64 (S, F f32); "
+            f"p2p f64 (dense leaves: f32 kernel values, f64 mfma sums)")
 
 
 # bytes per M2L launch (FETCH_SIZE x 2 + WRITE_SIZE x 1, calibrated on a plain
@@ -144,14 +149,17 @@ def named_phases(phase, order):
     return out
 
 
-def phase_rooflines(phase, info, ops, nc, precision, kernel_id, ranks=None):
+def phase_rooflines(phase, info, ops, nc, precision, kernel_id, ranks=None, proxy_from=None):
     """Algorithmic bytes or flops of each phase over its measured time, as a
     fraction of MI355X's peak (HBM 8 TB/s, FP64 78.6 TF/s): HBM-bound
     expansion passes count the bytes they must move (S / P / Pf / F once,
-    points and weights once per level), P2P counts (c_K + 2 nCols) flops per
-    ordered pair (c_K = 9 for 1/r, 11 for the Gaussian).  ranks: {level:
-    (kf, ks)} the path runs (default r_f x r_s); S and F are FP64 in the FP64
-    mode and FP32 in the mixed modes, P and Pf likewise."""
+    points and weights once per level that reads them), P2P counts
+    (c_K + 2 nCols) flops per ordered pair (c_K = 9 for 1/r, 11 for the
+    Gaussian).  ranks: {level: (kf, ks)} the path runs (default r_f x r_s);
+    S and F are FP64 in the FP64 mode and FP32 in the mixed modes, P and Pf
+    likewise.  proxy_from: the first level of the proxy passes (levels of
+    rank <= 32, mixed modes), whose boxes have no S / F; above it P2M and
+    L2P run at its parent level only, with M2M / L2L to the coarser ones."""
     n3, rf, rs = ops.n3, ops.rank_f, ops.rank_s
     L = info["tree_level"]
     N = info["npoints"]
@@ -173,16 +181,44 @@ def phase_rooflines(phase, info, ops, nc, precision, kernel_id, ranks=None):
             res[name] = {"bound": "hbm", "bytes": by, "GB/s": round(by / t / 1e9, 1),
                          "frac": round(by / t / hbm, 3)}
 
-    hbm_entry("p2m_all_levels", phase["p2m"], S + nlev * N * (24 + 8 * nc))
-    hbm_entry("vt_proxy", phase["vt_proxy"], S + P)
-    hbm_entry("u_expand", phase["u_expand"], Pf + S)
-    hbm_entry("l2p_all_levels", phase["l2l"], S + nlev * N * 24 + (2 * (nlev - 1) + 1) * N * nc * 8)
+    Lp = proxy_from if proxy_from is not None else L + 1
+    if Lp <= L:
+        pts, lpw = N * (24 + 8 * nc), N * nc * 8  # points + weights; one level's point-major results
+        nx = L - Lp + 1                           # proxy levels
+        Se = sum(boxes[l] for l in range(2, Lp)) * n3 * nc * eb  # S (then F) of the expansion levels
+        Px = sum(boxes[l] * nc * ranks[l][1] for l in range(Lp, L + 1)) * eb
+        Pfx = sum(boxes[l] * nc * ranks[l][0] for l in range(Lp, L + 1)) * eb
+        hbm_entry("p2m_all_levels", phase["p2m"], (pts + 2 * Se) if Lp > 2 else 0)
+        hbm_entry("vt_proxy", phase["vt_proxy"], Se + (P - Px) + nx * pts + Px)
+        hbm_entry("u_expand", phase["u_expand"], (Pf - Pfx) + Se)
+        hbm_entry("l2p_all_levels", phase["l2l"],
+                  (2 * Se + N * 24 + lpw if Lp > 2 else 0) + Pfx + nx * (N * 24 + 2 * lpw))
+    else:
+        hbm_entry("p2m_all_levels", phase["p2m"], S + nlev * N * (24 + 8 * nc))
+        hbm_entry("vt_proxy", phase["vt_proxy"], S + P)
+        hbm_entry("u_expand", phase["u_expand"], Pf + S)
+        hbm_entry("l2p_all_levels", phase["l2l"], S + nlev * N * 24 + (2 * (nlev - 1) + 1) * N * nc * 8)
     t = phase["near"] * 1e-3
     if t > 0:
         fl = info["near_pairs"] * (ck + 2 * nc)
         res["p2p"] = {"bound": "valu_fp64", "flops": fl, "TFLOP/s": round(fl / t / 1e12, 2),
                       "frac": round(fl / t / 78.6e12, 3)}
     return res
+
+
+def proxy_from(ops, mranks, L, args):
+    """First level of the proxy passes: the deepest levels whose kept ranks
+    are at most 32 (mixed modes without the cross-product-sum tile, orders
+    4..8, r_f >= 32); L + 1 when there are none."""
+    if mranks is None or not 4 <= args.order <= 8 or ops.rank_f < 32:
+        return L + 1
+    lp = L + 1
+    while lp > 2:
+        kf, ks, _ = mranks[lp - 3 if ops.nsets > 1 else 0]
+        if kf > 32 or ks > 32:
+            break
+        lp -= 1
+    return lp
 
 
 def kernel_name(args):
@@ -499,7 +535,8 @@ def main():
         "fp64_path": fp64,
         "phase_ms": named_phases(phase, args.order),
         "phase_rooflines": phase_rooflines(phase, info, ops, nc, args.precision, kernel.id,
-                                           {l: level_ranks(l) for l in range(2, info["tree_level"] + 1)}),
+                                           {l: level_ranks(l) for l in range(2, info["tree_level"] + 1)},
+                                           proxy_from(ops, mranks, info["tree_level"], args)),
         "ranks": [ops.rank_f, ops.rank_s],
         "m2l_pairs": info["m2l_pairs"], "near_pairs": info["near_pairs"],
         "precompute_s": round(t_pre, 3), "plan_s": round(t_plan, 3),

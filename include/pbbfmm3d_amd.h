@@ -225,10 +225,11 @@ PBB_API pbb_status pbb_ops_set_mixed_tolerance(pbb_ops *ops, double tol);
  * operators.  Takes effect from the next call on these operators; a batch
  * captured before keeps the operators it was captured with.               */
 PBB_API pbb_status pbb_ops_set_fp64_recompressed(pbb_ops *ops, int32_t enable);
-/* The ranks the mixed modes run per operator set (arrays of nsets: sets in
- * level order 2..L, or one set for a homogeneous kernel) and the discarded
- * relative Frobenius norm (tail may be NULL).  Computes them on the host if
- * needed; no device work.                                                 */
+/* The ranks the mixed modes keep per operator set (arrays of nsets: sets in
+ * level order 2..L, or one set for a homogeneous kernel; ks as kept, before
+ * the padding of the proxies to 32-deep chunks) and the discarded relative
+ * Frobenius norm (tail may be NULL).  Computes them on the host if needed;
+ * no device work.                                                         */
 PBB_API pbb_status pbb_ops_mixed_ranks(pbb_ops *ops, int32_t *kf, int32_t *ks, double *tail);
 
 PBB_API void pbb_ops_destroy(pbb_ops *ops);

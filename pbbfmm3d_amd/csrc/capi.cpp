@@ -934,7 +934,7 @@ PBB_API pbb_status pbb_ops_mixed_ranks(pbb_ops *ops, int32_t *kf, int32_t *ks, d
         ops->compute_mixed_sets();
         for (int s = 0; s < ops->h.nsets; s++) {
             kf[s] = ops->mset[s].kf;
-            ks[s] = ops->mset[s].ks;
+            ks[s] = ops->mset[s].ks_used;
             if (tail) tail[s] = ops->mset[s].tail;
         }
     });

@@ -27,7 +27,7 @@ def _ops():
     # the proxy passes run at the levels whose ranks fit 32 (and every deeper
     # one); level 6 keeps 16 directions each side (ks stored padded to one
     # 32-deep chunk), so its f16 x2 M2L takes 16-deep steps
-    assert [r[:2] for r in ranks[-2:]] == [(32, 32), (16, 32)], ranks
+    assert [r[:2] for r in ranks[-2:]] == [(32, 32), (16, 16)], ranks
     assert all(r[0] > 32 for r in ranks[:-2])
     return ops
 

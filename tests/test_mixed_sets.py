@@ -5,9 +5,8 @@ M2L blocks, and the accuracy the kept directions carry.
 The check is independent of the product's TSQR + Jacobi SVD: numpy/LAPACK
 singular values of [E_0 .. E_315] and [E_0; ..; E_315] give the smallest
 kf (multiple of 16) and ks (16 or a multiple of 32) whose discarded values
-hold at most tol of the Frobenius norm (ks is reported padded to 32); the
-projection of every E_d on those leading directions must reproduce E_d
-within that bound.
+hold at most tol of the Frobenius norm; the projection of every E_d on those
+leading directions must reproduce E_d within that bound.
 """
 import numpy as np
 import pytest
@@ -45,16 +44,16 @@ def test_ranks_against_numpy(tol):
         UL, sL, _ = np.linalg.svd(A, full_matrices=False)
         _, sR, VRt = np.linalg.svd(B, full_matrices=False)
         kf_ref, ks_ref = _rank(sL, rf, 16, 16, tol), _rank(sR, rs, 16, 32, tol)
-        if kf_ref >= rf and ks_ref >= rs:  # no gain: the set's own factors (ks padded to 32)
-            assert (kf, ks) == (rf, 32 * ((rs + 31) // 32)) and tail == 0.0
+        if kf_ref >= rf and ks_ref >= rs:  # no gain: the set's own factors
+            assert (kf, ks) == (rf, rs) and tail == 0.0
             continue
         # the boundary value itself may fall on either side of the bound
         # within rounding: allow one step
-        ks_pad = 32 * ((ks_ref + 31) // 32)
-        assert abs(kf - kf_ref) <= 16 and abs(ks - ks_pad) <= 32, (s, kf, kf_ref, ks, ks_ref)
+        assert abs(kf - kf_ref) <= 16 and abs(ks - ks_ref) <= 32, (s, kf, kf_ref, ks, ks_ref)
+        assert ks == 16 or ks % 32 == 0 or ks == rs
         assert 0 < tail <= tol * 1.0001
         # the kept directions reproduce every block within the bound
-        QL, QR = UL[:, :kf], VRt[:min(ks, ks_ref)].T
+        QL, QR = UL[:, :kf], VRt[:ks].T
         num = sum(np.linalg.norm(Ed - QL @ (QL.T @ Ed @ QR) @ QR.T) ** 2 for Ed in E[s])
         assert np.sqrt(num) <= 2.0 * tol * np.linalg.norm(A)
     # smooth kernel, small boxes: the deepest set keeps far fewer directions
@@ -64,7 +63,7 @@ def test_ranks_against_numpy(tol):
 def test_homogeneous_kernel_keeps_its_rank_and_tolerance_rules():
     ops = pbb.Operators(pbb.laplacian(), 1.0, 4, 4, 1e-5, host_precompute=True)
     (kf, ks, tail), = ops.mixed_ranks()
-    assert kf == ops.rank_f and ks == 32 * ((ops.rank_s + 31) // 32) and tail == 0.0
+    assert kf == ops.rank_f and ks == ops.rank_s and tail == 0.0
     # the tolerance is fixed once the sets are made
     with pytest.raises(pbb.capi.PBBError):
         ops.set_mixed_tolerance(1e-9)
