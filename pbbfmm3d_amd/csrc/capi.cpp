@@ -1780,6 +1780,17 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         return prec == PBB_PREC_MIXED_F16X2 && SE && mset(l).ks_used <= 16 && mset(l).kf <= 32 &&
                SE->nch[h.ops_set(l)] == 1;
     };
+    // the proxies of a proxy level keep compact rows (16 floats where the M2L
+    // takes 16-deep steps, else 32) inside the level's own region of P (rows
+    // of ldP), so that a box's proxies fill whole cache lines
+    auto p_level = [&](int l, int &ldp) -> float * {
+        if (l < Lp) {
+            ldp = ldP;
+            return W.P32;
+        }
+        ldp = k16(l) ? 16 : dev::PX_K;
+        return W.P32 + (size_t)p->lv[l].off * ncols * (ldP - ldp);
+    };
     for (int l = 2; l <= Lv; l++) {
         const DevLevel &D = p->lv[l];
         const int sl = h.ops_set(l);
@@ -1801,10 +1812,11 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         }
         if (l >= Lp) {  // proxies straight from the points (rows past ks are zero)
             dev::ProxyArgs x{};
-            x.ncols = ncols; x.nb = (int)D.nb; x.rank = mset(l).ks; x.ldp = ldP; x.off = D.off; x.len = D.len;
+            x.ncols = ncols; x.nb = (int)D.nb; x.rank = mset(l).ks; x.off = D.off; x.len = D.len;
             x.pstart = D.pstart; x.coord = D.coord; x.ctab = D.ctab; x.Tkz = o->d_Tkz;
             x.x = p->x; x.y = p->y; x.z = p->z; x.wsT = W.wsT;
-            x.op = o->dm_VTw + mset(l).vt_off; x.P = W.P32; x.npts = p->N;
+            x.op = o->dm_VTw + mset(l).vt_off; x.npts = p->N;
+            x.P = p_level(l, x.ldp);
             if (prec == PBB_PREC_MIXED_F16X2) {  // the proxy pass also finds the level's max |P|
                 dev::launch_zero_u32(W.xmax + l, 1, st);
                 x.xmax = W.xmax + l;
@@ -1824,12 +1836,14 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
             // (rows past ks come from zero rows of the split operator)
             const int used = 32 * SE->nch[sl];
             if (used > ldP) fail(PBB_ERR_INTERNAL, "proxy chunks past the padding");
+            int ldl = 0;
+            float *Pl = p_level(l, ldl) + (size_t)D.off * ncols * ldl;  // this level's first row
             if (prec == PBB_PREC_MIXED_F16X2 && l < Lp) {
                 dev::launch_zero_u32(W.xmax + l, 1, st);
-                dev::launch_absmax_rows(W.P32 + (size_t)D.off * ncols * ldP, D.nb * ncols, ldP, used, W.xmax + l, st);
+                dev::launch_absmax_rows(Pl, D.nb * ncols, ldl, used, W.xmax + l, st);
             }
             if (prec == PBB_PREC_MIXED_F16X2 || prec == PBB_PREC_MIXED_BF16X2)
-                dev::launch_split_p2(prec, W.P32 + (size_t)D.off * ncols * ldP, D.nb * ncols, ldP,
+                dev::launch_split_p2(prec, Pl, D.nb * ncols, ldl,
                                      k16(l) ? 1 : used / 32, prec == PBB_PREC_MIXED_F16X2 ? W.xmax + l : nullptr,
                                      st, k16(l) ? 16 : 32);
         }
@@ -1873,7 +1887,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
             a.TB = std::max(1, std::min(16, width_m / cptm));
         }
         a.side = 1 << l;
-        a.kpad = ldP; a.nch = SE->nch[sl]; a.escale = SE->escale[sl]; a.off = D.off; a.R = SE->R[sl];
+        a.nch = SE->nch[sl]; a.escale = SE->escale[sl]; a.off = D.off; a.R = SE->R[sl];
         a.tile_prefix[0] = 0;
         a.class_start[0] = 0;
         for (int c = 0; c < 8; c++) {
@@ -1882,7 +1896,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
         }
         a.order = D.order; a.coord = D.coord; a.map = D.map;
         a.Es = (const uint16_t *)SE->d + SE->off[sl];
-        a.P = W.P32; a.Pf = pf_level(l, a.ldf);
+        a.P = p_level(l, a.kpad); a.Pf = pf_level(l, a.ldf);
         a.xmax = prec == PBB_PREC_MIXED_F16X2 ? W.xmax + l : nullptr;
         // direction groups only where the launch fills the GPU many times over
         const int64_t wgs = (int64_t)a.tile_prefix[8] * nchunks_m * (a.row_split ? 2 : 1);

@@ -117,7 +117,7 @@ struct ProxyArgs {
     const double *ctab, *Tkz, *x, *y, *z;
     const double *wsT;         // P2M: point-major sorted weights
     const double *op;          // P2M: VT' (rank x n^3), L2P: U' (n^3 x rank), column-major
-    float *P;                  // P2M: proxies, rows 0..PX_K-1 written
+    float *P;                  // P2M: proxies, rows 0..min(ldp, PX_K)-1 written (those past are zero by construction)
     unsigned *xmax;            // P2M: atomic max of |P| as FP32 bits (cleared by the caller), or nullptr
     const float *Pf;           // L2P: M2L results
     double *lp;                // L2P: point-major results

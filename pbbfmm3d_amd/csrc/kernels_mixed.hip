@@ -1456,7 +1456,7 @@ __global__ __launch_bounds__(256) void p2m_proxy(ProxyArgs a)
                         acc[3] += w * y.w;
                     }
                     if (s_ps[j + 1] <= sub_end) {  // the box ends in this pass
-                        if (c < cn) {
+                        if (c < cn && 4 * rq < a.ldp) {  // rows past the stride are zero (ranks past the kept ones)
                             const float4 v = float4{(float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]};
                             *(float4 *)&a.P[((a.off + jb + j) * a.ncols + c0 + c) * a.ldp + 4 * rq] = v;
                             pmax = fmaxf(pmax, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
@@ -1563,7 +1563,7 @@ bool launch_p2m_proxy(const ProxyArgs &a, int n, int64_t npts, hipStream_t st)
 {
     if (n < 4 || n > 8) return false;
     const unsigned g = proxy_grid(a, npts);
-    if (a.ldp < PX_K || a.ldp % 4) fail(PBB_ERR_INTERNAL, "proxy stride must hold 32 rows in 16-byte units");
+    if (a.ldp < 16 || a.ldp % 4) fail(PBB_ERR_INTERNAL, "proxy stride must hold 16 rows in 16-byte units");
     switch (n) {
     case 4: p2m_proxy<4><<<g, 256, 0, st>>>(a); break;
     case 5: p2m_proxy<5><<<g, 256, 0, st>>>(a); break;
