@@ -1775,9 +1775,10 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
     // FP64 proxy stride r_s (a set that was not re-compressed)
     auto red = [&](int l) { return mixed || (reduced && mset(l).ks <= h.rs); };
     // f16 x2 sets of ks <= 16 (stored padded to one 32-deep chunk): the M2L
-    // runs 16-deep steps on proxies pre-split to 16 + 16 parts
+    // runs 16-deep steps on proxies pre-split to 16 + 16 parts (not with the
+    // cross-product-sum tile, which keeps its own kernel for its accuracy)
     auto k16 = [&](int l) {
-        return prec == PBB_PREC_MIXED_F16X2 && SE && mset(l).ks_used <= 16 && mset(l).kf <= 32 &&
+        return prec == PBB_PREC_MIXED_F16X2 && !split_acc && SE && mset(l).ks_used <= 16 && mset(l).kf <= 32 &&
                SE->nch[h.ops_set(l)] == 1;
     };
     // the proxies of a proxy level keep compact rows (16 floats where the M2L
