@@ -228,7 +228,7 @@ struct M2LMixArgs {
     const float *P;             // FP32 proxies, kpad per column
     const unsigned *xmax;       // max |P| of the level (FP16 scaling) or nullptr
     float *Pf;
-    int k16;          // f16 x2, ks at most 16: 16-deep steps (v_mfma_f32_16x16x16_f16) on 16-deep pre-split proxies
+    int k16;          // f16 x2, ks at most 16: two directions' 16-deep chunks per 32-deep step, 16-deep pre-split proxies
 };
 
 struct NearGradArgs {

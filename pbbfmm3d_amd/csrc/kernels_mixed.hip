@@ -53,12 +53,7 @@ __device__ __forceinline__ f32x4 mfma16(f16x8 a, f16x8 b, f32x4 c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
-// 16-deep steps: A[i][k] i = l % 16, k = 4*(l/16) + e; B[k][j] j = l % 16, k = 4*(l/16) + e (e < 4)
-typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ f32x4 mfma16(f16x4 a, f16x4 b, f32x4 c)
-{
-    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
-}
+
 
 // ---------------------------------------------------------------------------
 // Es from the FP64 operators: x = E_d[row + k*rf] * 2^escale, split in FP64.
@@ -196,21 +191,25 @@ void launch_split_p2(int variant, float *P, int64_t rows, int ld, int cpr, const
 constexpr int TB_MAX = 16;
 __device__ __attribute__((aligned(16))) float g_zero32[32];  // a zero k chunk (empty slots), never written
 
-// KD = 16 (f16 x2, sets whose ks is at most 16): 16-deep steps on proxies
-// pre-split as 16 high + 16 low parts in the first 64 bytes of each row
-// (half the proxy bytes per (box, direction) pair, which bound these levels),
-// E_d read as the first 16 k of its 32-deep chunk.
+// KD = 16 (f16 x2, sets whose ks is at most 16): each step pairs two
+// directions' 16-deep chunks into one 32-deep product, A = [E_d1 | E_d2],
+// B = [P(t+d1); P(t+d2)], so one MFMA adds both directions (half the steps,
+// barriers and MFMAs of one direction per step).  The proxies are pre-split
+// as 16 high + 16 low parts in the first 64 bytes of each row (half the
+// proxy bytes per (box, direction) pair); E_d is read as the first 16 k of
+// its 32-deep chunk.
 template <typename T, int NSPLIT, int RB, int NCB, int KD = 32>
 __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2LMixArgs a)
 {
-    static_assert(KD == 32 || (KD == 16 && NSPLIT == 2), "16-deep steps are for pre-split two-part proxies");
+    static_assert(KD == 32 || (KD == 16 && NSPLIT == 2), "paired 16-deep steps are for pre-split two-part proxies");
+    constexpr bool PAIR = KD == 16;
     typedef typename Vec8<T>::type V8;
-    typedef T VK __attribute__((ext_vector_type(KD / 4)));  // one lane's k run of a fragment
-    constexpr int NT = 256, W = 64 * NCB, R = 16 * RB;      // R: rows of this workgroup
-    constexpr int KU = KD / 8;          // 16-byte units of one E row's k chunk
-    constexpr int KS = KD + 8;          // LDS row stride of As (16-bit elements)
-    constexpr int PU = KD / 4;          // 16-byte units of one column's proxy chunk
-    constexpr int BS = KD + 4;          // LDS row stride of Bs (floats)
+    typedef V8 VK;
+    constexpr int NT = 256, W = 64 * NCB, R = 16 * RB;  // R: rows of this workgroup
+    constexpr int KU = 4;               // 16-byte units of one E row's 32-deep k chunk
+    constexpr int KS = 40;              // LDS row stride of As (16-bit elements)
+    constexpr int PU = 8;               // 16-byte units of one column's proxy chunk
+    constexpr int BS = 36;              // LDS row stride of Bs (floats)
     constexpr int NA = NSPLIT * R * KU;  // 16-byte units of one E chunk
     constexpr int AR = (NA + NT - 1) / NT;
     constexpr int BR = W * PU / NT;     // 16-byte units of one proxy chunk per thread
@@ -280,7 +279,7 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
     if (live) s_dir[pre + (wave > 0 ? s_wcnt[0] : 0) + (wave > 1 ? s_wcnt[1] : 0)] = (short)tid;
     if (tid == 0) s_nd = s_wcnt[0] + s_wcnt[1] + s_wcnt[2];
     __syncthreads();
-    const int nsteps = s_nd * nch;
+    const int nsteps = PAIR ? (s_nd + 1) / 2 : s_nd * nch;
     if (a.accumulate && nsteps == 0) return;  // nothing to add for this tile
 
     // Loads are unconditional, so that the compiler's wait counts stay exact:
@@ -289,29 +288,42 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
     u32x4 ra[AR];
     f32x4 pb0[BR], pb1[BR];  // proxy chunks of two consecutive steps
     const int last = nsteps - 1;
+    // step -> first direction (index into s_dir) and k chunk
+    auto step_dir = [&](int step, int &di, int &ch) __attribute__((always_inline)) {
+        if constexpr (PAIR) {
+            di = 2 * step;
+            ch = 0;
+        } else {
+            m2l_step(step, s_nd, nch, a.chunk_major, di, ch);
+        }
+    };
     auto loadE = [&](int step) __attribute__((always_inline)) {
         int di, ch;
-        m2l_step(min(step, last), s_nd, nch, a.chunk_major, di, ch);
-        const int k = s_dir[di];
-        const u32x4 *src = (const u32x4 *)((const T *)a.Es + ((size_t)c_dirs[o][k][3] * nch + ch) * NSPLIT * RE * 32);
+        step_dir(min(step, last), di, ch);
+        const int k0 = s_dir[di], k1 = s_dir[min(di + 1, s_nd - 1)];  // (a padding direction's proxies are zero)
+        const u32x4 *src0 = (const u32x4 *)((const T *)a.Es + ((size_t)c_dirs[o][k0][3] * nch + ch) * NSPLIT * RE * 32);
+        const u32x4 *src1 = (const u32x4 *)((const T *)a.Es + ((size_t)c_dirs[o][k1][3] * nch + ch) * NSPLIT * RE * 32);
 #pragma unroll
         for (int r = 0; r < AR; r++) {
             const int u = min(tid + r * NT, NA - 1), p = u / (R * KU), row = min(r0 + (u / KU) % R, RE - 1),
                       q = u % KU;
-            ra[r] = src[(p * RE + row) * 4 + q];  // rows of 32-deep chunks: 4 units each
+            if constexpr (PAIR) ra[r] = (q < 2 ? src0 : src1)[(p * RE + row) * 4 + (q & 1)];  // 16 k of each
+            else ra[r] = src0[(p * RE + row) * 4 + q];  // rows of 32-deep chunks: 4 units each
         }
     };
     auto loadP = [&](int step, f32x4 (&pb)[BR]) __attribute__((always_inline)) {
         int di, ch;
-        m2l_step(min(step, last), s_nd, nch, a.chunk_major, di, ch);
-        const int k = s_dir[di];
+        step_dir(min(step, last), di, ch);
 #pragma unroll
         for (int r = 0; r < BR; r++) {
             const int u = tid + r * NT, col = u / PU, k4 = u % PU;
             const int tb = col / cpt, cl = col - tb * cpt;
-            const int sid = tb < TB && step <= last ? s_sid[k * TB_MAX + tb] : -1;  // zeros past the last step
+            const int dd = PAIR ? di + (k4 >> 2) : di, kq = PAIR ? (k4 & 3) : k4;
+            const int k = s_dir[min(dd, s_nd - 1)];
+            // zeros past the last step and for the padding direction of an odd count
+            const int sid = tb < TB && step <= last && dd < s_nd ? s_sid[k * TB_MAX + tb] : -1;
             const float *src = sid >= 0 ? a.P + ((size_t)(a.off + sid) * a.ncols + c0 + cl) * a.kpad + ch * 32 : g_zero32;
-            pb[r] = *(const f32x4 *)(src + 4 * k4);
+            pb[r] = *(const f32x4 *)(src + 4 * kq);
         }
     };
     f32x4 acc[RB][NCB];
@@ -369,9 +381,12 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++) {
             const float *bp = &Bs[((wave * NCB + cb) * 16 + fr) * BS];
-            if constexpr (PRESPLIT) {  // plane p of k (KD/4)fq.. at floats (KD/2)p + (KD/8)fq
-                bfr[cb][0] = *(const VK *)(bp + (KD / 8) * fq);
-                bfr[cb][1] = *(const VK *)(bp + KD / 2 + (KD / 8) * fq);
+            if constexpr (PAIR) {  // [d1: 8 floats high, 8 low][d2: the same]; k 8fq.. of plane p
+                bfr[cb][0] = *(const VK *)(bp + 16 * (fq >> 1) + 4 * (fq & 1));
+                bfr[cb][1] = *(const VK *)(bp + 16 * (fq >> 1) + 8 + 4 * (fq & 1));
+            } else if constexpr (PRESPLIT) {  // plane p of k 8fq..8fq+7 at floats 16p + 4fq
+                bfr[cb][0] = *(const VK *)(bp + 4 * fq);
+                bfr[cb][1] = *(const VK *)(bp + 16 + 4 * fq);
             } else {
                 const float4 v0 = *(const float4 *)(bp + 8 * fq), v1 = *(const float4 *)(bp + 8 * fq + 4);
                 float x[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
@@ -391,7 +406,7 @@ __global__ __launch_bounds__(256, NCB * RB <= 24 ? 2 : 1) void m2l_mfma_split(M2
         for (int rb_ = 0; rb_ < RB; rb_++) {
             VK afr[NSPLIT];
 #pragma unroll
-            for (int p = 0; p < NSPLIT; p++) afr[p] = *(const VK *)&As[(p * R + rb_ * 16 + fr) * KS + (KD / 4) * fq];
+            for (int p = 0; p < NSPLIT; p++) afr[p] = *(const VK *)&As[(p * R + rb_ * 16 + fr) * KS + 8 * fq];
 #pragma unroll
             for (int cb = 0; cb < NCB; cb++) {
                 // smallest terms first
