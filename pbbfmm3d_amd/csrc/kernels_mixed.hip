@@ -1268,7 +1268,8 @@ template <int N>
 struct PxDims {
     static constexpr int N3 = N * N * N, KC = (N3 + 31) / 32, KP = 32 * KC;
     static constexpr int KS = KP + 8;  // row stride of the split operator (bf16): conflict-free 16-byte reads
-    static constexpr int TS = N * N + 1, ZS = N + 1;
+    static constexpr int ZS = N + 1;                  // stride of one axis's S_n in the pass's table
+    static constexpr int TAB = PX_PT * 3 * ZS;       // floats of the table
 };
 
 // The boxes [bA, bB) of range k: bA, the first box with pstart >= k*PX_PT,
@@ -1330,51 +1331,52 @@ __device__ __forceinline__ int px_pass_boxes(const ProxyArgs &a, int *s_ps, int 
 
 // Points [base, base + np) of a pass, whose boxes are jb + 0..nbx-1 with
 // first points s_ps[0..nbx-1]: vs[pp][r] = sum_i Ob[i][r] S_n,i(x_{base+pp}).
-// Per half of PX_HALF points: one thread per point evaluates S_n along the
-// three axes (FP64 Clenshaw) into Sxy[pp][l1*N + l2] and Sz[pp][l3]; then
-// wave w forms the 16 x 32 block of points 16w..16w+15 on the matrix cores.
-// Begins and ends with a barrier-separated use of tab (which the caller may
-// reuse after the final barrier).
+// All threads first find each point's box and evaluate S_n along each axis
+// (FP64 Clenshaw, one (point, axis) per thread) into tab[pp][axis][m]; then,
+// per half of PX_HALF points, wave w forms the 16 x 32 block of points
+// 16w..16w+15 on the matrix cores, with A = S_x S_y S_z (FP32 products)
+// split into three BF16 parts.  Ends with a barrier after which vs is
+// complete and tab is free for the caller.
 template <int N>
 __device__ __forceinline__ void px_point_vec(const ProxyArgs &a, const __bf16 *Ob, const int *s_ps, int nbx, int jb,
-                                             int base, int np, float *vs, float *tab)
+                                             int base, int np, float *vs, float *tab, int *s_bx)
 {
     typedef PxDims<N> D;
     typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-    float *Sxy = tab, *Sz = tab + PX_HALF * D::TS;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int pl = lane & 15, g = lane >> 4;
+    if (tid < np) {  // the last box of the pass that starts at or before the point
+        const int q = base + tid;
+        int lo = 0, hi = nbx - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_ps[mid] <= q) lo = mid;
+            else hi = mid - 1;
+        }
+        s_bx[tid] = lo;
+    }
+    __syncthreads();
+    const double ihalfL = 2.0 / a.len;
+    for (int t = tid; t < 3 * PX_PT; t += blockDim.x) {
+        const int pp = t % PX_PT, ax = t / PX_PT;
+        float *dst = tab + (pp * 3 + ax) * D::ZS;
+        if (pp < np) {
+            const uint32_t cc = (uint32_t)a.coord[jb + s_bx[pp]];
+            const double *xa = ax == 0 ? a.x : ax == 1 ? a.y : a.z;
+            double S[N];
+            sn_eval_n<N>(a.Tkz, ihalfL * (xa[base + pp] - a.ctab[(cc >> (10 * ax)) & 1023]), S);
+#pragma unroll
+            for (int m = 0; m < N; m++) dst[m] = (float)S[m];
+        } else {  // points past the pass: zeros (their results are not read)
+#pragma unroll
+            for (int m = 0; m < N; m++) dst[m] = 0.0f;
+        }
+    }
+    __syncthreads();
     for (int h0 = 0; h0 < PX_PT; h0 += PX_HALF) {
         if (h0 >= np) break;  // block-uniform
-        if (tid < PX_HALF) {
-            const int pp = h0 + tid;
-            if (pp < np) {
-                const int q = base + pp;
-                int lo = 0, hi = nbx - 1;  // the last box of the pass that starts at or before q
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (s_ps[mid] <= q) lo = mid;
-                    else hi = mid - 1;
-                }
-                const uint32_t cc = (uint32_t)a.coord[jb + lo];
-                const double ihalfL = 2.0 / a.len;
-                double Sx[N], Sy[N], Szd[N];
-                sn_eval_n<N>(a.Tkz, ihalfL * (a.x[q] - a.ctab[cc & 1023]), Sx);
-                sn_eval_n<N>(a.Tkz, ihalfL * (a.y[q] - a.ctab[(cc >> 10) & 1023]), Sy);
-                sn_eval_n<N>(a.Tkz, ihalfL * (a.z[q] - a.ctab[(cc >> 20) & 1023]), Szd);
-#pragma unroll
-                for (int l1 = 0; l1 < N; l1++)
-#pragma unroll
-                    for (int l2 = 0; l2 < N; l2++) Sxy[tid * D::TS + l1 * N + l2] = (float)(Sx[l1] * Sy[l2]);
-#pragma unroll
-                for (int m = 0; m < N; m++) Sz[tid * D::ZS + m] = (float)Szd[m];
-            } else {  // rows of points past the pass: zeros (their results are not read)
-                for (int m = 0; m < N * N; m++) Sxy[tid * D::TS + m] = 0.0f;
-                for (int m = 0; m < N; m++) Sz[tid * D::ZS + m] = 0.0f;
-            }
-        }
-        __syncthreads();
-        const int pt = 16 * wave + pl;  // this lane's point (row of A) within the half
+        const int pt = h0 + 16 * wave + pl;  // this lane's point (row of A)
+        const float *sx = tab + (pt * 3) * D::ZS, *sy = sx + D::ZS, *sz = sy + D::ZS;
         f32x4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};
 #pragma unroll 1
         for (int kc = 0; kc < D::KC; kc++) {
@@ -1384,8 +1386,8 @@ __device__ __forceinline__ void px_point_vec(const ProxyArgs &a, const __bf16 *O
                 const int k = 32 * kc + 8 * g + e;
                 float v = 0.0f;
                 if (k < D::N3) {
-                    const int l12 = k / N, l3 = k - l12 * N;
-                    v = Sxy[pt * D::TS + l12] * Sz[pt * D::ZS + l3];
+                    const int l12 = k / N, l3 = k - l12 * N, l1 = l12 / N, l2 = l12 - l1 * N;
+                    v = sx[l1] * sy[l2] * sz[l3];
                 }
                 const __bf16 p0 = (__bf16)v;
                 v -= (float)p0;
@@ -1418,8 +1420,8 @@ __device__ __forceinline__ void px_point_vec(const ProxyArgs &a, const __bf16 *O
             row[pl] = d0[v];
             row[16 + pl] = d1[v];
         }
-        __syncthreads();  // tab is free again
     }
+    __syncthreads();  // vs complete, tab free
 }
 
 template <int N>
@@ -1427,11 +1429,12 @@ __global__ __launch_bounds__(256) void p2m_proxy(ProxyArgs a)
 {
     typedef PxDims<N> D;
     constexpr int SWS = PX_K + 1;
-    constexpr int TAB = PX_HALF * (D::TS + D::ZS), UNI = TAB > PX_PT * SWS ? TAB : PX_PT * SWS;
+    constexpr int UNI = D::TAB > PX_PT * SWS ? D::TAB : PX_PT * SWS;
     __shared__ __attribute__((aligned(16))) __bf16 Ob[3 * PX_K * D::KS];
     __shared__ __attribute__((aligned(16))) float vs[PX_PT * PX_VS];
     __shared__ float uni[UNI];  // S_n tables of the rank-vector phase, then the pass's weights
     __shared__ int s_ps[PX_PT + 1];
+    __shared__ int s_bx[PX_PT];
     float *sw = uni;
     const int tid = threadIdx.x;
     px_stage_op<N>(a, false, Ob);
@@ -1451,7 +1454,7 @@ __global__ __launch_bounds__(256) void p2m_proxy(ProxyArgs a)
                 const int sub_end = min(base + PX_PT, p_end), np = sub_end - base;
                 __syncthreads();  // the previous pass's readers are done; Ob is staged
                 const int nbx = px_pass_boxes(a, s_ps, jb, bB, sub_end);
-                px_point_vec<N>(a, Ob, s_ps, nbx, jb, base, np, vs, uni);
+                px_point_vec<N>(a, Ob, s_ps, nbx, jb, base, np, vs, uni, s_bx);
                 for (int q = tid; q < PX_PT * PX_K; q += 256) {
                     const int pp = q / PX_K, cl = q - pp * PX_K;
                     sw[pp * SWS + cl] =
@@ -1496,8 +1499,9 @@ __global__ __launch_bounds__(256) void l2p_proxy(ProxyArgs a)
     typedef PxDims<N> D;
     __shared__ __attribute__((aligned(16))) __bf16 Ob[3 * PX_K * D::KS];
     __shared__ __attribute__((aligned(16))) float vs[PX_PT * PX_VS];
-    __shared__ float tab[PX_HALF * (D::TS + D::ZS)];
+    __shared__ float tab[D::TAB];
     __shared__ int s_ps[PX_PT + 1];
+    __shared__ int s_bx[PX_PT];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     px_stage_op<N>(a, true, Ob);
     const int64_t nranges = (a.npts + PX_PT - 1) / PX_PT;
@@ -1514,7 +1518,7 @@ __global__ __launch_bounds__(256) void l2p_proxy(ProxyArgs a)
                 const int sub_end = min(base + PX_PT, p_end), np = sub_end - base;
                 __syncthreads();
                 const int nbx = px_pass_boxes(a, s_ps, jb, bB, sub_end);
-                px_point_vec<N>(a, Ob, s_ps, nbx, jb, base, np, vs, tab);
+                px_point_vec<N>(a, Ob, s_ps, nbx, jb, base, np, vs, tab, s_bx);
                 // one wave per box: lane (cl, hf) holds half of the box's Pf row cl;
                 // the next box's row is loaded while this box's points run.
                 // Each output element gets one addition per launch, so the
