@@ -131,6 +131,11 @@ struct pbb_ops {
     std::vector<double> mset_emax;  // max |E'_d| per set
     std::vector<double> hm_E, hm_VTw, hm_Uw;  // host copies until uploaded
     double *dm_E = nullptr, *dm_VTw = nullptr, *dm_Uw = nullptr;
+    // the proxy passes' operators, split into three BF16 parts: per set, VT'
+    // then U', each [part][rank < 32][k < px_kp] (row stride px_kp + 8)
+    uint16_t *dm_px = nullptr;
+    int px_ks = 0;
+    size_t px_off(int set, int which) const { return ((size_t)set * 2 + which) * 3 * 32 * px_ks; }
     SplitE split[4];
     // VT'w and U'w of every set split into three BF16 parts for the mixed
     // modes' proxy / expansion products: set s at off[s], [3][R][32*nch[s]]
@@ -215,6 +220,43 @@ struct pbb_ops {
         dm_E = dupload(hm_E, "mixed E");
         dm_VTw = dupload(hm_VTw, "mixed VT");
         dm_Uw = dupload(hm_Uw, "mixed U");
+        {
+            // BF16 with round-to-nearest-even, and back
+            auto f2bf = [](float f) -> uint16_t {
+                uint32_t u;
+                std::memcpy(&u, &f, 4);
+                if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+                u += 0x7fffu + ((u >> 16) & 1u);
+                return (uint16_t)(u >> 16);
+            };
+            auto bf2f = [](uint16_t b) -> float {
+                const uint32_t u = (uint32_t)b << 16;
+                float f;
+                std::memcpy(&f, &u, 4);
+                return f;
+            };
+            const int n3 = h.n3, kp = 32 * ((n3 + 31) / 32);
+            px_ks = kp + 8;
+            std::vector<uint16_t> px((size_t)h.nsets * 2 * 3 * 32 * px_ks, 0);
+            for (int s = 0; s < h.nsets; s++) {
+                const MixedSet &m = mset[s];
+                for (int which = 0; which < 2; which++) {
+                    const int rank = which == 0 ? m.ks : m.kf;
+                    uint16_t *dst = px.data() + px_off(s, which);
+                    for (int r = 0; r < 32 && r < rank; r++)
+                        for (int k = 0; k < n3; k++) {
+                            double x = which == 0 ? hm_VTw[m.vt_off + r + (size_t)k * m.ks]
+                                                  : hm_Uw[m.u_off + k + (size_t)r * n3];
+                            for (int p = 0; p < 3; p++) {
+                                const uint16_t b = f2bf((float)x);
+                                dst[((size_t)p * 32 + r) * px_ks + k] = b;
+                                x -= (double)bf2f(b);
+                            }
+                        }
+                }
+            }
+            dm_px = dupload(px, "proxy operators");
+        }
         std::vector<double>().swap(hm_E);
         std::vector<double>().swap(hm_VTw);
         std::vector<double>().swap(hm_Uw);
@@ -357,7 +399,7 @@ struct pbb_ops {
             (void)hipGetDevice(&prev);
             (void)hipSetDevice(device);
             dfree(d_Tkz); dfree(d_Cw); dfree(d_VTw); dfree(d_Uw); dfree(d_E);
-            dfree(dm_E); dfree(dm_VTw); dfree(dm_Uw);
+            dfree(dm_E); dfree(dm_VTw); dfree(dm_Uw); dfree(dm_px);
             for (auto &x : split) dfree(x.d);
             dfree(vt3.d); dfree(u3.d);
             if (prev >= 0) (void)hipSetDevice(prev);
@@ -1816,7 +1858,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
             x.ncols = ncols; x.nb = (int)D.nb; x.rank = mset(l).ks; x.off = D.off; x.len = D.len;
             x.pstart = D.pstart; x.coord = D.coord; x.ctab = D.ctab; x.Tkz = o->d_Tkz;
             x.x = p->x; x.y = p->y; x.z = p->z; x.wsT = W.wsT;
-            x.op = o->dm_VTw + mset(l).vt_off; x.npts = p->N;
+            x.ob = o->dm_px + o->px_off(sl, 0); x.npts = p->N;
             x.P = p_level(l, x.ldp);
             if (prec == PBB_PREC_MIXED_F16X2) {  // the proxy pass also finds the level's max |P|
                 dev::launch_zero_u32(W.xmax + l, 1, st);
@@ -1991,7 +2033,7 @@ static void matvec_run(pbb_plan *p, pbb_ops *o, const Workspace &W, const double
             x.alpha = h.homog ? std::pow(1.0 / D.len, h.homogen_flipped) : 1.0;  // as the U product's scale
             x.pstart = D.pstart; x.coord = D.coord; x.ctab = D.ctab; x.Tkz = o->d_Tkz;
             x.x = p->x; x.y = p->y; x.z = p->z;
-            x.op = o->dm_Uw + mset(l).u_off; x.lp = W.lp; x.npts = p->N;
+            x.ob = o->dm_px + o->px_off(h.ops_set(l), 1); x.lp = W.lp; x.npts = p->N;
             x.Pf = pf_level(l, x.ldf);
             x.pf_vec = ((uintptr_t)(x.Pf + (size_t)D.off * ncols * x.ldf) & 15) == 0;
             dev::launch_l2p_proxy(x, n, p->N, st);

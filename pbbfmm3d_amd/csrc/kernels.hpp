@@ -116,7 +116,8 @@ struct ProxyArgs {
     const int32_t *pstart, *coord;
     const double *ctab, *Tkz, *x, *y, *z;
     const double *wsT;         // P2M: point-major sorted weights
-    const double *op;          // P2M: VT' (rank x n^3), L2P: U' (n^3 x rank), column-major
+    const void *ob;            // operator split into 3 BF16 parts, [part][rank < 32][k] (row stride
+                               // 32*ceil(n^3/32) + 8): P2M: VT' rows, L2P: U' columns
     float *P;                  // P2M: proxies, rows 0..min(ldp, PX_K)-1 written (those past are zero by construction)
     unsigned *xmax;            // P2M: atomic max of |P| as FP32 bits (cleared by the caller), or nullptr
     const float *Pf;           // L2P: M2L results

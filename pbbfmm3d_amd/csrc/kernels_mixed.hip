@@ -1253,13 +1253,14 @@ void launch_m2l_mixed(int variant, const M2LMixArgs &a, int ntiles, int ncolchun
 // ncols expansions S and F of the level's boxes, which at a few points per
 // box are several times the points' own data.
 // Range k holds the boxes whose first point lies in [k*PX_PT, (k+1)*PX_PT);
-// a workgroup stages the operator once and walks ranges k = blockIdx.x,
-// + gridDim.x, ..., each in passes of PX_PT points.
+// persistent workgroups walk ranges k = blockIdx.x, + gridDim.x, ..., each
+// in passes of PX_PT points.
 // The rank vectors of a pass (PX_PT points x 32 ranks, K = n^3) run on the
 // BF16 matrix cores in three parts per operand (6 products, FP32
 // accumulation), as the expansion route's proxy and expansion products do:
 // A = S_n of 16 points (products of the per-axis weights, split in
-// registers), B = the operator, split once into LDS as [part][rank][k].
+// registers), B = the operator, split once per operator set on the host into
+// [part][rank][k] (L2-resident, read straight into the fragments).
 // The box sums (P2M) and the Pf products (L2P) run in FP64 on the VALU.
 constexpr int PX_VS = PX_K + 4;  // LDS stride of a point's rank vector (16-byte aligned rows)
 constexpr int PX_HALF = 64;      // points whose S_n tables are staged at once (4 waves x 16)
@@ -1292,26 +1293,6 @@ __device__ __forceinline__ void px_range(const ProxyArgs &a, int64_t k, int &bA,
     bA = lo;
     const int b = bA + (int)threadIdx.x;
     bB = bA + __syncthreads_count(threadIdx.x < PX_PT && b < a.nb && (int64_t)a.pstart[b] < t1);
-}
-
-// the operator split into three BF16 parts, Ob[p][r][k] (ranks past a.rank
-// and k past n^3 zero); n3_major: the operator is n^3 x rank (U'), else
-// rank x n^3 (VT'), both column-major
-template <int N>
-__device__ __forceinline__ void px_stage_op(const ProxyArgs &a, bool n3_major, __bf16 *Ob)
-{
-    typedef PxDims<N> D;
-    for (int q = threadIdx.x; q < PX_K * D::KP; q += blockDim.x) {
-        const int r = q / D::KP, k = q - r * D::KP;
-        double x = 0.0;
-        if (r < a.rank && k < D::N3) x = n3_major ? a.op[k + (int64_t)r * D::N3] : a.op[r + (int64_t)k * a.rank];
-#pragma unroll
-        for (int p = 0; p < 3; p++) {
-            const __bf16 h = (__bf16)(float)x;
-            Ob[(p * PX_K + r) * D::KS + k] = h;
-            x -= (double)(float)h;
-        }
-    }
 }
 
 // The boxes of a pass: s_ps[j] = first point of box jb + j (j <= PX_PT, the
@@ -1430,14 +1411,13 @@ __global__ __launch_bounds__(256) void p2m_proxy(ProxyArgs a)
     typedef PxDims<N> D;
     constexpr int SWS = PX_K + 1;
     constexpr int UNI = D::TAB > PX_PT * SWS ? D::TAB : PX_PT * SWS;
-    __shared__ __attribute__((aligned(16))) __bf16 Ob[3 * PX_K * D::KS];
     __shared__ __attribute__((aligned(16))) float vs[PX_PT * PX_VS];
     __shared__ float uni[UNI];  // S_n tables of the rank-vector phase, then the pass's weights
     __shared__ int s_ps[PX_PT + 1];
     __shared__ int s_bx[PX_PT];
     float *sw = uni;
     const int tid = threadIdx.x;
-    px_stage_op<N>(a, false, Ob);
+    const __bf16 *Ob = (const __bf16 *)a.ob;
     const int64_t nranges = (a.npts + PX_PT - 1) / PX_PT;
     const int c = tid >> 3, rq = tid & 7;  // output phase: column c, ranks 4rq..4rq+3
     float pmax = 0.0f;                     // max |P| stored by this thread
@@ -1452,7 +1432,7 @@ __global__ __launch_bounds__(256) void p2m_proxy(ProxyArgs a)
             double acc[4] = {0.0, 0.0, 0.0, 0.0};  // the current box's sums (FP64), carried across passes
             for (int base = p_begin; base < p_end; base += PX_PT) {
                 const int sub_end = min(base + PX_PT, p_end), np = sub_end - base;
-                __syncthreads();  // the previous pass's readers are done; Ob is staged
+                __syncthreads();  // the previous pass's readers are done
                 const int nbx = px_pass_boxes(a, s_ps, jb, bB, sub_end);
                 px_point_vec<N>(a, Ob, s_ps, nbx, jb, base, np, vs, uni, s_bx);
                 for (int q = tid; q < PX_PT * PX_K; q += 256) {
@@ -1497,13 +1477,12 @@ template <int N>
 __global__ __launch_bounds__(256) void l2p_proxy(ProxyArgs a)
 {
     typedef PxDims<N> D;
-    __shared__ __attribute__((aligned(16))) __bf16 Ob[3 * PX_K * D::KS];
     __shared__ __attribute__((aligned(16))) float vs[PX_PT * PX_VS];
     __shared__ float tab[D::TAB];
     __shared__ int s_ps[PX_PT + 1];
     __shared__ int s_bx[PX_PT];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    px_stage_op<N>(a, true, Ob);
+    const __bf16 *Ob = (const __bf16 *)a.ob;
     const int64_t nranges = (a.npts + PX_PT - 1) / PX_PT;
     const int cl = lane >> 1, hf = lane & 1;  // output phase: column cl, ranks 16hf..16hf+15
     for (int64_t k = blockIdx.x; k < nranges; k += gridDim.x) {
