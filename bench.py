@@ -51,7 +51,7 @@ def dtype_label(precision, flags):
 # bytes per M2L launch (FETCH_SIZE x 2 + WRITE_SIZE x 1, calibrated on a plain
 # copy), averaged over the 6 launches of one f16x2 matvec of the default
 # workload (45.09 GB per matvec)
-M2L_TRAFFIC_F16X2_C3 = 7.514e9
+M2L_TRAFFIC_F16X2_C3 = 7.516e9
 TRAFFIC_SOURCE = "profiles/m2l_f16x2_c3_gaussian_traffic.txt"
 
 
