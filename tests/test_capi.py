@@ -125,3 +125,18 @@ def test_matvec_prec_rejects_bad_arguments():
     with pytest.raises(ValueError):
         matvec_host(ops, x, w, 1, precision=5)
     assert not out.any()
+
+
+def test_plan_flags_are_checked_before_device_work():
+    # an unknown flag bit fails as an argument error; the known flags pass the
+    # check (and then, without a device, fail as PBB_ERR_NO_DEVICE)
+    L = capi.lib()
+    ops = pbb.Operators(pbb.gaussian(0.3), 1.0, 3, 4, 1e-6)
+    dp = C.POINTER(C.c_double)
+    x = np.random.default_rng(2).random((64, 3)) - 0.5
+    h = C.c_void_p()
+    assert L.pbb_plan_create_ex(ops.handle, x.ctypes.data_as(dp), 64, 8, C.byref(h)) == capi.PBB_ERR_INVALID_ARG
+    assert b"flags" in L.pbb_last_error()
+    if L.pbb_device_count() == 0:
+        for f in (capi.PLAN_EXPANSION_ROUTE, capi.PLAN_REJECT_OUTSIDE | capi.PLAN_EXPANSION_ROUTE):
+            assert L.pbb_plan_create_ex(ops.handle, x.ctypes.data_as(dp), 64, f, C.byref(h)) == capi.PBB_ERR_NO_DEVICE
