@@ -12,8 +12,10 @@
 // Layouts:
 //   Es  [set][d][chunk][plane][R][32]  split E_d, rows padded to R = 16*RB,
 //                                     k padded to chunks of 32, zeros
-//   P32 [box][col][kpad]               FP32 proxies, k padded with zeros
-//   Pf32[box][col][rf]                 FP32 result
+//   P32 [box][col][kpad]               FP32 proxies, k padded with zeros (the
+//                                      proxy-pass levels keep rows of 16 or 32
+//                                      inside their own region)
+//   Pf32[box][col][rf]                 FP32 result (proxy-pass levels: rows of 32)
 // Fragment layout of v_mfma_f32_16x16x32_{bf16,f16} (lane l of 64, e < 8):
 //   A[i][k]: i = l % 16, k = 8*(l/16) + e     B[k][j]: j = l % 16, k = 8*(l/16) + e
 //   D[i][j]: j = l % 16, i = 4*(l/16) + v     (v = 0..3, FP32)
