@@ -1,5 +1,7 @@
 """Mixed-precision matvec (split 16-bit MFMA M2L, FP32 accumulation) against
-the FP64 GPU path and the FP64 direct sum.
+the FP64 GPU path, whose own parity with the oracle is checked here too (the
+FP64 direct sum is the reference of tests/test_gpu_workloads.py, at 4M
+points).
 
 Bars (relative L2): bf16 x3 and f16 x2 stay within 1e-6 of the FP64 GPU
 path; bf16 x2 (16 significand bits) within 1e-4.
