@@ -1500,10 +1500,12 @@ __global__ __launch_bounds__(256) void l2p_proxy(ProxyArgs a)
                 __syncthreads();
                 const int nbx = px_pass_boxes(a, s_ps, jb, bB, sub_end);
                 px_point_vec<N>(a, Ob, s_ps, nbx, jb, base, np, vs, tab, s_bx);
-                // one wave per box: lane (cl, hf) holds half of the box's Pf row cl;
-                // the next box's row is loaded while this box's points run.
-                // Each output element gets one addition per launch, so the
-                // no-return atomic adds leave the sums deterministic.
+                // every wave walks the pass's boxes and takes every 4th point of
+                // each (a box of many points keeps all waves busy); lane (cl, hf)
+                // holds half of the box's Pf row cl, and the next box's row is
+                // loaded while this box's points run.  Each output element gets
+                // one addition per launch, so the no-return atomic adds leave the
+                // sums deterministic.
                 auto load_row = [&](int j, float (&pr)[16]) __attribute__((always_inline)) {
                     const int jj = min(j, nbx - 1);
                     const float *row = a.Pf + ((a.off + jb + jj) * a.ncols + c0 + min(cl, cn - 1)) * a.ldf + 16 * hf;
@@ -1522,14 +1524,14 @@ __global__ __launch_bounds__(256) void l2p_proxy(ProxyArgs a)
                     }
                 };
                 float nxt[16];
-                if (wave < nbx) load_row(wave, nxt);
-                for (int j = wave; j < nbx; j += 4) {
+                load_row(0, nxt);  // (nbx >= 1)
+                for (int j = 0; j < nbx; j++) {
                     const int lo = max(s_ps[j], base) - base, hi = min(s_ps[j + 1], sub_end) - base;
                     double pf[16];  // rows past the rank are not written by the M2L
 #pragma unroll
                     for (int k = 0; k < 16; k++) pf[k] = 16 * hf + k < a.rank ? (double)nxt[k] : 0.0;
-                    if (j + 4 < nbx) load_row(j + 4, nxt);
-                    for (int qq = lo; qq < hi; qq++) {
+                    if (j + 1 < nbx) load_row(j + 1, nxt);
+                    for (int qq = lo + wave; qq < hi; qq += 4) {
                         const float4 *zv = (const float4 *)&vs[qq * PX_VS + 16 * hf];
                         double s = 0.0;
 #pragma unroll
