@@ -1500,12 +1500,14 @@ __global__ __launch_bounds__(256) void l2p_proxy(ProxyArgs a)
                 __syncthreads();
                 const int nbx = px_pass_boxes(a, s_ps, jb, bB, sub_end);
                 px_point_vec<N>(a, Ob, s_ps, nbx, jb, base, np, vs, tab, s_bx);
-                // every wave walks the pass's boxes and takes every 4th point of
-                // each (a box of many points keeps all waves busy); lane (cl, hf)
-                // holds half of the box's Pf row cl, and the next box's row is
-                // loaded while this box's points run.  Each output element gets
-                // one addition per launch, so the no-return atomic adds leave the
-                // sums deterministic.
+                // A pass of few boxes (more than 16 points each on average): every
+                // wave walks all of them and takes every 4th point of each, so a
+                // big box keeps all waves busy.  A pass of many small boxes: one
+                // wave per box (the waves do not all load every box's row).
+                // Lane (cl, hf) holds half of the box's Pf row cl, and the wave's
+                // next row is loaded while this box's points run.  Each output
+                // element gets one addition per launch, so the no-return atomic
+                // adds leave the sums deterministic.
                 auto load_row = [&](int j, float (&pr)[16]) __attribute__((always_inline)) {
                     const int jj = min(j, nbx - 1);
                     const float *row = a.Pf + ((a.off + jb + jj) * a.ncols + c0 + min(cl, cn - 1)) * a.ldf + 16 * hf;
@@ -1523,15 +1525,17 @@ __global__ __launch_bounds__(256) void l2p_proxy(ProxyArgs a)
                         for (int k = 0; k < 16; k++) pr[k] = row[k];
                     }
                 };
+                const bool split = nbx < 8;  // the same for the whole workgroup
+                const int j0 = split ? 0 : wave, js = split ? 1 : 4, q0 = split ? wave : 0, qs = split ? 4 : 1;
                 float nxt[16];
-                load_row(0, nxt);  // (nbx >= 1)
-                for (int j = 0; j < nbx; j++) {
+                if (j0 < nbx) load_row(j0, nxt);
+                for (int j = j0; j < nbx; j += js) {
                     const int lo = max(s_ps[j], base) - base, hi = min(s_ps[j + 1], sub_end) - base;
                     double pf[16];  // rows past the rank are not written by the M2L
 #pragma unroll
                     for (int k = 0; k < 16; k++) pf[k] = 16 * hf + k < a.rank ? (double)nxt[k] : 0.0;
-                    if (j + 1 < nbx) load_row(j + 1, nxt);
-                    for (int qq = lo + wave; qq < hi; qq += 4) {
+                    if (j + js < nbx) load_row(j + js, nxt);
+                    for (int qq = lo + q0; qq < hi; qq += qs) {
                         const float4 *zv = (const float4 *)&vs[qq * PX_VS + 16 * hf];
                         double s = 0.0;
 #pragma unroll
