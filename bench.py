@@ -51,8 +51,8 @@ def dtype_label(precision, flags):
 # bytes per M2L launch (FETCH_SIZE x 2 + WRITE_SIZE x 1, calibrated on a plain
 # copy), averaged over the 6 launches of one f16x2 matvec of the default
 # workload (45.09 GB per matvec)
-M2L_TRAFFIC_F16X2_C3 = 7.516e9
-TRAFFIC_SOURCE = "profiles/m2l_f16x2_c3_gaussian_traffic.txt"
+M2L_TRAFFIC_F16X2_4M_CLUSTERED = 7.516e9
+TRAFFIC_SOURCE = "profiles/m2l_f16x2_4m_clustered_gaussian_traffic.txt"
 
 
 def clustered_points(n, seed):
@@ -250,9 +250,10 @@ def main():
                          "direct sum ('' to skip)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 256-cloud forward+adjoint batch measured beside the headline")
-    ap.add_argument("--workload", default="c3", choices=["c3", "c5"],
-                    help="c3: the headline single cloud (default); c5: 256 independent clouds in one "
-                         "HIP-graph batch (a separate line, not the headline metric)")
+    ap.add_argument("--workload", default="clustered_4m", choices=["clustered_4m", "batch_256_clouds"],
+                    help="clustered_4m: the headline single cloud of 4M clustered points (default); "
+                         "batch_256_clouds: 256 independent clouds in one HIP-graph batch (a separate "
+                         "line, not the headline metric)")
     ap.add_argument("--m2l-tiles", default="",
                     help="JSON {level: [width, cols_per_box, flags]} of the headline precision's M2L tiles "
                          "(as the tuner reports them) set instead of running the tuner (profiled runs)")
@@ -280,8 +281,8 @@ def main():
     import pbbfmm3d_amd as pbb
     from pbbfmm3d_amd import _capi as capi
 
-    if args.workload == "c5":
-        run_c5(args, torch, pbb, capi)
+    if args.workload == "batch_256_clouds":
+        run_batch_line(args, torch, pbb, capi)
         return
 
     dev = torch.device("cuda:0")
@@ -438,7 +439,7 @@ def main():
     # profiles/; null for other workloads
     traffic = None
     if (args.precision, N, nc, args.level, args.order, args.kernel) == ("f16x2", 4_000_000, 32, 7, 6, "gaussian"):
-        traffic = M2L_TRAFFIC_F16X2_C3
+        traffic = M2L_TRAFFIC_F16X2_4M_CLUSTERED
     roofline = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
                 "frac": round(achieved / peak, 4), "traffic": traffic,
                 "traffic_source": TRAFFIC_SOURCE if traffic else None,
@@ -629,9 +630,9 @@ def batch_clouds(args, torch, pbb, capi, log_prefix="batch"):
             "tune_s": round(t_tune, 2)}
 
 
-def run_c5(args, torch, pbb, capi):
+def run_batch_line(args, torch, pbb, capi):
     """The 256-cloud batch as a line of its own (not the headline metric)."""
-    r = batch_clouds(args, torch, pbb, capi, "c5")
+    r = batch_clouds(args, torch, pbb, capi, "batch_256_clouds")
     line = {
         "metric": "targets/s, 256-cloud batch, forward + adjoint in one HIP graph", "value": r["targets_per_s"],
         "unit": "targets/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

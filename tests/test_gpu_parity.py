@@ -29,7 +29,7 @@ def _run(kernel, L, level, n, eps, xyz, w, ncols):
     return ops, plan, got, ref
 
 
-def test_laplacian_c1_shape(gpu):
+def test_laplacian_20k_uniform_level4(gpu):
     # the reference's CPU-runnable case: 20k uniform, level 4, order 4, eps 1e-5
     xyz = uniform_points(20000, seed=0)
     w = weights(20000, 1, seed=1)
@@ -61,7 +61,7 @@ def test_polynomial_kernels_exact(gpu, kernel):
     assert orc.rel_err(got, d) <= 1e-12
 
 
-def test_c2_shape_ncols8(gpu):
+def test_uniform_order5_ncols8(gpu):
     xyz = uniform_points(40000, seed=6)
     w = weights(40000, 8, seed=7)
     ops, plan, got, ref = _run(pbb.laplacian(), 1.0, 4, 5, 1e-7, xyz, w, 8)
