@@ -799,6 +799,8 @@ PBB_API pbb_status pbb_ops_create_from_arrays(const pbb_kernel_desc *kernel, dou
     });
 }
 
+}  // extern "C"
+
 // ---- operator cache files in the reference's format ----------------------
 //   K file: double box length L, int32 tree level (StartPrecompute,
 //           H2_3D_Tree.cpp:141-143), then per operator set the 316 E blocks,
@@ -851,6 +853,7 @@ std::string cache_params(const pbb_kernel_desc &k, double L, int tree_level, int
     return buf;
 }
 }  // namespace
+extern "C" {
 
 PBB_API pbb_status pbb_ops_write_cache(const pbb_ops *ops, const char *k_path, const char *u_path, const char *v_path)
 {
